@@ -1,0 +1,306 @@
+"""Python host interface over libandromeda_hip (ctypes), mirroring the reference's operator
+boundaries: integral matrices (sdFormIntegralMatrices / sdFormIntegralMatricesStokes), the LSQR
+routines of src/lib_lsqr.f, and the resistance drivers of src/lib_particle.f90.
+
+The native library is required: every call goes to the HIP implementation, and a missing or
+unloadable library raises immediately (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .mesh import Mesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libandromeda_hip.so")
+
+ROW_MAJOR, COL_MAJOR = 0, 1
+FORM_EXTERIOR, FORM_BOUNDED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_LEARNED = 0, 1, 2
+MLP_NPARAM = 9288
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+_L = C.POINTER(C.c_int64)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("nnodes", C.c_int32), ("nelem", C.c_int32), ("nwalls", C.c_int32),
+                ("xyz", _D), ("con", _I), ("bcid", _I), ("normmul", _D),
+                ("index_base", C.c_int32), ("itri", C.c_int32)]
+
+
+class _LsqrOpts(C.Structure):
+    _fields_ = [("maxit", C.c_int32), ("eps", C.c_double), ("precon", C.c_int32)]
+
+
+class _LsqrStats(C.Structure):
+    _fields_ = [("nit", C.c_int32 * 16), ("istop", C.c_int32 * 16), ("seconds", C.c_double)]
+
+
+class _ResOpts(C.Structure):
+    _fields_ = [("form", C.c_int32), ("precision", C.c_int32), ("mu", C.c_double), ("eps", C.c_double),
+                ("maxit", C.c_int32), ("center", C.c_double * 3)]
+
+
+class _ResReport(C.Structure):
+    _fields_ = [("nit", C.c_int32 * 6), ("status", C.c_int32), ("err", C.c_double * 2),
+                ("t_assemble", C.c_double), ("t_solve", C.c_double), ("quad_points", C.c_int64),
+                ("pairs_learned", C.c_int64), ("pairs_fallback", C.c_int64)]
+
+
+class AndromedaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libandromeda_hip.so; raises if it is absent (build with andromeda_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AndromedaError(f"{path} is missing: build it with `python -m andromeda_amd.build`")
+    lib = C.CDLL(path)
+    lib.andr_version.restype = C.c_char_p
+    lib.andr_last_error.restype = C.c_char_p
+    lib.andr_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
+    lib.andr_destroy.argtypes = [C.c_void_p]
+    lib.andr_last_error.argtypes = [C.c_void_p]
+    lib.andr_assemble_laplace.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, _D, _D, _D]
+    lib.andr_assemble_stokes.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.c_double,
+                                         C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), _D]
+    lib.andr_lsqr_dense.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _D, C.c_int32, _D, _D,
+                                    C.POINTER(_LsqrOpts), C.POINTER(_LsqrStats)]
+    lib.andr_resistance.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.POINTER(_ResOpts), _D, C.POINTER(_ResReport)]
+    lib.andr_resistance_batch.argtypes = [C.c_void_p, C.c_int32, C.POINTER(_Mesh), C.POINTER(_ResOpts), _D,
+                                          C.POINTER(_ResReport)]
+    lib.lsqr_precon_.argtypes = [_I, _I, _L, _D, _L, _I, _D]
+    lib.lsqr_solve_.argtypes = [_I, _I, _L, _I, _D, _I, _I, _D, _L, _I, _D, _D, _D]
+    lib.andr_quad_training_pairs.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_double, C.c_double, C.c_int64,
+                                             _F, _I, _L]
+    lib.andr_mlp_train.argtypes = [C.c_void_p, C.c_int64, _F, _I, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                   _F, _F]
+    lib.andr_mlp_infer.argtypes = [C.c_void_p, C.c_int64, _F, _F, _F]
+    lib.andr_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int64, _F, _I, _F, _F, _F]
+    lib.andr_set_quadrature_model.argtypes = [C.c_void_p, _F, C.c_float, C.c_double]
+    _lib = lib
+    return lib
+
+
+EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "andr_assemble_laplace",
+            "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
+            "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
+            "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model"]
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_F)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_I)
+
+
+class _MeshView:
+    """Holds the arrays referenced by the C struct."""
+
+    def __init__(self, mesh: Mesh, normmul=None, itri: int = 5):
+        m = mesh.contiguous()
+        self.m = m
+        self.normmul = np.ascontiguousarray(normmul if normmul is not None else np.ones(max(m.nwalls, 1)),
+                                            dtype=np.float64)
+        self.s = _Mesh(m.nnodes, m.nelem, max(m.nwalls, 1), _p(m.xyz), _pi(m.con), _pi(m.bcid),
+                       _p(self.normmul), 0, itri)
+
+
+@dataclass
+class ResistanceResult:
+    FT: np.ndarray           # (6, 6): row k = (force, torque) of rigid-body problem k
+    nit: np.ndarray          # (6,) iterations
+    status: int
+    err: tuple
+    t_assemble: float
+    t_solve: float
+    quad_points: int
+    pairs_learned: int
+    pairs_fallback: int
+
+
+class Context:
+    """One GPU + one HIP stream (andr_create / andr_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.andr_create(C.byref(h), device)
+        if rc != 0:
+            raise AndromedaError(f"andr_create failed ({rc}): no usable HIP device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.andr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc < 0:
+            msg = self.lib.andr_last_error(self.h)
+            raise AndromedaError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+        return rc
+
+    # ---- seam 1 ---------------------------------------------------------------------------------
+    def assemble_laplace(self, mesh: Mesh, normmul=None, itri: int = 5, layout: int = ROW_MAJOR):
+        mv = _MeshView(mesh, normmul, itri)
+        nn, ne = mv.m.nnodes, mv.m.nelem
+        nsp = nn + ne
+        shapeH = (nsp, nn) if layout == ROW_MAJOR else (nn, nsp)
+        shapeG = (nsp, ne) if layout == ROW_MAJOR else (ne, nsp)
+        H = np.zeros(shapeH)
+        G = np.zeros(shapeG)
+        err = np.zeros(2)
+        self._check(self.lib.andr_assemble_laplace(self.h, C.byref(mv.s), layout, _p(H), _p(G), _p(err)),
+                    "andr_assemble_laplace")
+        if layout == COL_MAJOR:
+            H, G = H.T, G.T
+        return H, G, err
+
+    def assemble_stokes(self, mesh: Mesh, normmul=None, itri: int = 5, extra: float = 0.0, pressure: bool = True):
+        mv = _MeshView(mesh, normmul, itri)
+        nn, ne = mv.m.nnodes, mv.m.nelem
+        nsp = nn + ne
+        T = [np.zeros((nsp, nn)) for _ in range(6)]
+        G = [np.zeros((nsp, ne)) for _ in range(6)]
+        prT = [np.zeros((nsp, nn)) for _ in range(3)] if pressure else None
+        prG = [np.zeros((nsp, ne)) for _ in range(3)] if pressure else None
+        err = np.zeros(2)
+        Tp = (_D * 6)(*[_p(x) for x in T])
+        Gp = (_D * 6)(*[_p(x) for x in G])
+        pTp = (_D * 3)(*[_p(x) for x in prT]) if pressure else (_D * 3)()
+        pGp = (_D * 3)(*[_p(x) for x in prG]) if pressure else (_D * 3)()
+        self._check(self.lib.andr_assemble_stokes(self.h, C.byref(mv.s), ROW_MAJOR, extra, Tp, Gp, pTp, pGp, _p(err)),
+                    "andr_assemble_stokes")
+        return T, G, prT, prG, err
+
+    # ---- seam 2 ---------------------------------------------------------------------------------
+    def lsqr_dense(self, A, B, X0=None, maxit: int = 0, eps: float = 1e-15, precon: bool = True):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        m, n = A.shape
+        B = np.asarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)  # column k contiguous
+        Xc = np.zeros((nrhs, n)) if X0 is None else np.ascontiguousarray(np.asarray(X0, dtype=np.float64).reshape(n, nrhs).T)
+        opts = _LsqrOpts(maxit, eps, 1 if precon else 0)
+        st = _LsqrStats()
+        self._check(self.lib.andr_lsqr_dense(self.h, m, n, _p(A), nrhs, _p(Bc), _p(Xc), C.byref(opts), C.byref(st)),
+                    "andr_lsqr_dense")
+        return Xc.T.copy(), {"nit": list(st.nit)[:nrhs], "istop": list(st.istop)[:nrhs], "seconds": st.seconds}
+
+    # ---- seam 3 ---------------------------------------------------------------------------------
+    def resistance_batch(self, meshes, form: int = FORM_EXTERIOR, precision: int = PREC_FP64, mu: float = 1.0,
+                         eps: float = 0.0, maxit: int = 0, center=(0.0, 0.0, 0.0), itri: int = 5, normmul=None):
+        views = [_MeshView(m, normmul, itri) for m in meshes]
+        arr = (_Mesh * len(views))(*[v.s for v in views])
+        opts = _ResOpts(form, precision, mu, eps, maxit, (C.c_double * 3)(*center))
+        FT = np.zeros((len(views), 36))
+        reps = (_ResReport * len(views))()
+        self._check(self.lib.andr_resistance_batch(self.h, len(views), arr, C.byref(opts), _p(FT), reps),
+                    "andr_resistance_batch")
+        out = []
+        for i, r in enumerate(reps):
+            out.append(ResistanceResult(FT[i].reshape(6, 6).copy(), np.array(list(r.nit)), r.status,
+                                        (r.err[0], r.err[1]), r.t_assemble, r.t_solve, r.quad_points,
+                                        r.pairs_learned, r.pairs_fallback))
+        return out
+
+    def resistance(self, mesh, **kw) -> ResistanceResult:
+        return self.resistance_batch([mesh], **kw)[0]
+
+    # ---- learned quadrature ---------------------------------------------------------------------
+    def quad_training_pairs(self, mesh, near_ratio: float = 4.0, rel_tol: float = 1e-8, max_pairs: int = 200000,
+                            itri: int = 5):
+        mv = _MeshView(mesh, None, itri)
+        feats = np.zeros((max_pairs, 6), dtype=np.float32)
+        labels = np.zeros(max_pairs, dtype=np.int32)
+        npairs = C.c_int64(0)
+        self._check(self.lib.andr_quad_training_pairs(self.h, C.byref(mv.s), near_ratio, rel_tol, max_pairs,
+                                                      _pf(feats), _pi(labels), C.byref(npairs)),
+                    "andr_quad_training_pairs")
+        k = npairs.value
+        return feats[:k].copy(), labels[:k].copy()
+
+    def mlp_train(self, feats, labels, params, epochs: int = 10, batch: int = 1024, lr: float = 1e-3, seed: int = 0):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        p = np.ascontiguousarray(params, dtype=np.float32).copy()
+        loss = np.zeros(epochs, dtype=np.float32)
+        self._check(self.lib.andr_mlp_train(self.h, feats.shape[0], _pf(feats), _pi(labels), epochs, batch, lr, seed,
+                                            _pf(p), _pf(loss)), "andr_mlp_train")
+        return p, loss
+
+    def mlp_infer(self, feats, params):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        probs = np.zeros((feats.shape[0], 8), dtype=np.float32)
+        self._check(self.lib.andr_mlp_infer(self.h, feats.shape[0], _pf(feats), _pf(params), _pf(probs)),
+                    "andr_mlp_infer")
+        return probs
+
+    def mlp_loss_grad(self, feats, labels, params):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._check(self.lib.andr_mlp_loss_grad(self.h, feats.shape[0], _pf(feats), _pi(labels), _pf(params),
+                                                _pf(loss), _pf(grad)), "andr_mlp_loss_grad")
+        return float(loss[0]), grad
+
+    def set_quadrature_model(self, params, confidence: float = 0.9, near_ratio: float = 4.0):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.andr_set_quadrature_model(self.h, _pf(params), confidence, near_ratio),
+                    "andr_set_quadrature_model")
+
+
+# ---- Fortran-style LSQR (lsqr_precon_ / lsqr_solve_), 1-based CSR --------------------------------
+def lsqr_precon(m: int, n: int, ia, ja, a):
+    """Calls lsqr_precon_ exactly as the Fortran host would: a is scaled in place, q returned."""
+    lib = load_library()
+    q = np.zeros(n)
+    mm, nn_, nnz = C.c_int32(m), C.c_int32(n), C.c_int64(a.size)
+    lib.lsqr_precon_(C.byref(mm), C.byref(nn_), C.byref(nnz), _p(q), ia.ctypes.data_as(_L), _pi(ja), _p(a))
+    return q
+
+
+def lsqr_solve(m: int, n: int, ia, ja, a, q, b, x, maxit: int = 0, eps: float = 1e-15):
+    lib = load_library()
+    mm, nn_, nnz = C.c_int32(m), C.c_int32(n), C.c_int64(a.size)
+    mi, ep, nit, ierr = C.c_int32(maxit), C.c_double(eps), C.c_int32(0), C.c_int32(0)
+    lib.lsqr_solve_(C.byref(mm), C.byref(nn_), C.byref(nnz), C.byref(mi), C.byref(ep), C.byref(nit), C.byref(ierr),
+                    _p(q), ia.ctypes.data_as(_L), _pi(ja), _p(a), _p(b), _p(x))
+    return {"nit": nit.value, "ierr": ierr.value, "maxit": mi.value}

@@ -1,0 +1,334 @@
+// C-ABI entry points of libandromeda_hip: context handling, the three seams (integral matrices,
+// least squares, resistance) and the Fortran-linkable LSQR routines.
+#include "andr_internal.h"
+#include "andr_assemble.h"
+#include "andr_solve.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+double andr_now() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+int andr_fail(andr_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  else fprintf(stderr, "andromeda_hip: %s\n", buf);
+  return code;
+}
+
+extern "C" const char *andr_version(void) { return "andromeda-hip 0.1 (gfx950)"; }
+
+extern "C" int andr_create(andr_ctx **out, int32_t device) {
+  if (!out) return ANDR_ERR_ARG;
+  *out = nullptr;
+  andr_ctx *ctx = new andr_ctx();
+  ctx->device = device;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    fprintf(stderr, "andromeda_hip: no HIP device available (%s)\n", hipGetErrorString(e));
+    delete ctx;
+    return ANDR_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    delete ctx;
+    return ANDR_ERR_ARG;
+  }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return ANDR_ERR_HIP;
+  }
+  if (andr_upload_rules(ctx) != ANDR_OK) {
+    fprintf(stderr, "andromeda_hip: %s\n", ctx->err.c_str());
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ANDR_ERR_HIP;
+  }
+  *out = ctx;
+  return ANDR_OK;
+}
+
+extern "C" int andr_destroy(andr_ctx *ctx) {
+  if (!ctx) return ANDR_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->d_model) (void)hipFree(ctx->d_model);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return ANDR_OK;
+}
+
+extern "C" const char *andr_last_error(const andr_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+// ---- seam 1 ----------------------------------------------------------------------------------
+static int max_err(andr_ctx *ctx, const double *d_err, int nn, int ne, double err[2]) {
+  std::vector<double> h(nn + ne);
+  ANDR_HIP(ctx, hipMemcpyAsync(h.data(), d_err, sizeof(double) * (nn + ne), hipMemcpyDeviceToHost, ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double eu = 0.0, eq = 0.0;
+  for (int i = 0; i < nn; ++i) eu = std::max(eu, h[i]);
+  for (int i = 0; i < ne; ++i) eq = std::max(eq, h[nn + i]);
+  if (err) {
+    err[0] = eu;
+    err[1] = std::max(eu, eq);
+  }
+  return ANDR_OK;
+}
+
+extern "C" int andr_assemble_laplace(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double *H, double *G,
+                                     double err[2]) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (layout != ANDR_ROW_MAJOR && layout != ANDR_COL_MAJOR) return andr_fail(ctx, ANDR_ERR_ARG, "bad layout");
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  int rc = andr_body_upload(ctx, mesh, b, true);
+  if (rc) return rc;
+  const int nn = b.g.nn, ne = b.g.ne, nsp = nn + ne;
+  DevBuf<double> dH, dG, dE;
+  if (dH.alloc((size_t)nsp * nn) || dG.alloc((size_t)nsp * ne) || dE.alloc(nsp))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "integral matrices (%d x %d)", nsp, nn + ne);
+  rc = andr_launch_laplace_rows(ctx, b, dH.p, dG.p, dE.p);
+  if (rc) return rc;
+  rc = andr_copy_out(ctx, dH.p, nsp, nn, layout, H);
+  if (rc) return rc;
+  rc = andr_copy_out(ctx, dG.p, nsp, ne, layout, G);
+  if (rc) return rc;
+  return max_err(ctx, dE.p, nn, ne, err);
+}
+
+extern "C" int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double extra_free_term,
+                                    double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (layout != ANDR_ROW_MAJOR && layout != ANDR_COL_MAJOR) return andr_fail(ctx, ANDR_ERR_ARG, "bad layout");
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  int rc = andr_body_upload(ctx, mesh, b, true);
+  if (rc) return rc;
+  const int nn = b.g.nn, ne = b.g.ne, nsp = nn + ne;
+  const size_t sT = (size_t)nsp * nn, sG = (size_t)nsp * ne;
+  DevBuf<double> dT, dG, dpT, dpG, dE;
+  if (dT.alloc(6 * sT) || dG.alloc(6 * sG) || dpT.alloc(3 * sT) || dpG.alloc(3 * sG) || dE.alloc(nsp))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "Stokes integral matrices (%zu doubles)", 9 * (sT + sG));
+  rc = andr_launch_stokes_rows(ctx, b, extra_free_term, dT.p, dG.p, dpT.p, dpG.p, dE.p);
+  if (rc) return rc;
+  for (int k = 0; k < 6; ++k) {
+    if (T && T[k] && (rc = andr_copy_out(ctx, dT.p + k * sT, nsp, nn, layout, T[k]))) return rc;
+    if (G && G[k] && (rc = andr_copy_out(ctx, dG.p + k * sG, nsp, ne, layout, G[k]))) return rc;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (prT && prT[k] && (rc = andr_copy_out(ctx, dpT.p + k * sT, nsp, nn, layout, prT[k]))) return rc;
+    if (prG && prG[k] && (rc = andr_copy_out(ctx, dpG.p + k * sG, nsp, ne, layout, prG[k]))) return rc;
+  }
+  return max_err(ctx, dE.p, nn, ne, err);
+}
+
+// ---- seam 2 ----------------------------------------------------------------------------------
+__global__ void k_csr_to_dense(int m, int64_t n, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
+                               const double *__restrict__ a, double *__restrict__ D) {
+  const int i = blockIdx.x;
+  for (int64_t p = ia[i] - 1 + threadIdx.x; p < ia[i + 1] - 1; p += blockDim.x) D[(size_t)i * n + (ja[p] - 1)] = a[p];
+}
+
+__global__ void k_pack_col(int n, const double *__restrict__ v, double *__restrict__ V) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * ANDR_NB) return;
+  V[t] = (t & 7) == 0 ? v[t >> 3] : 0.0;
+}
+__global__ void k_unpack_col(int n, const double *__restrict__ V, double *__restrict__ v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  v[t] = V[(size_t)t * ANDR_NB];
+}
+
+namespace {
+struct LsqrCache {
+  const double *key_a = nullptr;
+  int64_t nnz = 0;
+  int m = 0, n = 0;
+  DevBuf<double> D, q;
+  std::vector<std::pair<int64_t, double>> probe;
+};
+std::mutex g_mu;
+andr_ctx *g_ctx = nullptr;
+LsqrCache g_cache;
+
+void probe_fill(LsqrCache &c, const double *a, int64_t nnz) {
+  c.probe.clear();
+  const int64_t np = std::min<int64_t>(nnz, 64);
+  for (int64_t i = 0; i < np; ++i) {
+    int64_t p = (nnz - 1) * i / std::max<int64_t>(np - 1, 1);
+    c.probe.push_back({p, a[p]});
+  }
+}
+bool probe_ok(const LsqrCache &c, const double *a) {
+  for (auto &pr : c.probe)
+    if (a[pr.first] != pr.second) return false;
+  return true;
+}
+int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
+               DevBuf<double> &D) {
+  DevBuf<int64_t> dia;
+  DevBuf<int32_t> dja;
+  DevBuf<double> da;
+  if (D.alloc((size_t)m * n) || dia.alloc(m + 1) || dja.alloc(nnz) || da.alloc(nnz))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR matrix %d x %d", m, n);
+  ANDR_HIP(ctx, hipMemsetAsync(D.p, 0, sizeof(double) * (size_t)m * n, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(dia.p, ia, sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(dja.p, ja, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(da.p, a, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, ctx->stream, m, (int64_t)n, dia.p, dja.p, da.p, D.p);
+  ANDR_HIP(ctx, hipGetLastError());
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ANDR_OK;
+}
+}  // namespace
+
+andr_ctx *andr_default_ctx() {
+  if (!g_ctx) {
+    const char *env = getenv("ANDR_DEVICE");
+    int dev = env ? atoi(env) : 0;
+    if (andr_create(&g_ctx, dev) != ANDR_OK) g_ctx = nullptr;
+  }
+  return g_ctx;
+}
+
+extern "C" void andr_lsqr_forget(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_cache.D.release();
+  g_cache.q.release();
+  g_cache.key_a = nullptr;
+}
+
+extern "C" void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *nnz, double *q, const int64_t *ia,
+                             const int32_t *ja, double *a) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  andr_ctx *ctx = andr_default_ctx();
+  if (!ctx) {
+    fprintf(stderr, "andromeda_hip: lsqr_precon_: no device\n");
+    return;
+  }
+  (void)hipSetDevice(ctx->device);
+  LsqrCache &c = g_cache;
+  if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D) || c.q.alloc(*n) ||
+      andr_colscale(ctx, c.D.p, false, *n, *m, *n, c.q.p)) {
+    fprintf(stderr, "andromeda_hip: lsqr_precon_: %s\n", ctx->err.c_str());
+    c.key_a = nullptr;
+    return;
+  }
+  if (hipMemcpy(q, c.q.p, sizeof(double) * (*n), hipMemcpyDeviceToHost) != hipSuccess) return;
+  // A <- A diag(q) on the host array too (lsqr_matvec4, src/lib_lsqr.f:301-329)
+  for (int i = 0; i < *m; ++i)
+    for (int64_t p = ia[i] - 1; p < ia[i + 1] - 1; ++p) a[p] = a[p] * q[ja[p] - 1];
+  c.key_a = a;
+  c.nnz = *nnz;
+  c.m = *m;
+  c.n = *n;
+  probe_fill(c, a, *nnz);
+}
+
+extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *nnz, int32_t *maxit, const double *eps,
+                            int32_t *nit, int32_t *ierr, const double *q, const int64_t *ia, const int32_t *ja,
+                            const double *a, const double *b, double *x) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  andr_ctx *ctx = andr_default_ctx();
+  if (!ctx) {
+    *ierr = -1;
+    return;
+  }
+  (void)hipSetDevice(ctx->device);
+  LsqrCache &c = g_cache;
+  if (!(c.key_a == a && c.nnz == *nnz && c.m == *m && c.n == *n && probe_ok(c, a))) {
+    if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D)) {
+      *ierr = -2;
+      return;
+    }
+    c.key_a = a;
+    c.nnz = *nnz;
+    c.m = *m;
+    c.n = *n;
+    probe_fill(c, a, *nnz);
+  }
+  if (*maxit == 0) *maxit = 10 * (*m + *n + 50);
+  const int M = *m, N = *n;
+  DevBuf<double> dq, dB, dX, tmp;
+  if (dq.alloc(N) || dB.alloc((size_t)M * ANDR_NB) || dX.alloc((size_t)N * ANDR_NB) || tmp.alloc(std::max(M, N))) {
+    *ierr = -3;
+    return;
+  }
+  hipStream_t st = ctx->stream;
+  (void)hipMemcpyAsync(dq.p, q, sizeof(double) * N, hipMemcpyHostToDevice, st);
+  (void)hipMemcpyAsync(tmp.p, b, sizeof(double) * M, hipMemcpyHostToDevice, st);
+  hipLaunchKernelGGL(k_pack_col, dim3((M * ANDR_NB + 255) / 256), dim3(256), 0, st, M, tmp.p, dB.p);
+  (void)hipMemcpyAsync(tmp.p, x, sizeof(double) * N, hipMemcpyHostToDevice, st);
+  hipLaunchKernelGGL(k_pack_col, dim3((N * ANDR_NB + 255) / 256), dim3(256), 0, st, N, tmp.p, dX.p);
+  int it = 0, is = 0;
+  int rc = andr_lsqr_device(ctx, c.D.p, false, N, M, N, dq.p, dB.p, dX.p, 1, *maxit, *eps, &it, &is);
+  if (rc) {
+    fprintf(stderr, "andromeda_hip: lsqr_solve_: %s\n", ctx->err.c_str());
+    *ierr = rc;
+    return;
+  }
+  hipLaunchKernelGGL(k_unpack_col, dim3((N + 255) / 256), dim3(256), 0, st, N, dX.p, tmp.p);
+  (void)hipMemcpyAsync(x, tmp.p, sizeof(double) * N, hipMemcpyDeviceToHost, st);
+  (void)hipStreamSynchronize(st);
+  *nit = it;
+  *ierr = is > 3 ? is : 0;
+}
+
+extern "C" int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double *A, int32_t nrhs, const double *B,
+                               double *X, const andr_lsqr_opts *opts, andr_lsqr_stats *stats) {
+  if (!ctx || !A || !B || !X || m <= 0 || n <= 0 || nrhs <= 0 || nrhs > 16) return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_lsqr_dense arguments") : ANDR_ERR_ARG;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  const double t0 = andr_now();
+  andr_lsqr_opts o = opts ? *opts : andr_lsqr_opts{0, 1e-15, 1};
+  DevBuf<double> dA, dq, dB, dX;
+  if (dA.alloc((size_t)m * n) || dq.alloc(n) || dB.alloc((size_t)m * ANDR_NB) || dX.alloc((size_t)n * ANDR_NB))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "andr_lsqr_dense workspace");
+  ANDR_HIP(ctx, hipMemcpyAsync(dA.p, A, sizeof(double) * m * n, hipMemcpyHostToDevice, ctx->stream));
+  if (o.precon) {
+    int rc = andr_colscale(ctx, dA.p, false, n, (int)m, (int)n, dq.p);
+    if (rc) return rc;
+  }
+  std::vector<double> hb((size_t)m * ANDR_NB), hx((size_t)n * ANDR_NB);
+  for (int g0 = 0; g0 < nrhs; g0 += ANDR_NB) {
+    const int nr = std::min(ANDR_NB, nrhs - g0);
+    std::fill(hb.begin(), hb.end(), 0.0);
+    std::fill(hx.begin(), hx.end(), 0.0);
+    for (int k = 0; k < nr; ++k) {
+      for (int64_t i = 0; i < m; ++i) hb[i * ANDR_NB + k] = B[(g0 + k) * m + i];
+      for (int64_t i = 0; i < n; ++i) hx[i * ANDR_NB + k] = X[(g0 + k) * n + i];
+    }
+    ANDR_HIP(ctx, hipMemcpyAsync(dB.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, ctx->stream));
+    ANDR_HIP(ctx, hipMemcpyAsync(dX.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, ctx->stream));
+    int it[ANDR_NB], is[ANDR_NB];
+    int rc = andr_lsqr_device(ctx, dA.p, false, n, (int)m, (int)n, o.precon ? dq.p : nullptr, dB.p, dX.p, nr, o.maxit,
+                              o.eps, it, is);
+    if (rc) return rc;
+    ANDR_HIP(ctx, hipMemcpyAsync(hx.data(), dX.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, ctx->stream));
+    ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nr; ++k) {
+      for (int64_t i = 0; i < n; ++i) X[(g0 + k) * n + i] = hx[i * ANDR_NB + k];
+      if (stats) {
+        stats->nit[g0 + k] = it[k];
+        stats->istop[g0 + k] = is[k];
+      }
+    }
+  }
+  if (stats) stats->seconds = andr_now() - t0;
+  return ANDR_OK;
+}

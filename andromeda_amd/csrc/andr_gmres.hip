@@ -1,0 +1,496 @@
+// Restarted GMRES on a batch of dense square systems with up to ANDR_NB right-hand sides each:
+// the solver of the mixed-precision resistance path.
+//
+// Storage of A may be FP32; products accumulate in FP64 and every vector (Krylov basis, Hessenberg,
+// solution) is FP64. Each restart cycle starts from the FP64 residual r = b - A x, so the cycles are
+// the iterative-refinement loop around the Krylov solve. All systems of the batch advance together:
+// one launch per step covers every system (work lists map waves and row tiles to systems), so the
+// launch count does not grow with the number of bodies.
+//
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): both passes are block
+// products over the basis, which keeps the basis traffic streaming instead of latency bound.
+#include "andr_internal.h"
+#include "andr_solve.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int NB = ANDR_NB;
+constexpr int TILE = 256;  // rows per vector tile
+
+struct GSys {  // device view of one system
+  const void *A;
+  int64_t lda;
+  int n;
+  const double *B;
+  double *X;
+  double *R;     // residual / scratch n x NB
+  double *W;     // new Krylov vector n x NB
+  double *Q;     // (m+1) x n x NB basis
+  double *H;     // NB x (m+1) x m Hessenberg (column-major per rhs)
+  double *cs, *sn;  // NB x m
+  double *g;     // NB x (m+1)
+  double *coef;  // NB x (m+1) projection coefficients of the current pass
+  double *small; // per rhs: [0] bnorm, [1] rnorm (true), [2] est, [3] inv_h, [4] active, [5] used steps
+  int tile0;     // first vector tile of the system
+  int ntiles;
+};
+
+struct WaveMap {
+  int sys, row0;
+};
+struct TileMap {
+  int sys, row0;
+};
+
+// Y = sx .* (A X) + sy .* Y for every system; sx/sy per (system, rhs). part[wave][k] = sum Y^2.
+template <class ST>
+__global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap, int nwaves,
+                                               int which_x, int jq, int which_y, const double *__restrict__ sxy,
+                                               double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const WaveMap wm = wmap[wave];
+  const GSys &S = sys[wm.sys];
+  const int m = S.n, n = S.n, row0 = wm.row0;
+  const ST *A = (const ST *)S.A;
+  const double *X = which_x == 0 ? S.X : S.Q + (size_t)jq * S.n * NB;
+  double *Y = which_y == 0 ? S.R : S.W;
+  const int64_t lda = S.lda;
+  double acc[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+  const ST *arow[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) arow[r] = A + (size_t)min(row0 + r, m - 1) * lda;
+  for (int j = lane; j < n; j += 64) {
+    const double4 x0 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB);
+    const double4 x1 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB + 4);
+    const double xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const double a = (double)arow[r][j];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[r * 8 + k] += a * xv[k];
+    }
+  }
+#pragma unroll
+  for (int half = 32; half >= 1; half >>= 1) {
+    const bool hi = (lane & half) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const double keep = hi ? acc[i + half] : acc[i];
+      const double send = hi ? acc[i] : acc[i + half];
+      acc[i] = keep + __shfl_xor(send, half, 64);
+    }
+  }
+  const int r = lane >> 3, k = lane & 7;
+  double y = 0.0;
+  if (row0 + r < m) {
+    double *py = Y + (size_t)(row0 + r) * NB + k;
+    const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
+    y = sx * acc[0] + (sy != 0.0 ? sy * (*py) : 0.0);
+    *py = y;
+  }
+  double y2 = y * y;
+  y2 += __shfl_xor(y2, 8, 64);
+  y2 += __shfl_xor(y2, 16, 64);
+  y2 += __shfl_xor(y2, 32, 64);
+  if (lane < 8) part[(size_t)wave * NB + lane] = y2;
+}
+
+// per tile: partial dots  part[tile][i][k] = sum_rows Q_i[row][k] * W[row][k], i = 0..j
+__global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
+                                               int mstride, double *__restrict__ part) {
+  __shared__ double red[4][NB];
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n;
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double w[TILE / 32];
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    w[t] = row < n ? S.W[(size_t)row * NB + k] : 0.0;
+  }
+  for (int i = 0; i <= j; ++i) {
+    const double *Qi = S.Q + (size_t)i * n * NB;
+    double a = 0.0;
+#pragma unroll
+    for (int t = 0; t < TILE / 32; ++t) {
+      const int row = tm.row0 + rr + 32 * t;
+      if (row < n) a += Qi[(size_t)row * NB + k] * w[t];
+    }
+    a += __shfl_xor(a, 8, 64);
+    a += __shfl_xor(a, 16, 64);
+    a += __shfl_xor(a, 32, 64);
+    if (lane < 8) red[wid][lane] = a;
+    __syncthreads();
+    if (threadIdx.x < 8)
+      part[((size_t)blockIdx.x * mstride + i) * NB + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                                                   red[2][threadIdx.x] + red[3][threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// thread per (system, i, k): coef = sum over the system's tiles; H(i, j) += coef
+__global__ void kb_dots_final(const GSys *__restrict__ sys, int nsys, int j, int m, int mstride,
+                              const double *__restrict__ part, int first_pass) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = (j + 1) * NB;
+  if (t >= nsys * per) return;
+  const int s = t / per, rem = t % per, i = rem / NB, k = rem % NB;
+  const GSys &S = sys[s];
+  double c = 0.0;
+  for (int tl = 0; tl < S.ntiles; ++tl) c += part[((size_t)(S.tile0 + tl) * mstride + i) * NB + k];
+  S.coef[k * (m + 1) + i] = c;
+  double &h = S.H[(size_t)k * (m + 1) * m + (size_t)j * (m + 1) + i];
+  h = first_pass ? c : h + c;
+}
+
+// per tile: W -= sum_i Q_i coef_i ; part2[tile][k] = sum W^2
+__global__ __launch_bounds__(256) void kb_sub(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j, int m,
+                                              double *__restrict__ part2) {
+  __shared__ double red[4][NB];
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n;
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double w2 = 0.0;
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row >= n) continue;
+    double w = S.W[(size_t)row * NB + k];
+    for (int i = 0; i <= j; ++i) w -= S.Q[((size_t)i * n + row) * NB + k] * S.coef[k * (m + 1) + i];
+    S.W[(size_t)row * NB + k] = w;
+    w2 += w * w;
+  }
+  w2 += __shfl_xor(w2, 8, 64);
+  w2 += __shfl_xor(w2, 16, 64);
+  w2 += __shfl_xor(w2, 32, 64);
+  if (lane < 8) red[wid][lane] = w2;
+  __syncthreads();
+  if (threadIdx.x < 8)
+    part2[(size_t)blockIdx.x * NB + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// thread per (system, k): h(j+1, j) = ||W||, Givens rotations, residual estimate
+__global__ void kb_givens(const GSys *__restrict__ sys, int nsys, int j, int m, const double *__restrict__ part2,
+                          double tol) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * NB) return;
+  const int s = t / NB, k = t % NB;
+  const GSys &S = sys[s];
+  double *sm = S.small + k * 8;
+  double hn2 = 0.0;
+  for (int tl = 0; tl < S.ntiles; ++tl) hn2 += part2[(size_t)(S.tile0 + tl) * NB + k];
+  const double hn = sqrt(hn2);
+  double *Hc = S.H + (size_t)k * (m + 1) * m + (size_t)j * (m + 1);
+  Hc[j + 1] = hn;
+  const bool live = sm[4] > 0.0 && sm[5] < 0.0;  // active and not yet frozen in this cycle
+  // apply previous rotations
+  double *cs = S.cs + k * m, *sn = S.sn + k * m, *g = S.g + k * (m + 1);
+  for (int i = 0; i < j; ++i) {
+    const double a = Hc[i], b = Hc[i + 1];
+    Hc[i] = cs[i] * a + sn[i] * b;
+    Hc[i + 1] = -sn[i] * a + cs[i] * b;
+  }
+  const double a = Hc[j], b = Hc[j + 1];
+  const double rr = sqrt(a * a + b * b);
+  const double c = rr > 0.0 ? a / rr : 1.0, sg = rr > 0.0 ? b / rr : 0.0;
+  cs[j] = c;
+  sn[j] = sg;
+  Hc[j] = rr;
+  Hc[j + 1] = 0.0;
+  g[j + 1] = -sg * g[j];
+  g[j] = c * g[j];
+  const double est = fabs(g[j + 1]);
+  sm[2] = est;
+  const double tiny = 1e-300;
+  sm[3] = hn > tiny ? 1.0 / hn : 0.0;
+  if (live && (est <= 0.1 * tol * sm[0] || hn <= 1e-14 * rr)) sm[5] = (double)(j + 1);  // steps to use
+}
+
+// per tile: Q_{j+1} = W * inv_h
+__global__ __launch_bounds__(256) void kb_scale_next(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n;
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const double inv = S.small[k * 8 + 3];
+  double *Qn = S.Q + (size_t)(j + 1) * n * NB;
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row < n) Qn[(size_t)row * NB + k] = S.W[(size_t)row * NB + k] * inv;
+  }
+}
+
+// thread per (system, k): back substitution y = R^{-1} g over the steps used; y stored in coef
+__global__ void kb_solve_y(const GSys *__restrict__ sys, int nsys, int steps, int m) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * NB) return;
+  const int s = t / NB, k = t % NB;
+  const GSys &S = sys[s];
+  double *sm = S.small + k * 8;
+  double *y = S.coef + k * (m + 1);
+  int kk = sm[5] > 0.0 ? (int)sm[5] : steps;
+  if (sm[4] <= 0.0) kk = 0;
+  const double *H = S.H + (size_t)k * (m + 1) * m;
+  const double *g = S.g + k * (m + 1);
+  for (int i = 0; i <= m; ++i) y[i] = 0.0;
+  for (int i = kk - 1; i >= 0; --i) {
+    double v = g[i];
+    for (int c = i + 1; c < kk; ++c) v -= H[(size_t)c * (m + 1) + i] * y[c];
+    const double d = H[(size_t)i * (m + 1) + i];
+    y[i] = d != 0.0 ? v / d : 0.0;
+  }
+}
+
+// per tile: X += sum_j Q_j y_j
+__global__ __launch_bounds__(256) void kb_xupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int steps,
+                                                  int m) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n;
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const double *y = S.coef + k * (m + 1);
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row >= n) continue;
+    double x = S.X[(size_t)row * NB + k];
+    for (int i = 0; i < steps; ++i) x += S.Q[((size_t)i * n + row) * NB + k] * y[i];
+    S.X[(size_t)row * NB + k] = x;
+  }
+}
+
+// per tile: R = B (start of the residual product)
+__global__ __launch_bounds__(256) void kb_copy_b(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row < S.n) S.R[(size_t)row * NB + k] = S.B[(size_t)row * NB + k];
+  }
+}
+
+// thread per (system, k): true residual norm from the gemv partials; restart set-up
+__global__ void kb_restart(const GSys *__restrict__ sys, int nsys, int m, const double *__restrict__ part,
+                           const int *__restrict__ wave0, const int *__restrict__ nwv, double tol, int first,
+                           int nrhs, int *__restrict__ done) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * NB) return;
+  const int s = t / NB, k = t % NB;
+  const GSys &S = sys[s];
+  double *sm = S.small + k * 8;
+  double r2 = 0.0;
+  for (int w = 0; w < nwv[s]; ++w) r2 += part[(size_t)(wave0[s] + w) * NB + k];
+  const double rn = sqrt(r2);
+  if (first) {
+    sm[0] = rn;  // ||b|| (x = 0)
+    sm[4] = (k < nrhs && rn > 0.0) ? 1.0 : 0.0;
+  }
+  sm[1] = rn;
+  if (sm[4] > 0.0 && rn <= tol * sm[0]) sm[4] = 0.0;  // converged
+  sm[5] = -1.0;
+  double *g = S.g + k * (m + 1);
+  for (int i = 0; i <= m; ++i) g[i] = 0.0;
+  g[0] = sm[4] > 0.0 ? rn : 0.0;
+  sm[3] = (sm[4] > 0.0 && rn > 0.0) ? 1.0 / rn : 0.0;
+  if (sm[4] > 0.0) atomicAdd(done, 1);
+}
+
+// per tile: Q_0 = R * (1/||r||)
+__global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const double inv = S.small[k * 8 + 3];
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row < S.n) S.Q[(size_t)row * NB + k] = S.R[(size_t)row * NB + k] * inv;
+  }
+}
+
+}  // namespace
+
+int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int nrhs, int restart, int maxit,
+                     double tol, int *iters) {
+  if (nsys <= 0) return ANDR_OK;
+  if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
+  const int m = std::max(4, restart);
+  hipStream_t st = ctx->stream;
+  // work maps
+  std::vector<WaveMap> wmap;
+  std::vector<TileMap> tmap;
+  std::vector<int> wave0(nsys), nwv(nsys), tile0(nsys), ntl(nsys);
+  size_t vec_elems = 0, basis_elems = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int n = in[s].n;
+    wave0[s] = (int)wmap.size();
+    for (int r = 0; r < n; r += 8) wmap.push_back({s, r});
+    nwv[s] = (int)wmap.size() - wave0[s];
+    tile0[s] = (int)tmap.size();
+    for (int r = 0; r < n; r += TILE) tmap.push_back({s, r});
+    ntl[s] = (int)tmap.size() - tile0[s];
+    vec_elems += (size_t)n * NB;
+    basis_elems += (size_t)(m + 1) * n * NB;
+  }
+  const int nw = (int)wmap.size(), nt = (int)tmap.size();
+  const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1) + 8);
+  DevBuf<double> R, W, Q, SM, sxy, partW, partT, partT2;
+  DevBuf<GSys> dsys;
+  DevBuf<WaveMap> dwm;
+  DevBuf<TileMap> dtm;
+  DevBuf<int> dw0, dnw, ddone;
+  if (R.alloc(vec_elems) || W.alloc(vec_elems) || Q.alloc(basis_elems) || SM.alloc(small_per * nsys) ||
+      sxy.alloc((size_t)nsys * 2 * NB) || partW.alloc((size_t)nw * NB) || partT.alloc((size_t)nt * (m + 1) * NB) ||
+      partT2.alloc((size_t)nt * NB) || dsys.alloc(nsys) || dwm.alloc(nw) || dtm.alloc(nt) || dw0.alloc(nsys) ||
+      dnw.alloc(nsys) || ddone.alloc(1))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
+  std::vector<GSys> hs(nsys);
+  size_t vo = 0, bo = 0;
+  for (int s = 0; s < nsys; ++s) {
+    GSys &g = hs[s];
+    const int n = in[s].n;
+    g.A = in[s].A;
+    g.lda = in[s].lda;
+    g.n = n;
+    g.B = in[s].B;
+    g.X = in[s].X;
+    g.R = R.p + vo;
+    g.W = W.p + vo;
+    g.Q = Q.p + bo;
+    double *sp = SM.p + small_per * s;
+    g.H = sp;
+    sp += (size_t)NB * (m + 1) * m;
+    g.cs = sp;
+    sp += NB * m;
+    g.sn = sp;
+    sp += NB * m;
+    g.g = sp;
+    sp += NB * (m + 1);
+    g.coef = sp;
+    sp += NB * (m + 1);
+    g.small = sp;
+    g.tile0 = tile0[s];
+    g.ntiles = ntl[s];
+    vo += (size_t)n * NB;
+    bo += (size_t)(m + 1) * n * NB;
+  }
+  ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dwm.p, wmap.data(), sizeof(WaveMap) * nw, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dtm.p, tmap.data(), sizeof(TileMap) * nt, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dw0.p, wave0.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dnw.p, nwv.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemsetAsync(SM.p, 0, sizeof(double) * small_per * nsys, st));
+  for (int s = 0; s < nsys; ++s)
+    ANDR_HIP(ctx, hipMemsetAsync(in[s].X, 0, sizeof(double) * in[s].n * NB, st));
+  // scale factors: residual product (sx = -1, sy = 1) and Arnoldi product (sx = 1, sy = 0)
+  std::vector<double> h_res(2 * NB * (size_t)nsys), h_arn(2 * NB * (size_t)nsys);
+  for (int s = 0; s < nsys; ++s)
+    for (int k = 0; k < NB; ++k) {
+      h_res[s * 2 * NB + k] = -1.0;
+      h_res[s * 2 * NB + NB + k] = 1.0;
+      h_arn[s * 2 * NB + k] = 1.0;
+      h_arn[s * 2 * NB + NB + k] = 0.0;
+    }
+  DevBuf<double> d_res, d_arn;
+  if (d_res.alloc(h_res.size()) || d_arn.alloc(h_arn.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES scales");
+  ANDR_HIP(ctx, hipMemcpyAsync(d_res.p, h_res.data(), sizeof(double) * h_res.size(), hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
+
+  const int gw = (nw + 3) / 4;
+  const int gs = (nsys * NB + 63) / 64;
+  auto gemv = [&](int which_x, int jq, int which_y, const double *scales) {
+    if (fp32)
+      hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
+                         partW.p);
+    else
+      hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
+                         partW.p);
+  };
+  std::vector<int> it_count((size_t)nsys * NB, 0);
+  std::vector<double> hsmall(small_per * nsys);
+  auto read_small = [&]() -> int {
+    for (int s = 0; s < nsys; ++s) {
+      hipError_t e = hipMemcpyAsync(hsmall.data() + small_per * s, hs[s].small, sizeof(double) * NB * 8,
+                                    hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return andr_fail(ctx, ANDR_ERR_HIP, "GMRES state copy: %s", hipGetErrorString(e));
+    }
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    return ANDR_OK;
+  };
+  int total = 0;
+  bool first = true;
+  while (true) {
+    // residual r = b - A x (x = 0 on the first cycle), FP64 accumulation
+    hipLaunchKernelGGL(kb_copy_b, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p);
+    gemv(0, 0, 0, d_res.p);
+    ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, m, partW.p, dw0.p, dnw.p, tol, first ? 1 : 0,
+                       nrhs, ddone.p);
+    int hdone = 0;
+    ANDR_HIP(ctx, hipMemcpyAsync(&hdone, ddone.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    ANDR_HIP(ctx, hipGetLastError());
+    first = false;
+    if (hdone == 0 || total >= maxit) break;
+    hipLaunchKernelGGL(kb_q0, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p);
+    const int steps_max = std::min(m, maxit - total);
+    int steps = 0;
+    for (int j = 0; j < steps_max; ++j) {
+      gemv(1, j, 1, d_arn.p);  // W = A Q_j
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(kb_dots, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j, m + 1, partT.p);
+        const int nthr = nsys * (j + 1) * NB;
+        hipLaunchKernelGGL(kb_dots_final, dim3((nthr + 255) / 256), dim3(256), 0, st, dsys.p, nsys, j, m, m + 1,
+                           partT.p, pass == 0 ? 1 : 0);
+        hipLaunchKernelGGL(kb_sub, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j, m, partT2.p);
+      }
+      hipLaunchKernelGGL(kb_givens, dim3(gs), dim3(64), 0, st, dsys.p, nsys, j, m, partT2.p, tol);
+      hipLaunchKernelGGL(kb_scale_next, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j);
+      steps = j + 1;
+      if (steps % 8 == 0 && steps < steps_max) {  // stop the cycle once every live column is resolved
+        ANDR_HIP(ctx, hipGetLastError());
+        int e = read_small();
+        if (e) return e;
+        bool live = false;
+        for (int s = 0; s < nsys && !live; ++s)
+          for (int k = 0; k < NB; ++k) {
+            const double *sm = hsmall.data() + small_per * s + k * 8;
+            if (sm[4] > 0.0 && sm[5] < 0.0) live = true;
+          }
+        if (!live) break;
+      }
+    }
+    total += steps;
+    hipLaunchKernelGGL(kb_solve_y, dim3(gs), dim3(64), 0, st, dsys.p, nsys, steps, m);
+    hipLaunchKernelGGL(kb_xupdate, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, steps, m);
+    ANDR_HIP(ctx, hipGetLastError());
+    int e = read_small();
+    if (e) return e;
+    for (int s = 0; s < nsys; ++s)
+      for (int k = 0; k < NB; ++k) {
+        const double *sm = hsmall.data() + small_per * s + k * 8;
+        if (sm[4] > 0.0) it_count[(size_t)s * NB + k] += sm[5] > 0.0 ? (int)sm[5] : steps;
+      }
+  }
+  if (iters)
+    for (int s = 0; s < nsys; ++s)
+      for (int k = 0; k < nrhs; ++k) iters[s * nrhs + k] = it_count[(size_t)s * NB + k];
+  return ANDR_OK;
+}

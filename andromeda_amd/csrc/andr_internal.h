@@ -1,0 +1,103 @@
+// Internal declarations of libandromeda_hip: context, device buffers, launch helpers and the
+// quadrature tables in constant memory. gfx950 (MI355X) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/andromeda_hip.h"
+
+#define ANDR_PI 3.141592653589793238462643383
+
+struct andr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // learned quadrature model (BF16 copy lives in the device buffer below)
+  bool has_model = false;
+  float model_conf = 0.9f;
+  double model_near = 4.0;
+  void *d_model = nullptr;  // packed bf16 weights + fp32 biases
+};
+
+// status helpers ------------------------------------------------------------------------------
+int andr_fail(andr_ctx *ctx, int code, const char *fmt, ...);
+#define ANDR_HIP(ctx, call)                                                                       \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return andr_fail((ctx), ANDR_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                       hipGetErrorString(e_));                                                    \
+  } while (0)
+
+// RAII device buffer ---------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, sizeof(T) * count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Quadrature tables (constant memory). Rule ids 1..5: the Wandzura-Xiao rules (ITRI) with
+// 7/25/54/85/126 points; ids 6..12: Dunavant 3/4/6/7/12/13/19 points.
+#define ANDR_QTAB_N (297 + 64)
+struct QuadRule {
+  int offset, npts;
+};
+extern __constant__ double c_qd[ANDR_QTAB_N][4];  // l1, l2, l3, w
+extern __constant__ float c_qf[ANDR_QTAB_N][4];
+QuadRule andr_rule(int id);  // host side lookup
+int andr_upload_rules(andr_ctx *ctx);
+
+// Per-body device geometry -------------------------------------------------------------------
+struct BodyGeom {
+  int nn = 0, ne = 0, nwalls = 0;
+  const double *xyz = nullptr;   // 3*nn
+  const int *con = nullptr;      // 3*ne, 0-based
+  const int *bcid = nullptr;     // ne
+  const double *nrm = nullptr;   // 3*ne unit normals (CalElementNormal)
+  const double *area = nullptr;  // ne (Heron)
+  const double *cen = nullptr;   // 3*ne centroids (CalQMesh)
+  const double *mul = nullptr;   // ne: NORMMUL of the element's wall
+};
+
+// Host-side owner of a body's device arrays.
+struct DevBody {
+  DevBuf<double> xyz, nrm, area, cen, mul;
+  DevBuf<int> con, bcid, color_elems;
+  std::vector<int> color_off;  // host copy of color offsets (size ncolors+1)
+  DevBuf<int> d_color_off;
+  BodyGeom g;
+  int itri = 5;
+};
+int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors);
+
+// kernels exposed between translation units ----------------------------------------------------
+int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows);
+int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, double *G, double *prT,
+                            double *prG, double *err_rows);
+// transposes an m x n row-major device matrix into a column-major host buffer
+int andr_copy_out(andr_ctx *ctx, const double *d, int64_t m, int64_t n, int32_t layout, double *h);
+
+// Process-wide default context used by the Fortran-style entry points.
+andr_ctx *andr_default_ctx();
+
+double andr_now();

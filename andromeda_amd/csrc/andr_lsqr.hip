@@ -1,0 +1,537 @@
+// Multi-RHS LSQR on a dense device matrix: the plain FP64 drop-in for src/lib_lsqr.f.
+//
+// Every right-hand side follows its own Paige-Saunders recurrence (lsqr_LSQR, src/lib_lsqr.f:470-1207,
+// damp = 0, conlim = 1e8, nconv = 1), but the two products of an iteration -- A v and A^T u -- are
+// formed for all ANDR_NB columns in one pass over A, so the matrix is streamed from HBM twice per
+// iteration for the whole block instead of twice per iteration per right-hand side.
+//
+// The u and v vectors are kept unnormalised together with their norms (true u = U / beta, true
+// v = V / alpha); the products fold the normalisation into their scale factors, which removes the
+// separate scaling passes of the reference (dscal at :947, :974, :980).
+#include "andr_internal.h"
+#include "andr_solve.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int NB = ANDR_NB;
+
+__device__ inline double d2norm(double a, double b) {  // lsqr_d2norm (src/lib_lsqr.f:1236-1263)
+  double scale = fabs(a) + fabs(b);
+  if (scale == 0.0) return 0.0;
+  double x = a / scale, y = b / scale;
+  return scale * sqrt(x * x + y * y);
+}
+
+// Butterfly reduce-scatter of 64 per-lane values: afterwards lane l holds the wave sum of v[l].
+__device__ inline double wave_reduce_scatter64(double (&v)[64]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int half = 32; half >= 1; half >>= 1) {
+    const bool hi = (lane & half) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const double keep = hi ? v[i + half] : v[i];
+      const double send = hi ? v[i] : v[i + half];
+      v[i] = keep + __shfl_xor(send, half, 64);
+    }
+  }
+  return v[0];
+}
+
+// Y = sx .* (A X) + sy .* Y for NB interleaved columns; one wave per 8 rows.
+// part[wave][k] = sum of the squares of the new Y over the wave's rows.
+template <class ST>
+__global__ __launch_bounds__(256) void k_gemv_n(const ST *__restrict__ A, int64_t lda, int m, int n,
+                                                const double *__restrict__ X, double *__restrict__ Y,
+                                                const double *__restrict__ sx, const double *__restrict__ sy,
+                                                double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int row0 = wave * 8;
+  if (row0 >= m) return;
+  double acc[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+  const ST *arow[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) arow[r] = A + (size_t)min(row0 + r, m - 1) * lda;
+  for (int j = lane; j < n; j += 64) {
+    const double4 x0 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB);
+    const double4 x1 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB + 4);
+    const double xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const double a = (double)arow[r][j];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[r * 8 + k] += a * xv[k];
+    }
+  }
+  double s = wave_reduce_scatter64(acc);  // lane l: row row0 + l/8, column l%8
+  const int r = lane >> 3, k = lane & 7;
+  double y = 0.0;
+  if (row0 + r < m) {
+    double *py = Y + (size_t)(row0 + r) * NB + k;
+    y = sx[k] * s + sy[k] * (*py);
+    *py = y;
+  }
+  double y2 = y * y;
+  y2 += __shfl_xor(y2, 8, 64);
+  y2 += __shfl_xor(y2, 16, 64);
+  y2 += __shfl_xor(y2, 32, 64);
+  if (lane < 8) part[(size_t)wave * NB + lane] = y2;
+}
+
+// P[s][j][k] = sum_{i in split s} A[i][j] Y[i][k]; one column per thread, splits over blockIdx.y.
+template <class ST>
+__global__ __launch_bounds__(256) void k_gemv_t(const ST *__restrict__ A, int64_t lda, int m, int n,
+                                                const double *__restrict__ Y, int rows_per_split,
+                                                double *__restrict__ P) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int i0 = s * rows_per_split;
+  const int i1 = min(m, i0 + rows_per_split);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (j < n) {
+    const ST *col = A + j;
+    int i = i0;
+    for (; i + 4 <= i1; i += 4) {
+      const double a0 = (double)col[(size_t)i * lda], a1 = (double)col[(size_t)(i + 1) * lda];
+      const double a2 = (double)col[(size_t)(i + 2) * lda], a3 = (double)col[(size_t)(i + 3) * lda];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        acc[k] += a0 * Y[(size_t)i * NB + k] + a1 * Y[(size_t)(i + 1) * NB + k] + a2 * Y[(size_t)(i + 2) * NB + k] +
+                  a3 * Y[(size_t)(i + 3) * NB + k];
+    }
+    for (; i < i1; ++i) {
+      const double a = (double)col[(size_t)i * lda];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += a * Y[(size_t)i * NB + k];
+    }
+    double4 *p = reinterpret_cast<double4 *>(P + ((size_t)s * n + j) * NB);
+    p[0] = make_double4(acc[0], acc[1], acc[2], acc[3]);
+    p[1] = make_double4(acc[4], acc[5], acc[6], acc[7]);
+  }
+}
+
+// V = sx .* sum_s P[s] + sy .* V; partV[block][k] = sum V^2, partW[block][k] = sum W^2 (old W).
+__global__ __launch_bounds__(256) void k_combine_t(int n, int nsplit, const double *__restrict__ P,
+                                                   double *__restrict__ V, const double *__restrict__ W,
+                                                   const double *__restrict__ sx, const double *__restrict__ sy,
+                                                   double *__restrict__ partV, double *__restrict__ partW) {
+  __shared__ double red[2 * NB * 4];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // one (j, k) pair per thread
+  const int k = t & 7;
+  double v2 = 0.0, w2 = 0.0;
+  if (t < n * NB) {
+    double s = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) s += P[(size_t)sp * n * NB + t];
+    const double v = sx[k] * s + sy[k] * V[t];
+    V[t] = v;
+    v2 = v * v;
+    if (W) {
+      const double w = W[t];
+      w2 = w * w;
+    }
+  }
+  // reduce over lanes with equal k (lane & 7)
+  v2 += __shfl_xor(v2, 8, 64);
+  v2 += __shfl_xor(v2, 16, 64);
+  v2 += __shfl_xor(v2, 32, 64);
+  w2 += __shfl_xor(w2, 8, 64);
+  w2 += __shfl_xor(w2, 16, 64);
+  w2 += __shfl_xor(w2, 32, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < 8) {
+    red[wid * NB + lane] = v2;
+    red[4 * NB + wid * NB + lane] = w2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < 4; ++w) {
+      a += red[w * NB + threadIdx.x];
+      b += red[4 * NB + w * NB + threadIdx.x];
+    }
+    partV[(size_t)blockIdx.x * NB + threadIdx.x] = a;
+    partW[(size_t)blockIdx.x * NB + threadIdx.x] = b;
+  }
+}
+
+__device__ inline double sum_parts(const double *part, int np, int k) {
+  double s = 0.0;
+  for (int i = 0; i < np; ++i) s += part[(size_t)i * NB + k];
+  return s;
+}
+
+// ---- scalar kernels (one thread per column) -------------------------------------------------
+// start: U = b - A x_s is formed; beta = ||U||, btol = eps ||b|| / ||r0||; set scales for A^T u.
+__global__ void k_lsqr_start(LsqrState *st, const double *partU, int npU, const double *partB, int npB, double eps,
+                             double *sxt, double *syt) {
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  LsqrState &S = st[k];
+  const double nrm_r = sqrt(sum_parts(partU, npU, k));
+  const double nrm_b = sqrt(sum_parts(partB, npB, k));
+  S.atol = eps;
+  S.btol = eps * nrm_b / nrm_r;
+  S.ctol = 1.0 / 1.0e8;
+  S.anorm = S.dnorm = S.res2 = S.xnorm = S.xnorm1 = S.z = 0.0;
+  S.cs2 = -1.0;
+  S.sn2 = 0.0;
+  S.itn = 0;
+  S.istop = 0;
+  S.beta = nrm_r;
+  S.alpha = 0.0;
+  if (S.active && S.beta > 0.0) {
+    sxt[k] = 1.0 / S.beta;
+    syt[k] = 0.0;
+  } else {
+    sxt[k] = 0.0;
+    syt[k] = 0.0;
+  }
+}
+
+// after the first A^T u: alpha, w = v, the initial recurrence values (src/lib_lsqr.f:952-1008)
+__global__ void k_lsqr_init2(LsqrState *st, const double *partV, int npV, double *sxn, double *syn, double *inv_alpha) {
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  LsqrState &S = st[k];
+  S.alpha = S.beta > 0.0 ? sqrt(sum_parts(partV, npV, k)) : 0.0;
+  inv_alpha[k] = S.alpha > 0.0 ? 1.0 / S.alpha : 0.0;
+  const double arnorm = S.alpha * S.beta;
+  S.rhobar = S.alpha;
+  S.phibar = S.beta;
+  S.bnorm = S.beta;
+  S.rnorm = S.beta;
+  if (arnorm == 0.0) S.active = 0;  // exact solution x = 0 (istop 0)
+  if (S.active) {
+    sxn[k] = 1.0 / S.alpha;
+    syn[k] = -S.alpha / S.beta;
+  } else {
+    sxn[k] = 0.0;
+    syn[k] = 0.0;
+  }
+}
+
+// after U = A v - alpha u: beta, anorm; scales of the A^T product
+__global__ void k_lsqr_mid(LsqrState *st, const double *partU, int npU, double *sxt, double *syt) {
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  LsqrState &S = st[k];
+  if (!S.active) {
+    sxt[k] = 0.0;
+    syt[k] = 0.0;
+    return;
+  }
+  S.itn += 1;
+  const double beta = sqrt(sum_parts(partU, npU, k));
+  double temp = d2norm(S.alpha, beta);
+  temp = d2norm(temp, 0.0);
+  S.anorm = d2norm(S.anorm, temp);
+  const double alpha_old = S.alpha;
+  S.beta = beta;
+  if (beta > 0.0) {
+    sxt[k] = 1.0 / beta;
+    syt[k] = -beta / alpha_old;
+  } else {
+    sxt[k] = 0.0;
+    syt[k] = 1.0;  // v unchanged
+  }
+}
+
+// after V = A^T u - beta v: alpha, the plane rotations and the convergence tests
+// (src/lib_lsqr.f:1015-1152)
+__global__ void k_lsqr_end(LsqrState *st, const double *partV, const double *partW, int np, double *sxn, double *syn,
+                           double *t1o, double *t2o, double *inv_alpha, int *active_out) {
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  LsqrState &S = st[k];
+  if (!S.active) {
+    t1o[k] = 0.0;
+    t2o[k] = 1.0;
+    inv_alpha[k] = 0.0;
+    sxn[k] = 0.0;
+    syn[k] = 0.0;
+    active_out[k] = 0;
+    return;
+  }
+  if (S.beta > 0.0) S.alpha = sqrt(sum_parts(partV, np, k));
+  const double alpha = S.alpha, beta = S.beta;
+  const double wnorm = sqrt(sum_parts(partW, np, k));
+  const double rhbar1 = S.rhobar;
+  const double rho = d2norm(rhbar1, beta);
+  const double cs = rhbar1 / rho, sn = beta / rho;
+  const double theta = sn * alpha;
+  S.rhobar = -cs * alpha;
+  const double phi = cs * S.phibar;
+  S.phibar = sn * S.phibar;
+  const double tau = sn * phi;
+  const double t1 = phi / rho, t2 = -theta / rho, t3 = 1.0 / rho;
+  const double dknorm = t3 * wnorm;
+  S.dnorm = d2norm(S.dnorm, dknorm);
+  const double delta = S.sn2 * rho, gambar = -S.cs2 * rho;
+  const double rhs = phi - delta * S.z, zbar = rhs / gambar;
+  S.xnorm = d2norm(S.xnorm1, zbar);
+  const double gamma = d2norm(gambar, theta);
+  S.cs2 = gambar / gamma;
+  S.sn2 = theta / gamma;
+  S.z = rhs / gamma;
+  S.xnorm1 = d2norm(S.xnorm1, S.z);
+  const double acond = S.anorm * S.dnorm;
+  S.res2 = d2norm(S.res2, 0.0);
+  S.rnorm = d2norm(S.res2, S.phibar);
+  const double arnorm = alpha * fabs(tau);
+  const double test1 = S.rnorm / S.bnorm;
+  const double test2 = S.rnorm > 0.0 ? arnorm / (S.anorm * S.rnorm) : 0.0;
+  const double test3 = 1.0 / acond;
+  double tt1 = test1 / (1.0 + S.anorm * S.xnorm / S.bnorm);
+  const double rtol = S.btol + S.atol * S.anorm * S.xnorm / S.bnorm;
+  const double tt3 = 1.0 + test3, tt2 = 1.0 + test2;
+  tt1 = 1.0 + tt1;
+  int istop = 0;
+  if (S.itn >= S.itnlim) istop = 5;
+  if (tt3 <= 1.0) istop = 4;
+  if (tt2 <= 1.0) istop = 2;
+  if (tt1 <= 1.0) istop = 1;
+  if (test3 <= S.ctol) istop = 4;
+  if (test2 <= S.atol) istop = 2;
+  if (test1 <= rtol) istop = 1;
+  S.istop = istop;
+  t1o[k] = t1;
+  t2o[k] = t2;
+  inv_alpha[k] = alpha > 0.0 ? 1.0 / alpha : 1.0;
+  if (istop != 0) S.active = 0;
+  active_out[k] = 1;  // the update of this iteration still applies (src/lib_lsqr.f:1057-1062)
+  if (S.active) {
+    sxn[k] = 1.0 / alpha;
+    syn[k] = -alpha / beta;
+  } else {
+    sxn[k] = 0.0;
+    syn[k] = 0.0;
+  }
+}
+
+// x += t1 w ; w = V * inv_alpha + t2 w  (src/lib_lsqr.f:1057-1062); columns with t1 = 0, t2 = 1,
+// inv_alpha = 0 stay unchanged.
+__global__ void k_lsqr_update(int n, double *__restrict__ X, double *__restrict__ W, const double *__restrict__ V,
+                              const double *__restrict__ t1, const double *__restrict__ t2,
+                              const double *__restrict__ inv_alpha, const int *__restrict__ was_active) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  const int k = t & 7;
+  if (!was_active[k]) return;
+  const double w = W[t];
+  X[t] += t1[k] * w;
+  W[t] = V[t] * inv_alpha[k] + t2[k] * w;
+}
+
+__global__ void k_scale_div(int n, const double *__restrict__ X0, const double *__restrict__ q, double *__restrict__ Xs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  Xs[t] = q ? X0[t] / q[t >> 3] : X0[t];
+}
+
+__global__ void k_scale_mul_add(int n, const double *__restrict__ q, const double *__restrict__ Xl,
+                                const double *__restrict__ X0, double *__restrict__ X) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  X[t] = (q ? q[t >> 3] * Xl[t] : Xl[t]) + X0[t];
+}
+
+__global__ void k_sumsq_parts(int n, const double *__restrict__ Y, double *__restrict__ part) {
+  __shared__ double red[NB * 4];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double y2 = 0.0;
+  if (t < n * NB) y2 = Y[t] * Y[t];
+  y2 += __shfl_xor(y2, 8, 64);
+  y2 += __shfl_xor(y2, 16, 64);
+  y2 += __shfl_xor(y2, 32, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < 8) red[wid * NB + lane] = y2;
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double a = 0.0;
+    for (int w = 0; w < 4; ++w) a += red[w * NB + threadIdx.x];
+    part[(size_t)blockIdx.x * NB + threadIdx.x] = a;
+  }
+}
+
+// ---- column scaling (lsqr_precon) ------------------------------------------------------------
+template <class ST>
+__global__ __launch_bounds__(256) void k_colsq(const ST *__restrict__ A, int64_t lda, int m, int n, int rows_per_split,
+                                               double *__restrict__ P) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (j >= n) return;
+  const int i0 = s * rows_per_split, i1 = min(m, i0 + rows_per_split);
+  double acc = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const double a = (double)A[(size_t)i * lda + j];
+    acc += a * a;
+  }
+  P[(size_t)s * n + j] = acc;
+}
+
+__global__ void k_colq(int n, int nsplit, const double *__restrict__ P, double *__restrict__ q) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) s += P[(size_t)sp * n + j];
+  q[j] = 1.0 / sqrt(s);
+}
+
+template <class ST>
+__global__ __launch_bounds__(256) void k_colmul(ST *__restrict__ A, int64_t lda, int m, int n, const double *__restrict__ q) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double qj = q[j];
+  for (int i = blockIdx.y; i < m; i += gridDim.y) A[(size_t)i * lda + j] = (ST)((double)A[(size_t)i * lda + j] * qj);
+}
+
+int splits_for(int m, int n) {
+  const int panels = (n + 255) / 256;
+  int s = (2048 + panels - 1) / panels;  // aim at >= 2048 workgroups
+  s = std::max(1, std::min(s, (m + 63) / 64));
+  return s;
+}
+
+}  // namespace
+
+int andr_colscale(andr_ctx *ctx, void *A, bool fp32, int64_t lda, int m, int n, double *q) {
+  const int ns = splits_for(m, n);
+  const int rps = (m + ns - 1) / ns;
+  DevBuf<double> P;
+  if (P.alloc((size_t)ns * n)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scaling scratch");
+  dim3 g1((n + 255) / 256, ns);
+  if (fp32)
+    hipLaunchKernelGGL(k_colsq<float>, g1, dim3(256), 0, ctx->stream, (const float *)A, lda, m, n, rps, P.p);
+  else
+    hipLaunchKernelGGL(k_colsq<double>, g1, dim3(256), 0, ctx->stream, (const double *)A, lda, m, n, rps, P.p);
+  hipLaunchKernelGGL(k_colq, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ns, P.p, q);
+  dim3 g2((n + 255) / 256, std::min(m, 256));
+  if (fp32)
+    hipLaunchKernelGGL(k_colmul<float>, g2, dim3(256), 0, ctx->stream, (float *)A, lda, m, n, q);
+  else
+    hipLaunchKernelGGL(k_colmul<double>, g2, dim3(256), 0, ctx->stream, (double *)A, lda, m, n, q);
+  ANDR_HIP(ctx, hipGetLastError());
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ANDR_OK;
+}
+
+int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m, int n, const double *q,
+                     const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop) {
+  if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
+  if (m < 1 || n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
+  hipStream_t st = ctx->stream;
+  const int wavesN = (m + 7) / 8;
+  const int blocksN = (wavesN + 3) / 4;
+  const int ns = splits_for(m, n);
+  const int rps = (m + ns - 1) / ns;
+  const int blocksC = (n * NB + 255) / 256;
+  const int blocksM = (m * NB + 255) / 256;
+  DevBuf<double> U, V, W, Xl, X0, Xs, P, partU, partV, partW, partB, sc;
+  DevBuf<LsqrState> S;
+  DevBuf<int> act;
+  if (U.alloc((size_t)m * NB) || V.alloc((size_t)n * NB) || W.alloc((size_t)n * NB) || Xl.alloc((size_t)n * NB) ||
+      X0.alloc((size_t)n * NB) || Xs.alloc((size_t)n * NB) || P.alloc((size_t)ns * n * NB) ||
+      partU.alloc((size_t)wavesN * NB) || partV.alloc((size_t)blocksC * NB) || partW.alloc((size_t)blocksC * NB) ||
+      partB.alloc((size_t)blocksM * NB) || sc.alloc(8 * NB) || S.alloc(NB) || act.alloc(NB))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR workspace (m=%d n=%d)", m, n);
+  double *sxn = sc.p, *syn = sc.p + NB, *sxt = sc.p + 2 * NB, *syt = sc.p + 3 * NB;
+  double *t1 = sc.p + 4 * NB, *t2 = sc.p + 5 * NB, *ia = sc.p + 6 * NB, *one = sc.p + 7 * NB;
+  // host-side initial state
+  LsqrState hs[NB];
+  std::memset(hs, 0, sizeof(hs));
+  const int itnlim = maxit > 0 ? maxit : 10 * (m + n + 50);
+  for (int k = 0; k < NB; ++k) {
+    hs[k].active = k < nrhs ? 1 : 0;
+    hs[k].itnlim = itnlim;
+  }
+  double hsc[8 * NB];
+  for (int k = 0; k < NB; ++k) {
+    hsc[k] = -1.0;       // sxn: U = -A x_s + U
+    hsc[NB + k] = 1.0;   // syn
+    hsc[7 * NB + k] = 1.0;
+  }
+  ANDR_HIP(ctx, hipMemcpyAsync(S.p, hs, sizeof(hs), hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(sc.p, hsc, sizeof(hsc), hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(X0.p, X, sizeof(double) * n * NB, hipMemcpyDeviceToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(U.p, B, sizeof(double) * m * NB, hipMemcpyDeviceToDevice, st));
+  ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
+  ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
+  ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
+  auto gemv_n = [&](const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
+    if (fp32)
+      hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
+                         part);
+    else
+      hipLaunchKernelGGL(k_gemv_n<double>, dim3(blocksN), dim3(256), 0, st, (const double *)A, lda, m, n, Xin, Y, sx,
+                         sy, part);
+  };
+  auto gemv_t = [&](const double *Y) {
+    dim3 g((n + 255) / 256, ns);
+    if (fp32)
+      hipLaunchKernelGGL(k_gemv_t<float>, g, dim3(256), 0, st, (const float *)A, lda, m, n, Y, rps, P.p);
+    else
+      hipLaunchKernelGGL(k_gemv_t<double>, g, dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps, P.p);
+  };
+  // x_s = x0 / q ; U = b - A x_s ; ||U||, ||b||
+  hipLaunchKernelGGL(k_scale_div, dim3(blocksC), dim3(256), 0, st, n, X0.p, q, Xs.p);
+  gemv_n(Xs.p, U.p, sxn, syn, partU.p);
+  hipLaunchKernelGGL(k_sumsq_parts, dim3(blocksM), dim3(256), 0, st, m, B, partB.p);
+  hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, partB.p, blocksM, eps, sxt, syt);
+  // V = A^T u
+  gemv_t(U.p);
+  hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, (const double *)nullptr, sxt, syt,
+                     partV.p, partW.p);
+  hipLaunchKernelGGL(k_lsqr_init2, dim3(1), dim3(64), 0, st, S.p, partV.p, blocksC, sxn, syn, ia);
+  // w = v / alpha  (X update with t1 = 0, t2 = 0, W = V * inv_alpha)
+  {
+    double z[2 * NB];
+    for (int k = 0; k < NB; ++k) {
+      z[k] = 0.0;
+      z[NB + k] = 0.0;
+    }
+    ANDR_HIP(ctx, hipMemcpyAsync(t1, z, sizeof(double) * 2 * NB, hipMemcpyHostToDevice, st));
+    int allon[NB];
+    for (int k = 0; k < NB; ++k) allon[k] = 1;
+    ANDR_HIP(ctx, hipMemcpyAsync(act.p, allon, sizeof(allon), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
+  }
+  ANDR_HIP(ctx, hipGetLastError());
+  // main loop
+  const int poll = 8;
+  int hact[NB];
+  for (int it = 0;; ++it) {
+    gemv_n(V.p, U.p, sxn, syn, partU.p);
+    hipLaunchKernelGGL(k_lsqr_mid, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, sxt, syt);
+    gemv_t(U.p);
+    hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, W.p, sxt, syt, partV.p, partW.p);
+    hipLaunchKernelGGL(k_lsqr_end, dim3(1), dim3(64), 0, st, S.p, partV.p, partW.p, blocksC, sxn, syn, t1, t2, ia,
+                       act.p);
+    hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
+    if ((it + 1) % poll == 0 || it + 1 >= itnlim) {
+      ANDR_HIP(ctx, hipGetLastError());
+      ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+      ANDR_HIP(ctx, hipStreamSynchronize(st));
+      bool any = false;
+      for (int k = 0; k < nrhs; ++k) any = any || hs[k].active;
+      if (!any || it + 1 >= itnlim) break;
+    }
+  }
+  (void)hact;
+  // x = q .* x + x0
+  hipLaunchKernelGGL(k_scale_mul_add, dim3(blocksC), dim3(256), 0, st, n, q, Xl.p, X0.p, X);
+  ANDR_HIP(ctx, hipGetLastError());
+  ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  for (int k = 0; k < nrhs; ++k) {
+    if (nit) nit[k] = hs[k].itn;
+    if (istop) istop[k] = hs[k].istop;
+  }
+  (void)one;
+  return ANDR_OK;
+}

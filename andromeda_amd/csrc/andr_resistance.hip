@@ -1,0 +1,241 @@
+// Resistance tensors of rigid bodies: the six rigid-body Stokes problems per body, assembled and
+// solved on the device (replaces StokesInlet3sidesForceCRS / TorqueOmegaCRS,
+// src/lib_particle.f90:176-476, and the force/torque reductions :1751-1831).
+//
+// Plain FP64 path: FP64 assembly and storage, column-scaled LSQR (the reference solver) with the
+// six right-hand sides in one block. Mixed path: FP32 regular quadrature, FP64 singular blocks,
+// FP32 storage, batched restarted GMRES with FP64 vectors and residuals. Bodies are processed in
+// waves sized to the free device memory; each wave is one assembly launch for all its bodies.
+#include "andr_internal.h"
+#include "andr_assemble.h"
+#include "andr_solve.h"
+#include "andr_learned.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+struct Prepared {
+  DevBody body;
+  std::vector<int> node_wall;
+  int ne = 0, nn = 0;
+};
+
+// Rigid-body velocities at the nodes: [node][component][rhs]. Problems 0..2 translate along
+// e_x, e_y, e_z; 3..5 rotate about e_x, e_y, e_z through `center`. For the bounded form only the
+// nodes of wall 2 move (src/lib_particle.f90:220-256, 386-409).
+void rigid_velocities(const andr_mesh *m, const std::vector<int> &node_wall, bool exterior, const double c[3],
+                      std::vector<double> &U) {
+  const int nn = m->nnodes;
+  U.assign((size_t)nn * 3 * ANDR_NRHS, 0.0);
+  for (int i = 0; i < nn; ++i) {
+    if (!exterior && node_wall[i] != 2) continue;
+    const double r[3] = {m->xyz[3 * i] - c[0], m->xyz[3 * i + 1] - c[1], m->xyz[3 * i + 2] - c[2]};
+    double *u = U.data() + (size_t)i * 3 * ANDR_NRHS;
+    for (int k = 0; k < 3; ++k) u[k * ANDR_NRHS + k] = 1.0;
+    // e_x x r = (0, -z, y); e_y x r = (z, 0, -x); e_z x r = (-y, x, 0)
+    u[1 * ANDR_NRHS + 3] = -r[2];
+    u[2 * ANDR_NRHS + 3] = r[1];
+    u[0 * ANDR_NRHS + 4] = r[2];
+    u[2 * ANDR_NRHS + 4] = -r[0];
+    u[0 * ANDR_NRHS + 5] = -r[1];
+    u[1 * ANDR_NRHS + 5] = r[0];
+  }
+}
+
+}  // namespace
+
+static int check_opts(andr_ctx *ctx, const andr_resistance_opts *o) {
+  if (!o) return andr_fail(ctx, ANDR_ERR_ARG, "null options");
+  if (o->form != ANDR_FORM_EXTERIOR && o->form != ANDR_FORM_BOUNDED) return andr_fail(ctx, ANDR_ERR_ARG, "bad form");
+  if (o->precision < ANDR_PREC_FP64 || o->precision > ANDR_PREC_LEARNED)
+    return andr_fail(ctx, ANDR_ERR_ARG, "bad precision");
+  if (o->precision == ANDR_PREC_LEARNED && !ctx->has_model)
+    return andr_fail(ctx, ANDR_ERR_STATE, "learned precision needs andr_set_quadrature_model first");
+  if (!(o->mu > 0.0)) return andr_fail(ctx, ANDR_ERR_ARG, "mu must be positive");
+  return ANDR_OK;
+}
+
+extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes,
+                                     const andr_resistance_opts *opts, double *FT, andr_resistance_report *reports) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (nbodies <= 0 || !meshes || !FT) return andr_fail(ctx, ANDR_ERR_ARG, "andr_resistance_batch arguments");
+  int rc = check_opts(ctx, opts);
+  if (rc) return rc;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  const bool exterior = opts->form == ANDR_FORM_EXTERIOR;
+  const bool plain = opts->precision == ANDR_PREC_FP64;
+  const bool learned = opts->precision == ANDR_PREC_LEARNED;
+  const size_t esz = plain ? sizeof(double) : sizeof(float);
+  const double eps = opts->eps > 0.0 ? opts->eps : (plain ? 1e-12 : 1e-10);
+  const double extra = exterior ? 8.0 * ANDR_PI : 0.0;
+  if (reports) std::memset(reports, 0, sizeof(andr_resistance_report) * nbodies);
+
+  // memory plan: matrix + vectors (+ GMRES basis) per body, waves within a budget
+  size_t free_b = 0, total_b = 0;
+  ANDR_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+  const int restart = 48;
+  const size_t budget = (size_t)(0.80 * (double)free_b);
+  auto body_bytes = [&](int b) -> size_t {
+    const size_t n = 3 * (size_t)meshes[b].nelem;
+    const size_t lda = (n + 63) / 64 * 64;
+    size_t s = n * lda * esz + 4 * n * ANDR_NRHS * sizeof(double);
+    s += plain ? 10 * n * ANDR_NRHS * sizeof(double) : (size_t)(restart + 4) * n * ANDR_NRHS * sizeof(double);
+    s += (size_t)meshes[b].nnodes * 200 + (size_t)meshes[b].nelem * 300;
+    return s;
+  };
+  int b0 = 0;
+  while (b0 < nbodies) {
+    int b1 = b0;
+    size_t used = 0;
+    while (b1 < nbodies && (b1 == b0 || used + body_bytes(b1) <= budget)) used += body_bytes(b1++);
+    if (used > budget) return andr_fail(ctx, ANDR_ERR_ALLOC, "body %d needs %zu bytes, budget %zu", b0, used, budget);
+    const int nb = b1 - b0;
+    // ---- upload the bodies of the wave -----------------------------------------------------------
+    std::vector<std::unique_ptr<Prepared>> P(nb);
+    std::vector<DevBuf<char>> Abuf(nb);
+    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ubuf(nb), Ebuf(nb);
+    std::vector<SysJob> jobs(nb);
+    std::vector<int2> rowmap;
+    const double ta0 = andr_now();
+    for (int i = 0; i < nb; ++i) {
+      const andr_mesh *m = &meshes[b0 + i];
+      P[i].reset(new Prepared());
+      Prepared &p = *P[i];
+      // exterior form: normals into the body (out of the fluid), whatever the input orientation factor
+      std::vector<double> nm(std::max(1, m->nwalls), 1.0);
+      if (m->normmul)
+        for (int w = 0; w < m->nwalls; ++w) nm[w] = m->normmul[w];
+      if (exterior)
+        for (auto &v : nm) v = -std::fabs(v);
+      andr_mesh mm = *m;
+      mm.normmul = nm.data();
+      if (mm.nwalls <= 0) mm.nwalls = 1;
+      rc = andr_body_upload(ctx, &mm, p.body, false);
+      if (rc) return rc;
+      p.nn = m->nnodes;
+      p.ne = m->nelem;
+      p.node_wall.assign(p.nn, 0);
+      for (int e = 0; e < p.ne; ++e) {
+        const int w = m->bcid ? m->bcid[e] : 1;
+        for (int v = 0; v < 3; ++v) {
+          int nd = m->con[3 * e + v] - m->index_base;
+          p.node_wall[nd] = std::max(p.node_wall[nd], w);
+        }
+      }
+      std::vector<double> U;
+      rigid_velocities(m, p.node_wall, exterior, opts->center, U);
+      const int n = 3 * p.ne;
+      const int64_t lda = (n + 63) / 64 * 64;
+      if (Abuf[i].alloc((size_t)n * lda * esz) || Bbuf[i].alloc((size_t)n * ANDR_NRHS) ||
+          Xbuf[i].alloc((size_t)n * ANDR_NRHS) || Ubuf[i].alloc(U.size()) || Ebuf[i].alloc(p.ne))
+        return andr_fail(ctx, ANDR_ERR_ALLOC, "resistance system of body %d (n = %d)", b0 + i, n);
+      ANDR_HIP(ctx, hipMemcpyAsync(Ubuf[i].p, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice, ctx->stream));
+      ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const QuadRule qr = andr_rule(p.body.itri);
+      SysJob &J = jobs[i];
+      J.g = p.body.g;
+      J.A_ = Abuf[i].p;
+      J.lda = lda;
+      J.B = Bbuf[i].p;
+      J.U = Ubuf[i].p;
+      J.mu = opts->mu;
+      J.extra = extra;
+      J.qoff = qr.offset;
+      J.nq = qr.npts;
+      J.err_rows = Ebuf[i].p;
+      for (int r = 0; r < p.ne; ++r) rowmap.push_back(make_int2(i, r));
+    }
+    DevBuf<SysJob> djobs;
+    DevBuf<int2> drow;
+    if (djobs.alloc(nb) || drow.alloc(rowmap.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "job tables");
+    ANDR_HIP(ctx, hipMemcpyAsync(djobs.p, jobs.data(), sizeof(SysJob) * nb, hipMemcpyHostToDevice, ctx->stream));
+    ANDR_HIP(ctx, hipMemcpyAsync(drow.p, rowmap.data(), sizeof(int2) * rowmap.size(), hipMemcpyHostToDevice, ctx->stream));
+    LearnedStats ls{};
+    if (learned) rc = andr_launch_stokes_system_learned(ctx, djobs.p, drow.p, (int)rowmap.size(), &ls);
+    else rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain);
+    if (rc) return rc;
+    ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double ta1 = andr_now();
+    // ---- solve ----------------------------------------------------------------------------------
+    std::vector<int> iters((size_t)nb * 6, 0), stops((size_t)nb * 6, 0);
+    if (plain) {
+      for (int i = 0; i < nb; ++i) {
+        const int n = 3 * P[i]->ne;
+        DevBuf<double> q;
+        if (q.alloc(n)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scale");
+        rc = andr_colscale(ctx, Abuf[i].p, false, jobs[i].lda, n, n, q.p);
+        if (rc) return rc;
+        ANDR_HIP(ctx, hipMemsetAsync(Xbuf[i].p, 0, sizeof(double) * n * ANDR_NRHS, ctx->stream));
+        rc = andr_lsqr_device(ctx, Abuf[i].p, false, jobs[i].lda, n, n, q.p, Bbuf[i].p, Xbuf[i].p, 6, opts->maxit, eps,
+                              &iters[(size_t)i * 6], &stops[(size_t)i * 6]);
+        if (rc) return rc;
+      }
+    } else {
+      std::vector<GmresSys> gs(nb);
+      for (int i = 0; i < nb; ++i) gs[i] = {Abuf[i].p, jobs[i].lda, 3 * P[i]->ne, Bbuf[i].p, Xbuf[i].p};
+      rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
+                            iters.data());
+      if (rc) return rc;
+    }
+    ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double ts1 = andr_now();
+    // ---- force and torque on wall 1 ---------------------------------------------------------------
+    for (int i = 0; i < nb; ++i) {
+      Prepared &p = *P[i];
+      const int ne = p.ne, n = 3 * ne;
+      std::vector<double> X((size_t)n * ANDR_NRHS), area(ne), cen(3 * (size_t)ne), err(ne);
+      ANDR_HIP(ctx, hipMemcpyAsync(X.data(), Xbuf[i].p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
+      ANDR_HIP(ctx, hipMemcpyAsync(area.data(), p.body.g.area, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
+      ANDR_HIP(ctx, hipMemcpyAsync(cen.data(), p.body.g.cen, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, ctx->stream));
+      ANDR_HIP(ctx, hipMemcpyAsync(err.data(), Ebuf[i].p, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
+      ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const andr_mesh *m = &meshes[b0 + i];
+      double *ft = FT + (size_t)(b0 + i) * 36;
+      for (int k = 0; k < 6; ++k) {
+        double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        for (int e = 0; e < ne; ++e) {
+          if (m->bcid && m->bcid[e] != 1) continue;
+          const double q[3] = {X[(size_t)e * ANDR_NRHS + k], X[(size_t)(ne + e) * ANDR_NRHS + k],
+                               X[(size_t)(2 * ne + e) * ANDR_NRHS + k]};
+          const double r[3] = {cen[3 * e] - opts->center[0], cen[3 * e + 1] - opts->center[1],
+                               cen[3 * e + 2] - opts->center[2]};
+          for (int c = 0; c < 3; ++c) F[c] += area[e] * q[c];
+          T[0] += area[e] * (r[1] * q[2] - r[2] * q[1]);
+          T[1] += area[e] * (r[2] * q[0] - r[0] * q[2]);
+          T[2] += area[e] * (r[0] * q[1] - r[1] * q[0]);
+        }
+        for (int c = 0; c < 3; ++c) {
+          ft[6 * k + c] = F[c];
+          ft[6 * k + 3 + c] = T[c];
+        }
+      }
+      if (reports) {
+        andr_resistance_report &R = reports[b0 + i];
+        for (int k = 0; k < 6; ++k) {
+          R.nit[k] = iters[(size_t)i * 6 + k];
+          R.status = std::max(R.status, stops[(size_t)i * 6 + k] > 3 ? stops[(size_t)i * 6 + k] : 0);
+        }
+        R.err[0] = 0.0;
+        for (int e = 0; e < ne; ++e) R.err[1] = std::max(R.err[1], err[e]);
+        R.t_assemble = (ta1 - ta0) / nb;
+        R.t_solve = (ts1 - ta1) / nb;
+        const QuadRule qr = andr_rule(p.body.itri);
+        R.quad_points = learned ? ls.points / nb : (int64_t)ne * (ne - 1) * qr.npts;
+        R.pairs_learned = learned ? ls.learned / nb : 0;
+        R.pairs_fallback = learned ? ls.fallback / nb : 0;
+      }
+    }
+    b0 = b1;
+  }
+  return ANDR_OK;
+}
+
+extern "C" int andr_resistance(andr_ctx *ctx, const andr_mesh *mesh, const andr_resistance_opts *opts, double FT[36],
+                               andr_resistance_report *report) {
+  return andr_resistance_batch(ctx, 1, mesh, opts, FT, report);
+}

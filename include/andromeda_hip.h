@@ -1,0 +1,168 @@
+/*
+ * andromeda_hip.h -- C-ABI of the MI355X-native BEM hot path (assembly, solve, resistance).
+ *
+ * Plain C types only: pointers, sizes, 32/64-bit integers and doubles. Every entry point returns
+ * an int status (0 ok; > 0 solver status compatible with the LSQR istop code; < 0 argument,
+ * allocation or HIP error, message in andr_last_error()). Nothing aborts the host process.
+ *
+ * Host arrays stay owned by the caller and are never freed by the library. Device memory is owned
+ * by the context and released by andr_destroy(). One context = one GPU and one HIP stream; a
+ * context must not be used from two host threads at once.
+ *
+ * Each entry point names the reference routine (WildSmilodon/Andromeda v1.8) it replaces.
+ */
+#ifndef ANDROMEDA_HIP_H
+#define ANDROMEDA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANDR_OK 0
+#define ANDR_ERR_ARG (-1)
+#define ANDR_ERR_HIP (-2)
+#define ANDR_ERR_ALLOC (-3)
+#define ANDR_ERR_STATE (-4)
+
+/* storage order of matrices handed to / returned by the host */
+#define ANDR_ROW_MAJOR 0
+#define ANDR_COL_MAJOR 1 /* Fortran Hmat(nsp, nnodes) order, src/lib_int.f90:254-255 */
+
+typedef struct andr_ctx andr_ctx;
+
+/* Surface mesh of one body (flat 3-node triangles), the data of module mMesh
+ * (src/mod_mesh.f90:16-111) passed explicitly. */
+typedef struct {
+  int32_t nnodes, nelem, nwalls;
+  const double *xyz;     /* 3*nnodes: x,y,z of node 0, node 1, ...            */
+  const int32_t *con;    /* 3*nelem: vertex ids of every triangle               */
+  const int32_t *bcid;   /* nelem: wall id 1..nwalls (NULL: every element wall 1) */
+  const double *normmul; /* nwalls: NORMMUL factor per wall (NULL: all +1)    */
+  int32_t index_base;    /* 0 for C ids, 1 for Fortran ids in con              */
+  int32_t itri;          /* regular triangle rule, ITRI 1..5 (7..126 points); 0 -> 5 */
+} andr_mesh;
+
+/* ---- context -------------------------------------------------------------------------- */
+int andr_create(andr_ctx **ctx, int32_t device);
+int andr_destroy(andr_ctx *ctx);
+const char *andr_last_error(const andr_ctx *ctx);
+/* Library version string; never NULL. */
+const char *andr_version(void);
+
+/* ---- seam 1: integral matrices ----------------------------------------------------------
+ * Replaces sdFormIntegralMatrices (src/lib_int.f90:225-323; rows by sdIntRowU/sdIntRowQ,
+ * src/lib_sdBem.f90:438-733). H is nsp x nnodes and G is nsp x nelem, nsp = nnodes + nelem,
+ * node rows first, free term included. err[2] = {fspCerr, QspCerr} as logged by CoR_Integrals. */
+int andr_assemble_laplace(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double *H, double *G,
+                          double err[2]);
+
+/* Replaces sdFormIntegralMatricesStokes (src/lib_Stokes.f90:1391-1555; rows by sdIntRowUStokes
+ * :1904-2245 and sdIntRowQStokes :1562-1897). T[6] = {Txx,Txy,Txz,Tyy,Tyz,Tzz} are nsp x nnodes,
+ * G[6] the same components nsp x nelem, prT[3]/prG[3] the pressure matrices (pointers may be
+ * NULL to skip). extra_free_term: 0 reproduces the reference; 8*pi gives the exterior form. */
+int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double extra_free_term,
+                         double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]);
+
+/* ---- seam 2: least squares --------------------------------------------------------------
+ * Link-compatible with the Fortran routines of src/lib_lsqr.f:1-167 (gfortran/flang name mangling,
+ * all arguments by reference, CSR with 1-based INTEGER(8) row pointers and INTEGER columns).
+ * lsqr_precon_: q = 1/||column||_2, a <- a diag(q) in place (src/lib_lsqr.f:131-167).
+ * lsqr_solve_: hot start from x, maxit == 0 -> 10(m+n+50) written back, ierr = istop if > 3
+ * (src/lib_lsqr.f:33-127). Runs on the device of the process default context (ANDR_DEVICE env,
+ * default 0). The device copy made by lsqr_precon_ is reused by later lsqr_solve_ calls on the
+ * same array; andr_lsqr_forget() drops it. */
+void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *nnz, double *q, const int64_t *ia,
+                  const int32_t *ja, double *a);
+void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *nnz, int32_t *maxit, const double *eps,
+                 int32_t *nit, int32_t *ierr, const double *q, const int64_t *ia, const int32_t *ja,
+                 const double *a, const double *b, double *x);
+void andr_lsqr_forget(void);
+
+typedef struct {
+  int32_t maxit;  /* 0 -> 10(m+n+50) */
+  double eps;     /* atol = eps, btol = eps ||b|| / ||r0|| (src/lib_lsqr.f:84-100) */
+  int32_t precon; /* 1: column scaling as lsqr_precon, 0: none */
+} andr_lsqr_opts;
+typedef struct {
+  int32_t nit[16];
+  int32_t istop[16];
+  double seconds;
+} andr_lsqr_stats;
+/* Multi-RHS LSQR on a dense row-major m x n host matrix (no CSR), nrhs <= 16 right-hand sides
+ * (B and X are m x nrhs and n x nrhs, column k contiguous). X is the hot start. Each column
+ * follows its own LSQR recurrence; the products A V and A^T U run for all columns at once. */
+int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double *A, int32_t nrhs, const double *B,
+                    double *X, const andr_lsqr_opts *opts, andr_lsqr_stats *stats);
+
+/* ---- seam 3: resistance ------------------------------------------------------------------- */
+#define ANDR_FORM_EXTERIOR 0 /* single surface, rigid-body velocity on the body                */
+#define ANDR_FORM_BOUNDED 1  /* body (wall 1) at rest inside an outer wall 2 carrying the flow */
+
+#define ANDR_PREC_FP64 0    /* plain path: FP64 assembly, FP64 storage, LSQR (drop-in)         */
+#define ANDR_PREC_MIXED 1   /* FP32 regular quadrature, FP64 singular integrals, FP32 storage,
+                               GMRES with FP64 vectors and FP64 restart residuals            */
+#define ANDR_PREC_LEARNED 2 /* MIXED plus learned Gauss-order selection for near-field pairs */
+
+typedef struct {
+  int32_t form;      /* ANDR_FORM_* */
+  int32_t precision; /* ANDR_PREC_* */
+  double mu;         /* viscosity (subdomain diff), default 1 */
+  double eps;        /* solver tolerance (LSQR eps / GMRES relative residual) */
+  int32_t maxit;     /* 0 -> default */
+  double center[3];  /* torque and rotation reference point */
+} andr_resistance_opts;
+
+typedef struct {
+  int32_t nit[6];       /* iterations per rigid-body problem */
+  int32_t status;       /* max solver status */
+  double err[2];        /* integral self-checks (fspCerr-like, QspCerr-like) */
+  double t_assemble;    /* seconds */
+  double t_solve;       /* seconds */
+  int64_t quad_points;  /* regular quadrature points evaluated */
+  int64_t pairs_learned;  /* near-field pairs whose rule came from the network */
+  int64_t pairs_fallback; /* near-field pairs sent to the fixed-order rule */
+} andr_resistance_report;
+
+/* Replaces StokesInlet3sidesForceCRS / TorqueOmegaCRS (src/lib_particle.f90:176-476): six
+ * rigid-body problems (translations e_x, e_y, e_z, rotations e_k x (r - center)); FT[6*k + i] is
+ * component i of (force, torque) for problem k, force = sum_e A_e q_e and torque =
+ * sum_e A_e (c_e - center) x q_e over wall 1 (src/lib_particle.f90:1751-1831). */
+int andr_resistance(andr_ctx *ctx, const andr_mesh *mesh, const andr_resistance_opts *opts, double FT[36],
+                    andr_resistance_report *report);
+
+/* Many bodies in one call: assembly of all bodies in one launch per wave of bodies that fits the
+ * device memory, then all bodies solved together. FT is 36*nbodies; reports may be NULL. */
+int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes, const andr_resistance_opts *opts,
+                          double *FT, andr_resistance_report *reports);
+
+/* ---- learned quadrature (no reference counterpart: the reference uses one fixed rule) ----- */
+/* Network: 6 inputs -> 3 hidden layers of 64 (ReLU) -> 8 classes; classes 0..6 are the Dunavant
+ * rules with 3, 4, 6, 7, 12, 13, 19 points, class 7 the fixed-order rule of the mesh (ITRI).
+ * Parameters are FP32 masters (ANDR_MLP_NPARAM floats: W1 6x64, b1, W2 64x64, b2, W3 64x64,
+ * b3, W4 64x8, b4, weights stored input-major); the assembly kernel uses a BF16 copy. */
+#define ANDR_MLP_NPARAM 9288
+/* Features of near-field (source point, element) pairs of a body and the label of each pair:
+ * the smallest rule whose G and T integrals agree with the fixed-order rule to rel_tol.
+ * Up to max_pairs pairs (pairs with distance/size < near_ratio); returns the count in *npairs. */
+int andr_quad_training_pairs(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol,
+                             int64_t max_pairs, float *features /* 6*max_pairs */, int32_t *labels,
+                             int64_t *npairs);
+/* Trains the network on the device (forward, backward and Adam kernels) for `epochs` passes of
+ * minibatches; params in/out (ANDR_MLP_NPARAM floats). loss_out[epochs] gets the mean loss. */
+int andr_mlp_train(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, float *params, float *loss_out);
+/* Forward pass on the device: probabilities 8*npairs. */
+int andr_mlp_infer(andr_ctx *ctx, int64_t npairs, const float *features, const float *params, float *probs);
+/* One forward+backward pass on a batch: loss and the gradient of the mean cross-entropy. */
+int andr_mlp_loss_grad(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels,
+                       const float *params, float *loss, float *grad);
+/* Installs the network used by ANDR_PREC_LEARNED; confidence: minimum softmax probability of the
+ * chosen class, below it the pair falls back to the fixed-order rule. */
+int andr_set_quadrature_model(andr_ctx *ctx, const float *params, float confidence, double near_ratio);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

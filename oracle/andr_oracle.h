@@ -1,0 +1,110 @@
+/*
+ * CPU oracle for the Andromeda BEM hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference algorithm (Andromeda v1.8, Fortran), used by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * Nothing in the product (andromeda_amd/) may link, load or call it.
+ *
+ * Conventions: 0-based indices, row-major dense matrices, double precision throughout.
+ * Every function names the reference routine it restates (file:line of the reference).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Quadrature: Triangle_IntegrationInit + WandzuratXiaoBary (src/mod_trikotnik.f90:282-781).
+ * itri 1..5 -> 7/25/54/85/126 points. Returns the number of points (<= 126), or -1. */
+int orc_tri_rule(int itri, double *l1, double *l2, double *l3, double *w);
+
+/* Element geometry: CalElementNormal (src/lib_mesh.f90:716-769), Triangle_Area/Heron
+ * (src/mod_trikotnik.f90:786-802), CalQMesh centroids (src/lib_mesh.f90:577-629). */
+void orc_geometry(int nnodes, int nelem, const double *xyz, const int32_t *con,
+                  double *normal, double *area, double *centroid);
+
+/* Laplace kernel BemKernel (src/lib_sdBem.f90:911-936). */
+void orc_bem_kernel(const double s[3], const double f[3], const double n[3], double *G, double *H);
+
+/* Triangle_3DLapInt (src/lib_sdBem.f90:859-905): accumulates into out[4] = {G, H1, H2, H3}. */
+void orc_tri_lap_int(const double x1[3], const double x2[3], const double x3[3], const double s[3],
+                     const double n[3], double area, double out[4]);
+
+/* Analytic singular integrals (src/mod_singInt.f90): si_tri_lap (:127-167) and si_tri_stk
+ * (:34-123) with the Ren-Chan frame (:905-1108) and closed forms (:418-899).
+ * isrc = 1,2,3 (vertex) or 4 (centroid). Gout = {Gxx,Gxy,Gxz,Gyy,Gyz,Gzz}. */
+double orc_si_tri_lap(const double x1[3], const double x2[3], const double x3[3], double area, int isrc);
+void orc_si_tri_stk(const double x1[3], const double x2[3], const double x3[3], double area, int isrc,
+                    double Gout[6]);
+
+/* StokesKernel (src/lib_Stokes.f90:2450-2544): G[9], T[9] row-major 3x3, prG[3], prT[3]. */
+void orc_stokes_kernel(const double s[3], const double f[3], const double n[3],
+                       double G[9], double T[9], double prG[3], double prT[3]);
+
+/* Triangle_3DStokesInt (src/lib_Stokes.f90:2341-2446). acc[36] accumulates
+ * {Gxx,Gxy,Gxz,Gyy,Gyz,Gzz, prGx,prGy,prGz, Txx[3],Txy[3],Txz[3],Tyy[3],Tyz[3],Tzz[3],
+ *  prTx[3],prTy[3],prTz[3]}. */
+void orc_tri_stokes_int(const double x1[3], const double x2[3], const double x3[3], const double s[3],
+                        const double n[3], double area, double acc[36]);
+
+/* A body/problem description shared by the assembly routines. Walls are numbered 1..nwalls
+ * (bcid per element); normmul[w-1] multiplies the normal of wall w (NORMMUL keyword). */
+typedef struct {
+  int nnodes, nelem, nwalls;
+  const double *xyz;     /* 3*nnodes */
+  const int32_t *con;    /* 3*nelem, 0-based */
+  const int32_t *bcid;   /* nelem, 1..nwalls */
+  const double *normmul; /* nwalls */
+  int itri;              /* 1..5 */
+} orc_body;
+
+/* sdFormIntegralMatrices (src/lib_int.f90:225-323) with sdIntRowU/sdIntRowQ
+ * (src/lib_sdBem.f90:438-733). H is nsp x nnodes, G is nsp x nelem (row-major),
+ * nsp = nnodes + nelem (node rows first, then centroid rows). err[2] = {fspCerr, QspCerr}. */
+void orc_assemble_laplace(const orc_body *b, double *H, double *G, double err[2]);
+
+/* sdFormIntegralMatricesStokes (src/lib_Stokes.f90:1391-1555) with sdIntRowUStokes
+ * (:1904-2245) and sdIntRowQStokes (:1562-1897). T[6] = {Txx,Txy,Txz,Tyy,Tyz,Tzz} each
+ * nsp x nnodes; G[6] likewise nsp x nelem; prT[3] nsp x nnodes, prG[3] nsp x nelem
+ * (prT/prG may be NULL). rows: NULL for all nsp rows, else a list of nrows source rows.
+ * extra_free_term is added to the Txx/Tyy/Tzz free term of every row (0 reproduces the
+ * reference; 8*pi gives the single-surface exterior formulation). err[2] as above. */
+void orc_assemble_stokes(const orc_body *b, int nrows, const int32_t *rows, double extra_free_term,
+                         double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]);
+
+/* LSQR, restating src/lib_lsqr.f. CSR with int64 row pointers, 0-based. */
+typedef struct { int64_t m, n, nnz; const int64_t *ia; const int32_t *ja; double *a; } orc_csr;
+/* lsqr_precon (src/lib_lsqr.f:131-167,267-329): q = 1/||col||, A <- A diag(q) in place. */
+void orc_lsqr_precon(orc_csr *A, double *q);
+/* lsqr_solve + lsqr_LSQR (src/lib_lsqr.f:33-127,470-1263). x is the hot start and the result.
+ * maxit==0 -> 10(m+n+50), written back. Returns ierr (istop if > 3, else 0); *nit, *istop_out. */
+int orc_lsqr_solve(const orc_csr *A, int *maxit, double eps, int *nit, int *istop_out,
+                   const double *q, const double *b, double *x);
+
+/* Dense Dirichlet systems (single subdomain, every wall Dirichlet for u), restating
+ * sdSetUpXandB (src/lib_sdBem.f90:4-107), sdSetUpSysMrhsM_CRS (src/lib_CRS.f90:148-343),
+ * StokesBigSystemXB (src/lib_Stokes.f90:1180-1276) and formQrow (src/lib_StokesFull.f90:4-85).
+ * Laplace: sys = multi/diff * G[centroid rows], rhs = H[centroid rows]; b = rhs * (-u).
+ * Stokes: rows interleaved (3*e + een), columns blocked by component (en*nelem + e);
+ * sys = -multi/diff * G, rhs = T (columns en*nnodes + node); b = rhs * u. */
+void orc_laplace_dirichlet_system(const orc_body *b, const double *H, const double *G, double diff,
+                                  double *sys /* ne x ne */, double *rhs /* ne x nn */);
+void orc_stokes_dirichlet_system(const orc_body *b, double *const T[6], double *const G[6],
+                                 int row_offset, double diff,
+                                 double *sys /* 3ne x 3ne */, double *rhs /* 3ne x 3nn */);
+
+/* Resistance: the six rigid-body problems (translations e_x,e_y,e_z; rotations about the
+ * origin e_k x r) on one body, solved one after another with LSQR on the column-scaled
+ * system (src/lib_particle.f90:176-476), force = sum A_e q_e (:1751-1764) and torque
+ * = sum A_e c_e x q_e (:1793-1831) over the body wall (wall 1). exterior != 0 selects the
+ * single-surface exterior formulation (rigid-body velocity on the body, normals into the body,
+ * free term + 8*pi); exterior == 0 is the reference's bounded set-up: the body (wall 1) at rest
+ * inside an outer wall (wall 2) that carries the velocity, as in src/lib_particle.f90:341-476.
+ * FT[36]: FT[6*k + i] = component i of (F, T) for problem k. Returns the max LSQR ierr. */
+int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int maxit,
+                   double FT[36], int nits[6]);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,120 @@
+"""HIP path against the oracle on the same inputs (FP64 plain path: relative 1e-10 on every
+matrix entry scale; solver outputs after tight convergence), and the mixed path against the plain
+path (1e-6 on resistance tensors). Runs only on an MI355X."""
+import numpy as np
+import pytest
+
+from andromeda_amd import mesh as M
+from andromeda_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+@pytest.mark.parametrize("sub", [2, 3])
+def test_laplace_assembly(ctx, orc, sub):
+    m = M.icosphere(sub)
+    H, G, err = ctx.assemble_laplace(m)
+    Ho, Go, erro = orc.assemble_laplace(m)
+    assert _rel(H, Ho) < 1e-10 and _rel(G, Go) < 1e-10
+    assert abs(err[0] - erro[0]) < 1e-12 and abs(err[1] - erro[1]) < 1e-9
+    Hc, Gc, _ = ctx.assemble_laplace(m, layout=api.COL_MAJOR)
+    assert np.array_equal(Hc, H) and np.array_equal(Gc, G)
+
+
+def test_laplace_reference_mesh(ctx, orc):
+    import os
+    m = M.read_gmsh(os.path.join(os.path.dirname(__file__), "golden", "sphere-01.msh"))
+    H, G, err = ctx.assemble_laplace(m)
+    Ho, Go, _ = orc.assemble_laplace(m)
+    assert _rel(H, Ho) < 1e-10 and _rel(G, Go) < 1e-10
+
+
+def test_stokes_assembly(ctx, orc):
+    m = M.geodesic_sphere(3)
+    T, G, pT, pG, err = ctx.assemble_stokes(m, pressure=True)
+    To, Go, pTo, pGo, erro = orc.assemble_stokes(m, pressure=True)
+    for k in range(6):
+        assert _rel(T[k], To[k]) < 1e-10, k
+        assert _rel(G[k], Go[k]) < 1e-10, k
+    for k in range(3):
+        assert _rel(pT[k], pTo[k]) < 1e-10 and _rel(pG[k], pGo[k]) < 1e-10
+
+
+def test_stokes_assembly_two_walls_exterior_term(ctx, orc):
+    m = M.body_in_sphere(M.geodesic_sphere(2, 0.5), 8.0, 2)
+    T, G, _, _, _ = ctx.assemble_stokes(m, extra=8 * np.pi, pressure=False)
+    To, Go, _, _, _ = orc.assemble_stokes(m, extra=8 * np.pi)
+    for k in range(6):
+        assert _rel(T[k], To[k]) < 1e-10 and _rel(G[k], Go[k]) < 1e-10
+
+
+def test_lsqr_dense_laplace_system(ctx, orc):
+    m = M.icosphere(3)
+    Ho, Go, _ = orc.assemble_laplace(m)
+    sysM, rhsM = orc.laplace_dirichlet_system(m, Ho, Go)
+    b = rhsM @ (-m.xyz[:, 0])
+    ia, ja, a = orc.csr_from_dense(sysM)
+    q = orc.lsqr_precon(ia, ja, a, m.nelem, m.nelem)
+    xo, info = orc.lsqr_solve(ia, ja, a, m.nelem, m.nelem, q, b, eps=1e-12)
+    X, st = ctx.lsqr_dense(sysM, b, eps=1e-12)
+    assert abs(st["nit"][0] - info["nit"]) <= 2
+    assert np.linalg.norm(X[:, 0] - xo) / np.linalg.norm(xo) < 1e-9
+
+
+def test_fortran_lsqr_shim(ctx, orc):
+    rng = np.random.default_rng(11)
+    n = 300
+    A = rng.normal(size=(n, n)) + 10 * np.eye(n)
+    b = rng.normal(size=n)
+    ia, ja, a = orc.csr_from_dense(A)
+    qo = orc.lsqr_precon(ia, ja, a, n, n)
+    xo, info = orc.lsqr_solve(ia, ja, a, n, n, qo, b, eps=1e-14)
+    # the Fortran host passes 1-based CSR
+    ia1, ja1 = ia + 1, ja + 1
+    a1 = A.reshape(-1).copy()
+    q = api.lsqr_precon(n, n, ia1, ja1, a1)
+    assert _rel(q, qo) < 1e-14 and _rel(a1, a) < 1e-14
+    x = np.zeros(n)
+    r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, x, maxit=0, eps=1e-14)
+    assert r["ierr"] == 0 and r["maxit"] == 10 * (n + n + 50)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["sphere", "superellipsoid"])
+def test_resistance_plain_vs_oracle(ctx, orc, kind):
+    if kind == "sphere":
+        m = M.geodesic_sphere(3)
+    else:
+        m = M.random_superellipsoid(np.random.default_rng(3), 3)[0]
+    FTo, nits, ierr = orc.resistance(m, exterior=True, eps=1e-12)
+    r = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+    assert r.status == 0
+    assert _rel(r.FT, FTo) < 1e-9
+
+
+def test_resistance_bounded_vs_oracle(ctx, orc):
+    m = M.body_in_sphere(M.geodesic_sphere(2, 0.5), 16.0, 2)
+    FTo, _, _ = orc.resistance(m, exterior=False, eps=1e-12)
+    r = ctx.resistance(m, form=api.FORM_BOUNDED, precision=api.PREC_FP64, eps=1e-12)
+    assert _rel(r.FT, FTo) < 1e-9
+
+
+def test_mixed_vs_plain(ctx):
+    m = M.random_superellipsoid(np.random.default_rng(7), 8)[0]
+    p = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+    x = ctx.resistance(m, precision=api.PREC_MIXED)
+    for blk in [(slice(0, 3), slice(0, 3)), (slice(0, 3), slice(3, 6)), (slice(3, 6), slice(3, 6))]:
+        assert _rel(x.FT[blk], p.FT[blk]) < 1e-6
+
+
+def test_batch_equals_single(ctx):
+    bodies = [M.random_superellipsoid(np.random.default_rng(s), 4)[0] for s in range(3)]
+    for prec in (api.PREC_FP64, api.PREC_MIXED):
+        rb = ctx.resistance_batch(bodies, precision=prec)
+        for b, r in zip(bodies, rb):
+            s = ctx.resistance(b, precision=prec)
+            assert _rel(r.FT, s.FT) < 1e-12

@@ -1,0 +1,75 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every entry point the
+header declares; mesh generators and the gmsh reader produce valid closed surfaces."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from andromeda_amd import mesh as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "andromeda_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:andr|lsqr)_\w+)\s*\(", src)))
+
+
+def test_library_exports_header():
+    import ctypes
+    from andromeda_amd import api
+    if not os.path.exists(api.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(api.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(api.EXPORTED) <= set(names)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from andromeda_amd import api
+    with pytest.raises(api.AndromedaError):
+        api._lib = None
+        api.load_library(str(tmp_path / "nope.so"))
+    api._lib = None
+
+
+@pytest.mark.parametrize("nu", [1, 2, 3, 5])
+def test_geodesic_sphere(nu):
+    m = M.geodesic_sphere(nu)
+    assert m.nelem == 20 * nu * nu and m.nnodes == 10 * nu * nu + 2
+    x1, x2, x3 = (m.xyz[m.con[:, k]] for k in range(3))
+    n = np.cross(x2 - x1, x3 - x1)
+    assert np.abs(n.sum(0)).max() < 1e-12               # closed surface
+    assert (np.einsum("ij,ij->i", n, x1 + x2 + x3) > 0).all()  # outward
+    assert np.allclose(np.linalg.norm(m.xyz, axis=1), 1.0)
+    # every edge shared by exactly two triangles
+    e = np.sort(np.concatenate([m.con[:, [0, 1]], m.con[:, [1, 2]], m.con[:, [2, 0]]]), axis=1)
+    _, cnt = np.unique(e, axis=0, return_counts=True)
+    assert (cnt == 2).all()
+
+
+def test_superellipsoid_on_surface():
+    rng = np.random.default_rng(0)
+    m, info = M.random_superellipsoid(rng, 6)
+    a = np.array(info["axes"])
+    f = np.sum(np.abs(m.xyz / a) ** info["p"], axis=1)
+    assert np.abs(f - 1.0).max() < 1e-12
+
+
+def test_batch_sizes():
+    bodies = M.batch_bodies(8, 2)
+    for mesh, info in bodies:
+        assert 500 <= mesh.nelem <= 8000
+        assert mesh.nelem == 20 * info["nu"] ** 2
+
+
+def test_gmsh_reader_reference_mesh():
+    m = M.read_gmsh(os.path.join(ROOT, "tests", "golden", "sphere-01.msh"))
+    assert m.nelem == 836 and m.nnodes == 420
+    x1, x2, x3 = (m.xyz[m.con[:, k]] for k in range(3))
+    assert np.abs(np.cross(x2 - x1, x3 - x1).sum(0)).max() < 1e-10
