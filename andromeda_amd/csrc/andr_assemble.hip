@@ -298,6 +298,8 @@ int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, do
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * b.g.nn;
   if (lds <= 128 * 1024) {
+    ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_laplace_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
     hipLaunchKernelGGL(k_laplace_rows<true>, dim3(nsp), dim3(256), lds, ctx->stream, b.g, qr.offset, qr.npts,
                        b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows);
   } else {
@@ -489,6 +491,8 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * 9 * (size_t)b.g.nn;
   if (lds <= 144 * 1024) {
+    ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_stokes_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
     hipLaunchKernelGGL(k_stokes_rows<true>, dim3(nsp), dim3(256), lds, ctx->stream, b.g, qr.offset, qr.npts,
                        b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows);
   } else {

@@ -182,6 +182,13 @@ bool probe_ok(const LsqrCache &c, const double *a) {
 }
 int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
                DevBuf<double> &D) {
+  // the device scatter trusts these indices: check the 1-based CSR on the host first
+  if (m <= 0 || n <= 0 || nnz < 0 || ia[0] != 1 || ia[m] != nnz + 1)
+    return andr_fail(ctx, ANDR_ERR_ARG, "CSR row pointers inconsistent (m=%d nnz=%lld)", m, (long long)nnz);
+  for (int i = 0; i < m; ++i)
+    if (ia[i + 1] < ia[i]) return andr_fail(ctx, ANDR_ERR_ARG, "CSR row pointer decreases at row %d", i + 1);
+  for (int64_t p = 0; p < nnz; ++p)
+    if (ja[p] < 1 || ja[p] > n) return andr_fail(ctx, ANDR_ERR_ARG, "CSR column %d out of range", ja[p]);
   DevBuf<int64_t> dia;
   DevBuf<int32_t> dja;
   DevBuf<double> da;
