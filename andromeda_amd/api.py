@@ -91,6 +91,8 @@ def load_library(path: str = LIB_PATH):
     lib.andr_mlp_infer.argtypes = [C.c_void_p, C.c_int64, _F, _F, _F]
     lib.andr_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int64, _F, _I, _F, _F, _F]
     lib.andr_set_quadrature_model.argtypes = [C.c_void_p, _F, C.c_float, C.c_double]
+    lib.andr_profile.argtypes = [C.c_void_p, C.c_int32]
+    lib.andr_profile_read.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _L, _D, _D, _D]
     _lib = lib
     return lib
 
@@ -98,7 +100,8 @@ def load_library(path: str = LIB_PATH):
 EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "andr_assemble_laplace",
             "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
-            "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model"]
+            "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
+            "andr_profile_read"]
 
 
 def _p(a):
@@ -240,6 +243,20 @@ class Context:
 
     def resistance(self, mesh, **kw) -> ResistanceResult:
         return self.resistance_batch([mesh], **kw)[0]
+
+    # ---- measurement ----------------------------------------------------------------------------
+    def profile(self, enable: bool = True):
+        self._check(self.lib.andr_profile(self.h, 1 if enable else 0), "andr_profile")
+
+    def profile_read(self) -> dict:
+        out = {}
+        i = 0
+        name = C.create_string_buffer(128)
+        n, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        while self.lib.andr_profile_read(self.h, i, name, 128, C.byref(n), C.byref(ms), C.byref(by), C.byref(fl)) == 1:
+            out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": by.value, "flops": fl.value}
+            i += 1
+        return out
 
     # ---- learned quadrature ---------------------------------------------------------------------
     def quad_training_pairs(self, mesh, near_ratio: float = 4.0, rel_tol: float = 1e-8, max_pairs: int = 200000,

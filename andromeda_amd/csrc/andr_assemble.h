@@ -33,4 +33,7 @@ struct SysJob {
 };
 
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
-                              bool fp32_store);
+                              bool fp32_store, double alg_bytes, double alg_flops);
+// algorithmic work of a Stokes pair evaluation: G + T kernels, 107 flops per quadrature point
+#define ANDR_FLOPS_PER_STOKES_EVAL 107.0
+#define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0

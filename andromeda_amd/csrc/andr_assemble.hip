@@ -12,6 +12,7 @@
 #include "andr_wx_rules.h"
 #include "andr_dunavant_rules.h"
 #include "andr_assemble.h"
+#include "andr_prof.h"
 
 #include <algorithm>
 #include <cstdarg>
@@ -297,6 +298,9 @@ int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, do
   const int nsp = b.g.nn + b.g.ne;
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * b.g.nn;
+  const double evals = (double)nsp * b.g.ne * qr.npts;
+  const int tok = andr_prof_begin(ctx, "laplace_rows", 8.0 * nsp * ((double)b.g.nn + b.g.ne),
+                                  evals * ANDR_FLOPS_PER_LAPLACE_EVAL);
   if (lds <= 128 * 1024) {
     ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_laplace_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
@@ -306,6 +310,7 @@ int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, do
     hipLaunchKernelGGL(k_laplace_rows<false>, dim3(nsp), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts,
                        b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows);
   }
+  andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;
 }
@@ -490,6 +495,9 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
   const int nsp = b.g.nn + b.g.ne;
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * 9 * (size_t)b.g.nn;
+  const double evals = (double)nsp * b.g.ne * qr.npts;
+  const int tok = andr_prof_begin(ctx, "stokes_rows", 8.0 * 9 * nsp * ((double)b.g.nn + b.g.ne),
+                                  evals * ANDR_FLOPS_PER_STOKES_EVAL);
   if (lds <= 144 * 1024) {
     ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_stokes_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
@@ -499,6 +507,7 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
     hipLaunchKernelGGL(k_stokes_rows<false>, dim3(nsp), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts,
                        b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows);
   }
+  andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;
 }
@@ -638,8 +647,9 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
 }
 
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
-                              bool fp32_store) {
+                              bool fp32_store, double alg_bytes, double alg_flops) {
   if (nrows <= 0) return ANDR_OK;
+  const int tok = andr_prof_begin(ctx, fp32_quad ? "stokes_system_fp32" : "stokes_system_fp64", alg_bytes, alg_flops);
   if (!fp32_quad && !fp32_store)
     hipLaunchKernelGGL((k_stokes_system<double, double>), dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
   else if (fp32_quad && fp32_store)
@@ -648,6 +658,7 @@ int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d
     hipLaunchKernelGGL((k_stokes_system<double, float>), dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
   else
     return andr_fail(ctx, ANDR_ERR_ARG, "FP32 quadrature with FP64 storage is not offered");
+  andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;
 }

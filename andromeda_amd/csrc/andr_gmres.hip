@@ -11,6 +11,7 @@
 // products over the basis, which keeps the basis traffic streaming instead of latency bound.
 #include "andr_internal.h"
 #include "andr_solve.h"
+#include "andr_prof.h"
 
 #include <algorithm>
 #include <cmath>
@@ -415,13 +416,22 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
 
   const int gw = (nw + 3) / 4;
   const int gs = (nsys * NB + 63) / 64;
+  double sum_n2 = 0.0, sum_n = 0.0;
+  for (int s = 0; s < nsys; ++s) {
+    sum_n2 += (double)in[s].n * in[s].n;
+    sum_n += in[s].n;
+  }
+  const double gbytes = sum_n2 * (fp32 ? 4.0 : 8.0) + 3.0 * sum_n * NB * 8.0;
+  const double gflops = 2.0 * sum_n2 * nrhs;
   auto gemv = [&](int which_x, int jq, int which_y, const double *scales) {
+    const int tok = andr_prof_begin(ctx, fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64", gbytes, gflops);
     if (fp32)
       hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
                          partW.p);
     else
       hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
                          partW.p);
+    andr_prof_end(ctx, tok);
   };
   std::vector<int> it_count((size_t)nsys * NB, 0);
   std::vector<double> hsmall(small_per * nsys);

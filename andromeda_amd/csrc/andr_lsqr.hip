@@ -10,6 +10,7 @@
 // separate scaling passes of the reference (dscal at :947, :974, :980).
 #include "andr_internal.h"
 #include "andr_solve.h"
+#include "andr_prof.h"
 
 #include <cmath>
 #include <cstring>
@@ -463,20 +464,27 @@ int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m
   ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
   ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
   ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
+  const double abytes = (double)m * n * (fp32 ? 4.0 : 8.0);
+  const double vbytes = ((double)m + n) * NB * 8.0;
+  const double aflops = 2.0 * m * n * nrhs;
   auto gemv_n = [&](const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
+    const int tok = andr_prof_begin(ctx, "lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0, aflops);
     if (fp32)
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
     else
       hipLaunchKernelGGL(k_gemv_n<double>, dim3(blocksN), dim3(256), 0, st, (const double *)A, lda, m, n, Xin, Y, sx,
                          sy, part);
+    andr_prof_end(ctx, tok);
   };
   auto gemv_t = [&](const double *Y) {
+    const int tok = andr_prof_begin(ctx, "lsqr_gemv_t", abytes + vbytes + (double)ns * n * NB * 8.0, aflops);
     dim3 g((n + 255) / 256, ns);
     if (fp32)
       hipLaunchKernelGGL(k_gemv_t<float>, g, dim3(256), 0, st, (const float *)A, lda, m, n, Y, rps, P.p);
     else
       hipLaunchKernelGGL(k_gemv_t<double>, g, dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps, P.p);
+    andr_prof_end(ctx, tok);
   };
   // x_s = x0 / q ; U = b - A x_s ; ||U||, ||b||
   hipLaunchKernelGGL(k_scale_div, dim3(blocksC), dim3(256), 0, st, n, X0.p, q, Xs.p);

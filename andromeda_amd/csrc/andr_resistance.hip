@@ -156,8 +156,15 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     ANDR_HIP(ctx, hipMemcpyAsync(djobs.p, jobs.data(), sizeof(SysJob) * nb, hipMemcpyHostToDevice, ctx->stream));
     ANDR_HIP(ctx, hipMemcpyAsync(drow.p, rowmap.data(), sizeof(int2) * rowmap.size(), hipMemcpyHostToDevice, ctx->stream));
     LearnedStats ls{};
+    double alg_bytes = 0.0, alg_flops = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const double ne = P[i]->ne;
+      alg_bytes += 9.0 * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
+      alg_flops += ne * (ne - 1.0) * jobs[i].nq * ANDR_FLOPS_PER_STOKES_EVAL;
+    }
     if (learned) rc = andr_launch_stokes_system_learned(ctx, djobs.p, drow.p, (int)rowmap.size(), &ls);
-    else rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain);
+    else
+      rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain, alg_bytes, alg_flops);
     if (rc) return rc;
     ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double ta1 = andr_now();

@@ -54,11 +54,21 @@ def _icosahedron():
     return v, f
 
 
+_SPHERES: dict = {}
+
+
 def geodesic_sphere(nu: int, radius: float = 1.0) -> Mesh:
     """Class-I geodesic sphere: every icosahedron face split into nu*nu triangles, vertices
     projected radially onto the sphere. F = 20 nu^2, V = 10 nu^2 + 2; outward normals
     (counter-clockwise seen from outside). nu = 2**s gives the "icosphere subdivision s" sizes
     (s = 3: 1280 triangles, s = 5: 20480)."""
+    if nu not in _SPHERES:
+        _SPHERES[nu] = _geodesic_unit(nu)
+    s = _SPHERES[nu]
+    return Mesh(s.xyz * radius, s.con.copy(), s.bcid.copy(), s.name)
+
+
+def _geodesic_unit(nu: int) -> Mesh:
     v0, f0 = _icosahedron()
     key_to_id: dict = {}
     pts: list = []
@@ -99,7 +109,6 @@ def geodesic_sphere(nu: int, radius: float = 1.0) -> Mesh:
     n = np.cross(x2 - x1, x3 - x1)
     flip = np.einsum("ij,ij->i", n, x1 + x2 + x3) < 0
     con[flip] = con[flip][:, [0, 2, 1]]
-    xyz *= radius
     return Mesh(xyz, con, np.ones(con.shape[0], dtype=np.int32), f"sphere_nu{nu}")
 
 

@@ -1,0 +1,196 @@
+"""Resistance-tensor throughput on one MI355X.
+
+Workload: the batched configuration of the benchmark -- 512 seeded superellipsoids
+(|x/a|^p + |y/b|^p + |z/c|^p = 1, axes U[0.5, 2], p U[2, 8]) and 64 fibre-like bodies (aspect
+U[10, 50]), element counts log-uniform in [500, 8000] rounded to geodesic sizes 20 nu^2, six
+rigid-body problems per body (single-surface exterior Stokes). One step computes all 576 6x6
+resistance tensors with the fast form (batched, FP32 quadrature and storage, FP64 solve; with the
+learned Gauss-order selection when a trained model file is present). The plain FP64 path (FP64
+assembly, FP64 LSQR, the drop-in) and the fast form are both timed on a size-stratified sample
+of the same bodies, which also gives the fast form's error against FP64. The CPU oracle is
+timed on a bounded sample on the host cores.
+
+Prints one JSON line (see the driver contract) plus a human-readable summary on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "resistance tensors/sec at ≤1e-6 rel err, batched mixed+learned vs plain FP64 GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_bodies(n_super, n_fibre):
+    from andromeda_amd import mesh as M
+    return [m for m, _ in M.batch_bodies(n_super, n_fibre)]
+
+
+def stratified(bodies, k):
+    order = sorted(range(len(bodies)), key=lambda i: bodies[i].nelem)
+    if k >= len(order):
+        return order
+    return [order[int(round(j * (len(order) - 1) / (k - 1)))] for j in range(k)]
+
+
+def cpu_baseline(bodies, budget_s=20.0):
+    """Oracle (C, FP64, OpenMP assembly + serial LSQR) on the smallest bodies of the batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    order = sorted(range(len(bodies)), key=lambda i: bodies[i].nelem)
+    done, t_total, elems = 0, 0.0, []
+    for i in order:
+        t0 = time.perf_counter()
+        pyoracle.resistance(bodies[i], exterior=True, eps=1e-12)
+        t_total += time.perf_counter() - t0
+        done += 1
+        elems.append(bodies[i].nelem)
+        if t_total > budget_s * 0.5 or done >= 3:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": done / t_total, "unit": "resistance tensors/s", "cores": threads, "kind": "port",
+            "sample": f"{done} bodies of {elems} triangles (smallest of the batch), FP64 oracle, "
+                      f"{t_total:.1f} s; assembly on {threads} OpenMP threads, LSQR serial"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--bodies", type=int, default=576, help="total bodies (8/9 superellipsoids, 1/9 fibres)")
+    ap.add_argument("--sample", type=int, default=16, help="bodies of the FP64-vs-fast comparison")
+    ap.add_argument("--precision", choices=["mixed", "learned", "auto"], default="auto")
+    ap.add_argument("--model", default=os.path.join(ROOT, "andromeda_amd", "quad_model.npz"))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from andromeda_amd import api
+    from andromeda_amd.metrics import tensor_error
+
+    n_fibre = args.bodies // 9
+    n_super = args.bodies - n_fibre
+    t0 = time.perf_counter()
+    bodies = make_bodies(n_super, n_fibre)
+    log(f"[bench] {len(bodies)} bodies, {sum(b.nelem for b in bodies)} triangles, "
+        f"sum F^2 = {sum(b.nelem ** 2 for b in bodies):.3e} (mesh generation {time.perf_counter() - t0:.1f} s)")
+
+    ctx = api.Context(0)
+    prec = api.PREC_MIXED
+    prec_name = "mixed"
+    if args.precision in ("learned", "auto") and os.path.exists(args.model):
+        z = np.load(args.model)
+        ctx.set_quadrature_model(z["params"], float(z["confidence"]), float(z["near_ratio"]))
+        prec, prec_name = api.PREC_LEARNED, "learned"
+    elif args.precision == "learned":
+        raise SystemExit(f"--precision learned needs {args.model}")
+
+    for w in range(args.warmup):
+        tw = time.perf_counter()
+        ctx.resistance_batch(bodies, precision=prec)
+        log(f"[bench] warmup {w}: {time.perf_counter() - tw:.2f} s")
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    results = None
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        results = ctx.resistance_batch(bodies, precision=prec)
+        log(f"[bench] step {s}: {time.perf_counter() - ts:.2f} s")
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    value = args.steps * len(bodies) / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] fast form ({prec_name}): {value:.2f} tensors/s, {ms_per_step:.0f} ms/step")
+
+    # plain FP64 path and fast form on the same stratified sample
+    idx = stratified(bodies, args.sample)
+    sample = [bodies[i] for i in idx]
+    tp = time.perf_counter()
+    plain = ctx.resistance_batch(sample, precision=api.PREC_FP64, eps=1e-12)
+    t_plain = time.perf_counter() - tp
+    tf = time.perf_counter()
+    fast_s = ctx.resistance_batch(sample, precision=prec)
+    t_fast = time.perf_counter() - tf
+    errs = [tensor_error(f.FT, p.FT) for f, p in zip(fast_s, plain)]
+    plain_rate = len(sample) / t_plain
+    log(f"[bench] sample of {len(sample)}: plain FP64 {plain_rate:.2f} tensors/s, fast {len(sample) / t_fast:.2f}; "
+        f"max err {max(errs):.2e}; plain LSQR its {[int(p.nit.max()) for p in plain]}")
+
+    # dominant kernel of the timed region and its roofline
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None
+    roofline = None
+    kernels = {}
+    for name, v in prof.items():
+        sec = v["ms"] * 1e-3
+        kernels[name] = {"launches": v["launches"], "ms_total": round(v["ms"], 3),
+                         "GBps": round(v["bytes"] / sec / 1e9, 1) if sec > 0 else None,
+                         "TFLOPs": round(v["flops"] / sec / 1e12, 2) if sec > 0 else None}
+    if dom:
+        name, v = dom
+        avg_ms = v["ms"] / v["launches"]
+        if "gemv" in name:
+            ach = v["bytes"] / v["launches"] / (avg_ms * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                        "avg_launch_ms": round(avg_ms, 4)}
+        else:
+            ach = v["flops"] / v["launches"] / (avg_ms * 1e-3) / 1e12
+            peak = FP32_PEAK_TFLOPS if "fp32" in name else FP64_PEAK_TFLOPS
+            roofline = {"bound": "valu", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(ach / peak, 4), "traffic": None, "kernel": name,
+                        "avg_launch_ms": round(avg_ms, 4)}
+
+    cpu = None if args.no_cpu else cpu_baseline(bodies)
+    if cpu:
+        log(f"[bench] cpu oracle: {cpu['value']:.4f} tensors/s ({cpu['sample']})")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "resistance tensors/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 quadrature+storage, fp64 solve" if prec_name == "mixed" else
+                 "fp32 learned-order quadrature+storage, fp64 solve",
+        "data": "synthetic (seeded superellipsoids and fibres)",
+        "config": {"workload": f"{len(bodies)} bodies ({n_super} superellipsoids + {n_fibre} fibres), "
+                               f"F log-uniform [500, 8000], 6 rigid-body RHS, exterior Stokes",
+                   "bodies": len(bodies), "triangles": int(sum(b.nelem for b in bodies)),
+                   "form": prec_name},
+        "plain_fp64": {"sample_bodies": len(sample), "tensors_per_s": round(plain_rate, 3),
+                       "fast_tensors_per_s_same_sample": round(len(sample) / t_fast, 3),
+                       "speedup_same_sample": round(t_plain / t_fast, 2)},
+        "accuracy": {"max_tensor_err_vs_fp64": float(f"{max(errs):.3e}"), "bodies": len(sample)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -162,6 +162,14 @@ int andr_mlp_loss_grad(andr_ctx *ctx, int64_t npairs, const float *features, con
  * chosen class, below it the pair falls back to the fixed-order rule. */
 int andr_set_quadrature_model(andr_ctx *ctx, const float *params, float confidence, double near_ratio);
 
+/* ---- measurement (no reference counterpart: the reference only logs CPU timers) ------------
+ * Per-kernel timing with HIP events recorded on the context stream around each launch of the hot
+ * kernels, with the algorithmic bytes and flops of each launch. andr_profile(ctx, 1) resets and
+ * enables, 0 disables; andr_profile_read returns 1 and fills entry idx (0..), 0 past the end. */
+int andr_profile(andr_ctx *ctx, int32_t enable);
+int andr_profile_read(andr_ctx *ctx, int32_t idx, char *name, int32_t namelen, int64_t *launches, double *ms,
+                      double *bytes, double *flops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,3 +73,13 @@ def test_gmsh_reader_reference_mesh():
     assert m.nelem == 836 and m.nnodes == 420
     x1, x2, x3 = (m.xyz[m.con[:, k]] for k in range(3))
     assert np.abs(np.cross(x2 - x1, x3 - x1).sum(0)).max() < 1e-10
+
+
+def test_tensor_error_measure():
+    from andromeda_amd.metrics import tensor_error
+    R = np.diag([6.0, 6.0, 6.0, 8.0, 8.0, 8.0])
+    X = R.copy()
+    X[0, 4] = 1e-7  # coupling of a symmetric body: judged on sqrt(|K| |Omega|)
+    assert abs(tensor_error(X, R) - 1e-7 / np.sqrt(np.linalg.norm(R[:3, :3]) * np.linalg.norm(R[3:, 3:]))) < 1e-15
+    X = R * (1 + 1e-6)
+    assert abs(tensor_error(X, R) - 1e-6) < 1e-12
