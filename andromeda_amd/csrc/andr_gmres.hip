@@ -334,35 +334,67 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
   const int m = std::max(4, restart);
   hipStream_t st = ctx->stream;
-  // work maps
-  std::vector<WaveMap> wmap;
-  std::vector<TileMap> tmap;
-  std::vector<int> wave0(nsys), nwv(nsys), tile0(nsys), ntl(nsys);
+  // full work maps (sizes the partial-sum buffers); subsets are rebuilt as systems converge
   size_t vec_elems = 0, basis_elems = 0;
+  int nw_full = 0, nt_full = 0;
   for (int s = 0; s < nsys; ++s) {
-    const int n = in[s].n;
-    wave0[s] = (int)wmap.size();
-    for (int r = 0; r < n; r += 8) wmap.push_back({s, r});
-    nwv[s] = (int)wmap.size() - wave0[s];
-    tile0[s] = (int)tmap.size();
-    for (int r = 0; r < n; r += TILE) tmap.push_back({s, r});
-    ntl[s] = (int)tmap.size() - tile0[s];
-    vec_elems += (size_t)n * NB;
-    basis_elems += (size_t)(m + 1) * n * NB;
+    vec_elems += (size_t)in[s].n * NB;
+    basis_elems += (size_t)(m + 1) * in[s].n * NB;
+    nw_full += (in[s].n + 7) / 8;
+    nt_full += (in[s].n + TILE - 1) / TILE;
   }
-  const int nw = (int)wmap.size(), nt = (int)tmap.size();
   const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1) + 8);
-  DevBuf<double> R, W, Q, SM, sxy, partW, partT, partT2;
+  DevBuf<double> R, W, Q, SM, partW, partT, partT2;
   DevBuf<GSys> dsys;
-  DevBuf<WaveMap> dwm;
-  DevBuf<TileMap> dtm;
-  DevBuf<int> dw0, dnw, ddone;
+  DevBuf<int> ddone;
   if (R.alloc(vec_elems) || W.alloc(vec_elems) || Q.alloc(basis_elems) || SM.alloc(small_per * nsys) ||
-      sxy.alloc((size_t)nsys * 2 * NB) || partW.alloc((size_t)nw * NB) || partT.alloc((size_t)nt * (m + 1) * NB) ||
-      partT2.alloc((size_t)nt * NB) || dsys.alloc(nsys) || dwm.alloc(nw) || dtm.alloc(nt) || dw0.alloc(nsys) ||
-      dnw.alloc(nsys) || ddone.alloc(1))
+      partW.alloc((size_t)nw_full * NB) || partT.alloc((size_t)nt_full * (m + 1) * NB) ||
+      partT2.alloc((size_t)nt_full * NB) || dsys.alloc(nsys) || ddone.alloc(1))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
+  // a map set: waves of 8 rows and tiles of TILE rows over a list of systems
+  struct MapSet {
+    DevBuf<WaveMap> w;
+    DevBuf<TileMap> t;
+    DevBuf<int> w0, nw;
+    int n_w = 0, n_t = 0;
+    std::vector<int> tile0, ntiles;
+  };
+  MapSet cyc, live;
+  for (MapSet *ms : {&cyc, &live})
+    if (ms->w.alloc(nw_full) || ms->t.alloc(nt_full) || ms->w0.alloc(nsys) || ms->nw.alloc(nsys))
+      return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES work maps");
   std::vector<GSys> hs(nsys);
+  auto build = [&](const std::vector<int> &list, MapSet &ms) -> int {
+    std::vector<WaveMap> wm;
+    std::vector<TileMap> tm;
+    std::vector<int> w0(nsys, 0), nw(nsys, 0);
+    ms.tile0.assign(nsys, 0);
+    ms.ntiles.assign(nsys, 0);
+    for (int s : list) {
+      w0[s] = (int)wm.size();
+      for (int r = 0; r < in[s].n; r += 8) wm.push_back({s, r});
+      nw[s] = (int)wm.size() - w0[s];
+      ms.tile0[s] = (int)tm.size();
+      for (int r = 0; r < in[s].n; r += TILE) tm.push_back({s, r});
+      ms.ntiles[s] = (int)tm.size() - ms.tile0[s];
+    }
+    ms.n_w = (int)wm.size();
+    ms.n_t = (int)tm.size();
+    if (ms.n_w) ANDR_HIP(ctx, hipMemcpyAsync(ms.w.p, wm.data(), sizeof(WaveMap) * wm.size(), hipMemcpyHostToDevice, st));
+    if (ms.n_t) ANDR_HIP(ctx, hipMemcpyAsync(ms.t.p, tm.data(), sizeof(TileMap) * tm.size(), hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(ms.w0.p, w0.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(ms.nw.p, nw.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));  // host vectors go out of scope
+    return ANDR_OK;
+  };
+  auto use_tiles = [&](const MapSet &ms) -> int {  // the reductions index partials through S.tile0/ntiles
+    for (int s = 0; s < nsys; ++s) {
+      hs[s].tile0 = ms.tile0[s];
+      hs[s].ntiles = ms.ntiles[s];
+    }
+    ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
+    return ANDR_OK;
+  };
   size_t vo = 0, bo = 0;
   for (int s = 0; s < nsys; ++s) {
     GSys &g = hs[s];
@@ -387,19 +419,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     g.coef = sp;
     sp += NB * (m + 1);
     g.small = sp;
-    g.tile0 = tile0[s];
-    g.ntiles = ntl[s];
     vo += (size_t)n * NB;
     bo += (size_t)(m + 1) * n * NB;
   }
-  ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(dwm.p, wmap.data(), sizeof(WaveMap) * nw, hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(dtm.p, tmap.data(), sizeof(TileMap) * nt, hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(dw0.p, wave0.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(dnw.p, nwv.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
   ANDR_HIP(ctx, hipMemsetAsync(SM.p, 0, sizeof(double) * small_per * nsys, st));
-  for (int s = 0; s < nsys; ++s)
-    ANDR_HIP(ctx, hipMemsetAsync(in[s].X, 0, sizeof(double) * in[s].n * NB, st));
+  for (int s = 0; s < nsys; ++s) ANDR_HIP(ctx, hipMemsetAsync(in[s].X, 0, sizeof(double) * in[s].n * NB, st));
   // scale factors: residual product (sx = -1, sy = 1) and Arnoldi product (sx = 1, sy = 0)
   std::vector<double> h_res(2 * NB * (size_t)nsys), h_arn(2 * NB * (size_t)nsys);
   for (int s = 0; s < nsys; ++s)
@@ -414,23 +438,24 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemcpyAsync(d_res.p, h_res.data(), sizeof(double) * h_res.size(), hipMemcpyHostToDevice, st));
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
-  const int gw = (nw + 3) / 4;
   const int gs = (nsys * NB + 63) / 64;
-  double sum_n2 = 0.0, sum_n = 0.0;
-  for (int s = 0; s < nsys; ++s) {
-    sum_n2 += (double)in[s].n * in[s].n;
-    sum_n += in[s].n;
-  }
-  const double gbytes = sum_n2 * (fp32 ? 4.0 : 8.0) + 3.0 * sum_n * NB * 8.0;
-  const double gflops = 2.0 * sum_n2 * nrhs;
-  auto gemv = [&](int which_x, int jq, int which_y, const double *scales) {
-    const int tok = andr_prof_begin(ctx, fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64", gbytes, gflops);
+  auto gemv = [&](const MapSet &ms, int which_x, int jq, int which_y, const double *scales) {
+    if (ms.n_w == 0) return;
+    double n2 = 0.0, n1 = 0.0;
+    for (int s = 0; s < nsys; ++s)
+      if (ms.ntiles[s]) {
+        n2 += (double)in[s].n * in[s].n;
+        n1 += in[s].n;
+      }
+    const int tok = andr_prof_begin(ctx, fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64",
+                                    n2 * (fp32 ? 4.0 : 8.0) + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
+    const int gw = (ms.n_w + 3) / 4;
     if (fp32)
-      hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
-                         partW.p);
+      hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
+                         scales, partW.p);
     else
-      hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, dwm.p, nw, which_x, jq, which_y, scales,
-                         partW.p);
+      hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
+                         scales, partW.p);
     andr_prof_end(ctx, tok);
   };
   std::vector<int> it_count((size_t)nsys * NB, 0);
@@ -444,60 +469,82 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ANDR_HIP(ctx, hipStreamSynchronize(st));
     return ANDR_OK;
   };
+  auto sm_of = [&](int s, int k) { return hsmall.data() + small_per * s + k * 8; };
+  std::vector<int> cyc_list(nsys);
+  for (int s = 0; s < nsys; ++s) cyc_list[s] = s;
+  int rc = build(cyc_list, cyc);
+  if (rc) return rc;
+  if ((rc = use_tiles(cyc))) return rc;
   int total = 0;
   bool first = true;
   while (true) {
-    // residual r = b - A x (x = 0 on the first cycle), FP64 accumulation
-    hipLaunchKernelGGL(kb_copy_b, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p);
-    gemv(0, 0, 0, d_res.p);
+    // residual r = b - A x of every system of the last cycle (x = 0 on the first), FP64 accumulation
+    if (cyc.n_t) hipLaunchKernelGGL(kb_copy_b, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
+    gemv(cyc, 0, 0, 0, d_res.p);
     ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, m, partW.p, dw0.p, dnw.p, tol, first ? 1 : 0,
-                       nrhs, ddone.p);
-    int hdone = 0;
-    ANDR_HIP(ctx, hipMemcpyAsync(&hdone, ddone.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, m, partW.p, cyc.w0.p, cyc.nw.p, tol,
+                       first ? 1 : 0, nrhs, ddone.p);
     ANDR_HIP(ctx, hipGetLastError());
+    if ((rc = read_small())) return rc;
     first = false;
-    if (hdone == 0 || total >= maxit) break;
-    hipLaunchKernelGGL(kb_q0, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p);
+    // systems with an unconverged column form the next cycle
+    cyc_list.clear();
+    for (int s = 0; s < nsys; ++s)
+      for (int k = 0; k < NB; ++k)
+        if (sm_of(s, k)[4] > 0.0) {
+          cyc_list.push_back(s);
+          break;
+        }
+    if (cyc_list.empty() || total >= maxit) break;
+    if ((rc = build(cyc_list, cyc))) return rc;
+    if ((rc = build(cyc_list, live))) return rc;
+    if ((rc = use_tiles(live))) return rc;
+    hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
     const int steps_max = std::min(m, maxit - total);
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
-      gemv(1, j, 1, d_arn.p);  // W = A Q_j
+      gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
       for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(kb_dots, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j, m + 1, partT.p);
+        hipLaunchKernelGGL(kb_dots, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1, partT.p);
         const int nthr = nsys * (j + 1) * NB;
         hipLaunchKernelGGL(kb_dots_final, dim3((nthr + 255) / 256), dim3(256), 0, st, dsys.p, nsys, j, m, m + 1,
                            partT.p, pass == 0 ? 1 : 0);
-        hipLaunchKernelGGL(kb_sub, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j, m, partT2.p);
+        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p);
       }
       hipLaunchKernelGGL(kb_givens, dim3(gs), dim3(64), 0, st, dsys.p, nsys, j, m, partT2.p, tol);
-      hipLaunchKernelGGL(kb_scale_next, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, j);
+      hipLaunchKernelGGL(kb_scale_next, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j);
       steps = j + 1;
-      if (steps % 8 == 0 && steps < steps_max) {  // stop the cycle once every live column is resolved
+      if (steps % 8 == 0 && steps < steps_max) {  // drop systems whose columns are all resolved
         ANDR_HIP(ctx, hipGetLastError());
-        int e = read_small();
-        if (e) return e;
-        bool live = false;
-        for (int s = 0; s < nsys && !live; ++s)
-          for (int k = 0; k < NB; ++k) {
-            const double *sm = hsmall.data() + small_per * s + k * 8;
-            if (sm[4] > 0.0 && sm[5] < 0.0) live = true;
-          }
-        if (!live) break;
+        if ((rc = read_small())) return rc;
+        std::vector<int> still;
+        for (int s : cyc_list) {
+          if (!live.ntiles[s]) continue;
+          bool l = false;
+          for (int k = 0; k < NB; ++k)
+            if (sm_of(s, k)[4] > 0.0 && sm_of(s, k)[5] < 0.0) l = true;
+          if (l) still.push_back(s);
+        }
+        if (still.empty()) break;
+        int nlive = 0;
+        for (int s = 0; s < nsys; ++s) nlive += live.ntiles[s] ? 1 : 0;
+        if ((int)still.size() < nlive) {
+          if ((rc = build(still, live))) return rc;
+          if ((rc = use_tiles(live))) return rc;
+        }
       }
     }
     total += steps;
     hipLaunchKernelGGL(kb_solve_y, dim3(gs), dim3(64), 0, st, dsys.p, nsys, steps, m);
-    hipLaunchKernelGGL(kb_xupdate, dim3(nt), dim3(256), 0, st, dsys.p, dtm.p, steps, m);
+    hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, m);
     ANDR_HIP(ctx, hipGetLastError());
-    int e = read_small();
-    if (e) return e;
-    for (int s = 0; s < nsys; ++s)
+    if ((rc = read_small())) return rc;
+    for (int s : cyc_list)
       for (int k = 0; k < NB; ++k) {
-        const double *sm = hsmall.data() + small_per * s + k * 8;
+        const double *sm = sm_of(s, k);
         if (sm[4] > 0.0) it_count[(size_t)s * NB + k] += sm[5] > 0.0 ? (int)sm[5] : steps;
       }
+    if ((rc = use_tiles(cyc))) return rc;
   }
   if (iters)
     for (int s = 0; s < nsys; ++s)

@@ -145,16 +145,78 @@ def random_superellipsoid(rng: np.random.Generator, nu: int) -> tuple[Mesh, dict
     return m, {"axes": axes.tolist(), "p": p, "nu": nu, "kind": "superellipsoid"}
 
 
-def random_fibre(rng: np.random.Generator, nu: int) -> tuple[Mesh, dict]:
-    """Fibre-like body: superellipsoid with aspect ratio U[10, 50] along a random axis order,
-    exponent U[2, 8], cross-section radius 0.1 (so the half length is 1 to 5)."""
+def fibre(half_length: float, radius: float, p: float, nelem_target: int, axis: int = 0) -> Mesh:
+    """Slender superellipsoid |s/L|^p + |y/r|^p + |z/r|^p = 1 (L = half_length, r = radius) with
+    near-isotropic triangles: latitude rings spaced uniformly in arc length along the meridian,
+    n_phi nodes per ring chosen so that the ring spacing matches the circumferential spacing, and
+    triangle fans at the two tips. The long axis is `axis` (0, 1 or 2)."""
+    L, r = float(half_length), float(radius)
+    # meridian of the body in the (s, rho) half plane, sampled finely, then re-sampled by arc length
+    t = np.linspace(0.0, np.pi, 4001)
+    ct, st = np.cos(t), np.sin(t)
+    # superellipse |s/L|^p + |rho/r|^p = 1 along the ray of angle t in the (s/L, rho/r) plane
+    k = (np.abs(ct) ** p + np.abs(st) ** p) ** (-1.0 / p)
+    s_m, rho_m = L * ct * k, r * st * k
+    seg = np.hypot(np.diff(s_m), np.diff(rho_m))
+    arc = np.concatenate([[0.0], np.cumsum(seg)])
+    total = arc[-1]
+    circ = 2.0 * np.pi * r
+    # F ~ 2 n_phi n_ring with n_ring / n_phi ~ total / (circ / ... ) for isotropic cells
+    ratio = total / circ
+    n_phi = max(4, int(round(math.sqrt(nelem_target / (2.0 * ratio)))))
+    n_ring = max(2, int(round(nelem_target / (2.0 * n_phi))))
+    a = np.linspace(0.0, total, n_ring + 2)[1:-1]  # interior rings
+    s_r = np.interp(a, arc, s_m)
+    rho_r = np.interp(a, arc, rho_m)
+    phi = 2.0 * np.pi * np.arange(n_phi) / n_phi
+    pts = [np.array([L, 0.0, 0.0])]
+    for i in range(n_ring):
+        c, sn = np.cos(phi), np.sin(phi)
+        # superellipse cross-section |y/R|^p + |z/R|^p = 1 scaled to the ring radius
+        kk = (np.abs(c) ** p + np.abs(sn) ** p) ** (-1.0 / p)
+        ring = np.stack([np.full(n_phi, s_r[i]), rho_r[i] * c * kk, rho_r[i] * sn * kk], 1)
+        pts.extend(ring)
+    pts.append(np.array([-L, 0.0, 0.0]))
+    xyz = np.array(pts)
+    tris = []
+    top, bot = 0, len(pts) - 1
+
+    def rid(i, j):
+        return 1 + i * n_phi + (j % n_phi)
+
+    for j in range(n_phi):
+        tris.append((top, rid(0, j), rid(0, j + 1)))
+    for i in range(n_ring - 1):
+        for j in range(n_phi):  # each ring-to-ring quad split along a diagonal
+            tris.append((rid(i, j), rid(i + 1, j), rid(i + 1, j + 1)))
+            tris.append((rid(i, j), rid(i + 1, j + 1), rid(i, j + 1)))
+    for j in range(n_phi):
+        tris.append((bot, rid(n_ring - 1, j + 1), rid(n_ring - 1, j)))
+    con = np.array(tris, dtype=np.int32)
+    # the connectivity above is consistently oriented; a coarse ring (few nodes) makes the
+    # polyhedron locally non-convex, so orient globally by the sign of the enclosed volume
+    x1, x2, x3 = xyz[con[:, 0]], xyz[con[:, 1]], xyz[con[:, 2]]
+    vol6 = np.einsum("ij,ij->", np.cross(x2 - x1, x3 - x1), x1)
+    if vol6 < 0:
+        con = con[:, [0, 2, 1]]
+    perm = [0, 1, 2]
+    perm[0], perm[axis] = perm[axis], perm[0]
+    xyz = xyz[:, perm]
+    if axis != 0:  # an odd permutation of the axes flips the orientation
+        con = con[:, [0, 2, 1]]
+    return Mesh(np.ascontiguousarray(xyz), con, np.ones(con.shape[0], dtype=np.int32),
+                f"fibre_L{L:.3f}_r{r:.3f}_p{p:.2f}_F{con.shape[0]}")
+
+
+def random_fibre(rng: np.random.Generator, nelem_target: int) -> tuple[Mesh, dict]:
+    """Fibre-like body: slender superellipsoid of aspect ratio U[10, 50], exponent U[2, 8],
+    cross-section radius 0.1 (half length 1 to 5), long axis drawn at random, meshed with
+    near-isotropic triangles (see `fibre`)."""
     ar = float(rng.uniform(10.0, 50.0))
     p = float(rng.uniform(2.0, 8.0))
-    r = 0.1
-    axes = np.array([ar * r, r, r])
-    axes = axes[rng.permutation(3)]
-    m = superellipsoid(axes, p, nu)
-    return m, {"axes": axes.tolist(), "p": p, "nu": nu, "kind": "fibre", "aspect": ar}
+    axis = int(rng.integers(0, 3))
+    m = fibre(ar * 0.1, 0.1, p, nelem_target, axis)
+    return m, {"aspect": ar, "p": p, "axis": axis, "kind": "fibre", "nelem": m.nelem}
 
 
 def batch_bodies(n_super: int = 512, n_fibre: int = 64, seed0: int = 0):
@@ -169,7 +231,7 @@ def batch_bodies(n_super: int = 512, n_fibre: int = 64, seed0: int = 0):
         if k < n_super:
             m, info = random_superellipsoid(rng, nu)
         else:
-            m, info = random_fibre(rng, nu)
+            m, info = random_fibre(rng, 20 * nu * nu)
         info["seed"] = seed0 + k
         bodies.append((m, info))
     return bodies

@@ -45,7 +45,7 @@ def test_geodesic_sphere(nu):
     x1, x2, x3 = (m.xyz[m.con[:, k]] for k in range(3))
     n = np.cross(x2 - x1, x3 - x1)
     assert np.abs(n.sum(0)).max() < 1e-12               # closed surface
-    assert (np.einsum("ij,ij->i", n, x1 + x2 + x3) > 0).all()  # outward
+    assert (np.einsum("ij,ij->i", n, x1 + x2 + x3) > 0).all()  # outward (convex body)
     assert np.allclose(np.linalg.norm(m.xyz, axis=1), 1.0)
     # every edge shared by exactly two triangles
     e = np.sort(np.concatenate([m.con[:, [0, 1]], m.con[:, [1, 2]], m.con[:, [2, 0]]]), axis=1)
