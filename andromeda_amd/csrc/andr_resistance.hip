@@ -78,7 +78,8 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
   // memory plan: matrix + vectors (+ GMRES basis) per body, waves within a budget
   size_t free_b = 0, total_b = 0;
   ANDR_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-  const int restart = 48;
+  // long cycles: slender bodies need 150-400 Krylov steps and short restarts stagnate on them
+  const int restart = 128;
   const size_t budget = (size_t)(0.80 * (double)free_b);
   auto body_bytes = [&](int b) -> size_t {
     const size_t n = 3 * (size_t)meshes[b].nelem;

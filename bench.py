@@ -116,6 +116,9 @@ def main():
         results = ctx.resistance_batch(bodies, precision=prec)
         log(f"[bench] step {s}: {time.perf_counter() - ts:.2f} s")
     elapsed = time.perf_counter() - t0
+    worst = sorted(range(len(results)), key=lambda i: -int(results[i].nit.max()))[:4]
+    log("[bench] most iterations: " + ", ".join(f"body {i} F={bodies[i].nelem} its={int(results[i].nit.max())}"
+                                                for i in worst))
     prof = ctx.profile_read()
     ctx.profile(False)
     value = args.steps * len(bodies) / elapsed

@@ -64,8 +64,25 @@ def test_superellipsoid_on_surface():
 def test_batch_sizes():
     bodies = M.batch_bodies(8, 2)
     for mesh, info in bodies:
-        assert 500 <= mesh.nelem <= 8000
-        assert mesh.nelem == 20 * info["nu"] ** 2
+        assert 450 <= mesh.nelem <= 8100
+        if info["kind"] == "superellipsoid":
+            assert mesh.nelem == 20 * info["nu"] ** 2
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fibre_mesh(axis):
+    m = M.fibre(3.0, 0.1, 4.0, 2000, axis)
+    x1, x2, x3 = (m.xyz[m.con[:, k]] for k in range(3))
+    n = np.cross(x2 - x1, x3 - x1)
+    assert np.abs(n.sum(0)).max() < 1e-12
+    assert np.einsum("ij,ij->", n, x1) > 0  # outward: positive enclosed volume
+    d = np.concatenate([m.con[:, [0, 1]], m.con[:, [1, 2]], m.con[:, [2, 0]]])
+    assert len(set(map(tuple, d))) == len(d)  # consistently oriented 2-manifold
+    area = 0.5 * np.linalg.norm(n, axis=1)
+    longest = np.max([np.linalg.norm(x2 - x1, axis=1), np.linalg.norm(x3 - x2, axis=1),
+                      np.linalg.norm(x1 - x3, axis=1)], axis=0)
+    assert np.median(longest ** 2 / (2 * area)) < 3.0  # near-isotropic elements
+    assert np.abs(m.xyz[:, axis]).max() > 2.9
 
 
 def test_gmsh_reader_reference_mesh():
