@@ -90,7 +90,8 @@ def load_library(path: str = LIB_PATH):
                                    _F, _F]
     lib.andr_mlp_infer.argtypes = [C.c_void_p, C.c_int64, _F, _F, _F]
     lib.andr_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int64, _F, _I, _F, _F, _F]
-    lib.andr_set_quadrature_model.argtypes = [C.c_void_p, _F, C.c_float, C.c_double]
+    lib.andr_set_quadrature_model.argtypes = [C.c_void_p, _F, C.c_float, C.c_double, _I]
+    lib.andr_quad_far_calibration.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_double, C.c_double, _I]
     lib.andr_profile.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_profile_read.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _L, _D, _D, _D]
     _lib = lib
@@ -101,7 +102,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
-            "andr_profile_read"]
+            "andr_profile_read", "andr_quad_far_calibration"]
+FAR_BINS = 16
 
 
 def _p(a):
@@ -298,10 +300,20 @@ class Context:
                                                 _pf(loss), _pf(grad)), "andr_mlp_loss_grad")
         return float(loss[0]), grad
 
-    def set_quadrature_model(self, params, confidence: float = 0.9, near_ratio: float = 4.0):
+    def set_quadrature_model(self, params, confidence: float, near_ratio: float, far_table):
         params = np.ascontiguousarray(params, dtype=np.float32)
-        self._check(self.lib.andr_set_quadrature_model(self.h, _pf(params), confidence, near_ratio),
+        far = np.ascontiguousarray(far_table, dtype=np.int32)
+        if far.size != FAR_BINS:
+            raise ValueError(f"far_table needs {FAR_BINS} entries")
+        self._check(self.lib.andr_set_quadrature_model(self.h, _pf(params), confidence, near_ratio, _pi(far)),
                     "andr_set_quadrature_model")
+
+    def quad_far_calibration(self, mesh, near_ratio: float, rel_tol: float, itri: int = 5):
+        mv = _MeshView(mesh, None, itri)
+        far = np.zeros(FAR_BINS, dtype=np.int32)
+        self._check(self.lib.andr_quad_far_calibration(self.h, C.byref(mv.s), near_ratio, rel_tol, _pi(far)),
+                    "andr_quad_far_calibration")
+        return far
 
 
 # ---- Fortran-style LSQR (lsqr_precon_ / lsqr_solve_), 1-based CSR --------------------------------

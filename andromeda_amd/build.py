@@ -33,6 +33,39 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "resistance_host")
+
+
+def amdflang() -> str | None:
+    for p in ("/opt/rocm/bin/amdflang", "/opt/rocm/llvm/bin/amdflang"):
+        if os.path.exists(p):
+            return p
+    return None
+
+
+def build_fortran(verbose: bool = True) -> str | None:
+    """The ISO_C_BINDING module and the minimal Fortran host, linked against the library."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran host not built")
+        return None
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    iface = os.path.join(mod, "andromeda_hip_iface.o")
+    steps = [[fc, "-O2", "-J", mod, "-c", os.path.join(FORTRAN_DIR, "andromeda_hip_iface.f90"), "-o", iface],
+             [fc, "-O2", "-J", mod, os.path.join(FORTRAN_DIR, "resistance_host.f90"), iface, "-L" + HERE,
+              "-landromeda_hip", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_HOST]]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{r.stderr[-6000:]}")
+    if verbose:
+        print(f"built {FORTRAN_HOST}")
+    return FORTRAN_HOST
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 6)) as ex:
@@ -45,6 +78,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_fortran(verbose)
     return OUT
 
 

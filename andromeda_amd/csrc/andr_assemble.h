@@ -5,18 +5,9 @@
 // right-hand sides carried per body (6 rigid-body problems, padded to 8)
 #define ANDR_NRHS 8
 
-template <class ST>
-struct SysJobT {
-  BodyGeom g;
-  ST *A;                  // 3ne x lda, blocked rows (c ne + r) and columns (en ne + e)
-  int64_t lda;
-  double *B;              // 3ne x ANDR_NRHS
-  const double *U;        // nn x 3 x ANDR_NRHS prescribed velocities
-  double mu, extra;
-  int qoff, nq;
-  double *err_rows;       // ne (may be null)
-};
-
+// One body of a batch: geometry, output system A (3ne x lda, blocked rows c ne + r and columns
+// en ne + e, element type float or double), right-hand sides B (3ne x ANDR_NRHS) and the
+// prescribed nodal velocities U (nn x 3 x ANDR_NRHS).
 struct SysJob {
   BodyGeom g;
   void *A_;
@@ -24,8 +15,10 @@ struct SysJob {
   double *B;
   const double *U;
   double mu, extra;
-  int qoff, nq;
-  double *err_rows;
+  int qoff, nq;        // the fixed-order rule
+  double *err_rows;    // ne (may be null)
+  uint8_t *cls;        // ne x ne rule class per pair (learned form), null: fixed-order rule
+  const double *defl;  // per wall id (1..nwalls): scale of the rank-one null-space term, null: none
 #ifdef __HIPCC__
   template <class ST>
   __device__ ST *Aptr() const { return (ST *)A_; }
@@ -34,6 +27,8 @@ struct SysJob {
 
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
                               bool fp32_store, double alg_bytes, double alg_flops);
-// algorithmic work of a Stokes pair evaluation: G + T kernels, 107 flops per quadrature point
+// algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0
+// Dunavant rules (classes 0..6 of the learned form) in the quadrature table
+#define ANDR_DUN_OFF0 297

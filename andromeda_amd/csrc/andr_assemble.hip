@@ -13,14 +13,13 @@
 #include "andr_dunavant_rules.h"
 #include "andr_assemble.h"
 #include "andr_prof.h"
+#include "andr_pairs.h"
 
 #include <algorithm>
 #include <cstdarg>
 #include <chrono>
 #include <cstring>
 
-__constant__ double c_qd[ANDR_QTAB_N][4];
-__constant__ float c_qf[ANDR_QTAB_N][4];
 
 // rule id -> (offset, npts): 1..5 Wandzura-Xiao (ITRI), 6..12 Dunavant 3..19 points
 static const int k_rule_off[14] = {0, 0, 7, 32, 86, 171, 297, 300, 304, 310, 317, 329, 342, 361};
@@ -54,9 +53,9 @@ int andr_upload_rules(andr_ctx *ctx) {
     for (int j = 0; j < 4; ++j) tab.push_back(andr_dun_points[i][j]);
   if (tab.size() != 4 * ANDR_QTAB_N) return andr_fail(ctx, ANDR_ERR_STATE, "quadrature table size %zu", tab.size());
   std::vector<float> tf(tab.begin(), tab.end());
-  ANDR_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_qd), tab.data(), sizeof(double) * tab.size()));
-  ANDR_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_qf), tf.data(), sizeof(float) * tf.size()));
-  return ANDR_OK;
+  int rc = andr_upload_rules_here(ctx, tab.data(), tf.data());
+  if (rc) return rc;
+  return andr_upload_rules_learned(ctx, tab.data(), tf.data());
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -519,60 +518,6 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
 // Only centroid rows (the equations of an all-Dirichlet problem) are formed; the T matrix is never
 // stored. One workgroup per (job, row); a job is one body of a batch.
 // ------------------------------------------------------------------------------------------------
-// index of component (i, j) in the symmetric storage {xx, xy, xz, yy, yz, zz}
-__host__ __device__ constexpr int sym3(int i, int j) {
-  return i > j ? sym3(j, i) : (i == 0 ? j : (i == 1 ? 2 + j : 5));
-}
-
-template <class CT>
-__device__ inline void stokes_pair_regular(const double s[3], const double *x1, const double *x2, const double *x3,
-                                           double nx, double ny, double nz, double area, int qoff, int nq,
-                                           double Gs[6], double Tv[6][3]) {
-  // relative geometry formed in FP64, quadrature in CT
-  const CT d0 = (CT)(x3[0] - s[0]), d1 = (CT)(x3[1] - s[1]), d2 = (CT)(x3[2] - s[2]);
-  const CT a0 = (CT)(x1[0] - x3[0]), a1 = (CT)(x1[1] - x3[1]), a2 = (CT)(x1[2] - x3[2]);
-  const CT b0 = (CT)(x2[0] - x3[0]), b1 = (CT)(x2[1] - x3[1]), b2 = (CT)(x2[2] - x3[2]);
-  const CT n0 = (CT)nx, n1 = (CT)ny, n2 = (CT)nz;
-  CT g[6] = {0, 0, 0, 0, 0, 0};
-  CT t[6][3] = {{0}};
-  for (int q = 0; q < nq; ++q) {
-    CT l1, l2, l3, w;
-    if constexpr (sizeof(CT) == 8) {
-      l1 = c_qd[qoff + q][0]; l2 = c_qd[qoff + q][1]; l3 = c_qd[qoff + q][2]; w = c_qd[qoff + q][3];
-    } else {
-      l1 = c_qf[qoff + q][0]; l2 = c_qf[qoff + q][1]; l3 = c_qf[qoff + q][2]; w = c_qf[qoff + q][3];
-    }
-    const CT rx = d0 + l1 * a0 + l2 * b0;
-    const CT ry = d1 + l1 * a1 + l2 * b1;
-    const CT rz = d2 + l1 * a2 + l2 * b2;
-    const CT rxx = rx * rx, rxy = rx * ry, rxz = rx * rz, ryy = ry * ry, ryz = ry * rz, rzz = rz * rz;
-    CT ir;
-    if constexpr (sizeof(CT) == 8) ir = rsqrt(rxx + ryy + rzz);
-    else ir = __frsqrt_rn(rxx + ryy + rzz);
-    const CT ir2 = ir * ir, ir3 = ir2 * ir;
-    const CT wir = w * ir, wir3 = w * ir3;
-    g[0] += wir + rxx * wir3;
-    g[1] += rxy * wir3;
-    g[2] += rxz * wir3;
-    g[3] += wir + ryy * wir3;
-    g[4] += ryz * wir3;
-    g[5] += wir + rzz * wir3;
-    const CT ct = (CT)(-6.0) * ir2 * ir3 * (rx * n0 + ry * n1 + rz * n2);
-    const CT wc[3] = {w * l1 * ct, w * l2 * ct, w * l3 * ct};
-    const CT rr[6] = {rxx, rxy, rxz, ryy, ryz, rzz};
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-#pragma unroll
-      for (int v = 0; v < 3; ++v) t[k][v] += wc[v] * rr[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    Gs[k] = (double)g[k] * area;
-#pragma unroll
-    for (int v = 0; v < 3; ++v) Tv[k][v] = (double)t[k][v] * area;
-  }
-}
-
 template <class CT, class ST>
 __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict__ jobs,
                                                        const int2 *__restrict__ rowmap) {
@@ -603,15 +548,30 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
 #pragma unroll
         for (int v = 0; v < 3; ++v) Tv[k][v] = 0.0;
     } else {
+      int qoff = J.qoff, nq = J.nq;
+      if (J.cls) {  // learned form: Dunavant class 0..6 or the fixed-order rule (class 7)
+        const int c = J.cls[(size_t)r * ne + e];
+        if (c < 7) {
+          constexpr int dn[7] = {3, 4, 6, 7, 12, 13, 19};
+          constexpr int doff[7] = {0, 3, 7, 13, 20, 32, 45};
+          qoff = ANDR_DUN_OFF0 + doff[c];
+          nq = dn[c];
+        }
+      }
       stokes_pair_regular<CT>(s, x1, x2, x3, m * g.nrm[3 * e], m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2],
-                              g.area[e], J.qoff, J.nq, Gs, Tv);
+                              g.area[e], qoff, nq, Gs, Tv);
     }
     const double sc = -m * inv_mu;
+    // rank-one null-space term d_w A_r n_r n_e^T between elements of one closed wall
+    double dfl = 0.0;
+    if (J.defl && g.bcid[e] == g.bcid[r]) dfl = J.defl[g.bcid[r]] * g.area[r];
+    const double ne3[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
+    const double nr3[3] = {dfl * g.nrm[3 * r], dfl * g.nrm[3 * r + 1], dfl * g.nrm[3 * r + 2]};
     ST *ar[3] = {arow0, arow1, arow2};
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int en = 0; en < 3; ++en) ar[c][(size_t)en * ne + e] = (ST)(sc * Gs[sym3(c, en)]);
+      for (int en = 0; en < 3; ++en) ar[c][(size_t)en * ne + e] = (ST)(sc * Gs[sym3(c, en)] + nr3[c] * ne3[en]);
     // T . U for the rigid-body right-hand sides
 #pragma unroll
     for (int v = 0; v < 3; ++v) {

@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -487,6 +489,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   if ((rc = use_tiles(cyc))) return rc;
   int total = 0;
   bool first = true;
+  const bool debug = getenv("ANDR_DEBUG_GMRES") != nullptr;
   while (true) {
     // residual r = b - A x of every system of the last cycle (x = 0 on the first), FP64 accumulation
     if (cyc.n_t) hipLaunchKernelGGL(kb_copy_b, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
@@ -497,6 +500,15 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ANDR_HIP(ctx, hipGetLastError());
     if ((rc = read_small())) return rc;
     first = false;
+    if (debug) {  // per-cycle true residuals of the first systems (ANDR_DEBUG_GMRES=1)
+      for (int s = 0; s < std::min(nsys, 4); ++s) {
+        fprintf(stderr, "[gmres] it %d sys %d n %d:", total, s, in[s].n);
+        for (int k = 0; k < nrhs; ++k)
+          fprintf(stderr, " %.2e%s", sm_of(s, k)[1] / std::max(sm_of(s, k)[0], 1e-300),
+                  sm_of(s, k)[4] > 0.0 ? "" : (sm_of(s, k)[7] > 0.0 ? "s" : "c"));
+        fprintf(stderr, "\n");
+      }
+    }
     // systems with an unconverged column form the next cycle
     cyc_list.clear();
     for (int s = 0; s < nsys; ++s)

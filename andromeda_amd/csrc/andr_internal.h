@@ -62,10 +62,11 @@ struct DevBuf {
 struct QuadRule {
   int offset, npts;
 };
-extern __constant__ double c_qd[ANDR_QTAB_N][4];  // l1, l2, l3, w
-extern __constant__ float c_qf[ANDR_QTAB_N][4];
+// Every translation unit that integrates keeps its own constant-memory copy of the table
+// (andr_pairs.h) and an upload function; andr_upload_rules fills them all.
 QuadRule andr_rule(int id);  // host side lookup
 int andr_upload_rules(andr_ctx *ctx);
+int andr_upload_rules_learned(andr_ctx *ctx, const double *tab, const float *tabf);
 
 // Per-body device geometry -------------------------------------------------------------------
 struct BodyGeom {

@@ -1,5 +1,5 @@
 // Learned Gauss-order selection for near-field pairs (mixed path): device network, training
-// kernels and the assembly variant that consults it.
+// kernels and the classification pass that feeds the assembly.
 #pragma once
 #include "andr_internal.h"
 #include "andr_assemble.h"
@@ -10,5 +10,7 @@ struct LearnedStats {
   int64_t fallback;  // near-field pairs sent to the fixed-order rule
 };
 
-int andr_launch_stokes_system_learned(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows,
-                                      LearnedStats *stats);
+// classifies every pair of the listed rows (writes SysJob::cls), then assembles with those rules;
+// stats[njobs] (may be null) gets the counts of each job
+int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJob *d_jobs, const int2 *d_rowmap,
+                                      int nrows, int njobs, LearnedStats *stats);

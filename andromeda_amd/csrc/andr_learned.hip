@@ -1,24 +1,671 @@
-// Learned quadrature: placeholder until the network kernels land.
+// Learned Gauss-order selection (no counterpart in the reference, which integrates every regular
+// pair with one fixed rule, src/lib_Stokes.f90:2341-2446).
+//
+// A small network (6 -> 64 -> 64 -> 64 -> 8, ReLU) predicts, for a near-field (collocation point,
+// element) pair, the cheapest triangle rule that reproduces the fixed-order integral: classes
+// 0..6 are the Dunavant rules with 3, 4, 6, 7, 12, 13 and 19 points, class 7 the mesh's own
+// fixed-order rule. Far pairs take a rule from a distance table calibrated on the same labels.
+// Labels are produced here from FP64 integrals (the fixed-order rule is the judge), the network
+// is trained here (forward, backward and Adam kernels), and inference runs in a classification
+// pass before the assembly with BF16 weights in LDS and FP32 accumulation.
 #include "andr_learned.h"
+#include "andr_pairs.h"
+#include "andr_prof.h"
+#include "andr_singular.h"
 
-int andr_launch_stokes_system_learned(andr_ctx *ctx, const SysJob *, const int2 *, int, LearnedStats *) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "learned quadrature not built");
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+namespace {
+
+constexpr int NIN = 6, NH = 64, NOUT = 8;
+constexpr int P_W1 = 0, P_B1 = P_W1 + NIN * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH, P_W3 = P_B2 + NH,
+              P_B3 = P_W3 + NH * NH, P_W4 = P_B3 + NH, P_B4 = P_W4 + NH * NOUT, NPARAM = P_B4 + NOUT;
+static_assert(NPARAM == ANDR_MLP_NPARAM, "parameter layout");
+constexpr int NCLS_DUN = 7;   // Dunavant classes; class 7 = fixed-order rule
+constexpr int DUN_ID0 = 6;    // rule id of the first Dunavant rule in the quadrature table
+
+// ---- pair geometry and features ---------------------------------------------------------------
+struct PairGeo {
+  double dh;      // distance from the collocation point to the element centroid / longest edge
+  double aspect;  // longest edge^2 / (2 area)
+  double cosn;    // cosine of the angle between the two element normals
+};
+
+__device__ inline PairGeo pair_geo(const double s[3], const double ns[3], const double *x1, const double *x2,
+                                   const double *x3, const double ne[3], double area) {
+  PairGeo p;
+  const double c[3] = {(x1[0] + x2[0] + x3[0]) / 3.0, (x1[1] + x2[1] + x3[1]) / 3.0, (x1[2] + x2[2] + x3[2]) / 3.0};
+  const double d = sqrt((s[0] - c[0]) * (s[0] - c[0]) + (s[1] - c[1]) * (s[1] - c[1]) + (s[2] - c[2]) * (s[2] - c[2]));
+  auto l2 = [](const double *a, const double *b) {
+    return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+  };
+  const double L2 = fmax(l2(x1, x2), fmax(l2(x2, x3), l2(x3, x1)));
+  p.dh = d / sqrt(L2);
+  p.aspect = L2 / (2.0 * area);
+  p.cosn = ns[0] * ne[0] + ns[1] * ne[1] + ns[2] * ne[2];
+  return p;
 }
-extern "C" int andr_quad_training_pairs(andr_ctx *ctx, const andr_mesh *, double, double, int64_t, float *, int32_t *,
-                                        int64_t *) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "not built");
+
+__device__ inline void features(const PairGeo &p, int kernel, float f[NIN]) {
+  f[0] = (float)log(p.dh);
+  f[1] = (float)log(p.aspect);
+  f[2] = (float)p.cosn;
+  f[3] = kernel == 0 ? 1.f : 0.f;
+  f[4] = kernel == 1 ? 1.f : 0.f;
+  f[5] = kernel == 2 ? 1.f : 0.f;
 }
-extern "C" int andr_mlp_train(andr_ctx *ctx, int64_t, const float *, const int32_t *, int32_t, int32_t, float, uint64_t,
-                              float *, float *) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "not built");
+
+__device__ inline int far_bin(double dh, double near_ratio) {
+  int b = (int)floor(2.0 * log2(dh / near_ratio));
+  return b < 0 ? 0 : (b > ANDR_FAR_BINS - 1 ? ANDR_FAR_BINS - 1 : b);
 }
-extern "C" int andr_mlp_infer(andr_ctx *ctx, int64_t, const float *, const float *, float *) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "not built");
+
+// Laplace pair in FP64: out = {G, H1, H2, H3} (BemKernel, r = s - f)
+__device__ inline void lap_pair(const double s[3], const double *x1, const double *x2, const double *x3,
+                                const double n[3], double area, int qoff, int nq, double out[4]) {
+  double g = 0, h1 = 0, h2 = 0, h3 = 0;
+  for (int q = 0; q < nq; ++q) {
+    const double l1 = c_qd[qoff + q][0], l2 = c_qd[qoff + q][1], l3 = c_qd[qoff + q][2], w = c_qd[qoff + q][3];
+    const double rx = s[0] - (l1 * x1[0] + l2 * x2[0] + l3 * x3[0]);
+    const double ry = s[1] - (l1 * x1[1] + l2 * x2[1] + l3 * x3[1]);
+    const double rz = s[2] - (l1 * x1[2] + l2 * x2[2] + l3 * x3[2]);
+    const double ira = rsqrt(rx * rx + ry * ry + rz * rz);
+    const double fg = (0.25 / ANDR_PI) * ira;
+    const double fh = (rx * n[0] + ry * n[1] + rz * n[2]) * ira * ira * fg;
+    g += w * fg;
+    h1 += w * fh * l1;
+    h2 += w * fh * l2;
+    h3 += w * fh * l3;
+  }
+  out[0] = g * area;
+  out[1] = h1 * area;
+  out[2] = h2 * area;
+  out[3] = h3 * area;
 }
-extern "C" int andr_mlp_loss_grad(andr_ctx *ctx, int64_t, const float *, const int32_t *, const float *, float *,
-                                  float *) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "not built");
+
+// values of one kernel family with one rule: Laplace 4, Stokes-G 6, Stokes-H 18
+__device__ inline int kernel_values(int kernel, const double s[3], const double *x1, const double *x2,
+                                    const double *x3, const double n[3], double area, int qoff, int nq,
+                                    double v[18]) {
+  if (kernel == 0) {
+    lap_pair(s, x1, x2, x3, n, area, qoff, nq, v);
+    return 4;
+  }
+  double G[6], T[6][3];
+  stokes_pair_regular<double>(s, x1, x2, x3, n[0], n[1], n[2], area, qoff, nq, G, T);
+  if (kernel == 1) {
+    for (int i = 0; i < 6; ++i) v[i] = G[i];
+    return 6;
+  }
+  for (int k = 0; k < 6; ++k)
+    for (int t = 0; t < 3; ++t) v[3 * k + t] = T[k][t];
+  return 18;
 }
-extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *, float, double) {
-  return andr_fail(ctx, ANDR_ERR_STATE, "not built");
+
+// smallest Dunavant class whose values agree with the fixed-order values to rel_tol, else 7
+__device__ inline int pair_label(int kernel, const double s[3], const double *x1, const double *x2, const double *x3,
+                                 const double n[3], double area, int qoff_ref, int nq_ref, const int *dun_off,
+                                 const int *dun_n, double rel_tol) {
+  double ref[18], v[18];
+  const int nv = kernel_values(kernel, s, x1, x2, x3, n, area, qoff_ref, nq_ref, ref);
+  double scale = 0.0;
+  for (int i = 0; i < nv; ++i) scale = fmax(scale, fabs(ref[i]));
+  for (int c = 0; c < NCLS_DUN; ++c) {
+    kernel_values(kernel, s, x1, x2, x3, n, area, dun_off[c], dun_n[c], v);
+    double err = 0.0;
+    for (int i = 0; i < nv; ++i) err = fmax(err, fabs(v[i] - ref[i]));
+    if (err <= rel_tol * scale) return c;
+  }
+  return NCLS_DUN;
+}
+
+struct RuleTab {
+  int off[8], n[8];
+};
+
+// one block per centroid row: near pairs -> (features, label) records; far pairs -> bin maxima
+__global__ __launch_bounds__(256) void k_pair_labels(BodyGeom g, int qoff_ref, int nq_ref, RuleTab rt,
+                                                     double near_ratio, double rel_tol, int mode, int64_t max_pairs,
+                                                     float *__restrict__ feats, int *__restrict__ labels,
+                                                     unsigned long long *__restrict__ counter,
+                                                     int *__restrict__ far_max) {
+  const int r = blockIdx.x;
+  const int ne = g.ne;
+  const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
+  const double ns[3] = {g.nrm[3 * r], g.nrm[3 * r + 1], g.nrm[3 * r + 2]};
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    if (e == r) continue;
+    const double *x1 = g.xyz + 3 * g.con[3 * e], *x2 = g.xyz + 3 * g.con[3 * e + 1], *x3 = g.xyz + 3 * g.con[3 * e + 2];
+    const double n[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
+    const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g.area[e]);
+    if (mode == 0 && pg.dh < near_ratio) {
+      for (int kernel = 0; kernel < 3; ++kernel) {
+        const int lab = pair_label(kernel, s, x1, x2, x3, n, g.area[e], qoff_ref, nq_ref, rt.off, rt.n, rel_tol);
+        const unsigned long long idx = atomicAdd(counter, 1ull);
+        if ((int64_t)idx < max_pairs) {
+          float f[NIN];
+          features(pg, kernel, f);
+          for (int i = 0; i < NIN; ++i) feats[idx * NIN + i] = f[i];
+          labels[idx] = lab;
+        }
+      }
+    } else if (mode == 1 && pg.dh >= near_ratio && pg.dh < near_ratio * 256.0) {
+      const int b = far_bin(pg.dh, near_ratio);
+      int lab = pair_label(1, s, x1, x2, x3, n, g.area[e], qoff_ref, nq_ref, rt.off, rt.n, rel_tol);
+      lab = max(lab, pair_label(2, s, x1, x2, x3, n, g.area[e], qoff_ref, nq_ref, rt.off, rt.n, rel_tol));
+      atomicMax(far_max + b, lab);
+    }
+  }
+}
+
+// ---- generic small FP32 GEMM: C = alpha op(A) op(B) + beta C (row-major) -------------------------
+__global__ void k_sgemm(int M, int N, int K, const float *__restrict__ A, int lda, int ta, const float *__restrict__ B,
+                        int ldb, int tb, float *__restrict__ C, int ldc, float alpha, float beta) {
+  __shared__ float As[16][17], Bs[16][17];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  float acc = 0.f;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int ka = k0 + tx, kb = k0 + ty;
+    As[ty][tx] = (row < M && ka < K) ? (ta ? A[(size_t)ka * lda + row] : A[(size_t)row * lda + ka]) : 0.f;
+    Bs[ty][tx] = (kb < K && col < N) ? (tb ? B[(size_t)col * ldb + kb] : B[(size_t)kb * ldb + col]) : 0.f;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    __syncthreads();
+  }
+  if (row < M && col < N) C[(size_t)row * ldc + col] = alpha * acc + (beta != 0.f ? beta * C[(size_t)row * ldc + col] : 0.f);
+}
+
+// Z += b (broadcast over rows); H = relu(Z) (H may alias nothing)
+__global__ void k_bias_relu(int M, int N, float *__restrict__ Z, const float *__restrict__ b, float *__restrict__ H,
+                            int relu) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * N) return;
+  const float z = Z[t] + b[t % N];
+  Z[t] = z;
+  if (H) H[t] = relu ? fmaxf(z, 0.f) : z;
+}
+
+// softmax cross-entropy over NOUT logits: P, per-sample loss, dZ = (P - onehot) * scale
+__global__ void k_softmax_xent(int M, const float *__restrict__ Z, const int *__restrict__ lab, float *__restrict__ P,
+                               float *__restrict__ loss, float *__restrict__ dZ, float scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const float *z = Z + (size_t)i * NOUT;
+  float mx = z[0];
+  for (int k = 1; k < NOUT; ++k) mx = fmaxf(mx, z[k]);
+  float e[NOUT], s = 0.f;
+  for (int k = 0; k < NOUT; ++k) {
+    e[k] = expf(z[k] - mx);
+    s += e[k];
+  }
+  for (int k = 0; k < NOUT; ++k) {
+    const float p = e[k] / s;
+    if (P) P[(size_t)i * NOUT + k] = p;
+    if (dZ) dZ[(size_t)i * NOUT + k] = (p - (lab && lab[i] == k ? 1.f : 0.f)) * scale;
+  }
+  if (loss && lab) loss[i] = -(z[lab[i]] - mx - logf(s));
+}
+
+__global__ void k_relu_bwd(int n, float *__restrict__ dH, const float *__restrict__ Z) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && !(Z[t] > 0.f)) dH[t] = 0.f;
+}
+
+__global__ void k_colsum(int M, int N, const float *__restrict__ X, float *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.f;
+  for (int i = 0; i < M; ++i) s += X[(size_t)i * N + j];
+  out[j] = s;
+}
+
+__global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float gg = g[t];
+  const float mm = b1 * m[t] + (1.f - b1) * gg;
+  const float vv = b2 * v[t] + (1.f - b2) * gg * gg;
+  m[t] = mm;
+  v[t] = vv;
+  p[t] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+__global__ void k_gather(int M, const int *__restrict__ idx, const float *__restrict__ F, const int *__restrict__ L,
+                         float *__restrict__ Fo, int *__restrict__ Lo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const int j = idx[i];
+  for (int k = 0; k < NIN; ++k) Fo[(size_t)i * NIN + k] = F[(size_t)j * NIN + k];
+  Lo[i] = L[j];
+}
+
+// Buffers of one forward/backward pass on a minibatch of at most B samples.
+struct MlpWork {
+  DevBuf<float> Z1, H1, Z2, H2, Z3, H3, Z4, P, dZ4, dH, dZb, loss;
+  int B = 0;
+  int alloc(int b) {
+    B = b;
+    if (Z1.alloc((size_t)b * NH) || H1.alloc((size_t)b * NH) || Z2.alloc((size_t)b * NH) || H2.alloc((size_t)b * NH) ||
+        Z3.alloc((size_t)b * NH) || H3.alloc((size_t)b * NH) || Z4.alloc((size_t)b * NOUT) || P.alloc((size_t)b * NOUT) ||
+        dZ4.alloc((size_t)b * NOUT) || dH.alloc((size_t)b * NH) || dZb.alloc((size_t)b * NH) || loss.alloc(b))
+      return 1;
+    return 0;
+  }
+};
+
+void sgemm(hipStream_t st, int M, int N, int K, const float *A, int lda, int ta, const float *B, int ldb, int tb,
+           float *C, int ldc, float alpha = 1.f, float beta = 0.f) {
+  dim3 grid((N + 15) / 16, (M + 15) / 16);
+  hipLaunchKernelGGL(k_sgemm, grid, dim3(16, 16), 0, st, M, N, K, A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
+}
+
+// forward pass of M samples (X: M x NIN) with parameters p
+void mlp_forward(hipStream_t st, int M, const float *X, const float *p, MlpWork &w) {
+  const int g1 = (M * NH + 255) / 256, g4 = (M * NOUT + 255) / 256;
+  sgemm(st, M, NH, NIN, X, NIN, 0, p + P_W1, NH, 0, w.Z1.p, NH);
+  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z1.p, p + P_B1, w.H1.p, 1);
+  sgemm(st, M, NH, NH, w.H1.p, NH, 0, p + P_W2, NH, 0, w.Z2.p, NH);
+  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z2.p, p + P_B2, w.H2.p, 1);
+  sgemm(st, M, NH, NH, w.H2.p, NH, 0, p + P_W3, NH, 0, w.Z3.p, NH);
+  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z3.p, p + P_B3, w.H3.p, 1);
+  sgemm(st, M, NOUT, NH, w.H3.p, NH, 0, p + P_W4, NOUT, 0, w.Z4.p, NOUT);
+  hipLaunchKernelGGL(k_bias_relu, dim3(g4), dim3(256), 0, st, M, NOUT, w.Z4.p, p + P_B4, (float *)nullptr, 0);
+}
+
+// backward of the mean cross-entropy: grad (NPARAM) from a forward pass already in w
+void mlp_backward(hipStream_t st, int M, const float *X, const int *lab, const float *p, MlpWork &w, float *grad) {
+  const int gm = (M + 255) / 256, g1 = (M * NH + 255) / 256;
+  hipLaunchKernelGGL(k_softmax_xent, dim3(gm), dim3(256), 0, st, M, w.Z4.p, lab, w.P.p, w.loss.p, w.dZ4.p,
+                     1.f / (float)M);
+  // layer 4
+  sgemm(st, NH, NOUT, M, w.H3.p, NH, 1, w.dZ4.p, NOUT, 0, grad + P_W4, NOUT);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NOUT, w.dZ4.p, grad + P_B4);
+  sgemm(st, M, NH, NOUT, w.dZ4.p, NOUT, 0, p + P_W4, NOUT, 1, w.dH.p, NH);
+  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dH.p, w.Z3.p);
+  // layer 3
+  sgemm(st, NH, NH, M, w.H2.p, NH, 1, w.dH.p, NH, 0, grad + P_W3, NH);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dH.p, grad + P_B3);
+  sgemm(st, M, NH, NH, w.dH.p, NH, 0, p + P_W3, NH, 1, w.dZb.p, NH);
+  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dZb.p, w.Z2.p);
+  // layer 2
+  sgemm(st, NH, NH, M, w.H1.p, NH, 1, w.dZb.p, NH, 0, grad + P_W2, NH);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dZb.p, grad + P_B2);
+  sgemm(st, M, NH, NH, w.dZb.p, NH, 0, p + P_W2, NH, 1, w.dH.p, NH);
+  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dH.p, w.Z1.p);
+  // layer 1
+  sgemm(st, NIN, NH, M, X, NIN, 1, w.dH.p, NH, 0, grad + P_W1, NH);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dH.p, grad + P_B1);
+}
+
+// ---- inference inside the assembly: BF16 weights in LDS, FP32 accumulation ---------------------
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+struct ModelView {
+  const uint16_t *w;  // bf16 weights, NPARAM entries (biases included, but biases read from b)
+  const float *b;     // fp32 parameters (biases used)
+};
+
+// Class and confidence of one feature vector. Weights (BF16) and biases sit in LDS and are read at
+// wave-uniform addresses (broadcast); the thread's activation vector lives in its own LDS row `h`
+// (stride CLS_HSTRIDE) so the 64 accumulators of a layer stay in registers.
+constexpr int CLS_THREADS = 128, CLS_HSTRIDE = NH + 1;
+constexpr int CB1 = 0, CB2 = NH, CB3 = 2 * NH, CB4 = 3 * NH, NBIAS = 3 * NH + NOUT;  // compact biases
+
+__device__ inline int mlp_classify(const float f[NIN], const uint16_t *sw, const float *sb, float *h, float *conf) {
+  float acc[NH];
+#pragma unroll
+  for (int o = 0; o < NH; ++o) acc[o] = sb[CB1 + o];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i)
+#pragma unroll
+    for (int o = 0; o < NH; ++o) acc[o] += f[i] * bf16_to_f(sw[P_W1 + i * NH + o]);
+#pragma unroll
+  for (int o = 0; o < NH; ++o) h[o] = fmaxf(acc[o], 0.f);
+  for (int layer = 0; layer < 2; ++layer) {
+    const int pw = layer == 0 ? P_W2 : P_W3, pb = layer == 0 ? CB2 : CB3;
+#pragma unroll
+    for (int o = 0; o < NH; ++o) acc[o] = sb[pb + o];
+    for (int i = 0; i < NH; ++i) {
+      const float hi = h[i];
+#pragma unroll
+      for (int o = 0; o < NH; ++o) acc[o] += hi * bf16_to_f(sw[pw + i * NH + o]);
+    }
+#pragma unroll
+    for (int o = 0; o < NH; ++o) h[o] = fmaxf(acc[o], 0.f);
+  }
+  float z[NOUT];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) z[o] = sb[CB4 + o];
+  for (int i = 0; i < NH; ++i) {
+    const float hi = h[i];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) z[o] += hi * bf16_to_f(sw[P_W4 + i * NOUT + o]);
+  }
+  float mx = z[0];
+  int best = 0;
+#pragma unroll
+  for (int o = 1; o < NOUT; ++o)
+    if (z[o] > mx) {
+      mx = z[o];
+      best = o;
+    }
+  float s = 0.f;
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) s += expf(z[o] - mx);
+  *conf = 1.f / s;
+  return best;
+}
+
+// one block per (job, row): rule class of every pair of the row
+__global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restrict__ jobs,
+                                                          const int2 *__restrict__ rowmap,
+                                                          const uint16_t *__restrict__ wbf, const float *__restrict__ wf,
+                                                          const int *__restrict__ far_table,
+                                                          const int *__restrict__ npts_of_cls, double near_ratio,
+                                                          float min_conf, unsigned long long *__restrict__ stats) {
+  __shared__ uint16_t sw[NPARAM];
+  __shared__ float sb[NBIAS];
+  __shared__ float sh[CLS_THREADS * CLS_HSTRIDE];
+  __shared__ int sfar[ANDR_FAR_BINS];
+  __shared__ unsigned long long red[3];
+  float *h = sh + threadIdx.x * CLS_HSTRIDE;
+  for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sw[i] = wbf[i];
+  for (int i = threadIdx.x; i < NBIAS; i += blockDim.x)
+    sb[i] = i < CB2 ? wf[P_B1 + i] : (i < CB3 ? wf[P_B2 + i - CB2] : (i < CB4 ? wf[P_B3 + i - CB3] : wf[P_B4 + i - CB4]));
+  if (threadIdx.x < ANDR_FAR_BINS) sfar[threadIdx.x] = far_table[threadIdx.x];
+  if (threadIdx.x < 3) red[threadIdx.x] = 0ull;
+  __syncthreads();
+  const int2 jr = rowmap[blockIdx.x];
+  const SysJob &J = jobs[jr.x];
+  const BodyGeom &g = J.g;
+  const int r = jr.y, ne = g.ne;
+  const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
+  const double ns[3] = {g.nrm[3 * r], g.nrm[3 * r + 1], g.nrm[3 * r + 2]};
+  unsigned long long pts = 0, nl = 0, nf = 0;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    uint8_t cls = NCLS_DUN;
+    if (e != r) {
+      const double *x1 = g.xyz + 3 * g.con[3 * e], *x2 = g.xyz + 3 * g.con[3 * e + 1], *x3 = g.xyz + 3 * g.con[3 * e + 2];
+      const double n[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
+      const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g.area[e]);
+      if (pg.dh >= near_ratio) {
+        cls = (uint8_t)sfar[far_bin(pg.dh, near_ratio)];
+      } else {
+        float f[NIN], cg, ch;
+        features(pg, 1, f);
+        const int a = mlp_classify(f, sw, sb, h, &cg);
+        features(pg, 2, f);
+        const int b = mlp_classify(f, sw, sb, h, &ch);
+        if (cg < min_conf || ch < min_conf || a == NCLS_DUN || b == NCLS_DUN) {
+          cls = NCLS_DUN;
+          ++nf;
+        } else {
+          cls = (uint8_t)max(a, b);
+          ++nl;
+        }
+      }
+      pts += cls == NCLS_DUN ? J.nq : npts_of_cls[cls];
+    }
+    J.cls[(size_t)r * ne + e] = cls;
+  }
+  atomicAdd(&red[0], pts);
+  atomicAdd(&red[1], nl);
+  atomicAdd(&red[2], nf);
+  __syncthreads();
+  if (threadIdx.x < 3) atomicAdd(stats + 3 * (size_t)jr.x + threadIdx.x, red[threadIdx.x]);
+}
+
+RuleTab dunavant_tab(int qoff_fixed, int nq_fixed) {
+  RuleTab t;
+  for (int c = 0; c < NCLS_DUN; ++c) {
+    QuadRule q = andr_rule(DUN_ID0 + c);
+    t.off[c] = q.offset;
+    t.n[c] = q.npts;
+  }
+  t.off[NCLS_DUN] = qoff_fixed;
+  t.n[NCLS_DUN] = nq_fixed;
+  return t;
+}
+
+}  // namespace
+
+// ---- C-ABI -----------------------------------------------------------------------------------------
+static int labels_common(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol, int mode,
+                         int64_t max_pairs, float *features_out, int32_t *labels_out, int64_t *npairs,
+                         int32_t *far_table) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (!(near_ratio > 0.0) || !(rel_tol > 0.0)) return andr_fail(ctx, ANDR_ERR_ARG, "near_ratio and rel_tol must be > 0");
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  int rc = andr_body_upload(ctx, mesh, b, false);
+  if (rc) return rc;
+  const QuadRule ref = andr_rule(b.itri);
+  RuleTab rt = dunavant_tab(ref.offset, ref.npts);
+  DevBuf<float> df;
+  DevBuf<int> dl, dfar;
+  DevBuf<unsigned long long> cnt;
+  const int64_t mp = mode == 0 ? std::max<int64_t>(max_pairs, 1) : 1;
+  if (df.alloc((size_t)mp * 6) || dl.alloc(mp) || dfar.alloc(ANDR_FAR_BINS) || cnt.alloc(1))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "training pair buffers");
+  ANDR_HIP(ctx, hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), ctx->stream));
+  ANDR_HIP(ctx, hipMemsetAsync(dfar.p, 0xff, sizeof(int) * ANDR_FAR_BINS, ctx->stream));  // -1: bin not seen
+  hipLaunchKernelGGL(k_pair_labels, dim3(b.g.ne), dim3(256), 0, ctx->stream, b.g, ref.offset, ref.npts, rt, near_ratio,
+                     rel_tol, mode, mp, df.p, dl.p, cnt.p, dfar.p);
+  ANDR_HIP(ctx, hipGetLastError());
+  unsigned long long n = 0;
+  ANDR_HIP(ctx, hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (mode == 0) {
+    const int64_t k = std::min<int64_t>((int64_t)n, max_pairs);
+    if (k > 0) {
+      ANDR_HIP(ctx, hipMemcpy(features_out, df.p, sizeof(float) * 6 * k, hipMemcpyDeviceToHost));
+      ANDR_HIP(ctx, hipMemcpy(labels_out, dl.p, sizeof(int) * k, hipMemcpyDeviceToHost));
+    }
+    if (npairs) *npairs = k;
+  } else {
+    ANDR_HIP(ctx, hipMemcpy(far_table, dfar.p, sizeof(int) * ANDR_FAR_BINS, hipMemcpyDeviceToHost));
+  }
+  return ANDR_OK;
+}
+
+extern "C" int andr_quad_training_pairs(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol,
+                                        int64_t max_pairs, float *features_out, int32_t *labels_out, int64_t *npairs) {
+  if (!features_out || !labels_out || max_pairs <= 0) return andr_fail(ctx, ANDR_ERR_ARG, "output buffers");
+  return labels_common(ctx, mesh, near_ratio, rel_tol, 0, max_pairs, features_out, labels_out, npairs, nullptr);
+}
+
+extern "C" int andr_quad_far_calibration(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol,
+                                         int32_t *far_table) {
+  if (!far_table) return andr_fail(ctx, ANDR_ERR_ARG, "far_table");
+  return labels_common(ctx, mesh, near_ratio, rel_tol, 1, 0, nullptr, nullptr, nullptr, far_table);
+}
+
+extern "C" int andr_mlp_loss_grad(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels,
+                                  const float *params, float *loss, float *grad) {
+  if (!ctx || npairs <= 0 || npairs > (1 << 24) || !features || !labels || !params || !grad)
+    return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_mlp_loss_grad arguments") : ANDR_ERR_ARG;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  const int M = (int)npairs;
+  for (int64_t i = 0; i < npairs; ++i)
+    if (labels[i] < 0 || labels[i] >= NOUT) return andr_fail(ctx, ANDR_ERR_ARG, "label out of range");
+  MlpWork w;
+  DevBuf<float> X, p, gr;
+  DevBuf<int> L;
+  if (w.alloc(M) || X.alloc((size_t)M * NIN) || p.alloc(NPARAM) || gr.alloc(NPARAM) || L.alloc(M))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "MLP buffers");
+  hipStream_t st = ctx->stream;
+  ANDR_HIP(ctx, hipMemcpyAsync(X.p, features, sizeof(float) * M * NIN, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(L.p, labels, sizeof(int) * M, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(p.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, st));
+  mlp_forward(st, M, X.p, p.p, w);
+  mlp_backward(st, M, X.p, L.p, p.p, w, gr.p);
+  ANDR_HIP(ctx, hipGetLastError());
+  std::vector<float> hl(M);
+  ANDR_HIP(ctx, hipMemcpyAsync(hl.data(), w.loss.p, sizeof(float) * M, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(grad, gr.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  double s = 0.0;
+  for (float v : hl) s += v;
+  if (loss) *loss = (float)(s / M);
+  return ANDR_OK;
+}
+
+extern "C" int andr_mlp_infer(andr_ctx *ctx, int64_t npairs, const float *features, const float *params, float *probs) {
+  if (!ctx || npairs <= 0 || !features || !params || !probs)
+    return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_mlp_infer arguments") : ANDR_ERR_ARG;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  const int chunk = 1 << 16;
+  MlpWork w;
+  DevBuf<float> X, p;
+  if (w.alloc(chunk) || X.alloc((size_t)chunk * NIN) || p.alloc(NPARAM)) return andr_fail(ctx, ANDR_ERR_ALLOC, "MLP buffers");
+  hipStream_t st = ctx->stream;
+  ANDR_HIP(ctx, hipMemcpyAsync(p.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, st));
+  for (int64_t i0 = 0; i0 < npairs; i0 += chunk) {
+    const int M = (int)std::min<int64_t>(chunk, npairs - i0);
+    ANDR_HIP(ctx, hipMemcpyAsync(X.p, features + i0 * NIN, sizeof(float) * M * NIN, hipMemcpyHostToDevice, st));
+    mlp_forward(st, M, X.p, p.p, w);
+    hipLaunchKernelGGL(k_softmax_xent, dim3((M + 255) / 256), dim3(256), 0, st, M, w.Z4.p, (const int *)nullptr, w.P.p,
+                       (float *)nullptr, (float *)nullptr, 1.f);
+    ANDR_HIP(ctx, hipGetLastError());
+    ANDR_HIP(ctx, hipMemcpyAsync(probs + i0 * NOUT, w.P.p, sizeof(float) * M * NOUT, hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+  }
+  return ANDR_OK;
+}
+
+extern "C" int andr_mlp_train(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels,
+                              int32_t epochs, int32_t batch, float lr, uint64_t seed, float *params, float *loss_out) {
+  if (!ctx || npairs <= 0 || npairs > (1 << 26) || !features || !labels || !params || epochs <= 0 || batch <= 0)
+    return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_mlp_train arguments") : ANDR_ERR_ARG;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  for (int64_t i = 0; i < npairs; ++i)
+    if (labels[i] < 0 || labels[i] >= NOUT) return andr_fail(ctx, ANDR_ERR_ARG, "label out of range");
+  const int N = (int)npairs, B = std::min<int>(batch, N);
+  MlpWork w;
+  DevBuf<float> X, Xb, p, gr, m, v, lossacc;
+  DevBuf<int> L, Lb, idx;
+  if (w.alloc(B) || X.alloc((size_t)N * NIN) || Xb.alloc((size_t)B * NIN) || p.alloc(NPARAM) || gr.alloc(NPARAM) ||
+      m.alloc(NPARAM) || v.alloc(NPARAM) || L.alloc(N) || Lb.alloc(B) || idx.alloc(N))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "MLP training buffers");
+  hipStream_t st = ctx->stream;
+  ANDR_HIP(ctx, hipMemcpyAsync(X.p, features, sizeof(float) * (size_t)N * NIN, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(L.p, labels, sizeof(int) * N, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(p.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemsetAsync(m.p, 0, sizeof(float) * NPARAM, st));
+  ANDR_HIP(ctx, hipMemsetAsync(v.p, 0, sizeof(float) * NPARAM, st));
+  std::mt19937_64 rng(seed);
+  std::vector<int> perm(N);
+  std::vector<float> hl(B);
+  int64_t t = 0;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::iota(perm.begin(), perm.end(), 0);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    ANDR_HIP(ctx, hipMemcpyAsync(idx.p, perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
+    double lsum = 0.0;
+    int nb = 0;
+    for (int i0 = 0; i0 + B <= N; i0 += B) {
+      hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, st, B, idx.p + i0, X.p, L.p, Xb.p, Lb.p);
+      mlp_forward(st, B, Xb.p, p.p, w);
+      mlp_backward(st, B, Xb.p, Lb.p, p.p, w, gr.p);
+      ++t;
+      const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+      hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, NPARAM, p.p, gr.p, m.p, v.p, lr, b1, b2,
+                         eps, c1, c2);
+      if ((nb & 15) == 0) {  // sample the loss every 16 minibatches
+        ANDR_HIP(ctx, hipMemcpyAsync(hl.data(), w.loss.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+        ANDR_HIP(ctx, hipStreamSynchronize(st));
+        double s = 0.0;
+        for (float x : hl) s += x;
+        lsum += s / B;
+      }
+      ++nb;
+    }
+    ANDR_HIP(ctx, hipGetLastError());
+    if (loss_out) loss_out[ep] = (float)(lsum / std::max(1, (nb + 15) / 16));
+  }
+  ANDR_HIP(ctx, hipMemcpyAsync(params, p.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  return ANDR_OK;
+}
+
+extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *params, float confidence, double near_ratio,
+                                         const int32_t *far_table) {
+  if (!ctx || !params || !far_table || !(near_ratio > 0.0) || !(confidence >= 0.f && confidence <= 1.f))
+    return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_set_quadrature_model arguments") : ANDR_ERR_ARG;
+  for (int b = 0; b < ANDR_FAR_BINS; ++b)
+    if (far_table[b] < 0 || far_table[b] > NCLS_DUN) return andr_fail(ctx, ANDR_ERR_ARG, "far_table class range");
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  std::vector<uint16_t> wb(NPARAM);
+  for (int i = 0; i < NPARAM; ++i) {  // round to nearest even BF16
+    uint32_t u;
+    std::memcpy(&u, &params[i], 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    wb[i] = (uint16_t)(u >> 16);
+  }
+  const size_t bytes = NPARAM * (sizeof(uint16_t) + sizeof(float)) + ANDR_FAR_BINS * sizeof(int) + 8 * sizeof(int);
+  if (ctx->d_model) (void)hipFree(ctx->d_model);
+  ctx->d_model = nullptr;
+  ANDR_HIP(ctx, hipMalloc(&ctx->d_model, bytes));
+  char *base = (char *)ctx->d_model;
+  float *fp = (float *)base;
+  uint16_t *bp = (uint16_t *)(base + NPARAM * sizeof(float));
+  int *far = (int *)(base + NPARAM * (sizeof(float) + sizeof(uint16_t)));
+  int *npts = far + ANDR_FAR_BINS;
+  int hn[8];
+  for (int c = 0; c < NCLS_DUN; ++c) hn[c] = andr_rule(DUN_ID0 + c).npts;
+  hn[NCLS_DUN] = 0;
+  ANDR_HIP(ctx, hipMemcpy(fp, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+  ANDR_HIP(ctx, hipMemcpy(bp, wb.data(), sizeof(uint16_t) * NPARAM, hipMemcpyHostToDevice));
+  ANDR_HIP(ctx, hipMemcpy(far, far_table, sizeof(int) * ANDR_FAR_BINS, hipMemcpyHostToDevice));
+  ANDR_HIP(ctx, hipMemcpy(npts, hn, sizeof(hn), hipMemcpyHostToDevice));
+  ctx->has_model = true;
+  ctx->model_conf = confidence;
+  ctx->model_near = near_ratio;
+  return ANDR_OK;
+}
+
+int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJob *d_jobs, const int2 *d_rowmap,
+                                      int nrows, int njobs, LearnedStats *stats) {
+  if (!ctx->has_model) return andr_fail(ctx, ANDR_ERR_STATE, "no quadrature model");
+  char *base = (char *)ctx->d_model;
+  const float *fp = (const float *)base;
+  const uint16_t *bp = (const uint16_t *)(base + NPARAM * sizeof(float));
+  const int *far = (const int *)(base + NPARAM * (sizeof(float) + sizeof(uint16_t)));
+  const int *npts = far + ANDR_FAR_BINS;
+  DevBuf<unsigned long long> st;
+  if (st.alloc(3 * (size_t)njobs)) return andr_fail(ctx, ANDR_ERR_ALLOC, "stats");
+  ANDR_HIP(ctx, hipMemsetAsync(st.p, 0, 3 * sizeof(unsigned long long) * njobs, ctx->stream));
+  const int tok = andr_prof_begin(ctx, "classify_pairs", 0.0, 0.0);
+  hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(CLS_THREADS), 0, ctx->stream, d_jobs, d_rowmap, bp, fp, far, npts,
+                     ctx->model_near, ctx->model_conf, st.p);
+  andr_prof_end(ctx, tok);
+  ANDR_HIP(ctx, hipGetLastError());
+  std::vector<unsigned long long> hs(3 * (size_t)njobs);
+  ANDR_HIP(ctx, hipMemcpyAsync(hs.data(), st.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double bytes = 0.0, points = 0.0;
+  for (int i = 0; i < njobs; ++i) {
+    bytes += 9.0 * h_jobs[i].g.ne * (double)h_jobs[i].g.ne * 4.0;
+    points += (double)hs[3 * (size_t)i];
+    if (stats) {
+      stats[i].points = (int64_t)hs[3 * (size_t)i];
+      stats[i].learned = (int64_t)hs[3 * (size_t)i + 1];
+      stats[i].fallback = (int64_t)hs[3 * (size_t)i + 2];
+    }
+  }
+  return andr_launch_stokes_system(ctx, d_jobs, d_rowmap, nrows, true, true, bytes, points * ANDR_FLOPS_PER_STOKES_EVAL);
+}
+
+int andr_upload_rules_learned(andr_ctx *ctx, const double *tab, const float *tabf) {
+  return andr_upload_rules_here(ctx, tab, tabf);
 }

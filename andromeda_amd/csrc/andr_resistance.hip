@@ -47,6 +47,56 @@ void rigid_velocities(const andr_mesh *m, const std::vector<int> &node_wall, boo
   }
 }
 
+// Null-space deflation for the iterative (mixed and learned) forms. On a closed surface the
+// single-layer operator annihilates the normal density, int_S G(x,y) n(y) dS_y = 0, so the
+// discrete system has a near-null right vector v (v_e = n_e) and left vector w (w_r = A_r n_r).
+// Discretisation error leaves a small component of the right-hand side along w that no
+// traction field fits; least squares (the reference's LSQR) leaves that direction unresolved,
+// and GMRES stalls on it. Adding d w v^T, with d of the size of the smooth-mode eigenvalues,
+// moves the direction into the bulk of the spectrum and makes the system consistent. Force and
+// torque are blind to the normal density on a closed polyhedron (sum A_e n_e = 0 and
+// sum A_e c_e x n_e = 0 exactly), so they are unchanged. Only walls that are closed, consistently
+// oriented surfaces (every edge shared by two of its elements in opposite directions) get the
+// term; defl[w] = 0 for the others. defl is indexed by wall id 1..nwalls.
+void deflation_scales(const andr_mesh *m, const double *nm, double mu, std::vector<double> &defl) {
+  const int nw = std::max(1, m->nwalls), ne = m->nelem, base = m->index_base;
+  defl.assign(nw + 1, 0.0);
+  std::vector<double> S(nw + 1, 0.0), S2(nw + 1, 0.0);
+  std::vector<int> cnt(nw + 1, 0);
+  std::vector<std::vector<std::pair<uint64_t, int>>> edges(nw + 1);
+  for (int e = 0; e < ne; ++e) {
+    const int w = m->bcid ? m->bcid[e] : 1;
+    const int v[3] = {m->con[3 * e] - base, m->con[3 * e + 1] - base, m->con[3 * e + 2] - base};
+    const double *p0 = m->xyz + 3 * v[0], *p1 = m->xyz + 3 * v[1], *p2 = m->xyz + 3 * v[2];
+    const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    S[w] += area;
+    S2[w] += area * area;
+    ++cnt[w];
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t i = (uint32_t)v[k], j = (uint32_t)v[(k + 1) % 3];
+      const uint64_t key = ((uint64_t)std::min(i, j) << 32) | std::max(i, j);
+      edges[w].push_back({key, i < j ? 1 : -1});
+    }
+  }
+  for (int w = 1; w <= nw; ++w) {
+    auto &E = edges[w];
+    if (E.empty()) continue;
+    std::sort(E.begin(), E.end());
+    bool closed = E.size() % 2 == 0;
+    for (size_t k = 0; closed && k < E.size(); k += 2)
+      closed = E[k].first == E[k + 1].first && E[k].second + E[k + 1].second == 0 &&
+               (k + 2 >= E.size() || E[k + 2].first != E[k].first);
+    if (!closed) continue;
+    // smooth-mode eigenvalue scale: a sphere of radius a has 16 pi a / 3 for uniform traction
+    const double a = std::sqrt(S[w] / (4.0 * ANDR_PI));
+    const double lam = (8.0 * ANDR_PI / 3.0) * a;
+    defl[w] = (-nm[w - 1] / mu) * lam / (std::sqrt(S2[w]) * std::sqrt((double)cnt[w]));
+  }
+}
+
 }  // namespace
 
 static int check_opts(andr_ctx *ctx, const andr_resistance_opts *o) {
@@ -85,6 +135,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     const size_t n = 3 * (size_t)meshes[b].nelem;
     const size_t lda = (n + 63) / 64 * 64;
     size_t s = n * lda * esz + 4 * n * ANDR_NRHS * sizeof(double);
+    if (learned) s += (size_t)meshes[b].nelem * meshes[b].nelem;  // rule class per pair
     s += plain ? 10 * n * ANDR_NRHS * sizeof(double) : (size_t)(restart + 4) * n * ANDR_NRHS * sizeof(double);
     s += (size_t)meshes[b].nnodes * 200 + (size_t)meshes[b].nelem * 300;
     return s;
@@ -99,7 +150,8 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     // ---- upload the bodies of the wave -----------------------------------------------------------
     std::vector<std::unique_ptr<Prepared>> P(nb);
     std::vector<DevBuf<char>> Abuf(nb);
-    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ubuf(nb), Ebuf(nb);
+    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ubuf(nb), Ebuf(nb), Dbuf(nb);
+    std::vector<DevBuf<uint8_t>> Cbuf(nb);
     std::vector<SysJob> jobs(nb);
     std::vector<int2> rowmap;
     const double ta0 = andr_now();
@@ -149,6 +201,19 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       J.qoff = qr.offset;
       J.nq = qr.npts;
       J.err_rows = Ebuf[i].p;
+      J.cls = nullptr;
+      J.defl = nullptr;
+      if (!plain) {
+        std::vector<double> dh;
+        deflation_scales(m, nm.data(), opts->mu, dh);
+        if (Dbuf[i].alloc(dh.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "deflation scales");
+        ANDR_HIP(ctx, hipMemcpy(Dbuf[i].p, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice));
+        J.defl = Dbuf[i].p;
+      }
+      if (learned) {
+        if (Cbuf[i].alloc((size_t)p.ne * p.ne)) return andr_fail(ctx, ANDR_ERR_ALLOC, "rule classes of body %d", b0 + i);
+        J.cls = Cbuf[i].p;
+      }
       for (int r = 0; r < p.ne; ++r) rowmap.push_back(make_int2(i, r));
     }
     DevBuf<SysJob> djobs;
@@ -156,14 +221,14 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     if (djobs.alloc(nb) || drow.alloc(rowmap.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "job tables");
     ANDR_HIP(ctx, hipMemcpyAsync(djobs.p, jobs.data(), sizeof(SysJob) * nb, hipMemcpyHostToDevice, ctx->stream));
     ANDR_HIP(ctx, hipMemcpyAsync(drow.p, rowmap.data(), sizeof(int2) * rowmap.size(), hipMemcpyHostToDevice, ctx->stream));
-    LearnedStats ls{};
+    std::vector<LearnedStats> ls(nb, LearnedStats{0, 0, 0});
     double alg_bytes = 0.0, alg_flops = 0.0;
     for (int i = 0; i < nb; ++i) {
       const double ne = P[i]->ne;
       alg_bytes += 9.0 * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
       alg_flops += ne * (ne - 1.0) * jobs[i].nq * ANDR_FLOPS_PER_STOKES_EVAL;
     }
-    if (learned) rc = andr_launch_stokes_system_learned(ctx, djobs.p, drow.p, (int)rowmap.size(), &ls);
+    if (learned) rc = andr_launch_stokes_system_learned(ctx, jobs.data(), djobs.p, drow.p, (int)rowmap.size(), nb, ls.data());
     else
       rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain, alg_bytes, alg_flops);
     if (rc) return rc;
@@ -233,9 +298,9 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         R.t_assemble = (ta1 - ta0) / nb;
         R.t_solve = (ts1 - ta1) / nb;
         const QuadRule qr = andr_rule(p.body.itri);
-        R.quad_points = learned ? ls.points / nb : (int64_t)ne * (ne - 1) * qr.npts;
-        R.pairs_learned = learned ? ls.learned / nb : 0;
-        R.pairs_fallback = learned ? ls.fallback / nb : 0;
+        R.quad_points = learned ? ls[i].points : (int64_t)ne * (ne - 1) * qr.npts;
+        R.pairs_learned = learned ? ls[i].learned : 0;
+        R.pairs_fallback = learned ? ls[i].fallback : 0;
       }
     }
     b0 = b1;

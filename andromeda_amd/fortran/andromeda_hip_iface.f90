@@ -1,0 +1,163 @@
+! andromeda_hip: ISO_C_BINDING interface of the MI355X BEM library (include/andromeda_hip.h).
+!
+! A Fortran host (the reference's and.inp/.msh/.bic drivers) uses this module to hand its mesh
+! arrays (module mMesh, src/mod_mesh.f90:16-111) to the device assembly, solve and resistance
+! entry points. The LSQR seam needs no interface: lsqr_precon_ and lsqr_solve_ are exported with
+! the Fortran name mangling and the argument list of src/lib_lsqr.f:1-167, so the reference's
+! implicit-interface calls (e.g. src/lib_CRS.f90:77-85) link against the library unchanged.
+module andromeda_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: ANDR_OK = 0
+  integer(c_int), parameter :: ANDR_ROW_MAJOR = 0, ANDR_COL_MAJOR = 1
+  integer(c_int), parameter :: ANDR_FORM_EXTERIOR = 0, ANDR_FORM_BOUNDED = 1
+  integer(c_int), parameter :: ANDR_PREC_FP64 = 0, ANDR_PREC_MIXED = 1, ANDR_PREC_LEARNED = 2
+  integer(c_int), parameter :: ANDR_MLP_NPARAM = 9288, ANDR_FAR_BINS = 16
+
+  ! andr_mesh: pointers to the host's arrays (xyz(3,nnodes), con(3,nelem), bcid(nelem),
+  ! normmul(nwalls)); index_base = 1 for Fortran node ids
+  type, bind(C) :: andr_mesh
+    integer(c_int32_t) :: nnodes, nelem, nwalls
+    type(c_ptr) :: xyz, con, bcid, normmul
+    integer(c_int32_t) :: index_base, itri
+  end type
+
+  type, bind(C) :: andr_lsqr_opts
+    integer(c_int32_t) :: maxit
+    real(c_double) :: eps
+    integer(c_int32_t) :: precon
+  end type
+
+  type, bind(C) :: andr_lsqr_stats
+    integer(c_int32_t) :: nit(16), istop(16)
+    real(c_double) :: seconds
+  end type
+
+  type, bind(C) :: andr_resistance_opts
+    integer(c_int32_t) :: form, precision
+    real(c_double) :: mu, eps
+    integer(c_int32_t) :: maxit
+    real(c_double) :: center(3)
+  end type
+
+  type, bind(C) :: andr_resistance_report
+    integer(c_int32_t) :: nit(6), status
+    real(c_double) :: err(2), t_assemble, t_solve
+    integer(c_int64_t) :: quad_points, pairs_learned, pairs_fallback
+  end type
+
+  interface
+    integer(c_int) function andr_create(ctx, device) bind(C, name="andr_create")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int32_t), value :: device
+    end function
+
+    integer(c_int) function andr_destroy(ctx) bind(C, name="andr_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    type(c_ptr) function andr_last_error(ctx) bind(C, name="andr_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! sdFormIntegralMatrices (src/lib_int.f90:225-323): H(nsp, nnodes), G(nsp, nelem)
+    integer(c_int) function andr_assemble_laplace(ctx, mesh, layout, H, G, err) &
+        bind(C, name="andr_assemble_laplace")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: layout
+      real(c_double), intent(out) :: H(*), G(*)
+      real(c_double), intent(out) :: err(2)
+    end function
+
+    ! sdFormIntegralMatricesStokes (src/lib_Stokes.f90:1391-1555): T(6) and G(6) hold c_loc of
+    ! the component matrices Txx..Tzz, Gxx..Gzz; prT/prG may be c_null_ptr
+    integer(c_int) function andr_assemble_stokes(ctx, mesh, layout, extra_free_term, T, G, prT, prG, err) &
+        bind(C, name="andr_assemble_stokes")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: layout
+      real(c_double), value :: extra_free_term
+      type(c_ptr), intent(in) :: T(6), G(6)
+      type(c_ptr), value :: prT, prG
+      real(c_double), intent(out) :: err(2)
+    end function
+
+    integer(c_int) function andr_lsqr_dense(ctx, m, n, A, nrhs, B, X, opts, stats) &
+        bind(C, name="andr_lsqr_dense")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, andr_lsqr_opts, andr_lsqr_stats
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: m, n
+      real(c_double), intent(in) :: A(*)
+      integer(c_int32_t), value :: nrhs
+      real(c_double), intent(in) :: B(*)
+      real(c_double), intent(inout) :: X(*)
+      type(andr_lsqr_opts), intent(in) :: opts
+      type(andr_lsqr_stats), intent(out) :: stats
+    end function
+
+    ! StokesInlet3sidesForceCRS / TorqueOmegaCRS (src/lib_particle.f90:176-476)
+    integer(c_int) function andr_resistance(ctx, mesh, opts, FT, report) bind(C, name="andr_resistance")
+      import :: c_int, c_ptr, c_double, andr_mesh, andr_resistance_opts, andr_resistance_report
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      type(andr_resistance_opts), intent(in) :: opts
+      real(c_double), intent(out) :: FT(36)
+      type(andr_resistance_report), intent(out) :: report
+    end function
+
+    integer(c_int) function andr_resistance_batch(ctx, nbodies, meshes, opts, FT, reports) &
+        bind(C, name="andr_resistance_batch")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh, andr_resistance_opts, andr_resistance_report
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nbodies
+      type(andr_mesh), intent(in) :: meshes(*)
+      type(andr_resistance_opts), intent(in) :: opts
+      real(c_double), intent(out) :: FT(*)
+      type(andr_resistance_report), intent(out) :: reports(*)
+    end function
+
+    integer(c_int) function andr_set_quadrature_model(ctx, params, confidence, near_ratio, far_table) &
+        bind(C, name="andr_set_quadrature_model")
+      import :: c_int, c_ptr, c_float, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: params(*)
+      real(c_float), value :: confidence
+      real(c_double), value :: near_ratio
+      integer(c_int32_t), intent(in) :: far_table(*)
+    end function
+  end interface
+
+contains
+
+  ! Message of the last failed call on ctx as a Fortran string.
+  function andr_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: n
+    type(c_ptr) :: cp
+    interface
+      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+        import :: c_size_t, c_ptr
+        type(c_ptr), value :: s
+      end function
+    end interface
+    cp = andr_last_error(ctx)
+    if (.not. c_associated(cp)) then
+      msg = ""
+      return
+    end if
+    n = int(c_strlen(cp))
+    call c_f_pointer(cp, p, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function
+
+end module andromeda_hip

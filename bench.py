@@ -41,11 +41,20 @@ def make_bodies(n_super, n_fibre):
     return [m for m, _ in M.batch_bodies(n_super, n_fibre)]
 
 
-def stratified(bodies, k):
-    order = sorted(range(len(bodies)), key=lambda i: bodies[i].nelem)
+def _spread(idx, bodies, k):
+    order = sorted(idx, key=lambda i: bodies[i].nelem)
     if k >= len(order):
         return order
+    if k == 1:
+        return [order[len(order) // 2]]
     return [order[int(round(j * (len(order) - 1) / (k - 1)))] for j in range(k)]
+
+
+def stratified(bodies, k, n_fibre):
+    """k bodies spread over the size range: a quarter of them fibres, the rest superellipsoids."""
+    n_super = len(bodies) - n_fibre
+    kf = min(n_fibre, k // 4)
+    return sorted(_spread(range(n_super), bodies, k - kf) + _spread(range(n_super, len(bodies)), bodies, kf))
 
 
 def cpu_baseline(bodies, budget_s=20.0):
@@ -99,7 +108,7 @@ def main():
     prec_name = "mixed"
     if args.precision in ("learned", "auto") and os.path.exists(args.model):
         z = np.load(args.model)
-        ctx.set_quadrature_model(z["params"], float(z["confidence"]), float(z["near_ratio"]))
+        ctx.set_quadrature_model(z["params"], float(z["confidence"]), float(z["near_ratio"]), z["far_table"])
         prec, prec_name = api.PREC_LEARNED, "learned"
     elif args.precision == "learned":
         raise SystemExit(f"--precision learned needs {args.model}")
@@ -126,7 +135,7 @@ def main():
     log(f"[bench] fast form ({prec_name}): {value:.2f} tensors/s, {ms_per_step:.0f} ms/step")
 
     # plain FP64 path and fast form on the same stratified sample
-    idx = stratified(bodies, args.sample)
+    idx = stratified(bodies, args.sample, n_fibre)
     sample = [bodies[i] for i in idx]
     tp = time.perf_counter()
     plain = ctx.resistance_batch(sample, precision=api.PREC_FP64, eps=1e-12)
@@ -138,6 +147,8 @@ def main():
     plain_rate = len(sample) / t_plain
     log(f"[bench] sample of {len(sample)}: plain FP64 {plain_rate:.2f} tensors/s, fast {len(sample) / t_fast:.2f}; "
         f"max err {max(errs):.2e}; plain LSQR its {[int(p.nit.max()) for p in plain]}")
+    log("[bench] sample (body, F, fast its, err): " + ", ".join(
+        f"({i}, {bodies[i].nelem}, {int(f.nit.max())}, {e:.1e})" for i, f, e in zip(idx, fast_s, errs)))
 
     # dominant kernel of the timed region and its roofline
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None

@@ -158,9 +158,19 @@ int andr_mlp_infer(andr_ctx *ctx, int64_t npairs, const float *features, const f
 /* One forward+backward pass on a batch: loss and the gradient of the mean cross-entropy. */
 int andr_mlp_loss_grad(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels,
                        const float *params, float *loss, float *grad);
-/* Installs the network used by ANDR_PREC_LEARNED; confidence: minimum softmax probability of the
- * chosen class, below it the pair falls back to the fixed-order rule. */
-int andr_set_quadrature_model(andr_ctx *ctx, const float *params, float confidence, double near_ratio);
+/* Installs the rule selection used by ANDR_PREC_LEARNED. Pairs with distance/size below near_ratio
+ * go through the network; confidence is the minimum softmax probability of the chosen class, below
+ * it the pair falls back to the fixed-order rule. Pairs farther out take far_table[b], the class
+ * calibrated on training pairs for bin b = floor(2 log2(d / (h near_ratio))), b = 0..15
+ * (beyond the last bin: far_table[15]). */
+#define ANDR_FAR_BINS 16
+int andr_set_quadrature_model(andr_ctx *ctx, const float *params, float confidence, double near_ratio,
+                              const int32_t *far_table);
+/* Far-field calibration: for pairs with distance/size in [near_ratio, near_ratio * 256) the
+ * largest label per far bin (as in far_table), over the centroid rows of a mesh; -1 for a bin
+ * that no pair of the mesh falls in. */
+int andr_quad_far_calibration(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol,
+                              int32_t *far_table);
 
 /* ---- measurement (no reference counterpart: the reference only logs CPU timers) ------------
  * Per-kernel timing with HIP events recorded on the context stream around each launch of the hot
