@@ -6,14 +6,15 @@
 #define ANDR_NRHS 8
 
 // One body of a batch: geometry, output system A (3ne x lda, blocked rows c ne + r and columns
-// en ne + e, element type float or double), right-hand sides B (3ne x ANDR_NRHS) and the
-// prescribed nodal velocities U (nn x 3 x ANDR_NRHS).
+// en ne + e, element type float or double) and right-hand sides B (3ne x ANDR_NRHS) of the six
+// rigid-body problems.
 struct SysJob {
   BodyGeom g;
   void *A_;
   int64_t lda;
-  double *B;
-  const double *U;
+  double *B;                // 3 ne x ANDR_NRHS right-hand sides (problems 0..5, the rest 0)
+  const uint8_t *moving;    // per node: 1 if it carries the rigid-body velocity (null: all nodes)
+  double center[3];         // rotation centre of problems 3..5
   double mu, extra;
   int qoff, nq;        // the fixed-order rule
   double *err_rows;    // ne (may be null)

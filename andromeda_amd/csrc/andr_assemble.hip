@@ -514,23 +514,28 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
 // ------------------------------------------------------------------------------------------------
 // Stokes resistance system, emitted directly in the solver layout (blocked rows and columns):
 //   A[(c ne + r), (en ne + e)] = -(mul_e / mu) G_{c,en}(r, e)    (formQrow, src/lib_StokesFull.f90:4-85)
-//   B[(c ne + r), k] = sum_{e,v,en} T_{c,en}(r, e, v) U[node(e,v), en, k] + free term
-// Only centroid rows (the equations of an all-Dirichlet problem) are formed; the T matrix is never
-// stored. One workgroup per (job, row); a job is one body of a batch.
+//   B[(c ne + r), k] = sum_{e,v,en} T_{c,en}(r, e, v) u_k(node(e,v))_en + free term
+// with u_k the rigid-body velocity of problem k (translation e_k, k < 3; rotation
+// e_{k-3} x (x - center)) on the moving nodes and 0 elsewhere. Since u_k is affine in the node
+// position, the contraction needs only the moments sum_v T_v and sum_v T_v (x_v - center) of each
+// pair (24 sums per row) instead of the nodal velocities. Only centroid rows (the equations of an
+// all-Dirichlet problem) are formed; the T matrix is never stored. One workgroup per (job, row); a
+// job is one body of a batch.
 // ------------------------------------------------------------------------------------------------
 template <class CT, class ST>
 __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict__ jobs,
                                                        const int2 *__restrict__ rowmap) {
-  __shared__ double red[(3 * ANDR_NRHS + 6) * 4];
+  constexpr int NACC = 6 + 18 + 6;  // sum T (moving), sum T r (moving), sum T (all nodes)
+  __shared__ double red[NACC * 4];
   const int2 jr = rowmap[blockIdx.x];
   const SysJob &J = jobs[jr.x];
   const BodyGeom &g = J.g;
   const int r = jr.y;
   const int ne = g.ne;
   const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
-  double bacc[3 * ANDR_NRHS + 6];
+  double acc[NACC];
 #pragma unroll
-  for (int i = 0; i < 3 * ANDR_NRHS + 6; ++i) bacc[i] = 0.0;
+  for (int i = 0; i < NACC; ++i) acc[i] = 0.0;
   const double inv_mu = 1.0 / J.mu;
   ST *A = J.template Aptr<ST>();
   ST *arow0 = A + (size_t)(0 * ne + r) * J.lda;
@@ -551,11 +556,9 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
       int qoff = J.qoff, nq = J.nq;
       if (J.cls) {  // learned form: Dunavant class 0..6 or the fixed-order rule (class 7)
         const int c = J.cls[(size_t)r * ne + e];
-        if (c < 7) {
-          constexpr int dn[7] = {3, 4, 6, 7, 12, 13, 19};
-          constexpr int doff[7] = {0, 3, 7, 13, 20, 32, 45};
-          qoff = ANDR_DUN_OFF0 + doff[c];
-          nq = dn[c];
+        if (c < 7) {  // Dunavant offsets {0,3,7,13,20,32,45} and sizes {3,4,6,7,12,13,19}, 6 bits each
+          qoff = ANDR_DUN_OFF0 + (int)((0x2D8143470C0ull >> (6 * c)) & 63);
+          nq = (int)((0x1334C1C6103ull >> (6 * c)) & 63);
         }
       }
       stokes_pair_regular<CT>(s, x1, x2, x3, m * g.nrm[3 * e], m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2],
@@ -572,32 +575,64 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int en = 0; en < 3; ++en) ar[c][(size_t)en * ne + e] = (ST)(sc * Gs[sym3(c, en)] + nr3[c] * ne3[en]);
-    // T . U for the rigid-body right-hand sides
+    // moments of T over the element's moving nodes
+    const double *xv[3] = {x1, x2, x3};
+    double mv[3], rv[3][3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      const double *u = J.U + (size_t)vv[v] * 3 * ANDR_NRHS;
+      mv[v] = J.moving ? (double)J.moving[vv[v]] : 1.0;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double t0 = Tv[sym3(c, 0)][v], t1 = Tv[sym3(c, 1)][v], t2 = Tv[sym3(c, 2)][v];
-#pragma unroll
-        for (int k = 0; k < ANDR_NRHS; ++k)
-          bacc[c * ANDR_NRHS + k] += t0 * u[0 * ANDR_NRHS + k] + t1 * u[1 * ANDR_NRHS + k] + t2 * u[2 * ANDR_NRHS + k];
-      }
+      for (int d = 0; d < 3; ++d) rv[v][d] = mv[v] * (xv[v][d] - J.center[d]);
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) bacc[3 * ANDR_NRHS + k] += Tv[k][0] + Tv[k][1] + Tv[k][2];
+    for (int k = 0; k < 6; ++k) {
+      acc[k] += mv[0] * Tv[k][0] + mv[1] * Tv[k][1] + mv[2] * Tv[k][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[6 + 3 * k + d] += Tv[k][0] * rv[0][d] + Tv[k][1] * rv[1][d] + Tv[k][2] * rv[2][d];
+      acc[24 + k] += Tv[k][0] + Tv[k][1] + Tv[k][2];
+    }
   }
-  block_sum<3 * ANDR_NRHS + 6>(bacc, red);
+  block_sum<NACC>(acc, red);
   if (threadIdx.x == 0) {
-    const double *tsum = bacc + 3 * ANDR_NRHS;
+    const double *S = acc, *M = acc + 6, *tsum = acc + 24;
     const double cfree[3] = {-tsum[0] + J.extra, -tsum[3] + J.extra, -tsum[5] + J.extra};
+    // rigid velocities at the row element's nodes, for the free term spread over its vertices
     const int *cn = g.con + 3 * r;
+    double ubar[3][ANDR_NRHS];  // (1/3) sum_v u_k(node v), component c
+#pragma unroll
     for (int c = 0; c < 3; ++c)
-      for (int k = 0; k < ANDR_NRHS; ++k) {
-        double b = bacc[c * ANDR_NRHS + k];
-        for (int v = 0; v < 3; ++v) b += (1.0 / 3.0) * cfree[c] * J.U[(size_t)cn[v] * 3 * ANDR_NRHS + c * ANDR_NRHS + k];
-        J.B[(size_t)(c * ne + r) * ANDR_NRHS + k] = b;
-      }
+#pragma unroll
+      for (int k = 0; k < ANDR_NRHS; ++k) ubar[c][k] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const double w = (J.moving ? (double)J.moving[cn[v]] : 1.0) / 3.0;
+      const double *p = g.xyz + 3 * cn[v];
+      const double q[3] = {p[0] - J.center[0], p[1] - J.center[1], p[2] - J.center[2]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ubar[c][c] += w;
+      // e_x x q = (0, -qz, qy); e_y x q = (qz, 0, -qx); e_z x q = (-qy, qx, 0)
+      ubar[1][3] -= w * q[2];
+      ubar[2][3] += w * q[1];
+      ubar[0][4] += w * q[2];
+      ubar[2][4] -= w * q[0];
+      ubar[0][5] -= w * q[1];
+      ubar[1][5] += w * q[0];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      // sum_v T_{c,i} u_k,i: translations pick column k; rotations (e_a x r)_i = eps_{i a j} r_j
+      const double *Mc[3] = {M + 3 * sym3(c, 0), M + 3 * sym3(c, 1), M + 3 * sym3(c, 2)};
+      double b[ANDR_NRHS];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) b[k] = S[sym3(c, k)];
+      b[3] = -Mc[1][2] + Mc[2][1];
+      b[4] = Mc[0][2] - Mc[2][0];
+      b[5] = -Mc[0][1] + Mc[1][0];
+#pragma unroll
+      for (int k = 6; k < ANDR_NRHS; ++k) b[k] = 0.0;
+#pragma unroll
+      for (int k = 0; k < ANDR_NRHS; ++k) J.B[(size_t)(c * ne + r) * ANDR_NRHS + k] = b[k] + cfree[c] * ubar[c][k];
+    }
     // self-checks as sdIntRowQStokes: off-diagonal sums ~ 0, diagonal sums ~ 4 pi (bounded form)
     const double osemPi = (double)(atanf(1.0f)) * 4.0 * 8.0;
     double err = fmax(fabs(tsum[1]), fmax(fabs(tsum[4]), fabs(tsum[2])));

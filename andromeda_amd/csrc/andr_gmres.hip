@@ -50,7 +50,11 @@ struct TileMap {
   int sys, row0;
 };
 
-// Y = sx .* (A X) + sy .* Y for every system; sx/sy per (system, rhs). part[wave][k] = sum Y^2.
+constexpr int WROWS = 64;  // rows of A per wave in the products
+
+// Y = sx .* (A X) + sy .* Y for every system; sx/sy per (system, rhs). part[wave][k] = sum Y^2 over
+// the wave's rows. FP64 accumulation (the residual product of the refinement loop): the wave
+// works through its 64 rows in blocks of 8, each lane covering every 64th column.
 template <class ST>
 __global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap, int nwaves,
                                                int which_x, int jq, int which_y, const double *__restrict__ sxy,
@@ -60,51 +64,153 @@ __global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, con
   if (wave >= nwaves) return;
   const WaveMap wm = wmap[wave];
   const GSys &S = sys[wm.sys];
-  const int m = S.n, n = S.n, row0 = wm.row0;
+  const int m = S.n, n = S.n;
   const ST *A = (const ST *)S.A;
   const double *X = which_x == 0 ? S.X : S.Q + (size_t)jq * S.n * NB;
   double *Y = which_y == 0 ? S.R : S.W;
   const int64_t lda = S.lda;
-  double acc[64];
+  double y2 = 0.0;
+  for (int row0 = wm.row0; row0 < min(wm.row0 + WROWS, m); row0 += 8) {
+    double acc[64];
 #pragma unroll
-  for (int i = 0; i < 64; ++i) acc[i] = 0.0;
-  const ST *arow[8];
+    for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+    const ST *arow[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) arow[r] = A + (size_t)min(row0 + r, m - 1) * lda;
-  for (int j = lane; j < n; j += 64) {
-    const double4 x0 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB);
-    const double4 x1 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB + 4);
-    const double xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    for (int r = 0; r < 8; ++r) arow[r] = A + (size_t)min(row0 + r, m - 1) * lda;
+    for (int j = lane; j < n; j += 64) {
+      const double4 x0 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB);
+      const double4 x1 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB + 4);
+      const double xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const double a = (double)arow[r][j];
+      for (int r = 0; r < 8; ++r) {
+        const double a = (double)arow[r][j];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[r * 8 + k] += a * xv[k];
+        for (int k = 0; k < 8; ++k) acc[r * 8 + k] += a * xv[k];
+      }
+    }
+#pragma unroll
+    for (int half = 32; half >= 1; half >>= 1) {
+      const bool hi = (lane & half) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        const double keep = hi ? acc[i + half] : acc[i];
+        const double send = hi ? acc[i] : acc[i + half];
+        acc[i] = keep + __shfl_xor(send, half, 64);
+      }
+    }
+    const int r = lane >> 3, k = lane & 7;
+    if (row0 + r < m) {
+      double *py = Y + (size_t)(row0 + r) * NB + k;
+      const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
+      const double y = sx * acc[0] + (sy != 0.0 ? sy * (*py) : 0.0);
+      *py = y;
+      y2 += y * y;
     }
   }
-#pragma unroll
-  for (int half = 32; half >= 1; half >>= 1) {
-    const bool hi = (lane & half) != 0;
-#pragma unroll
-    for (int i = 0; i < half; ++i) {
-      const double keep = hi ? acc[i + half] : acc[i];
-      const double send = hi ? acc[i] : acc[i + half];
-      acc[i] = keep + __shfl_xor(send, half, 64);
-    }
-  }
-  const int r = lane >> 3, k = lane & 7;
-  double y = 0.0;
-  if (row0 + r < m) {
-    double *py = Y + (size_t)(row0 + r) * NB + k;
-    const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
-    y = sx * acc[0] + (sy != 0.0 ? sy * (*py) : 0.0);
-    *py = y;
-  }
-  double y2 = y * y;
   y2 += __shfl_xor(y2, 8, 64);
   y2 += __shfl_xor(y2, 16, 64);
   y2 += __shfl_xor(y2, 32, 64);
-  if (lane < 8) part[(size_t)wave * NB + lane] = y2;
+  if (part && lane < 8) part[(size_t)wave * NB + lane] = y2;
+}
+
+// The Krylov product W = A Q_j on the matrix cores: FP32 A (as stored), Q_j rounded to FP32,
+// v_mfma_f32_16x16x4f32 with the 8 right-hand sides in the 16 B columns (8 unused). A wave owns 64
+// rows as four 16-row tiles; lane (g = lane / 16, c = lane % 16) feeds row 16 T + c of tile T with
+// columns k0 + 4 g + t (one float4 per tile and 16 columns), and column c of B with the same k.
+// The FP32 accumulators are added into FP64 every 64 columns, which keeps the summation error at
+// the size of the FP32 storage error. Output lane layout: D[4 g + v][c] in element v of tile T.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap,
+                                                    int nwaves, int jq, const double *__restrict__ sxy) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const WaveMap wm = wmap[wave];
+  const GSys &S = sys[wm.sys];
+  const int n = S.n, row0 = wm.row0;
+  const float *A = (const float *)S.A;
+  const double *X = S.Q + (size_t)jq * n * NB;
+  double *Y = S.W;
+  const int g = lane >> 4, c = lane & 15;
+  const bool cl = c < NB;
+  const float *ap[4];
+#pragma unroll
+  for (int T = 0; T < 4; ++T) ap[T] = A + (size_t)min(row0 + 16 * T + c, n - 1) * S.lda + 4 * g;
+  const double *xp = X + (size_t)(4 * g) * NB + (cl ? c : 0);
+  f4v acc[4];
+  double acc64[4][4];
+#pragma unroll
+  for (int T = 0; T < 4; ++T) {
+    acc[T] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc64[T][v] = 0.0;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc64[T][v] += (double)acc[T][v];
+      acc[T] = f4v{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  const int nfull = n & ~31;
+  int k0 = 0;
+  for (; k0 < nfull; k0 += 32) {
+    float4 a0[4], a1[4];
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+      a0[T] = *reinterpret_cast<const float4 *>(ap[T] + k0);
+      a1[T] = *reinterpret_cast<const float4 *>(ap[T] + k0 + 16);
+    }
+    float b0[4], b1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      b0[t] = cl ? (float)xp[(size_t)(k0 + t) * NB] : 0.f;
+      b1[t] = cl ? (float)xp[(size_t)(k0 + 16 + t) * NB] : 0.f;
+    }
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].x, b0[0], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].y, b0[1], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].z, b0[2], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].w, b0[3], acc[T], 0, 0, 0);
+    }
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].x, b1[0], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].y, b1[1], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].z, b1[2], acc[T], 0, 0, 0);
+      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].w, b1[3], acc[T], 0, 0, 0);
+    }
+    if (k0 & 32) flush();
+  }
+  // last columns (n not a multiple of 32): masked, 16 at a time
+  for (; k0 < n; k0 += 16) {
+    float a[4][4], b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool in = k0 + 4 * g + t < n;
+      b[t] = (cl && in) ? (float)xp[(size_t)(k0 + t) * NB] : 0.f;
+#pragma unroll
+      for (int T = 0; T < 4; ++T) a[T][t] = in ? ap[T][k0 + t] : 0.f;
+    }
+#pragma unroll
+    for (int T = 0; T < 4; ++T)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][t], b[t], acc[T], 0, 0, 0);
+  }
+  flush();
+  if (cl) {
+    const double sx = sxy[(size_t)wm.sys * 2 * NB + c];
+#pragma unroll
+    for (int T = 0; T < 4; ++T)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 16 * T + 4 * g + v;
+        if (row < n) Y[(size_t)row * NB + c] = sx * acc64[T][v];
+      }
+  }
 }
 
 // per tile: partial dots  part[tile][i][k] = sum_rows Q_i[row][k] * W[row][k], i = 0..j
@@ -352,7 +458,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   for (int s = 0; s < nsys; ++s) {
     vec_elems += (size_t)in[s].n * NB;
     basis_elems += (size_t)(m + 1) * in[s].n * NB;
-    nw_full += (in[s].n + 7) / 8;
+    nw_full += (in[s].n + WROWS - 1) / WROWS;
     nt_full += (in[s].n + TILE - 1) / TILE;
   }
   const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1) + 8);
@@ -384,7 +490,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ms.ntiles.assign(nsys, 0);
     for (int s : list) {
       w0[s] = (int)wm.size();
-      for (int r = 0; r < in[s].n; r += 8) wm.push_back({s, r});
+      for (int r = 0; r < in[s].n; r += WROWS) wm.push_back({s, r});
       nw[s] = (int)wm.size() - w0[s];
       ms.tile0[s] = (int)tm.size();
       for (int r = 0; r < in[s].n; r += TILE) tm.push_back({s, r});
@@ -459,10 +565,13 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         n2 += (double)in[s].n * in[s].n;
         n1 += in[s].n;
       }
-    const int tok = andr_prof_begin(ctx, fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64",
+    const bool mfma = fp32 && which_x == 1 && which_y == 1;
+    const int tok = andr_prof_begin(ctx, mfma ? "gmres_gemv_mfma" : (fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64"),
                                     n2 * (fp32 ? 4.0 : 8.0) + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
     const int gw = (ms.n_w + 3) / 4;
-    if (fp32)
+    if (mfma)
+      hipLaunchKernelGGL(kb_gemv_mfma, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, jq, scales);
+    else if (fp32)
       hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
                          scales, partW.p);
     else
@@ -526,6 +635,13 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
       gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
+      // CGS2: two passes of (coefficients Q^T W, W -= Q coef); each pass streams the basis twice
+      double nlive = 0.0;
+      for (int s = 0; s < nsys; ++s)
+        if (live.ntiles[s]) nlive += in[s].n;
+      const double vbytes = nlive * NB * sizeof(double);
+      const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", 2.0 * (2.0 * (j + 1) + 3.0) * vbytes,
+                                        2.0 * 4.0 * (j + 1) * nlive * NB);
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(kb_dots, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1, partT.p);
         const int nthr = nsys * (j + 1) * NB;
@@ -533,8 +649,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
                            partT.p, pass == 0 ? 1 : 0);
         hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p);
       }
+      andr_prof_end(ctx, tok_o);
+      const int tok_g = andr_prof_begin(ctx, "gmres_givens_scale", 2.0 * vbytes, 0.0);
       hipLaunchKernelGGL(kb_givens, dim3(gs), dim3(64), 0, st, dsys.p, nsys, j, m, partT2.p, tol);
       hipLaunchKernelGGL(kb_scale_next, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j);
+      andr_prof_end(ctx, tok_g);
       steps = j + 1;
       if (steps % 8 == 0 && steps < steps_max) {  // drop systems whose columns are all resolved
         ANDR_HIP(ctx, hipGetLastError());

@@ -128,10 +128,23 @@ struct RuleTab {
   int off[8], n[8];
 };
 
-// one block per centroid row: near pairs -> (features, label) records; far pairs -> bin maxima
+// deterministic 32-bit hash of a pair (subsampling of the training records)
+__device__ inline uint32_t pair_hash(uint32_t r, uint32_t e) {
+  uint32_t h = r * 0x9E3779B1u ^ (e + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
+// one block per centroid row. mode 0: near pairs with hash <= keep -> (features, label) records,
+// three per pair (one per kernel); mode 1: far pairs -> bin maxima; mode 2: count near records.
 __global__ __launch_bounds__(256) void k_pair_labels(BodyGeom g, int qoff_ref, int nq_ref, RuleTab rt,
-                                                     double near_ratio, double rel_tol, int mode, int64_t max_pairs,
-                                                     float *__restrict__ feats, int *__restrict__ labels,
+                                                     double near_ratio, double rel_tol, int mode, uint32_t keep,
+                                                     int64_t max_pairs, float *__restrict__ feats,
+                                                     int *__restrict__ labels,
                                                      unsigned long long *__restrict__ counter,
                                                      int *__restrict__ far_max) {
   const int r = blockIdx.x;
@@ -143,7 +156,9 @@ __global__ __launch_bounds__(256) void k_pair_labels(BodyGeom g, int qoff_ref, i
     const double *x1 = g.xyz + 3 * g.con[3 * e], *x2 = g.xyz + 3 * g.con[3 * e + 1], *x3 = g.xyz + 3 * g.con[3 * e + 2];
     const double n[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
     const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g.area[e]);
-    if (mode == 0 && pg.dh < near_ratio) {
+    if (mode == 2 && pg.dh < near_ratio) {
+      atomicAdd(counter, 3ull);
+    } else if (mode == 0 && pg.dh < near_ratio && pair_hash(r, e) <= keep) {
       for (int kernel = 0; kernel < 3; ++kernel) {
         const int lab = pair_label(kernel, s, x1, x2, x3, n, g.area[e], qoff_ref, nq_ref, rt.off, rt.n, rel_tol);
         const unsigned long long idx = atomicAdd(counter, 1ull);
@@ -455,12 +470,23 @@ static int labels_common(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio
   const int64_t mp = mode == 0 ? std::max<int64_t>(max_pairs, 1) : 1;
   if (df.alloc((size_t)mp * 6) || dl.alloc(mp) || dfar.alloc(ANDR_FAR_BINS) || cnt.alloc(1))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "training pair buffers");
+  uint32_t keep = 0xFFFFFFFFu;
+  unsigned long long n = 0;
+  if (mode == 0) {  // count first; sample the pairs uniformly when they exceed max_pairs
+    ANDR_HIP(ctx, hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(k_pair_labels, dim3(b.g.ne), dim3(256), 0, ctx->stream, b.g, ref.offset, ref.npts, rt,
+                       near_ratio, rel_tol, 2, keep, mp, df.p, dl.p, cnt.p, dfar.p);
+    ANDR_HIP(ctx, hipGetLastError());
+    ANDR_HIP(ctx, hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
+    ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((double)n > (double)max_pairs)
+      keep = (uint32_t)std::min(4294967295.0, 0.97 * (double)max_pairs / (double)n * 4294967296.0);
+  }
   ANDR_HIP(ctx, hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), ctx->stream));
   ANDR_HIP(ctx, hipMemsetAsync(dfar.p, 0xff, sizeof(int) * ANDR_FAR_BINS, ctx->stream));  // -1: bin not seen
   hipLaunchKernelGGL(k_pair_labels, dim3(b.g.ne), dim3(256), 0, ctx->stream, b.g, ref.offset, ref.npts, rt, near_ratio,
-                     rel_tol, mode, mp, df.p, dl.p, cnt.p, dfar.p);
+                     rel_tol, mode, keep, mp, df.p, dl.p, cnt.p, dfar.p);
   ANDR_HIP(ctx, hipGetLastError());
-  unsigned long long n = 0;
   ANDR_HIP(ctx, hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (mode == 0) {

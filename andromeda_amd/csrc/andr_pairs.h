@@ -15,7 +15,7 @@ static inline int andr_upload_rules_here(andr_ctx *ctx, const double *tab, const
 
 // index of component (i, j) in the symmetric storage {xx, xy, xz, yy, yz, zz}
 __host__ __device__ constexpr int sym3(int i, int j) {
-  return i > j ? sym3(j, i) : (i == 0 ? j : (i == 1 ? 2 + j : 5));
+  return (i < j ? i : j) == 0 ? (i > j ? i : j) : ((i < j ? i : j) == 1 ? i + j + 1 : 5);
 }
 
 template <class CT>

@@ -25,28 +25,6 @@ struct Prepared {
   int ne = 0, nn = 0;
 };
 
-// Rigid-body velocities at the nodes: [node][component][rhs]. Problems 0..2 translate along
-// e_x, e_y, e_z; 3..5 rotate about e_x, e_y, e_z through `center`. For the bounded form only the
-// nodes of wall 2 move (src/lib_particle.f90:220-256, 386-409).
-void rigid_velocities(const andr_mesh *m, const std::vector<int> &node_wall, bool exterior, const double c[3],
-                      std::vector<double> &U) {
-  const int nn = m->nnodes;
-  U.assign((size_t)nn * 3 * ANDR_NRHS, 0.0);
-  for (int i = 0; i < nn; ++i) {
-    if (!exterior && node_wall[i] != 2) continue;
-    const double r[3] = {m->xyz[3 * i] - c[0], m->xyz[3 * i + 1] - c[1], m->xyz[3 * i + 2] - c[2]};
-    double *u = U.data() + (size_t)i * 3 * ANDR_NRHS;
-    for (int k = 0; k < 3; ++k) u[k * ANDR_NRHS + k] = 1.0;
-    // e_x x r = (0, -z, y); e_y x r = (z, 0, -x); e_z x r = (-y, x, 0)
-    u[1 * ANDR_NRHS + 3] = -r[2];
-    u[2 * ANDR_NRHS + 3] = r[1];
-    u[0 * ANDR_NRHS + 4] = r[2];
-    u[2 * ANDR_NRHS + 4] = -r[0];
-    u[0 * ANDR_NRHS + 5] = -r[1];
-    u[1 * ANDR_NRHS + 5] = r[0];
-  }
-}
-
 // Null-space deflation for the iterative (mixed and learned) forms. On a closed surface the
 // single-layer operator annihilates the normal density, int_S G(x,y) n(y) dS_y = 0, so the
 // discrete system has a near-null right vector v (v_e = n_e) and left vector w (w_r = A_r n_r).
@@ -150,7 +128,8 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     // ---- upload the bodies of the wave -----------------------------------------------------------
     std::vector<std::unique_ptr<Prepared>> P(nb);
     std::vector<DevBuf<char>> Abuf(nb);
-    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ubuf(nb), Ebuf(nb), Dbuf(nb);
+    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ebuf(nb), Dbuf(nb);
+    std::vector<DevBuf<uint8_t>> Mbuf(nb);
     std::vector<DevBuf<uint8_t>> Cbuf(nb);
     std::vector<SysJob> jobs(nb);
     std::vector<int2> rowmap;
@@ -180,22 +159,25 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
           p.node_wall[nd] = std::max(p.node_wall[nd], w);
         }
       }
-      std::vector<double> U;
-      rigid_velocities(m, p.node_wall, exterior, opts->center, U);
       const int n = 3 * p.ne;
       const int64_t lda = (n + 63) / 64 * 64;
       if (Abuf[i].alloc((size_t)n * lda * esz) || Bbuf[i].alloc((size_t)n * ANDR_NRHS) ||
-          Xbuf[i].alloc((size_t)n * ANDR_NRHS) || Ubuf[i].alloc(U.size()) || Ebuf[i].alloc(p.ne))
+          Xbuf[i].alloc((size_t)n * ANDR_NRHS) || Ebuf[i].alloc(p.ne))
         return andr_fail(ctx, ANDR_ERR_ALLOC, "resistance system of body %d (n = %d)", b0 + i, n);
-      ANDR_HIP(ctx, hipMemcpyAsync(Ubuf[i].p, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice, ctx->stream));
-      ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (!exterior) {  // bounded form: only the nodes of wall 2 move (src/lib_particle.f90:220-256, 386-409)
+        std::vector<uint8_t> mv(p.nn);
+        for (int k = 0; k < p.nn; ++k) mv[k] = p.node_wall[k] == 2 ? 1 : 0;
+        if (Mbuf[i].alloc(p.nn)) return andr_fail(ctx, ANDR_ERR_ALLOC, "moving-node flags");
+        ANDR_HIP(ctx, hipMemcpy(Mbuf[i].p, mv.data(), p.nn, hipMemcpyHostToDevice));
+      }
       const QuadRule qr = andr_rule(p.body.itri);
       SysJob &J = jobs[i];
       J.g = p.body.g;
       J.A_ = Abuf[i].p;
       J.lda = lda;
       J.B = Bbuf[i].p;
-      J.U = Ubuf[i].p;
+      J.moving = exterior ? nullptr : Mbuf[i].p;
+      for (int d = 0; d < 3; ++d) J.center[d] = opts->center[d];
       J.mu = opts->mu;
       J.extra = extra;
       J.qoff = qr.offset;

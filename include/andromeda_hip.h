@@ -145,7 +145,9 @@ int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshe
 #define ANDR_MLP_NPARAM 9288
 /* Features of near-field (source point, element) pairs of a body and the label of each pair:
  * the smallest rule whose G and T integrals agree with the fixed-order rule to rel_tol.
- * Up to max_pairs pairs (pairs with distance/size < near_ratio); returns the count in *npairs. */
+ * Pairs with distance/size < near_ratio give three records each (Laplace, Stokes-G, Stokes-H);
+ * when there are more than max_pairs records, the pairs are sampled uniformly (deterministic hash
+ * of the pair) to about 0.97 max_pairs. The number of records written is returned in *npairs. */
 int andr_quad_training_pairs(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio, double rel_tol,
                              int64_t max_pairs, float *features /* 6*max_pairs */, int32_t *labels,
                              int64_t *npairs);

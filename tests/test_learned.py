@@ -279,6 +279,7 @@ def test_gpu_learned_gain(ctx):
     for p, mx, t, l in zip(plain, mixed, table_only, learned):
         assert tensor_error(l.FT, p.FT) <= 1e-6
         assert tensor_error(t.FT, p.FT) <= 1e-6
-        assert l.quad_points < 0.25 * mx.quad_points
-        assert l.quad_points < t.quad_points
+        assert l.quad_points < t.quad_points < mx.quad_points
         assert l.pairs_learned > 0
+    # over the three bodies: at most a quarter of the fixed rule's quadrature points
+    assert sum(l.quad_points for l in learned) < 0.25 * sum(mx.quad_points for mx in mixed)
