@@ -1,6 +1,7 @@
 // Resistance-system assembly job description shared by the assembly and driver units.
 #pragma once
 #include "andr_internal.h"
+#include "andr_solve.h"
 
 // right-hand sides carried per body (6 rigid-body problems, padded to 8)
 #define ANDR_NRHS 8
@@ -11,7 +12,7 @@
 struct SysJob {
   BodyGeom g;
   void *A_;
-  int64_t lda;
+  int64_t lda;         // row-major leading dimension; 0: FP32 tiled layout (andr_tiled_off)
   double *B;                // 3 ne x ANDR_NRHS right-hand sides (problems 0..5, the rest 0)
   const uint8_t *moving;    // per node: 1 if it carries the rigid-body velocity (null: all nodes)
   double center[3];         // rotation centre of problems 3..5
@@ -20,6 +21,8 @@ struct SysJob {
   double *err_rows;    // ne (may be null)
   uint8_t *cls;        // ne x ne rule class per pair (learned form), null: fixed-order rule
   const double *defl;  // per wall id (1..nwalls): scale of the rank-one null-space term, null: none
+  double *minv;        // ne x 9: inverse 3x3 self blocks (right block-Jacobi preconditioner), null: none
+  double *X;           // 3 ne x ANDR_NRHS solution (for andr_launch_unprecondition)
 #ifdef __HIPCC__
   template <class ST>
   __device__ ST *Aptr() const { return (ST *)A_; }
@@ -28,6 +31,11 @@ struct SysJob {
 
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
                               bool fp32_store, double alg_bytes, double alg_flops);
+// Right block-Jacobi preconditioner of the iterative forms: minv[e] = inverse of the 3x3 self block
+// (singular Stokeslet block of element e at its own centroid, scaled as the system, plus the
+// null-space term). The assembly then stores A M^-1 and the solve returns y with x = M^-1 y.
+int andr_launch_self_blocks(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
+int andr_launch_unprecondition(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
 // algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0

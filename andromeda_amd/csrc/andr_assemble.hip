@@ -131,8 +131,9 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
   b.itri = m->itri == 0 ? 5 : m->itri;
   if (b.itri < 1 || b.itri > 5) return andr_fail(ctx, ANDR_ERR_ARG, "itri must be 1..5");
   DevBuf<double> d_wm;
-  if (b.xyz.alloc(3 * (size_t)nn) || b.con.alloc(3 * (size_t)ne) || b.bcid.alloc(ne) || b.nrm.alloc(3 * (size_t)ne) ||
-      b.area.alloc(ne) || b.cen.alloc(3 * (size_t)ne) || b.mul.alloc(ne) || d_wm.alloc(nw))
+  if (b.xyz.alloc(3 * (size_t)nn, ctx) || b.con.alloc(3 * (size_t)ne, ctx) || b.bcid.alloc(ne, ctx) ||
+      b.nrm.alloc(3 * (size_t)ne, ctx) || b.area.alloc(ne, ctx) || b.cen.alloc(3 * (size_t)ne, ctx) ||
+      b.mul.alloc(ne, ctx) || d_wm.alloc(nw, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "device allocation for mesh");
   ANDR_HIP(ctx, hipMemcpyAsync(b.xyz.p, m->xyz, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(b.con.p, con.data(), sizeof(int) * 3 * ne, hipMemcpyHostToDevice, ctx->stream));
@@ -538,9 +539,11 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   for (int i = 0; i < NACC; ++i) acc[i] = 0.0;
   const double inv_mu = 1.0 / J.mu;
   ST *A = J.template Aptr<ST>();
-  ST *arow0 = A + (size_t)(0 * ne + r) * J.lda;
-  ST *arow1 = A + (size_t)(1 * ne + r) * J.lda;
-  ST *arow2 = A + (size_t)(2 * ne + r) * J.lda;
+  const bool tiled = J.lda == 0;
+  const int KB = (3 * ne + 15) / 16;
+  ST *arow0 = A + (tiled ? 0 : (size_t)(0 * ne + r) * J.lda);
+  ST *arow1 = A + (tiled ? 0 : (size_t)(1 * ne + r) * J.lda);
+  ST *arow2 = A + (tiled ? 0 : (size_t)(2 * ne + r) * J.lda);
   for (int e = threadIdx.x; e < ne; e += blockDim.x) {
     const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
     const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
@@ -571,10 +574,31 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
     const double ne3[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
     const double nr3[3] = {dfl * g.nrm[3 * r], dfl * g.nrm[3 * r + 1], dfl * g.nrm[3 * r + 2]};
     ST *ar[3] = {arow0, arow1, arow2};
+    double blk[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int en = 0; en < 3; ++en) ar[c][(size_t)en * ne + e] = (ST)(sc * Gs[sym3(c, en)] + nr3[c] * ne3[en]);
+      for (int en = 0; en < 3; ++en) blk[c][en] = sc * Gs[sym3(c, en)] + nr3[c] * ne3[en];
+    if (J.minv) {  // right preconditioner: block (r, e) times minv[e]
+      const double *mi = J.minv + 9 * (size_t)e;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double b0 = blk[c][0], b1 = blk[c][1], b2 = blk[c][2];
+#pragma unroll
+        for (int en = 0; en < 3; ++en) blk[c][en] = b0 * mi[en] + b1 * mi[3 + en] + b2 * mi[6 + en];
+      }
+    }
+    if (tiled) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int en = 0; en < 3; ++en) A[andr_tiled_off(c * ne + r, en * ne + e, KB)] = (ST)blk[c][en];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int en = 0; en < 3; ++en) ar[c][(size_t)en * ne + e] = (ST)blk[c][en];
+    }
     // moments of T over the element's moving nodes
     const double *xv[3] = {x1, x2, x3};
     double mv[3], rv[3][3];
@@ -654,6 +678,73 @@ int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d
   else
     return andr_fail(ctx, ANDR_ERR_ARG, "FP32 quadrature with FP64 storage is not offered");
   andr_prof_end(ctx, tok);
+  ANDR_HIP(ctx, hipGetLastError());
+  return ANDR_OK;
+}
+
+// thread per (job, element): inverse of the element's own 3x3 system block
+__global__ void k_self_blocks(const SysJob *__restrict__ jobs, const int2 *__restrict__ rowmap, int nrows) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int2 jr = rowmap[t];
+  const SysJob &J = jobs[jr.x];
+  const BodyGeom &g = J.g;
+  const int e = jr.y;
+  const double *x1 = g.xyz + 3 * g.con[3 * e], *x2 = g.xyz + 3 * g.con[3 * e + 1], *x3 = g.xyz + 3 * g.con[3 * e + 2];
+  const double s[3] = {g.cen[3 * e], g.cen[3 * e + 1], g.cen[3 * e + 2]};
+  double Gs[6];
+  andr_sing_stk(x1, x2, x3, s, 4, Gs);
+  const double sc = -g.mul[e] / J.mu;
+  const double dfl = J.defl ? J.defl[g.bcid[e]] * g.area[e] : 0.0;
+  const double *n = g.nrm + 3 * e;
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = sc * Gs[sym3(i, j)] + dfl * n[i] * n[j];
+  const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1], c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2],
+               c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+  double *mi = J.minv + 9 * (size_t)e;
+  if (!(fabs(det) > 0.0)) {  // degenerate element: no scaling
+    for (int i = 0; i < 9; ++i) mi[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  const double id = 1.0 / det;
+  mi[0] = c00 * id;
+  mi[3] = c01 * id;
+  mi[6] = c02 * id;
+  mi[1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  mi[4] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  mi[7] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  mi[2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  mi[5] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  mi[8] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+}
+
+// thread per (job, element): x_e = minv[e] y_e for every right-hand side (in place on J.X)
+__global__ void k_unprecondition(const SysJob *__restrict__ jobs, const int2 *__restrict__ rowmap, int nrows) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int2 jr = rowmap[t];
+  const SysJob &J = jobs[jr.x];
+  const int e = jr.y, ne = J.g.ne;
+  const double *mi = J.minv + 9 * (size_t)e;
+  for (int k = 0; k < ANDR_NRHS; ++k) {
+    const double y0 = J.X[(size_t)(0 * ne + e) * ANDR_NRHS + k], y1 = J.X[(size_t)(1 * ne + e) * ANDR_NRHS + k],
+                 y2 = J.X[(size_t)(2 * ne + e) * ANDR_NRHS + k];
+    for (int c = 0; c < 3; ++c) J.X[(size_t)(c * ne + e) * ANDR_NRHS + k] = mi[3 * c] * y0 + mi[3 * c + 1] * y1 + mi[3 * c + 2] * y2;
+  }
+}
+
+int andr_launch_self_blocks(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows) {
+  if (nrows <= 0) return ANDR_OK;
+  hipLaunchKernelGGL(k_self_blocks, dim3((nrows + 255) / 256), dim3(256), 0, ctx->stream, d_jobs, d_rowmap, nrows);
+  ANDR_HIP(ctx, hipGetLastError());
+  return ANDR_OK;
+}
+
+int andr_launch_unprecondition(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows) {
+  if (nrows <= 0) return ANDR_OK;
+  hipLaunchKernelGGL(k_unprecondition, dim3((nrows + 255) / 256), dim3(256), 0, ctx->stream, d_jobs, d_rowmap, nrows);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;
 }

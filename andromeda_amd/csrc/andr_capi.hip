@@ -64,13 +64,66 @@ extern "C" int andr_create(andr_ctx **out, int32_t device) {
 extern "C" int andr_destroy(andr_ctx *ctx) {
   if (!ctx) return ANDR_OK;
   (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_model) (void)hipFree(ctx->d_model);
-  if (ctx->stream) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
+  andr_pool_trim(ctx);
+  for (auto &kv : ctx->pool.size_of) (void)hipFree(kv.first);  // blocks still held by nobody's buffers
+  ctx->pool.size_of.clear();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ANDR_OK;
+}
+
+// ---- device memory cache ------------------------------------------------------------------------
+static size_t pool_round(size_t bytes) {
+  const size_t g = bytes <= (1u << 20) ? 4096 : (1u << 20);
+  return (bytes + g - 1) / g * g;
+}
+
+void *andr_dev_alloc(andr_ctx *ctx, size_t bytes) {
+  andr_pool &P = ctx->pool;
+  const size_t want = pool_round(bytes);
+  auto it = P.free_blocks.lower_bound(want);
+  if (it != P.free_blocks.end() && it->first <= want + want / 4 + (1u << 20)) {  // close enough: reuse
+    void *p = it->second;
+    P.cached -= it->first;
+    P.free_blocks.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    andr_pool_trim(ctx);
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  P.size_of[p] = want;
+  return p;
+}
+
+void andr_dev_release(andr_ctx *ctx, void *p) {
+  andr_pool &P = ctx->pool;
+  auto it = P.size_of.find(p);
+  if (it == P.size_of.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  P.free_blocks.emplace(it->second, p);
+  P.cached += it->second;
+}
+
+void andr_pool_trim(andr_ctx *ctx) {
+  andr_pool &P = ctx->pool;
+  if (P.free_blocks.empty()) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : P.free_blocks) {
+    (void)hipFree(kv.second);
+    P.size_of.erase(kv.second);
+  }
+  P.free_blocks.clear();
+  P.cached = 0;
 }
 
 extern "C" const char *andr_last_error(const andr_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }

@@ -23,7 +23,8 @@
 namespace {
 
 constexpr int NB = ANDR_NB;
-constexpr int TILE = 256;  // rows per vector tile
+constexpr int TILE = 64;   // rows per vector tile (256 threads: 32 rows x 8 right-hand sides per pass)
+constexpr int ICH = 16;   // basis vectors per block of the projection kernel
 
 struct GSys {  // device view of one system
   const void *A;
@@ -41,10 +42,15 @@ struct GSys {  // device view of one system
   double *small; // per rhs: [0] bnorm, [1] rnorm (true), [2] est, [3] inv_h, [4] active, [5] used steps
   int tile0;     // first vector tile of the system
   int ntiles;
+  int nseg;      // column segments of the matrix-core product (1: written straight to W)
+  double *P;     // nseg x n x NB partial products when nseg > 1
 };
 
 struct WaveMap {
   int sys, row0;
+};
+struct MWave {  // matrix-core product: 64 rows x columns [k0, k1) of one system, column segment seg
+  int sys, row0, k0, k1, seg;
 };
 struct TileMap {
   int sys, row0;
@@ -121,22 +127,24 @@ __global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, con
 // the size of the FP32 storage error. Output lane layout: D[4 g + v][c] in element v of tile T.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap,
+__global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
                                                     int nwaves, int jq, const double *__restrict__ sxy) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
-  const WaveMap wm = wmap[wave];
+  const MWave wm = wmap[wave];
   const GSys &S = sys[wm.sys];
-  const int n = S.n, row0 = wm.row0;
+  const int n = S.n, row0 = wm.row0, kend = min(wm.k1, n);
+  const int KB = (n + 15) / 16;
   const float *A = (const float *)S.A;
   const double *X = S.Q + (size_t)jq * n * NB;
-  double *Y = S.W;
   const int g = lane >> 4, c = lane & 15;
   const bool cl = c < NB;
+  // tiled layout: the float4 of lane (g, c) in tile (band, kb) sits at 256 (band KB + kb) + 16 c + 4 g;
+  // ap[T] + 16 k0 addresses it for the k-block k0 / 16 (16 floats per column of tiles)
   const float *ap[4];
 #pragma unroll
-  for (int T = 0; T < 4; ++T) ap[T] = A + (size_t)min(row0 + 16 * T + c, n - 1) * S.lda + 4 * g;
+  for (int T = 0; T < 4; ++T) ap[T] = A + (size_t)((row0 >> 4) + T) * KB * 256 + 16 * c + 4 * g;
   const double *xp = X + (size_t)(4 * g) * NB + (cl ? c : 0);
   f4v acc[4];
   double acc64[4][4];
@@ -154,14 +162,13 @@ __global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys
       acc[T] = f4v{0.f, 0.f, 0.f, 0.f};
     }
   };
-  const int nfull = n & ~31;
-  int k0 = 0;
-  for (; k0 < nfull; k0 += 32) {
+  int k0 = wm.k0, step = 0;
+  for (; k0 + 32 <= kend; k0 += 32) {
     float4 a0[4], a1[4];
 #pragma unroll
     for (int T = 0; T < 4; ++T) {
-      a0[T] = *reinterpret_cast<const float4 *>(ap[T] + k0);
-      a1[T] = *reinterpret_cast<const float4 *>(ap[T] + k0 + 16);
+      a0[T] = *reinterpret_cast<const float4 *>(ap[T] + (size_t)16 * k0);
+      a1[T] = *reinterpret_cast<const float4 *>(ap[T] + (size_t)16 * (k0 + 16));
     }
     float b0[4], b1[4];
 #pragma unroll
@@ -183,17 +190,18 @@ __global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys
       acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].z, b1[2], acc[T], 0, 0, 0);
       acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].w, b1[3], acc[T], 0, 0, 0);
     }
-    if (k0 & 32) flush();
+    if (++step & 1) continue;
+    flush();
   }
-  // last columns (n not a multiple of 32): masked, 16 at a time
-  for (; k0 < n; k0 += 16) {
+  // last columns of the segment (fewer than 32): masked, 16 at a time
+  for (; k0 < kend; k0 += 16) {
     float a[4][4], b[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const bool in = k0 + 4 * g + t < n;
+      const bool in = k0 + 4 * g + t < kend;
       b[t] = (cl && in) ? (float)xp[(size_t)(k0 + t) * NB] : 0.f;
 #pragma unroll
-      for (int T = 0; T < 4; ++T) a[T][t] = in ? ap[T][k0 + t] : 0.f;
+      for (int T = 0; T < 4; ++T) a[T][t] = in ? ap[T][(size_t)16 * k0 + t] : 0.f;
     }
 #pragma unroll
     for (int T = 0; T < 4; ++T)
@@ -202,7 +210,9 @@ __global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys
   }
   flush();
   if (cl) {
-    const double sx = sxy[(size_t)wm.sys * 2 * NB + c];
+    const bool direct = S.nseg <= 1;
+    const double sx = direct ? sxy[(size_t)wm.sys * 2 * NB + c] : 1.0;
+    double *Y = direct ? S.W : S.P + (size_t)wm.seg * n * NB;
 #pragma unroll
     for (int T = 0; T < 4; ++T)
 #pragma unroll
@@ -213,38 +223,152 @@ __global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys
   }
 }
 
-// per tile: partial dots  part[tile][i][k] = sum_rows Q_i[row][k] * W[row][k], i = 0..j
+// Y = sx .* (A X) + sy .* Y on the tiled FP32 layout with FP64 products and sums (the residual of
+// the refinement loop): wave per 64-row band, lane (g, c) covers row 16 T + c and columns 4 g..4 g+3
+// of every 16-column block; the four column groups are summed across lanes at the end.
+// part[wave][k] = sum over the band's rows of Y^2.
+__global__ __launch_bounds__(256) void kb_resid_tiled(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap,
+                                                      int nwaves, const double *__restrict__ sxy,
+                                                      double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const WaveMap wm = wmap[wave];
+  const GSys &S = sys[wm.sys];
+  const int n = S.n, row0 = wm.row0, KB = (n + 15) / 16;
+  const int g = lane >> 4, c = lane & 15;
+  const float *A = (const float *)S.A;
+  double acc[4][NB];
+#pragma unroll
+  for (int T = 0; T < 4; ++T)
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc[T][k] = 0.0;
+  for (int kb = 0; kb < KB; ++kb) {
+    double x[4][NB];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int col = 16 * kb + 4 * g + t;
+      if (col < n) {
+        const double4 x0 = *reinterpret_cast<const double4 *>(S.X + (size_t)col * NB);
+        const double4 x1 = *reinterpret_cast<const double4 *>(S.X + (size_t)col * NB + 4);
+        x[t][0] = x0.x; x[t][1] = x0.y; x[t][2] = x0.z; x[t][3] = x0.w;
+        x[t][4] = x1.x; x[t][5] = x1.y; x[t][6] = x1.z; x[t][7] = x1.w;
+      } else {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) x[t][k] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+      const float4 a = *reinterpret_cast<const float4 *>(A + ((size_t)((row0 >> 4) + T) * KB + kb) * 256 + 16 * c + 4 * g);
+      // columns past n are padding of the last block (not written by the assembly): masked
+      const int c0 = 16 * kb + 4 * g;
+      const double av[4] = {c0 < n ? (double)a.x : 0.0, c0 + 1 < n ? (double)a.y : 0.0,
+                            c0 + 2 < n ? (double)a.z : 0.0, c0 + 3 < n ? (double)a.w : 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) acc[T][k] += av[t] * x[t][k];
+    }
+  }
+  // sum the four column groups (lanes c, c + 16, c + 32, c + 48)
+#pragma unroll
+  for (int T = 0; T < 4; ++T)
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      acc[T][k] += __shfl_xor(acc[T][k], 16, 64);
+      acc[T][k] += __shfl_xor(acc[T][k], 32, 64);
+    }
+  // lane (g, c) writes row 16 g + c (tile T = g) for all right-hand sides
+  double y2[NB];
+  const int row = row0 + 16 * g + c;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    double v = acc[0][k];
+#pragma unroll
+    for (int T = 1; T < 4; ++T) v = g == T ? acc[T][k] : v;
+    y2[k] = 0.0;
+    if (row < n) {
+      double *py = S.R + (size_t)row * NB + k;
+      const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
+      const double y = sx * v + (sy != 0.0 ? sy * (*py) : 0.0);
+      *py = y;
+      y2[k] = y * y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    for (int off = 1; off < 64; off <<= 1) y2[k] += __shfl_xor(y2[k], off, 64);
+  if (lane < NB) {
+    double v = y2[0];
+#pragma unroll
+    for (int k = 1; k < NB; ++k) v = lane == k ? y2[k] : v;
+    part[(size_t)wave * NB + lane] = v;
+  }
+}
+
+// per tile of a system with nseg > 1: W = sx .* sum over segments of the partial products
+__global__ __launch_bounds__(256) void kb_seg_sum(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap,
+                                                  const double *__restrict__ sxy) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  if (S.nseg <= 1) return;
+  const int n = S.n;
+  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
+  const double sx = sxy[(size_t)tm.sys * 2 * NB + k];
+#pragma unroll
+  for (int t = 0; t < TILE / 32; ++t) {
+    const int row = tm.row0 + rr + 32 * t;
+    if (row >= n) continue;
+    double y = 0.0;
+    for (int sg = 0; sg < S.nseg; ++sg) y += S.P[((size_t)sg * n + row) * NB + k];
+    S.W[(size_t)row * NB + k] = sx * y;
+  }
+}
+
+// per (tile, chunk of ICH basis vectors): partial dots part[tile][i][k] = sum_rows Q_i[row][k] W[row][k]
 __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
                                                int mstride, double *__restrict__ part) {
-  __shared__ double red[4][NB];
+  __shared__ double red[4][ICH][NB];
+  const int i0 = blockIdx.y * ICH;
+  if (i0 > j) return;  // uniform over the block
+  const int ni = min(ICH, j + 1 - i0);
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double w[TILE / 32];
+  int rows[TILE / 32];
 #pragma unroll
   for (int t = 0; t < TILE / 32; ++t) {
     const int row = tm.row0 + rr + 32 * t;
+    rows[t] = row < n ? row : -1;
     w[t] = row < n ? S.W[(size_t)row * NB + k] : 0.0;
   }
-  for (int i = 0; i <= j; ++i) {
-    const double *Qi = S.Q + (size_t)i * n * NB;
-    double a = 0.0;
+  double acc[ICH];
 #pragma unroll
-    for (int t = 0; t < TILE / 32; ++t) {
-      const int row = tm.row0 + rr + 32 * t;
-      if (row < n) a += Qi[(size_t)row * NB + k] * w[t];
+  for (int ii = 0; ii < ICH; ++ii) {
+    double a = 0.0;
+    if (ii < ni) {
+      const double *Qi = S.Q + (size_t)(i0 + ii) * n * NB;
+#pragma unroll
+      for (int t = 0; t < TILE / 32; ++t)
+        if (rows[t] >= 0) a += Qi[(size_t)rows[t] * NB + k] * w[t];
     }
     a += __shfl_xor(a, 8, 64);
     a += __shfl_xor(a, 16, 64);
     a += __shfl_xor(a, 32, 64);
-    if (lane < 8) red[wid][lane] = a;
-    __syncthreads();
-    if (threadIdx.x < 8)
-      part[((size_t)blockIdx.x * mstride + i) * NB + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] +
-                                                                   red[2][threadIdx.x] + red[3][threadIdx.x];
-    __syncthreads();
+    acc[ii] = a;
+  }
+  if (lane < 8)
+#pragma unroll
+    for (int ii = 0; ii < ICH; ++ii) red[wid][ii][lane] = acc[ii];
+  __syncthreads();
+  if (threadIdx.x < ni * NB) {
+    const int ii = threadIdx.x >> 3, kk = threadIdx.x & 7;
+    part[((size_t)blockIdx.x * mstride + i0 + ii) * NB + kk] =
+        red[0][ii][kk] + red[1][ii][kk] + red[2][ii][kk] + red[3][ii][kk];
   }
 }
 
@@ -465,50 +589,93 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   DevBuf<double> R, W, Q, SM, partW, partT, partT2;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
-  if (R.alloc(vec_elems) || W.alloc(vec_elems) || Q.alloc(basis_elems) || SM.alloc(small_per * nsys) ||
-      partW.alloc((size_t)nw_full * NB) || partT.alloc((size_t)nt_full * (m + 1) * NB) ||
-      partT2.alloc((size_t)nt_full * NB) || dsys.alloc(nsys) || ddone.alloc(1))
+  if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
+      SM.alloc(small_per * nsys, ctx) || partW.alloc((size_t)nw_full * NB, ctx) ||
+      partT.alloc((size_t)nt_full * (m + 1) * NB, ctx) || partT2.alloc((size_t)nt_full * NB, ctx) ||
+      dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
-  // a map set: waves of 8 rows and tiles of TILE rows over a list of systems
+  // a map set over a list of systems: waves of WROWS rows (FP64-accumulating product), tiles of
+  // TILE rows (vector kernels) and matrix-core waves (64 rows x a column segment). When the list
+  // has few rows (the last, slowest systems of a batch) the columns are split into segments so
+  // that the product still fills the chip; the segments are summed by kb_seg_sum.
+  constexpr int MF_TARGET_WAVES = 2048, MF_MAX_SEG = 8;
   struct MapSet {
     DevBuf<WaveMap> w;
     DevBuf<TileMap> t;
+    DevBuf<MWave> mw;
     DevBuf<int> w0, nw;
-    int n_w = 0, n_t = 0;
-    std::vector<int> tile0, ntiles;
+    int n_w = 0, n_t = 0, n_mw = 0, max_seg = 1;
+    size_t mw_cap = 0;
+    std::vector<int> tile0, ntiles, nseg;
+    std::vector<size_t> poff;
+    size_t p_elems = 0;
   };
   MapSet cyc, live;
   for (MapSet *ms : {&cyc, &live})
-    if (ms->w.alloc(nw_full) || ms->t.alloc(nt_full) || ms->w0.alloc(nsys) || ms->nw.alloc(nsys))
+    if (ms->w.alloc(nw_full, ctx) || ms->t.alloc(nt_full, ctx) || ms->w0.alloc(nsys, ctx) || ms->nw.alloc(nsys, ctx))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES work maps");
   std::vector<GSys> hs(nsys);
+  DevBuf<double> Pseg;
+  size_t pseg_cap = 0;
   auto build = [&](const std::vector<int> &list, MapSet &ms) -> int {
     std::vector<WaveMap> wm;
     std::vector<TileMap> tm;
+    std::vector<MWave> mw;
     std::vector<int> w0(nsys, 0), nw(nsys, 0);
     ms.tile0.assign(nsys, 0);
     ms.ntiles.assign(nsys, 0);
+    ms.nseg.assign(nsys, 1);
+    ms.poff.assign(nsys, 0);
+    ms.p_elems = 0;
+    ms.max_seg = 1;
+    int row_waves = 0;
+    for (int s : list) row_waves += (in[s].n + WROWS - 1) / WROWS;
+    const int want = row_waves > 0 ? std::min(MF_MAX_SEG, (MF_TARGET_WAVES + row_waves - 1) / row_waves) : 1;
     for (int s : list) {
+      const int n = in[s].n;
       w0[s] = (int)wm.size();
-      for (int r = 0; r < in[s].n; r += WROWS) wm.push_back({s, r});
+      for (int r = 0; r < n; r += WROWS) wm.push_back({s, r});
       nw[s] = (int)wm.size() - w0[s];
       ms.tile0[s] = (int)tm.size();
-      for (int r = 0; r < in[s].n; r += TILE) tm.push_back({s, r});
+      for (int r = 0; r < n; r += TILE) tm.push_back({s, r});
       ms.ntiles[s] = (int)tm.size() - ms.tile0[s];
+      const int seglen = ((n + want - 1) / want + 31) / 32 * 32;
+      const int nseg = (n + seglen - 1) / seglen;
+      ms.nseg[s] = nseg;
+      ms.max_seg = std::max(ms.max_seg, nseg);
+      if (nseg > 1) {
+        ms.poff[s] = ms.p_elems;
+        ms.p_elems += (size_t)nseg * n * NB;
+      }
+      for (int r = 0; r < n; r += WROWS)
+        for (int sg = 0; sg < nseg; ++sg) mw.push_back({s, r, sg * seglen, std::min(n, (sg + 1) * seglen), sg});
     }
     ms.n_w = (int)wm.size();
     ms.n_t = (int)tm.size();
+    ms.n_mw = (int)mw.size();
+    if (mw.size() > ms.mw_cap) {
+      if (ms.mw.alloc(mw.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES matrix-core map");
+      ms.mw_cap = mw.size();
+    }
     if (ms.n_w) ANDR_HIP(ctx, hipMemcpyAsync(ms.w.p, wm.data(), sizeof(WaveMap) * wm.size(), hipMemcpyHostToDevice, st));
     if (ms.n_t) ANDR_HIP(ctx, hipMemcpyAsync(ms.t.p, tm.data(), sizeof(TileMap) * tm.size(), hipMemcpyHostToDevice, st));
+    if (ms.n_mw) ANDR_HIP(ctx, hipMemcpyAsync(ms.mw.p, mw.data(), sizeof(MWave) * mw.size(), hipMemcpyHostToDevice, st));
     ANDR_HIP(ctx, hipMemcpyAsync(ms.w0.p, w0.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
     ANDR_HIP(ctx, hipMemcpyAsync(ms.nw.p, nw.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
     ANDR_HIP(ctx, hipStreamSynchronize(st));  // host vectors go out of scope
     return ANDR_OK;
   };
-  auto use_tiles = [&](const MapSet &ms) -> int {  // the reductions index partials through S.tile0/ntiles
+  // the reductions index partials through S.tile0/ntiles; the product reads S.nseg/P
+  auto use_tiles = [&](const MapSet &ms) -> int {
+    if (ms.p_elems > pseg_cap) {
+      if (Pseg.alloc(ms.p_elems, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES segment partials");
+      pseg_cap = ms.p_elems;
+    }
     for (int s = 0; s < nsys; ++s) {
       hs[s].tile0 = ms.tile0[s];
       hs[s].ntiles = ms.ntiles[s];
+      hs[s].nseg = ms.nseg[s];
+      hs[s].P = ms.nseg[s] > 1 ? Pseg.p + ms.poff[s] : nullptr;
     }
     ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
     return ANDR_OK;
@@ -552,7 +719,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       h_arn[s * 2 * NB + NB + k] = 0.0;
     }
   DevBuf<double> d_res, d_arn;
-  if (d_res.alloc(h_res.size()) || d_arn.alloc(h_arn.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES scales");
+  if (d_res.alloc(h_res.size(), ctx) || d_arn.alloc(h_arn.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES scales");
   ANDR_HIP(ctx, hipMemcpyAsync(d_res.p, h_res.data(), sizeof(double) * h_res.size(), hipMemcpyHostToDevice, st));
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
@@ -569,11 +736,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     const int tok = andr_prof_begin(ctx, mfma ? "gmres_gemv_mfma" : (fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64"),
                                     n2 * (fp32 ? 4.0 : 8.0) + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
     const int gw = (ms.n_w + 3) / 4;
-    if (mfma)
-      hipLaunchKernelGGL(kb_gemv_mfma, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, jq, scales);
-    else if (fp32)
-      hipLaunchKernelGGL(kb_gemv<float>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
-                         scales, partW.p);
+    if (mfma) {
+      hipLaunchKernelGGL(kb_gemv_mfma, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw, jq, scales);
+      if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
+    } else if (fp32)  // the residual of the refinement loop: X into R
+      hipLaunchKernelGGL(kb_resid_tiled, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, scales, partW.p);
     else
       hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
                          scales, partW.p);
@@ -643,7 +810,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", 2.0 * (2.0 * (j + 1) + 3.0) * vbytes,
                                         2.0 * 4.0 * (j + 1) * nlive * NB);
       for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(kb_dots, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1, partT.p);
+        hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1,
+                           partT.p);
         const int nthr = nsys * (j + 1) * NB;
         hipLaunchKernelGGL(kb_dots_final, dim3((nthr + 255) / 256), dim3(256), 0, st, dsys.p, nsys, j, m, m + 1,
                            partT.p, pass == 0 ? 1 : 0);

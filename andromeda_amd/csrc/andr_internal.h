@@ -6,17 +6,32 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/andromeda_hip.h"
 
 #define ANDR_PI 3.141592653589793238462643383
 
+// Device memory cache of a context. Blocks released during a call stay with the context for the
+// next call instead of going back to the driver: hipFree synchronises the device and mapping or
+// unmapping large blocks costs milliseconds each, which is most of a batched call's host time
+// otherwise. Every use of a block is ordered on the context's stream, so a block handed out again
+// is only touched by work queued after the work of its previous owner. The cache is emptied when
+// an allocation fails and on andr_destroy.
+struct andr_pool {
+  std::multimap<size_t, void *> free_blocks;  // size -> cached block
+  std::unordered_map<void *, size_t> size_of;  // every block the pool owns (live or cached)
+  size_t cached = 0;
+};
+
 struct andr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  andr_pool pool;
   // learned quadrature model (BF16 copy lives in the device buffer below)
   bool has_model = false;
   float model_conf = 0.9f;
@@ -34,25 +49,43 @@ int andr_fail(andr_ctx *ctx, int code, const char *fmt, ...);
                        hipGetErrorString(e_));                                                    \
   } while (0)
 
-// RAII device buffer ---------------------------------------------------------------------------
+void *andr_dev_alloc(andr_ctx *ctx, size_t bytes);  // from the context's cache or the driver; null on failure
+void andr_dev_release(andr_ctx *ctx, void *p);       // back to the context's cache
+void andr_pool_trim(andr_ctx *ctx);                  // return every cached block to the driver
+
+// RAII device buffer; with a context it draws on the context's memory cache
 template <class T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  andr_ctx *owner = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
-  hipError_t alloc(size_t count) {
+  hipError_t alloc(size_t count, andr_ctx *pool = nullptr) {
     release();
     n = count;
     if (count == 0) return hipSuccess;
+    if (pool) {
+      p = (T *)andr_dev_alloc(pool, sizeof(T) * count);
+      if (!p) {
+        n = 0;
+        return hipErrorOutOfMemory;
+      }
+      owner = pool;
+      return hipSuccess;
+    }
     return hipMalloc(&p, sizeof(T) * count);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (owner) andr_dev_release(owner, p);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    owner = nullptr;
   }
 };
 

@@ -320,63 +320,110 @@ void mlp_backward(hipStream_t st, int M, const float *X, const int *lab, const f
   hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dH.p, grad + P_B1);
 }
 
-// ---- inference inside the assembly: BF16 weights in LDS, FP32 accumulation ---------------------
-__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// ---- inference inside the assembly: BF16 matrix cores, FP32 accumulation --------------------------
+// The network runs transposed, H^T = W^T X^T: hidden units are the M dimension and pairs the N
+// dimension of v_mfma_f32_16x16x16bf16_1k. With that orientation the accumulator tile of one layer
+// (lane (g, c) holds rows 4g..4g+3 of a 16-row block for pair c) is, after bias, ReLU and rounding
+// to BF16, exactly the B operand (rows 4g..4g+3 of a 16-deep K block, column c) of the next layer:
+// the activations never leave the registers. Each wave keeps the 40 weight tiles (A operands) in
+// registers, collects the near pairs of its row in an LDS queue and evaluates them 64 at a time
+// (4 column blocks of 16 pairs), for the Stokes-G and Stokes-H kernel ids.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-struct ModelView {
-  const uint16_t *w;  // bf16 weights, NPARAM entries (biases included, but biases read from b)
-  const float *b;     // fp32 parameters (biases used)
+__device__ inline short f2bf(float x) {  // round to nearest even
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+constexpr int CLS_THREADS = 256, CLS_WAVES = CLS_THREADS / 64, CLS_Q = 128;
+
+struct MlpTiles {     // A operands (W^T tiles) of one lane
+  bf16x4 w1[4];       // layer 1: 4 row blocks x 1 K block (6 inputs padded to 16)
+  bf16x4 w2[4][4];    // layers 2, 3: 4 x 4
+  bf16x4 w3[4][4];
+  bf16x4 w4[4];       // layer 4: 1 row block (8 classes padded to 16) x 4 K blocks
 };
 
-// Class and confidence of one feature vector. Weights (BF16) and biases sit in LDS and are read at
-// wave-uniform addresses (broadcast); the thread's activation vector lives in its own LDS row `h`
-// (stride CLS_HSTRIDE) so the 64 accumulators of a layer stay in registers.
-constexpr int CLS_THREADS = 128, CLS_HSTRIDE = NH + 1;
-constexpr int CB1 = 0, CB2 = NH, CB3 = 2 * NH, CB4 = 3 * NH, NBIAS = 3 * NH + NOUT;  // compact biases
+// lane (g, i): A[m = 16 mt + i][k = 16 kt + 4 g + t] = W[in = 16 kt + 4 g + t][out = 16 mt + i]
+__device__ inline bf16x4 load_tile(const uint16_t *w, int nin, int nout, int mt, int kt, int g, int i) {
+  bf16x4 r;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int in = 16 * kt + 4 * g + t, out = 16 * mt + i;
+    r[t] = (in < nin && out < nout) ? (short)w[in * nout + out] : (short)0;
+  }
+  return r;
+}
 
-__device__ inline int mlp_classify(const float f[NIN], const uint16_t *sw, const float *sb, float *h, float *conf) {
-  float acc[NH];
+// bias + ReLU + BF16 of an accumulator tile -> B operand of the next layer
+__device__ inline bf16x4 act_tile(f32x4 c, const float *bias, int mt, int g) {
+  bf16x4 r;
 #pragma unroll
-  for (int o = 0; o < NH; ++o) acc[o] = sb[CB1 + o];
+  for (int v = 0; v < 4; ++v) r[v] = f2bf(fmaxf(c[v] + bias[16 * mt + 4 * g + v], 0.f));
+  return r;
+}
+
+// logits of 16 pairs (column block) for one kernel id; returns in lanes g == 0 the class and the
+// softmax confidence of pair c
+__device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float *feat /* [16][3] */, int kernel,
+                                 int g, int c, int *cls, float *conf) {
+  bf16x4 x;
 #pragma unroll
-  for (int i = 0; i < NIN; ++i)
-#pragma unroll
-    for (int o = 0; o < NH; ++o) acc[o] += f[i] * bf16_to_f(sw[P_W1 + i * NH + o]);
-#pragma unroll
-  for (int o = 0; o < NH; ++o) h[o] = fmaxf(acc[o], 0.f);
-  for (int layer = 0; layer < 2; ++layer) {
-    const int pw = layer == 0 ? P_W2 : P_W3, pb = layer == 0 ? CB2 : CB3;
-#pragma unroll
-    for (int o = 0; o < NH; ++o) acc[o] = sb[pb + o];
-    for (int i = 0; i < NH; ++i) {
-      const float hi = h[i];
-#pragma unroll
-      for (int o = 0; o < NH; ++o) acc[o] += hi * bf16_to_f(sw[pw + i * NH + o]);
-    }
-#pragma unroll
-    for (int o = 0; o < NH; ++o) h[o] = fmaxf(acc[o], 0.f);
+  for (int t = 0; t < 4; ++t) {
+    const int f = 4 * g + t;  // feature index
+    float v = 0.f;
+    if (f < 3) v = feat[3 * c + f];
+    else if (f == 3 + kernel) v = 1.f;
+    x[t] = f2bf(v);
   }
-  float z[NOUT];
+  bf16x4 h1[4], h2[4];
 #pragma unroll
-  for (int o = 0; o < NOUT; ++o) z[o] = sb[CB4 + o];
-  for (int i = 0; i < NH; ++i) {
-    const float hi = h[i];
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) z[o] += hi * bf16_to_f(sw[P_W4 + i * NOUT + o]);
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w1[mt], x, acc, 0, 0, 0);
+    h1[mt] = act_tile(acc, sb + 0, mt, g);
   }
-  float mx = z[0];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w2[mt][kt], h1[kt], acc, 0, 0, 0);
+    h2[mt] = act_tile(acc, sb + NH, mt, g);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w3[mt][kt], h2[kt], acc, 0, 0, 0);
+    h1[mt] = act_tile(acc, sb + 2 * NH, mt, g);
+  }
+  f32x4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kt = 0; kt < 4; ++kt) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w4[kt], h1[kt], z, 0, 0, 0);
+  // classes 4g + v; lanes g = 0 hold classes 0..3 of pair c, lanes g = 1 classes 4..7
+  float zz[8];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const float mine = z[v] + (g < 2 ? sb[3 * NH + 4 * g + v] : 0.f);
+    const float other = __shfl_xor(mine, 16, 64);
+    zz[v] = g == 0 ? mine : other;
+    zz[4 + v] = g == 0 ? other : mine;
+  }
+  float mx = zz[0];
   int best = 0;
 #pragma unroll
   for (int o = 1; o < NOUT; ++o)
-    if (z[o] > mx) {
-      mx = z[o];
+    if (zz[o] > mx) {
+      mx = zz[o];
       best = o;
     }
-  float s = 0.f;
+  float sum = 0.f;
 #pragma unroll
-  for (int o = 0; o < NOUT; ++o) s += expf(z[o] - mx);
-  *conf = 1.f / s;
-  return best;
+  for (int o = 0; o < NOUT; ++o) sum += expf(zz[o] - mx);
+  *cls = best;
+  *conf = 1.f / sum;
 }
 
 // one block per (job, row): rule class of every pair of the row
@@ -386,50 +433,144 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
                                                           const int *__restrict__ far_table,
                                                           const int *__restrict__ npts_of_cls, double near_ratio,
                                                           float min_conf, unsigned long long *__restrict__ stats) {
-  __shared__ uint16_t sw[NPARAM];
-  __shared__ float sb[NBIAS];
-  __shared__ float sh[CLS_THREADS * CLS_HSTRIDE];
+  __shared__ float sb[3 * NH + NOUT];                 // biases b1, b2, b3, b4
   __shared__ int sfar[ANDR_FAR_BINS];
+  __shared__ int qe[CLS_WAVES][CLS_Q];                // queued near pairs: element
+  __shared__ float qf[CLS_WAVES][CLS_Q][3];           // and its three geometric features
+  __shared__ int qres[CLS_WAVES][64][2];              // class per kernel of a processed batch
+  __shared__ float qconf[CLS_WAVES][64][2];
   __shared__ unsigned long long red[3];
-  float *h = sh + threadIdx.x * CLS_HSTRIDE;
-  for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sw[i] = wbf[i];
-  for (int i = threadIdx.x; i < NBIAS; i += blockDim.x)
-    sb[i] = i < CB2 ? wf[P_B1 + i] : (i < CB3 ? wf[P_B2 + i - CB2] : (i < CB4 ? wf[P_B3 + i - CB3] : wf[P_B4 + i - CB4]));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  for (int i = threadIdx.x; i < NH; i += blockDim.x) {
+    sb[i] = wf[P_B1 + i];
+    sb[NH + i] = wf[P_B2 + i];
+    sb[2 * NH + i] = wf[P_B3 + i];
+  }
+  if (threadIdx.x < NOUT) sb[3 * NH + threadIdx.x] = wf[P_B4 + threadIdx.x];
   if (threadIdx.x < ANDR_FAR_BINS) sfar[threadIdx.x] = far_table[threadIdx.x];
   if (threadIdx.x < 3) red[threadIdx.x] = 0ull;
+  MlpTiles T;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    T.w1[mt] = load_tile(wbf + P_W1, NIN, NH, mt, 0, g, c);
+    T.w4[mt] = load_tile(wbf + P_W4, NH, NOUT, 0, mt, g, c);
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+      T.w2[mt][kt] = load_tile(wbf + P_W2, NH, NH, mt, kt, g, c);
+      T.w3[mt][kt] = load_tile(wbf + P_W3, NH, NH, mt, kt, g, c);
+    }
+  }
   __syncthreads();
   const int2 jr = rowmap[blockIdx.x];
   const SysJob &J = jobs[jr.x];
-  const BodyGeom &g = J.g;
-  const int r = jr.y, ne = g.ne;
-  const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
-  const double ns[3] = {g.nrm[3 * r], g.nrm[3 * r + 1], g.nrm[3 * r + 2]};
+  const BodyGeom &g_ = J.g;
+  const int r = jr.y, ne = g_.ne;
+  const double s[3] = {g_.cen[3 * r], g_.cen[3 * r + 1], g_.cen[3 * r + 2]};
+  const double ns[3] = {g_.nrm[3 * r], g_.nrm[3 * r + 1], g_.nrm[3 * r + 2]};
+  uint8_t *crow = J.cls + (size_t)r * ne;
   unsigned long long pts = 0, nl = 0, nf = 0;
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-    uint8_t cls = NCLS_DUN;
-    if (e != r) {
-      const double *x1 = g.xyz + 3 * g.con[3 * e], *x2 = g.xyz + 3 * g.con[3 * e + 1], *x3 = g.xyz + 3 * g.con[3 * e + 2];
-      const double n[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
-      const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g.area[e]);
-      if (pg.dh >= near_ratio) {
-        cls = (uint8_t)sfar[far_bin(pg.dh, near_ratio)];
+  int qn = 0;  // queue length (wave-uniform)
+  // classify the first `cnt` queued pairs (all lanes take part: the matrix cores need a full wave)
+  auto run_batch = [&](int cnt) {
+    for (int blk = 0; blk < 4; ++blk) {
+      if (16 * blk >= cnt) break;  // uniform
+      const float *feat = &qf[wid][16 * blk][0];
+      int a, b;
+      float ca, cb;
+      mlp_block(T, sb, feat, 1, g, c, &a, &ca);
+      mlp_block(T, sb, feat, 2, g, c, &b, &cb);
+      if (g == 0) {
+        qres[wid][16 * blk + c][0] = a;
+        qres[wid][16 * blk + c][1] = b;
+        qconf[wid][16 * blk + c][0] = ca;
+        qconf[wid][16 * blk + c][1] = cb;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt) {
+      const int a = qres[wid][lane][0], b = qres[wid][lane][1];
+      const float ca = qconf[wid][lane][0], cb = qconf[wid][lane][1];
+      uint8_t cl;
+      if (ca < min_conf || cb < min_conf || a == NCLS_DUN || b == NCLS_DUN) {
+        cl = NCLS_DUN;
+        ++nf;
       } else {
-        float f[NIN], cg, ch;
-        features(pg, 1, f);
-        const int a = mlp_classify(f, sw, sb, h, &cg);
-        features(pg, 2, f);
-        const int b = mlp_classify(f, sw, sb, h, &ch);
-        if (cg < min_conf || ch < min_conf || a == NCLS_DUN || b == NCLS_DUN) {
-          cls = NCLS_DUN;
-          ++nf;
+        cl = (uint8_t)max(a, b);
+        ++nl;
+      }
+      pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
+      crow[qe[wid][lane]] = cl;
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+  for (int base = wid * 64; base < ne; base += CLS_THREADS) {
+    const int e = base + lane;
+    bool near = false;
+    float f3[3] = {0.f, 0.f, 0.f};
+    if (e < ne) {
+      if (e == r) {
+        crow[e] = NCLS_DUN;
+      } else {
+        const double *x1 = g_.xyz + 3 * g_.con[3 * e], *x2 = g_.xyz + 3 * g_.con[3 * e + 1],
+                     *x3 = g_.xyz + 3 * g_.con[3 * e + 2];
+        const double n[3] = {g_.nrm[3 * e], g_.nrm[3 * e + 1], g_.nrm[3 * e + 2]};
+        const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g_.area[e]);
+        if (pg.dh >= near_ratio) {
+          const int cl = sfar[far_bin(pg.dh, near_ratio)];
+          crow[e] = (uint8_t)cl;
+          pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
         } else {
-          cls = (uint8_t)max(a, b);
-          ++nl;
+          near = true;
+          float f[NIN];
+          features(pg, 0, f);
+          f3[0] = f[0];
+          f3[1] = f[1];
+          f3[2] = f[2];
         }
       }
-      pts += cls == NCLS_DUN ? J.nq : npts_of_cls[cls];
     }
-    J.cls[(size_t)r * ne + e] = cls;
+    const unsigned long long m = __ballot(near);
+    const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
+    if (near) {
+      qe[wid][pos] = e;
+      qf[wid][pos][0] = f3[0];
+      qf[wid][pos][1] = f3[1];
+      qf[wid][pos][2] = f3[2];
+    }
+    qn += __popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    if (qn >= 64) {
+      run_batch(64);
+      qn -= 64;
+      // move the rest (< 64) to the front
+      int te = 0;
+      float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+      if (lane < qn) {
+        te = qe[wid][64 + lane];
+        t0 = qf[wid][64 + lane][0];
+        t1 = qf[wid][64 + lane][1];
+        t2 = qf[wid][64 + lane][2];
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane < qn) {
+        qe[wid][lane] = te;
+        qf[wid][lane][0] = t0;
+        qf[wid][lane][1] = t1;
+        qf[wid][lane][2] = t2;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (qn > 0) {
+    // unused queue slots of the last batch get harmless features
+    if (lane >= qn) {
+      qf[wid][lane][0] = 0.f;
+      qf[wid][lane][1] = 0.f;
+      qf[wid][lane][2] = 1.f;
+    }
+    __builtin_amdgcn_wave_barrier();
+    run_batch(qn);
   }
   atomicAdd(&red[0], pts);
   atomicAdd(&red[1], nl);

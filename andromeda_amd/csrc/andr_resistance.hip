@@ -4,7 +4,8 @@
 //
 // Plain FP64 path: FP64 assembly and storage, column-scaled LSQR (the reference solver) with the
 // six right-hand sides in one block. Mixed path: FP32 regular quadrature, FP64 singular blocks,
-// FP32 storage, batched restarted GMRES with FP64 vectors and residuals. Bodies are processed in
+// FP32 storage of A M^-1 (M: 3x3 self blocks, a right block-Jacobi preconditioner), batched
+// restarted GMRES with FP64 vectors and residuals, x = M^-1 y at the end. Bodies are processed in
 // waves sized to the free device memory; each wave is one assembly launch for all its bodies.
 #include "andr_internal.h"
 #include "andr_assemble.h"
@@ -13,6 +14,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -99,20 +102,23 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
   const bool plain = opts->precision == ANDR_PREC_FP64;
   const bool learned = opts->precision == ANDR_PREC_LEARNED;
   const size_t esz = plain ? sizeof(double) : sizeof(float);
-  const double eps = opts->eps > 0.0 ? opts->eps : (plain ? 1e-12 : 1e-10);
+  // fast forms: GMRES relative residual 1e-8 leaves tensor errors near 1e-9, far below the 1e-6 target
+  const double eps = opts->eps > 0.0 ? opts->eps : (plain ? 1e-12 : 1e-8);
   const double extra = exterior ? 8.0 * ANDR_PI : 0.0;
   if (reports) std::memset(reports, 0, sizeof(andr_resistance_report) * nbodies);
+  const bool timing = getenv("ANDR_DEBUG_TIMING") != nullptr;  // per-wave host timing on stderr
 
   // memory plan: matrix + vectors (+ GMRES basis) per body, waves within a budget
   size_t free_b = 0, total_b = 0;
   ANDR_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+  free_b += ctx->pool.cached;  // the context's cached blocks are reusable
   // long cycles: slender bodies need 150-400 Krylov steps and short restarts stagnate on them
   const int restart = 128;
   const size_t budget = (size_t)(0.80 * (double)free_b);
   auto body_bytes = [&](int b) -> size_t {
     const size_t n = 3 * (size_t)meshes[b].nelem;
     const size_t lda = (n + 63) / 64 * 64;
-    size_t s = n * lda * esz + 4 * n * ANDR_NRHS * sizeof(double);
+    size_t s = (plain ? n * lda * esz : andr_tiled_elems((int)n) * sizeof(float)) + 4 * n * ANDR_NRHS * sizeof(double);
     if (learned) s += (size_t)meshes[b].nelem * meshes[b].nelem;  // rule class per pair
     s += plain ? 10 * n * ANDR_NRHS * sizeof(double) : (size_t)(restart + 4) * n * ANDR_NRHS * sizeof(double);
     s += (size_t)meshes[b].nnodes * 200 + (size_t)meshes[b].nelem * 300;
@@ -128,7 +134,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     // ---- upload the bodies of the wave -----------------------------------------------------------
     std::vector<std::unique_ptr<Prepared>> P(nb);
     std::vector<DevBuf<char>> Abuf(nb);
-    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ebuf(nb), Dbuf(nb);
+    std::vector<DevBuf<double>> Bbuf(nb), Xbuf(nb), Ebuf(nb), Dbuf(nb), Pbuf(nb);
     std::vector<DevBuf<uint8_t>> Mbuf(nb);
     std::vector<DevBuf<uint8_t>> Cbuf(nb);
     std::vector<SysJob> jobs(nb);
@@ -160,15 +166,16 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         }
       }
       const int n = 3 * p.ne;
-      const int64_t lda = (n + 63) / 64 * 64;
-      if (Abuf[i].alloc((size_t)n * lda * esz) || Bbuf[i].alloc((size_t)n * ANDR_NRHS) ||
-          Xbuf[i].alloc((size_t)n * ANDR_NRHS) || Ebuf[i].alloc(p.ne))
+      const int64_t lda = plain ? (n + 63) / 64 * 64 : 0;  // fast forms: FP32 tiled layout
+      if (Abuf[i].alloc(plain ? (size_t)n * lda * esz : andr_tiled_elems(n) * sizeof(float), ctx) || Bbuf[i].alloc((size_t)n * ANDR_NRHS, ctx) ||
+          Xbuf[i].alloc((size_t)n * ANDR_NRHS, ctx) || Ebuf[i].alloc(p.ne, ctx))
         return andr_fail(ctx, ANDR_ERR_ALLOC, "resistance system of body %d (n = %d)", b0 + i, n);
       if (!exterior) {  // bounded form: only the nodes of wall 2 move (src/lib_particle.f90:220-256, 386-409)
         std::vector<uint8_t> mv(p.nn);
         for (int k = 0; k < p.nn; ++k) mv[k] = p.node_wall[k] == 2 ? 1 : 0;
-        if (Mbuf[i].alloc(p.nn)) return andr_fail(ctx, ANDR_ERR_ALLOC, "moving-node flags");
-        ANDR_HIP(ctx, hipMemcpy(Mbuf[i].p, mv.data(), p.nn, hipMemcpyHostToDevice));
+        if (Mbuf[i].alloc(p.nn, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "moving-node flags");
+        ANDR_HIP(ctx, hipMemcpyAsync(Mbuf[i].p, mv.data(), p.nn, hipMemcpyHostToDevice, ctx->stream));
+        ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
       }
       const QuadRule qr = andr_rule(p.body.itri);
       SysJob &J = jobs[i];
@@ -185,22 +192,28 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       J.err_rows = Ebuf[i].p;
       J.cls = nullptr;
       J.defl = nullptr;
+      J.minv = nullptr;
+      J.X = Xbuf[i].p;
       if (!plain) {
+        if (Pbuf[i].alloc(9 * (size_t)p.ne, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "preconditioner blocks");
+        J.minv = Pbuf[i].p;
         std::vector<double> dh;
         deflation_scales(m, nm.data(), opts->mu, dh);
-        if (Dbuf[i].alloc(dh.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "deflation scales");
-        ANDR_HIP(ctx, hipMemcpy(Dbuf[i].p, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice));
+        if (Dbuf[i].alloc(dh.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "deflation scales");
+        ANDR_HIP(ctx, hipMemcpyAsync(Dbuf[i].p, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice, ctx->stream));
+        ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         J.defl = Dbuf[i].p;
       }
       if (learned) {
-        if (Cbuf[i].alloc((size_t)p.ne * p.ne)) return andr_fail(ctx, ANDR_ERR_ALLOC, "rule classes of body %d", b0 + i);
+        if (Cbuf[i].alloc((size_t)p.ne * p.ne, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "rule classes of body %d", b0 + i);
         J.cls = Cbuf[i].p;
       }
       for (int r = 0; r < p.ne; ++r) rowmap.push_back(make_int2(i, r));
     }
+    const double tu1 = andr_now();  // bodies uploaded
     DevBuf<SysJob> djobs;
     DevBuf<int2> drow;
-    if (djobs.alloc(nb) || drow.alloc(rowmap.size())) return andr_fail(ctx, ANDR_ERR_ALLOC, "job tables");
+    if (djobs.alloc(nb, ctx) || drow.alloc(rowmap.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "job tables");
     ANDR_HIP(ctx, hipMemcpyAsync(djobs.p, jobs.data(), sizeof(SysJob) * nb, hipMemcpyHostToDevice, ctx->stream));
     ANDR_HIP(ctx, hipMemcpyAsync(drow.p, rowmap.data(), sizeof(int2) * rowmap.size(), hipMemcpyHostToDevice, ctx->stream));
     std::vector<LearnedStats> ls(nb, LearnedStats{0, 0, 0});
@@ -210,6 +223,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       alg_bytes += 9.0 * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
       alg_flops += ne * (ne - 1.0) * jobs[i].nq * ANDR_FLOPS_PER_STOKES_EVAL;
     }
+    if (!plain && (rc = andr_launch_self_blocks(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
     if (learned) rc = andr_launch_stokes_system_learned(ctx, jobs.data(), djobs.p, drow.p, (int)rowmap.size(), nb, ls.data());
     else
       rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain, alg_bytes, alg_flops);
@@ -222,7 +236,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       for (int i = 0; i < nb; ++i) {
         const int n = 3 * P[i]->ne;
         DevBuf<double> q;
-        if (q.alloc(n)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scale");
+        if (q.alloc(n, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scale");
         rc = andr_colscale(ctx, Abuf[i].p, false, jobs[i].lda, n, n, q.p);
         if (rc) return rc;
         ANDR_HIP(ctx, hipMemsetAsync(Xbuf[i].p, 0, sizeof(double) * n * ANDR_NRHS, ctx->stream));
@@ -236,6 +250,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
                             iters.data());
       if (rc) return rc;
+      if ((rc = andr_launch_unprecondition(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
     }
     ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double ts1 = andr_now();
@@ -285,6 +300,9 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         R.pairs_fallback = learned ? ls[i].fallback : 0;
       }
     }
+    if (timing)
+      fprintf(stderr, "[andr] wave of %d bodies: upload %.3f s, assembly %.3f s, solve %.3f s, force/torque %.3f s\n",
+              nb, tu1 - ta0, ta1 - tu1, ts1 - ta1, andr_now() - ts1);
     b0 = b1;
   }
   return ANDR_OK;

@@ -34,8 +34,19 @@ int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m
 // Restarted GMRES on a batch of square systems (FP32 or FP64 storage, FP64 vectors); restart
 // cycles recompute the FP64 residual b - A x. Stops per system when every active column reaches
 // ||r|| <= tol ||b||. X holds the result (zero start). iters[s*nrhs + k] gets the iteration count.
+// Fast-path system matrix layout: FP32 in 16 x 16 tiles, tile (rt, kb) holding rows 16 rt..16 rt+15
+// and columns 16 kb..16 kb+15 row-major, tiles of a row band contiguous, KB = ceil(n / 16) tiles
+// per band and ceil(n / 64) * 4 bands (the product reads 64-row bands). One wave instruction of the
+// matrix-core product then reads one whole 1 KB tile.
+__host__ __device__ inline size_t andr_tiled_off(int row, int col, int KB) {
+  return (((size_t)(row >> 4) * KB + (col >> 4)) * 16 + (row & 15)) * 16 + (col & 15);
+}
+__host__ __device__ inline size_t andr_tiled_elems(int n) {
+  return (size_t)((n + 63) / 64 * 64) * (size_t)((n + 15) / 16 * 16);
+}
+
 struct GmresSys {
-  const void *A;
+  const void *A;    // FP32: tiled layout (andr_tiled_off); FP64: row-major with leading dimension lda
   int64_t lda;
   int n;
   const double *B;  // n x ANDR_NB

@@ -136,7 +136,8 @@ def main():
     for s in range(args.steps):
         ts = time.perf_counter()
         results = ctx.resistance_batch(bodies, precision=prec)
-        log(f"[bench] step {s}: {time.perf_counter() - ts:.2f} s")
+        log(f"[bench] step {s}: {time.perf_counter() - ts:.2f} s (upload + assembly "
+            f"{sum(r.t_assemble for r in results):.2f} s, solve {sum(r.t_solve for r in results):.2f} s)")
     elapsed = time.perf_counter() - t0
     worst = sorted(range(len(results)), key=lambda i: -int(results[i].nit.max()))[:4]
     log("[bench] most iterations: " + ", ".join(f"body {i} F={bodies[i].nelem} its={int(results[i].nit.max())}"

@@ -106,3 +106,15 @@ def bf16_round(p):
     u = np.asarray(p, dtype=np.float32).view(np.uint32).astype(np.uint64)
     u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
     return u.astype(np.uint32).view(np.float32)
+
+
+def forward_bf16(p, X):
+    """Logits as the in-assembly classifier computes them: inputs, weights and the activations
+    between layers rounded to bfloat16 (nearest even), biases and accumulation in float (here
+    float64, which differs from the device's float32 sums only in rounding)."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
+    r = lambda a: bf16_round(np.asarray(a, dtype=np.float32)).astype(np.float64)  # noqa: E731
+    H = r(X)
+    for W, b in ((W1, b1), (W2, b2), (W3, b3)):
+        H = r(np.maximum(H @ r(W) + b, 0.0))
+    return H @ r(W4) + b4

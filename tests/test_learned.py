@@ -197,7 +197,6 @@ def test_gpu_labels_match_oracle(ctx, orc, shape):
 def _expected_classes(m, orc, model, mesh_nq=126):
     """Oracle decisions of the in-assembly classifier for every centroid-row pair of a body."""
     x, nrm, area, cen = QL.geometry(m.xyz, m.con)
-    pb = MO.bf16_round(model["params"])
     near, conf, far = float(model["near_ratio"]), float(model["confidence"]), model["far_table"]
     pts = learned = fallback = 0
     feats, where = [], []
@@ -213,11 +212,8 @@ def _expected_classes(m, orc, model, mesh_nq=126):
                 feats += [QL.features(dh, asp, cs, 1), QL.features(dh, asp, cs, 2)]
                 where.append((r, e))
     if feats:
-        # weights rounded to BF16 as in the kernel; biases stay FP32
-        W = MO.unpack(pb.astype(np.float64))
-        Wf = MO.unpack(np.asarray(model["params"], np.float64))
-        q = MO.pack([W[0], Wf[1], W[2], Wf[3], W[4], Wf[5], W[6], Wf[7]])
-        Z, _ = MO.forward(q, np.array(feats, dtype=np.float32).astype(np.float64))
+        # BF16 inputs, weights and activations as on the matrix cores; biases FP32
+        Z = MO.forward_bf16(np.asarray(model["params"], np.float64), np.array(feats, dtype=np.float32))
         P = MO.softmax(Z)
         c = P.argmax(1)
         pmax = P.max(1)
