@@ -12,7 +12,7 @@
 struct SysJob {
   BodyGeom g;
   void *A_;
-  int64_t lda;         // row-major leading dimension; 0: FP32 tiled layout (andr_tiled_off)
+  int64_t lda;         // row-major leading dimension; 0: the six symmetric blocks (andr_sym_off)
   double *B;                // 3 ne x ANDR_NRHS right-hand sides (problems 0..5, the rest 0)
   const uint8_t *moving;    // per node: 1 if it carries the rigid-body velocity (null: all nodes)
   double center[3];         // rotation centre of problems 3..5

@@ -539,8 +539,8 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   for (int i = 0; i < NACC; ++i) acc[i] = 0.0;
   const double inv_mu = 1.0 / J.mu;
   ST *A = J.template Aptr<ST>();
-  const bool tiled = J.lda == 0;
-  const int KB = (3 * ne + 15) / 16;
+  const bool tiled = J.lda == 0;  // symmetric-block layout: S_q = sc G_q only (no null-space term, no M^-1)
+  const int KB = (ne + 15) / 16;
   ST *arow0 = A + (tiled ? 0 : (size_t)(0 * ne + r) * J.lda);
   ST *arow1 = A + (tiled ? 0 : (size_t)(1 * ne + r) * J.lda);
   ST *arow2 = A + (tiled ? 0 : (size_t)(2 * ne + r) * J.lda);
@@ -568,32 +568,31 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
                               g.area[e], qoff, nq, Gs, Tv);
     }
     const double sc = -m * inv_mu;
-    // rank-one null-space term d_w A_r n_r n_e^T between elements of one closed wall
-    double dfl = 0.0;
-    if (J.defl && g.bcid[e] == g.bcid[r]) dfl = J.defl[g.bcid[r]] * g.area[r];
-    const double ne3[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
-    const double nr3[3] = {dfl * g.nrm[3 * r], dfl * g.nrm[3 * r + 1], dfl * g.nrm[3 * r + 2]};
-    ST *ar[3] = {arow0, arow1, arow2};
-    double blk[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int en = 0; en < 3; ++en) blk[c][en] = sc * Gs[sym3(c, en)] + nr3[c] * ne3[en];
-    if (J.minv) {  // right preconditioner: block (r, e) times minv[e]
-      const double *mi = J.minv + 9 * (size_t)e;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double b0 = blk[c][0], b1 = blk[c][1], b2 = blk[c][2];
-#pragma unroll
-        for (int en = 0; en < 3; ++en) blk[c][en] = b0 * mi[en] + b1 * mi[3 + en] + b2 * mi[6 + en];
-      }
-    }
     if (tiled) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) A[andr_sym_off(q, r, e, KB)] = (ST)(sc * Gs[q]);
+    } else {
+      // full 3 x 3 block, optionally with the rank-one null-space term d_w A_r n_r n_e^T between
+      // elements of one closed wall and the right preconditioner M^-1 (block (r, e) times minv[e])
+      double dfl = 0.0;
+      if (J.defl && g.bcid[e] == g.bcid[r]) dfl = J.defl[g.bcid[r]] * g.area[r];
+      const double ne3[3] = {g.nrm[3 * e], g.nrm[3 * e + 1], g.nrm[3 * e + 2]};
+      const double nr3[3] = {dfl * g.nrm[3 * r], dfl * g.nrm[3 * r + 1], dfl * g.nrm[3 * r + 2]};
+      ST *ar[3] = {arow0, arow1, arow2};
+      double blk[3][3];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int en = 0; en < 3; ++en) A[andr_tiled_off(c * ne + r, en * ne + e, KB)] = (ST)blk[c][en];
-    } else {
+        for (int en = 0; en < 3; ++en) blk[c][en] = sc * Gs[sym3(c, en)] + nr3[c] * ne3[en];
+      if (J.minv) {
+        const double *mi = J.minv + 9 * (size_t)e;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double b0 = blk[c][0], b1 = blk[c][1], b2 = blk[c][2];
+#pragma unroll
+          for (int en = 0; en < 3; ++en) blk[c][en] = b0 * mi[en] + b1 * mi[3 + en] + b2 * mi[6 + en];
+        }
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll

@@ -44,6 +44,13 @@ struct GSys {  // device view of one system
   int ntiles;
   int nseg;      // column segments of the matrix-core product (1: written straight to W)
   double *P;     // nseg x n x NB partial products when nseg > 1
+  // structured Stokes operator (fp32 path)
+  StokesOp op;
+  int nep;       // op.ne rounded up to 32
+  float *Z32;    // 3 x NB x nep: z = M^-1 q_j in FP32, [component][rhs][element]
+  double *Z64;   // 3 ne x NB: z = M^-1 x in FP64 (residual)
+  double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
+  int et0, net;  // element tiles (32 elements) of the system in the current map
 };
 
 struct WaveMap {
@@ -119,191 +126,275 @@ __global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, con
   if (part && lane < 8) part[(size_t)wave * NB + lane] = y2;
 }
 
-// The Krylov product W = A Q_j on the matrix cores: FP32 A (as stored), Q_j rounded to FP32,
-// v_mfma_f32_16x16x4f32 with the 8 right-hand sides in the 16 B columns (8 unused). A wave owns 64
-// rows as four 16-row tiles; lane (g = lane / 16, c = lane % 16) feeds row 16 T + c of tile T with
-// columns k0 + 4 g + t (one float4 per tile and 16 columns), and column c of B with the same k.
-// The FP32 accumulators are added into FP64 every 64 columns, which keeps the summation error at
-// the size of the FP32 storage error. Output lane layout: D[4 g + v][c] in element v of tile T.
 typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int MAXW = ANDR_OP_MAXW;
 
-__global__ __launch_bounds__(256) void kb_gemv_mfma(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
-                                                    int nwaves, int jq, const double *__restrict__ sxy) {
+// ---- structured Stokes operator --------------------------------------------------------------
+// per tile of 32 elements (256 threads = 32 elements x 8 right-hand sides): z_e = M^-1_e v_e for the
+// three components of element e, v = Q_j (mode 0, z to Z32 in FP32) or v = X (mode 1, z to Z64 in
+// FP64); partial sums of n_e . z_e per wall (1..MAXW) and rhs for the null-space term.
+__global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, const TileMap *__restrict__ emap, int mode,
+                                               int jq, double *__restrict__ part_s) {
+  __shared__ double red[4][MAXW][NB];
+  const TileMap tm = emap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const StokesOp &O = S.op;
+  const int ne = O.ne, n = S.n;
+  const int k = threadIdx.x & 7, el = threadIdx.x >> 3;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int e = tm.row0 + el;
+  const double *v = mode == 0 ? S.Q + (size_t)jq * n * NB : S.X;
+  double dz = 0.0;
+  int w = 0;
+  if (e < ne) {
+    const double q0 = v[(size_t)(0 * ne + e) * NB + k], q1 = v[(size_t)(1 * ne + e) * NB + k],
+                 q2 = v[(size_t)(2 * ne + e) * NB + k];
+    const double *mi = O.minv + 9 * (size_t)e;
+    const double z[3] = {mi[0] * q0 + mi[1] * q1 + mi[2] * q2, mi[3] * q0 + mi[4] * q1 + mi[5] * q2,
+                         mi[6] * q0 + mi[7] * q1 + mi[8] * q2};
+    if (mode == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) S.Z32[((size_t)c * NB + k) * S.nep + e] = (float)z[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) S.Z64[(size_t)(c * ne + e) * NB + k] = z[c];
+    }
+    w = O.bcid[e];
+    dz = O.nrm[3 * e] * z[0] + O.nrm[3 * e + 1] * z[1] + O.nrm[3 * e + 2] * z[2];
+  } else if (mode == 0 && e < S.nep) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) S.Z32[((size_t)c * NB + k) * S.nep + e] = 0.f;
+  }
+#pragma unroll
+  for (int ww = 0; ww < MAXW; ++ww) {
+    double a = w == ww + 1 ? dz : 0.0;
+    a += __shfl_xor(a, 8, 64);
+    a += __shfl_xor(a, 16, 64);
+    a += __shfl_xor(a, 32, 64);
+    if (lane < 8) red[wid][ww][lane] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < MAXW * NB) {
+    const int ww = threadIdx.x / NB, kk = threadIdx.x % NB;
+    part_s[((size_t)blockIdx.x * MAXW + ww) * NB + kk] = red[0][ww][kk] + red[1][ww][kk] + red[2][ww][kk] + red[3][ww][kk];
+  }
+}
+
+// thread per (system, wall, rhs): sdef = sum of the system's tile partials
+__global__ void kb_sred(const GSys *__restrict__ sys, int nsys, const double *__restrict__ part_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * MAXW * NB) return;
+  const int s = t / (MAXW * NB), r = t % (MAXW * NB);
+  const GSys &S = sys[s];
+  double a = 0.0;
+  for (int i = 0; i < S.net; ++i) a += part_s[(size_t)(S.et0 + i) * MAXW * NB + r];
+  S.sdef[r] = a;
+}
+
+// null-space term of row (element r, component c) for rhs k
+__device__ inline double defl_term(const StokesOp &O, const double *sdef, int r, int c, int k) {
+  if (!O.defl) return 0.0;
+  const int w = O.bcid[r];
+  if (w < 1 || w > MAXW) return 0.0;
+  const double d = O.defl[w];
+  return d != 0.0 ? d * O.area[r] * O.nrm[3 * r + c] * sdef[(w - 1) * NB + k] : 0.0;
+}
+
+// The Krylov product W = A M^-1 Q_j on the matrix cores: y_c = sum_en S_{c,en} z_en (+ the null-space
+// term), S the six symmetric FP32 blocks, z = M^-1 q_j rounded to FP32, v_mfma_f32_16x16x4f32 with
+// the 8 right-hand sides in the 16 B columns (8 unused). A wave owns 32 element rows (two 16-row
+// bands) over a column segment; lane (g = lane / 16, c = lane % 16) feeds row 16 T + c with columns
+// 16 kb + 4 g + t of every block (one float4 per block and band: 1 KB contiguous per instruction)
+// and column c of B with the same columns. Nine block products reuse six blocks. The FP32
+// accumulators are added into FP64 every 64 columns. Output lane layout: D[4 g + v][c].
+__global__ __launch_bounds__(256) void kb_gemv_sym(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
+                                                   int nwaves) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
   const MWave wm = wmap[wave];
   const GSys &S = sys[wm.sys];
-  const int n = S.n, row0 = wm.row0, kend = min(wm.k1, n);
-  const int KB = (n + 15) / 16;
-  const float *A = (const float *)S.A;
-  const double *X = S.Q + (size_t)jq * n * NB;
+  const StokesOp &O = S.op;
+  const int ne = O.ne, row0 = wm.row0, kend = min(wm.k1, ne), KBe = (ne + 15) / 16;
   const int g = lane >> 4, c = lane & 15;
   const bool cl = c < NB;
-  // tiled layout: the float4 of lane (g, c) in tile (band, kb) sits at 256 (band KB + kb) + 16 c + 4 g;
-  // ap[T] + 16 k0 addresses it for the k-block k0 / 16 (16 floats per column of tiles)
-  const float *ap[4];
+  const float *ab[2];
 #pragma unroll
-  for (int T = 0; T < 4; ++T) ap[T] = A + (size_t)((row0 >> 4) + T) * KB * 256 + 16 * c + 4 * g;
-  const double *xp = X + (size_t)(4 * g) * NB + (cl ? c : 0);
-  f4v acc[4];
-  double acc64[4][4];
+  for (int T = 0; T < 2; ++T) ab[T] = O.S + (size_t)((row0 >> 4) + T) * KBe * 1536 + 16 * c + 4 * g;
+  const float *zb[3];
 #pragma unroll
-  for (int T = 0; T < 4; ++T) {
-    acc[T] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int en = 0; en < 3; ++en) zb[en] = S.Z32 + ((size_t)en * NB + (cl ? c : 0)) * S.nep + 4 * g;
+  f4v acc[2][3];
+  double acc64[2][3][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc64[T][v] = 0.0;
-  }
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int co = 0; co < 3; ++co) {
+      acc[T][co] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc64[T][co][v] = 0.0;
+    }
   auto flush = [&]() {
 #pragma unroll
-    for (int T = 0; T < 4; ++T) {
+    for (int T = 0; T < 2; ++T)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc64[T][v] += (double)acc[T][v];
-      acc[T] = f4v{0.f, 0.f, 0.f, 0.f};
-    }
+      for (int co = 0; co < 3; ++co) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc64[T][co][v] += (double)acc[T][co][v];
+        acc[T][co] = f4v{0.f, 0.f, 0.f, 0.f};
+      }
   };
-  int k0 = wm.k0, step = 0;
-  for (; k0 + 32 <= kend; k0 += 32) {
-    float4 a0[4], a1[4];
+  int kb = wm.k0 >> 4, step = 0;
+  for (; 16 * kb + 16 <= kend; ++kb) {
+    float4 a[2][6], z[3];
 #pragma unroll
-    for (int T = 0; T < 4; ++T) {
-      a0[T] = *reinterpret_cast<const float4 *>(ap[T] + (size_t)16 * k0);
-      a1[T] = *reinterpret_cast<const float4 *>(ap[T] + (size_t)16 * (k0 + 16));
-    }
-    float b0[4], b1[4];
+    for (int T = 0; T < 2; ++T)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      b0[t] = cl ? (float)xp[(size_t)(k0 + t) * NB] : 0.f;
-      b1[t] = cl ? (float)xp[(size_t)(k0 + 16 + t) * NB] : 0.f;
-    }
+      for (int q = 0; q < 6; ++q) a[T][q] = *reinterpret_cast<const float4 *>(ab[T] + (size_t)kb * 1536 + q * 256);
 #pragma unroll
-    for (int T = 0; T < 4; ++T) {
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].x, b0[0], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].y, b0[1], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].z, b0[2], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[T].w, b0[3], acc[T], 0, 0, 0);
-    }
+    for (int en = 0; en < 3; ++en)
+      z[en] = cl ? *reinterpret_cast<const float4 *>(zb[en] + 16 * kb) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int T = 0; T < 4; ++T) {
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].x, b1[0], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].y, b1[1], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].z, b1[2], acc[T], 0, 0, 0);
-      acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[T].w, b1[3], acc[T], 0, 0, 0);
-    }
-    if (++step & 1) continue;
-    flush();
+    for (int T = 0; T < 2; ++T)
+#pragma unroll
+      for (int co = 0; co < 3; ++co)
+#pragma unroll
+        for (int en = 0; en < 3; ++en) {
+          const float4 av = a[T][sym3(co, en)];
+          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, z[en].x, acc[T][co], 0, 0, 0);
+          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, z[en].y, acc[T][co], 0, 0, 0);
+          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, z[en].z, acc[T][co], 0, 0, 0);
+          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, z[en].w, acc[T][co], 0, 0, 0);
+        }
+    if ((++step & 3) == 0) flush();
   }
-  // last columns of the segment (fewer than 32): masked, 16 at a time
-  for (; k0 < kend; k0 += 16) {
-    float a[4][4], b[4];
+  if (16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
+    float a[2][6][4], z[3][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const bool in = k0 + 4 * g + t < kend;
-      b[t] = (cl && in) ? (float)xp[(size_t)(k0 + t) * NB] : 0.f;
+      const bool in = 16 * kb + 4 * g + t < kend;
 #pragma unroll
-      for (int T = 0; T < 4; ++T) a[T][t] = in ? ap[T][(size_t)16 * k0 + t] : 0.f;
+      for (int en = 0; en < 3; ++en) z[en][t] = (in && cl) ? zb[en][16 * kb + t] : 0.f;
+#pragma unroll
+      for (int T = 0; T < 2; ++T)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a[T][q][t] = in ? ab[T][(size_t)kb * 1536 + q * 256 + t] : 0.f;
     }
 #pragma unroll
-    for (int T = 0; T < 4; ++T)
+    for (int T = 0; T < 2; ++T)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][t], b[t], acc[T], 0, 0, 0);
+      for (int co = 0; co < 3; ++co)
+#pragma unroll
+        for (int en = 0; en < 3; ++en)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][sym3(co, en)][t], z[en][t], acc[T][co], 0, 0, 0);
   }
   flush();
   if (cl) {
     const bool direct = S.nseg <= 1;
-    const double sx = direct ? sxy[(size_t)wm.sys * 2 * NB + c] : 1.0;
-    double *Y = direct ? S.W : S.P + (size_t)wm.seg * n * NB;
+    double *Y = direct ? S.W : S.P + (size_t)wm.seg * S.n * NB;
 #pragma unroll
-    for (int T = 0; T < 4; ++T)
+    for (int T = 0; T < 2; ++T)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = row0 + 16 * T + 4 * g + v;
-        if (row < n) Y[(size_t)row * NB + c] = sx * acc64[T][v];
+        const int r = row0 + 16 * T + 4 * g + v;
+        if (r >= ne) continue;
+#pragma unroll
+        for (int co = 0; co < 3; ++co) {
+          double y = acc64[T][co][v];
+          if (wm.seg == 0) y += defl_term(O, S.sdef, r, co, c);
+          Y[(size_t)(co * ne + r) * NB + c] = y;
+        }
       }
   }
 }
 
-// Y = sx .* (A X) + sy .* Y on the tiled FP32 layout with FP64 products and sums (the residual of
-// the refinement loop): wave per 64-row band, lane (g, c) covers row 16 T + c and columns 4 g..4 g+3
-// of every 16-column block; the four column groups are summed across lanes at the end.
-// part[wave][k] = sum over the band's rows of Y^2.
-__global__ __launch_bounds__(256) void kb_resid_tiled(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap,
-                                                      int nwaves, const double *__restrict__ sxy,
-                                                      double *__restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wave >= nwaves) return;
-  const WaveMap wm = wmap[wave];
-  const GSys &S = sys[wm.sys];
-  const int n = S.n, row0 = wm.row0, KB = (n + 15) / 16;
+// R = B - A M^-1 X for the structured operator in FP64 (the residual of the refinement loop), from
+// z = M^-1 X already in Z64. Block of 4 waves per 64 element rows of one system (one 16-row band per
+// wave); the 16 x 3 x 8 values of z for the current column block are staged in LDS and shared.
+// Lane (g, c) covers row c of its band and columns 4 g..4 g+3; the column groups are summed across
+// lanes at the end. part[4 block + wave][k] = sum over the band's rows of R^2 (rows of all three
+// components).
+__global__ __launch_bounds__(256) void kb_resid_sym(const GSys *__restrict__ sys, const WaveMap *__restrict__ bmap,
+                                                    double *__restrict__ part) {
+  __shared__ double zc[3][16][NB];
+  const WaveMap bm = bmap[blockIdx.x];
+  const GSys &S = sys[bm.sys];
+  const StokesOp &O = S.op;
+  const int ne = O.ne, KBe = (ne + 15) / 16;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
-  const float *A = (const float *)S.A;
-  double acc[4][NB];
+  const int band = (bm.row0 >> 4) + wid;
+  const float *ab = O.S + (size_t)band * KBe * 1536 + 16 * c + 4 * g;
+  double acc[3][NB];
 #pragma unroll
-  for (int T = 0; T < 4; ++T)
+  for (int co = 0; co < 3; ++co)
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc[T][k] = 0.0;
-  for (int kb = 0; kb < KB; ++kb) {
-    double x[4][NB];
+    for (int k = 0; k < NB; ++k) acc[co][k] = 0.0;
+  const bool band_in = 16 * band < ne;  // all waves take part in the staging
+  for (int kb = 0; kb < KBe; ++kb) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * 16 * NB; i += blockDim.x) {
+      const int en = i / (16 * NB), rem = i % (16 * NB), col = rem / NB, k = rem % NB;
+      const int e = 16 * kb + col;
+      zc[en][col][k] = e < ne ? S.Z64[(size_t)(en * ne + e) * NB + k] : 0.0;
+    }
+    __syncthreads();
+    if (!band_in) continue;
+    double a[6][4];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const float4 v = *reinterpret_cast<const float4 *>(ab + (size_t)kb * 1536 + q * 256);
+      const int c0 = 16 * kb + 4 * g;
+      a[q][0] = c0 < ne ? (double)v.x : 0.0;
+      a[q][1] = c0 + 1 < ne ? (double)v.y : 0.0;
+      a[q][2] = c0 + 2 < ne ? (double)v.z : 0.0;
+      a[q][3] = c0 + 3 < ne ? (double)v.w : 0.0;
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int col = 16 * kb + 4 * g + t;
-      if (col < n) {
-        const double4 x0 = *reinterpret_cast<const double4 *>(S.X + (size_t)col * NB);
-        const double4 x1 = *reinterpret_cast<const double4 *>(S.X + (size_t)col * NB + 4);
-        x[t][0] = x0.x; x[t][1] = x0.y; x[t][2] = x0.z; x[t][3] = x0.w;
-        x[t][4] = x1.x; x[t][5] = x1.y; x[t][6] = x1.z; x[t][7] = x1.w;
-      } else {
+      const int col = 4 * g + t;
 #pragma unroll
-        for (int k = 0; k < NB; ++k) x[t][k] = 0.0;
+      for (int k = 0; k < NB; ++k) {
+        const double z0 = zc[0][col][k], z1 = zc[1][col][k], z2 = zc[2][col][k];
+        acc[0][k] += a[0][t] * z0 + a[1][t] * z1 + a[2][t] * z2;
+        acc[1][k] += a[1][t] * z0 + a[3][t] * z1 + a[4][t] * z2;
+        acc[2][k] += a[2][t] * z0 + a[4][t] * z1 + a[5][t] * z2;
       }
     }
-#pragma unroll
-    for (int T = 0; T < 4; ++T) {
-      const float4 a = *reinterpret_cast<const float4 *>(A + ((size_t)((row0 >> 4) + T) * KB + kb) * 256 + 16 * c + 4 * g);
-      // columns past n are padding of the last block (not written by the assembly): masked
-      const int c0 = 16 * kb + 4 * g;
-      const double av[4] = {c0 < n ? (double)a.x : 0.0, c0 + 1 < n ? (double)a.y : 0.0,
-                            c0 + 2 < n ? (double)a.z : 0.0, c0 + 3 < n ? (double)a.w : 0.0};
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int k = 0; k < NB; ++k) acc[T][k] += av[t] * x[t][k];
-    }
   }
-  // sum the four column groups (lanes c, c + 16, c + 32, c + 48)
 #pragma unroll
-  for (int T = 0; T < 4; ++T)
+  for (int co = 0; co < 3; ++co)
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-      acc[T][k] += __shfl_xor(acc[T][k], 16, 64);
-      acc[T][k] += __shfl_xor(acc[T][k], 32, 64);
+      acc[co][k] += __shfl_xor(acc[co][k], 16, 64);
+      acc[co][k] += __shfl_xor(acc[co][k], 32, 64);
     }
-  // lane (g, c) writes row 16 g + c (tile T = g) for all right-hand sides
-  double y2[NB];
-  const int row = row0 + 16 * g + c;
+  double r2[NB];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    double v = acc[0][k];
+  for (int k = 0; k < NB; ++k) r2[k] = 0.0;
+  const int r = 16 * band + c;
+  if (band_in && g == 0 && r < ne) {
 #pragma unroll
-    for (int T = 1; T < 4; ++T) v = g == T ? acc[T][k] : v;
-    y2[k] = 0.0;
-    if (row < n) {
-      double *py = S.R + (size_t)row * NB + k;
-      const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
-      const double y = sx * v + (sy != 0.0 ? sy * (*py) : 0.0);
-      *py = y;
-      y2[k] = y * y;
-    }
+    for (int co = 0; co < 3; ++co)
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const double y = acc[co][k] + defl_term(O, S.sdef, r, co, k);
+        double *pr = S.R + (size_t)(co * ne + r) * NB + k;
+        const double v = *pr - y;  // R holds B
+        *pr = v;
+        r2[k] += v * v;
+      }
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k)
-    for (int off = 1; off < 64; off <<= 1) y2[k] += __shfl_xor(y2[k], off, 64);
+    for (int off = 1; off < 64; off <<= 1) r2[k] += __shfl_xor(r2[k], off, 64);
   if (lane < NB) {
-    double v = y2[0];
+    double v = r2[0];
 #pragma unroll
-    for (int k = 1; k < NB; ++k) v = lane == k ? y2[k] : v;
-    part[(size_t)wave * NB + lane] = v;
+    for (int k = 1; k < NB; ++k) v = lane == k ? r2[k] : v;
+    part[((size_t)blockIdx.x * 4 + wid) * NB + lane] = v;
   }
 }
 
@@ -577,82 +668,113 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   const int m = std::max(4, restart);
   hipStream_t st = ctx->stream;
   // full work maps (sizes the partial-sum buffers); subsets are rebuilt as systems converge
-  size_t vec_elems = 0, basis_elems = 0;
-  int nw_full = 0, nt_full = 0;
+  // fp32: the structured Stokes operator of the fast path; otherwise a dense FP64 matrix
+  const bool structured = fp32;
+  size_t vec_elems = 0, basis_elems = 0, z32_elems = 0;
+  int nw_full = 0, nt_full = 0, net_full = 0;
   for (int s = 0; s < nsys; ++s) {
-    vec_elems += (size_t)in[s].n * NB;
-    basis_elems += (size_t)(m + 1) * in[s].n * NB;
-    nw_full += (in[s].n + WROWS - 1) / WROWS;
-    nt_full += (in[s].n + TILE - 1) / TILE;
+    const int n = in[s].n, ne = in[s].op.ne;
+    if (structured && (!in[s].op.S || !in[s].op.minv || 3 * ne != n))
+      return andr_fail(ctx, ANDR_ERR_ARG, "structured GMRES system %d is incomplete", s);
+    vec_elems += (size_t)n * NB;
+    basis_elems += (size_t)(m + 1) * n * NB;
+    nw_full += structured ? 4 * ((ne + 63) / 64) : (n + WROWS - 1) / WROWS;
+    nt_full += (n + TILE - 1) / TILE;
+    net_full += structured ? (ne + 31) / 32 : 0;
+    z32_elems += structured ? (size_t)3 * NB * ((ne + 31) / 32 * 32) : 0;
   }
-  const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1) + 8);
-  DevBuf<double> R, W, Q, SM, partW, partT, partT2;
+  // per system: Hessenberg, rotations, g, coefficients; the per-rhs state (8 doubles per rhs) of all
+  // systems sits in one separate array so that the host reads it with one copy
+  const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1));
+  const size_t state_per = (size_t)NB * 8;
+  DevBuf<double> R, W, Q, SM, ST8, partW, partT, partT2, Z64, SDEF, partS;
+  DevBuf<float> Z32;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
   if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
-      SM.alloc(small_per * nsys, ctx) || partW.alloc((size_t)nw_full * NB, ctx) ||
+      SM.alloc(small_per * nsys, ctx) || ST8.alloc(state_per * nsys, ctx) || partW.alloc((size_t)nw_full * NB, ctx) ||
       partT.alloc((size_t)nt_full * (m + 1) * NB, ctx) || partT2.alloc((size_t)nt_full * NB, ctx) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
+  if (structured && (Z64.alloc(vec_elems, ctx) || Z32.alloc(z32_elems, ctx) ||
+                     SDEF.alloc((size_t)nsys * MAXW * NB, ctx) || partS.alloc((size_t)net_full * MAXW * NB, ctx)))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES operator workspace");
   // a map set over a list of systems: waves of WROWS rows (FP64-accumulating product), tiles of
   // TILE rows (vector kernels) and matrix-core waves (64 rows x a column segment). When the list
   // has few rows (the last, slowest systems of a batch) the columns are split into segments so
   // that the product still fills the chip; the segments are summed by kb_seg_sum.
   constexpr int MF_TARGET_WAVES = 2048, MF_MAX_SEG = 8;
+  // structured operator: w = residual blocks of 64 element rows (4 partial entries each), mw =
+  // matrix-core waves of 32 element rows, et = element tiles of 32 (z = M^-1 v and wall sums)
   struct MapSet {
     DevBuf<WaveMap> w;
-    DevBuf<TileMap> t;
+    DevBuf<TileMap> t, et;
     DevBuf<MWave> mw;
     DevBuf<int> w0, nw;
-    int n_w = 0, n_t = 0, n_mw = 0, max_seg = 1;
+    int n_w = 0, n_t = 0, n_et = 0, n_mw = 0, max_seg = 1;
     size_t mw_cap = 0;
-    std::vector<int> tile0, ntiles, nseg;
+    std::vector<int> tile0, ntiles, et0, net, nseg;
     std::vector<size_t> poff;
     size_t p_elems = 0;
   };
   MapSet cyc, live;
   for (MapSet *ms : {&cyc, &live})
-    if (ms->w.alloc(nw_full, ctx) || ms->t.alloc(nt_full, ctx) || ms->w0.alloc(nsys, ctx) || ms->nw.alloc(nsys, ctx))
+    if (ms->w.alloc(nw_full, ctx) || ms->t.alloc(nt_full, ctx) || ms->w0.alloc(nsys, ctx) || ms->nw.alloc(nsys, ctx) ||
+        (structured && ms->et.alloc(net_full, ctx)))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES work maps");
   std::vector<GSys> hs(nsys);
   DevBuf<double> Pseg;
   size_t pseg_cap = 0;
   auto build = [&](const std::vector<int> &list, MapSet &ms) -> int {
     std::vector<WaveMap> wm;
-    std::vector<TileMap> tm;
+    std::vector<TileMap> tm, em;
     std::vector<MWave> mw;
     std::vector<int> w0(nsys, 0), nw(nsys, 0);
     ms.tile0.assign(nsys, 0);
     ms.ntiles.assign(nsys, 0);
+    ms.et0.assign(nsys, 0);
+    ms.net.assign(nsys, 0);
     ms.nseg.assign(nsys, 1);
     ms.poff.assign(nsys, 0);
     ms.p_elems = 0;
     ms.max_seg = 1;
     int row_waves = 0;
-    for (int s : list) row_waves += (in[s].n + WROWS - 1) / WROWS;
+    for (int s : list) row_waves += structured ? (in[s].op.ne + 31) / 32 : 0;
     const int want = row_waves > 0 ? std::min(MF_MAX_SEG, (MF_TARGET_WAVES + row_waves - 1) / row_waves) : 1;
     for (int s : list) {
-      const int n = in[s].n;
-      w0[s] = (int)wm.size();
-      for (int r = 0; r < n; r += WROWS) wm.push_back({s, r});
-      nw[s] = (int)wm.size() - w0[s];
+      const int n = in[s].n, ne = in[s].op.ne;
+      if (structured) {
+        w0[s] = 4 * (int)wm.size();
+        for (int r = 0; r < ne; r += 64) wm.push_back({s, r});
+        nw[s] = 4 * (int)wm.size() - w0[s];
+      } else {
+        w0[s] = (int)wm.size();
+        for (int r = 0; r < n; r += WROWS) wm.push_back({s, r});
+        nw[s] = (int)wm.size() - w0[s];
+      }
       ms.tile0[s] = (int)tm.size();
       for (int r = 0; r < n; r += TILE) tm.push_back({s, r});
       ms.ntiles[s] = (int)tm.size() - ms.tile0[s];
-      const int seglen = ((n + want - 1) / want + 31) / 32 * 32;
-      const int nseg = (n + seglen - 1) / seglen;
+      if (!structured) continue;
+      ms.et0[s] = (int)em.size();
+      for (int e = 0; e < ne; e += 32) em.push_back({s, e});
+      ms.net[s] = (int)em.size() - ms.et0[s];
+      const int seglen = ((ne + want - 1) / want + 31) / 32 * 32;
+      const int nseg = (ne + seglen - 1) / seglen;
       ms.nseg[s] = nseg;
       ms.max_seg = std::max(ms.max_seg, nseg);
       if (nseg > 1) {
         ms.poff[s] = ms.p_elems;
         ms.p_elems += (size_t)nseg * n * NB;
       }
-      for (int r = 0; r < n; r += WROWS)
-        for (int sg = 0; sg < nseg; ++sg) mw.push_back({s, r, sg * seglen, std::min(n, (sg + 1) * seglen), sg});
+      for (int r = 0; r < ne; r += 32)
+        for (int sg = 0; sg < nseg; ++sg) mw.push_back({s, r, sg * seglen, std::min(ne, (sg + 1) * seglen), sg});
     }
     ms.n_w = (int)wm.size();
     ms.n_t = (int)tm.size();
+    ms.n_et = (int)em.size();
     ms.n_mw = (int)mw.size();
+    if (ms.n_et) ANDR_HIP(ctx, hipMemcpyAsync(ms.et.p, em.data(), sizeof(TileMap) * em.size(), hipMemcpyHostToDevice, st));
     if (mw.size() > ms.mw_cap) {
       if (ms.mw.alloc(mw.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES matrix-core map");
       ms.mw_cap = mw.size();
@@ -674,16 +796,26 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     for (int s = 0; s < nsys; ++s) {
       hs[s].tile0 = ms.tile0[s];
       hs[s].ntiles = ms.ntiles[s];
+      hs[s].et0 = ms.et0[s];
+      hs[s].net = ms.net[s];
       hs[s].nseg = ms.nseg[s];
       hs[s].P = ms.nseg[s] > 1 ? Pseg.p + ms.poff[s] : nullptr;
     }
     ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
     return ANDR_OK;
   };
-  size_t vo = 0, bo = 0;
+  size_t vo = 0, bo = 0, zo = 0;
   for (int s = 0; s < nsys; ++s) {
     GSys &g = hs[s];
     const int n = in[s].n;
+    g.op = in[s].op;
+    if (structured) {
+      g.nep = (g.op.ne + 31) / 32 * 32;
+      g.Z32 = Z32.p + zo;
+      g.Z64 = Z64.p + vo;
+      g.sdef = SDEF.p + (size_t)s * MAXW * NB;
+      zo += (size_t)3 * NB * g.nep;
+    }
     g.A = in[s].A;
     g.lda = in[s].lda;
     g.n = n;
@@ -702,12 +834,12 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     g.g = sp;
     sp += NB * (m + 1);
     g.coef = sp;
-    sp += NB * (m + 1);
-    g.small = sp;
+    g.small = ST8.p + state_per * s;
     vo += (size_t)n * NB;
     bo += (size_t)(m + 1) * n * NB;
   }
   ANDR_HIP(ctx, hipMemsetAsync(SM.p, 0, sizeof(double) * small_per * nsys, st));
+  ANDR_HIP(ctx, hipMemsetAsync(ST8.p, 0, sizeof(double) * state_per * nsys, st));
   for (int s = 0; s < nsys; ++s) ANDR_HIP(ctx, hipMemsetAsync(in[s].X, 0, sizeof(double) * in[s].n * NB, st));
   // scale factors: residual product (sx = -1, sy = 1) and Arnoldi product (sx = 1, sy = 0)
   std::vector<double> h_res(2 * NB * (size_t)nsys), h_arn(2 * NB * (size_t)nsys);
@@ -724,40 +856,45 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
   const int gs = (nsys * NB + 63) / 64;
+  const int gsd = (nsys * MAXW * NB + 255) / 256;
+  // which_x 0: residual R = B - A X (R holds B); 1: Krylov product W = A Q_j
   auto gemv = [&](const MapSet &ms, int which_x, int jq, int which_y, const double *scales) {
     if (ms.n_w == 0) return;
     double n2 = 0.0, n1 = 0.0;
     for (int s = 0; s < nsys; ++s)
       if (ms.ntiles[s]) {
-        n2 += (double)in[s].n * in[s].n;
+        const double ne = in[s].op.ne;
+        n2 += structured ? 6.0 * ne * ne : (double)in[s].n * in[s].n;
         n1 += in[s].n;
       }
-    const bool mfma = fp32 && which_x == 1 && which_y == 1;
-    const int tok = andr_prof_begin(ctx, mfma ? "gmres_gemv_mfma" : (fp32 ? "gmres_gemv_fp32" : "gmres_gemv_fp64"),
-                                    n2 * (fp32 ? 4.0 : 8.0) + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
-    const int gw = (ms.n_w + 3) / 4;
-    if (mfma) {
-      hipLaunchKernelGGL(kb_gemv_mfma, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw, jq, scales);
-      if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
-    } else if (fp32)  // the residual of the refinement loop: X into R
-      hipLaunchKernelGGL(kb_resid_tiled, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, scales, partW.p);
-    else
-      hipLaunchKernelGGL(kb_gemv<double>, dim3(gw), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq, which_y,
-                         scales, partW.p);
+    if (structured) {
+      const bool krylov = which_x == 1;
+      const int tok = andr_prof_begin(ctx, krylov ? "gmres_gemv_mfma" : "gmres_resid_fp64",
+                                      n2 * 4.0 + 2.0 * n1 * NB * 8.0, 2.0 * 1.5 * n2 * nrhs);
+      hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
+      hipLaunchKernelGGL(kb_sred, dim3(gsd), dim3(256), 0, st, dsys.p, nsys, partS.p);
+      if (krylov) {
+        hipLaunchKernelGGL(kb_gemv_sym, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
+        if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
+      } else {
+        hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
+      }
+      andr_prof_end(ctx, tok);
+      return;
+    }
+    const int tok = andr_prof_begin(ctx, "gmres_gemv_fp64", n2 * 8.0 + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
+    hipLaunchKernelGGL(kb_gemv<double>, dim3((ms.n_w + 3) / 4), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq,
+                       which_y, scales, partW.p);
     andr_prof_end(ctx, tok);
   };
   std::vector<int> it_count((size_t)nsys * NB, 0);
-  std::vector<double> hsmall(small_per * nsys);
+  std::vector<double> hsmall(state_per * nsys);
   auto read_small = [&]() -> int {
-    for (int s = 0; s < nsys; ++s) {
-      hipError_t e = hipMemcpyAsync(hsmall.data() + small_per * s, hs[s].small, sizeof(double) * NB * 8,
-                                    hipMemcpyDeviceToHost, st);
-      if (e != hipSuccess) return andr_fail(ctx, ANDR_ERR_HIP, "GMRES state copy: %s", hipGetErrorString(e));
-    }
+    ANDR_HIP(ctx, hipMemcpyAsync(hsmall.data(), ST8.p, sizeof(double) * state_per * nsys, hipMemcpyDeviceToHost, st));
     ANDR_HIP(ctx, hipStreamSynchronize(st));
     return ANDR_OK;
   };
-  auto sm_of = [&](int s, int k) { return hsmall.data() + small_per * s + k * 8; };
+  auto sm_of = [&](int s, int k) { return hsmall.data() + state_per * s + k * 8; };
   std::vector<int> cyc_list(nsys);
   for (int s = 0; s < nsys; ++s) cyc_list[s] = s;
   int rc = build(cyc_list, cyc);

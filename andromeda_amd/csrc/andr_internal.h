@@ -15,6 +15,11 @@
 
 #define ANDR_PI 3.141592653589793238462643383
 
+// index of component (i, j) in the symmetric storage {xx, xy, xz, yy, yz, zz}
+__host__ __device__ constexpr int sym3(int i, int j) {
+  return (i < j ? i : j) == 0 ? (i > j ? i : j) : ((i < j ? i : j) == 1 ? i + j + 1 : 5);
+}
+
 // Device memory cache of a context. Blocks released during a call stay with the context for the
 // next call instead of going back to the driver: hipFree synchronises the device and mapping or
 // unmapping large blocks costs milliseconds each, which is most of a batched call's host time

@@ -13,10 +13,6 @@ static inline int andr_upload_rules_here(andr_ctx *ctx, const double *tab, const
   return ANDR_OK;
 }
 
-// index of component (i, j) in the symmetric storage {xx, xy, xz, yy, yz, zz}
-__host__ __device__ constexpr int sym3(int i, int j) {
-  return (i < j ? i : j) == 0 ? (i > j ? i : j) : ((i < j ? i : j) == 1 ? i + j + 1 : 5);
-}
 
 template <class CT>
 __device__ inline void stokes_pair_regular(const double s[3], const double *x1, const double *x2, const double *x3,

@@ -62,7 +62,7 @@ void deflation_scales(const andr_mesh *m, const double *nm, double mu, std::vect
       edges[w].push_back({key, i < j ? 1 : -1});
     }
   }
-  for (int w = 1; w <= nw; ++w) {
+  for (int w = 1; w <= std::min(nw, ANDR_OP_MAXW); ++w) {  // the operator carries the term for walls 1..4
     auto &E = edges[w];
     if (E.empty()) continue;
     std::sort(E.begin(), E.end());
@@ -118,7 +118,8 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
   auto body_bytes = [&](int b) -> size_t {
     const size_t n = 3 * (size_t)meshes[b].nelem;
     const size_t lda = (n + 63) / 64 * 64;
-    size_t s = (plain ? n * lda * esz : andr_tiled_elems((int)n) * sizeof(float)) + 4 * n * ANDR_NRHS * sizeof(double);
+    size_t s = (plain ? n * lda * esz : andr_sym_elems(meshes[b].nelem) * sizeof(float)) +
+               4 * n * ANDR_NRHS * sizeof(double);
     if (learned) s += (size_t)meshes[b].nelem * meshes[b].nelem;  // rule class per pair
     s += plain ? 10 * n * ANDR_NRHS * sizeof(double) : (size_t)(restart + 4) * n * ANDR_NRHS * sizeof(double);
     s += (size_t)meshes[b].nnodes * 200 + (size_t)meshes[b].nelem * 300;
@@ -166,8 +167,8 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         }
       }
       const int n = 3 * p.ne;
-      const int64_t lda = plain ? (n + 63) / 64 * 64 : 0;  // fast forms: FP32 tiled layout
-      if (Abuf[i].alloc(plain ? (size_t)n * lda * esz : andr_tiled_elems(n) * sizeof(float), ctx) || Bbuf[i].alloc((size_t)n * ANDR_NRHS, ctx) ||
+      const int64_t lda = plain ? (n + 63) / 64 * 64 : 0;  // fast forms: symmetric blocks, FP32
+      if (Abuf[i].alloc(plain ? (size_t)n * lda * esz : andr_sym_elems(p.ne) * sizeof(float), ctx) || Bbuf[i].alloc((size_t)n * ANDR_NRHS, ctx) ||
           Xbuf[i].alloc((size_t)n * ANDR_NRHS, ctx) || Ebuf[i].alloc(p.ne, ctx))
         return andr_fail(ctx, ANDR_ERR_ALLOC, "resistance system of body %d (n = %d)", b0 + i, n);
       if (!exterior) {  // bounded form: only the nodes of wall 2 move (src/lib_particle.f90:220-256, 386-409)
@@ -246,7 +247,11 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
       }
     } else {
       std::vector<GmresSys> gs(nb);
-      for (int i = 0; i < nb; ++i) gs[i] = {Abuf[i].p, jobs[i].lda, 3 * P[i]->ne, Bbuf[i].p, Xbuf[i].p};
+      for (int i = 0; i < nb; ++i) {
+        const BodyGeom &g = P[i]->body.g;
+        StokesOp op{(const float *)Abuf[i].p, P[i]->ne, jobs[i].minv, g.area, g.nrm, g.bcid, jobs[i].defl};
+        gs[i] = {nullptr, 0, 3 * P[i]->ne, op, Bbuf[i].p, Xbuf[i].p};
+      }
       rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
                             iters.data());
       if (rc) return rc;

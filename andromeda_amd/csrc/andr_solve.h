@@ -34,21 +34,37 @@ int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m
 // Restarted GMRES on a batch of square systems (FP32 or FP64 storage, FP64 vectors); restart
 // cycles recompute the FP64 residual b - A x. Stops per system when every active column reaches
 // ||r|| <= tol ||b||. X holds the result (zero start). iters[s*nrhs + k] gets the iteration count.
-// Fast-path system matrix layout: FP32 in 16 x 16 tiles, tile (rt, kb) holding rows 16 rt..16 rt+15
-// and columns 16 kb..16 kb+15 row-major, tiles of a row band contiguous, KB = ceil(n / 16) tiles
-// per band and ceil(n / 64) * 4 bands (the product reads 64-row bands). One wave instruction of the
-// matrix-core product then reads one whole 1 KB tile.
-__host__ __device__ inline size_t andr_tiled_off(int row, int col, int KB) {
-  return (((size_t)(row >> 4) * KB + (col >> 4)) * 16 + (row & 15)) * 16 + (col & 15);
+// Fast-path system matrix: the six distinct Stokeslet blocks S_q(r, e) = -(mul_e / mu) G_q(r, e),
+// q = {xx, xy, xz, yy, yz, zz}, of the centroid-collocation system (the full matrix has 3 x 3 blocks
+// with G_{c,en} = G_{en,c}). FP32 in 16 x 16 tiles: tile (band, kb) of block q holds element rows
+// 16 band.. and columns 16 kb.. row-major; for one (band, kb) the six q tiles are contiguous, and
+// the bands of a 32-row pair follow each other. KBe = ceil(ne / 16); ceil(ne / 32) * 2 bands.
+__host__ __device__ inline size_t andr_sym_off(int q, int r, int e, int KBe) {
+  return ((((size_t)(r >> 4) * KBe + (e >> 4)) * 6 + q) * 16 + (r & 15)) * 16 + (e & 15);
 }
-__host__ __device__ inline size_t andr_tiled_elems(int n) {
-  return (size_t)((n + 63) / 64 * 64) * (size_t)((n + 15) / 16 * 16);
+__host__ __device__ inline size_t andr_sym_elems(int ne) {
+  return (size_t)((ne + 31) / 32 * 32) * (size_t)((ne + 15) / 16 * 16) * 6;
 }
 
+// The structured operator of the fast resistance path, in GMRES's variable y (x = M^-1 y):
+//   A M^-1 y = S z + sum_w d_w (A_r n_r) (sum_{e in w} n_e . z_e),   z = M^-1 y,
+// with S the symmetric Stokeslet blocks above, M the 3 x 3 self blocks (block-Jacobi) and the
+// rank-one null-space term of every closed wall w.
+struct StokesOp {
+  const float *S;            // andr_sym_off layout
+  int ne;
+  const double *minv;        // ne x 9
+  const double *area, *nrm;  // ne, 3 ne
+  const int *bcid;           // ne, wall ids 1..nwalls
+  const double *defl;        // per wall id: d_w (0: no term); walls above ANDR_OP_MAXW carry none
+};
+#define ANDR_OP_MAXW 4
+
 struct GmresSys {
-  const void *A;    // FP32: tiled layout (andr_tiled_off); FP64: row-major with leading dimension lda
+  const void *A;    // FP64 row-major with leading dimension lda (fp32 = false)
   int64_t lda;
   int n;
+  StokesOp op;      // fp32 = true: the structured Stokes operator, n = 3 op.ne
   const double *B;  // n x ANDR_NB
   double *X;        // n x ANDR_NB
 };
