@@ -7,8 +7,9 @@
 // one launch per step covers every system (work lists map waves and row tiles to systems), so the
 // launch count does not grow with the number of bodies.
 //
-// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): both passes are block
-// products over the basis, which keeps the basis traffic streaming instead of latency bound.
+// Orthogonalisation is classical Gram-Schmidt with a second pass where the first one lost more
+// than 1/sqrt(2) of the vector's norm (the DGKS test, decided on the device per system): both
+// passes are block products over the basis, which keeps the basis traffic streaming.
 #include "andr_internal.h"
 #include "andr_solve.h"
 #include "andr_prof.h"
@@ -418,14 +419,18 @@ __global__ __launch_bounds__(256) void kb_seg_sum(const GSys *__restrict__ sys, 
 }
 
 // per (tile, chunk of ICH basis vectors): partial dots part[tile][i][k] = sum_rows Q_i[row][k] W[row][k]
+// Pass 0 also stores sum W^2 of the tile in slot mstride - 1 (the norm before projection, for the
+// reorthogonalisation test); pass 1 is skipped for systems whose test passed.
 __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
-                                               int mstride, double *__restrict__ part) {
+                                               int mstride, double *__restrict__ part, int pass) {
   __shared__ double red[4][ICH][NB];
+  __shared__ double rw[4][NB];
   const int i0 = blockIdx.y * ICH;
   if (i0 > j) return;  // uniform over the block
   const int ni = min(ICH, j + 1 - i0);
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
+  if (pass == 1 && S.small[6] > 0.5) return;  // uniform: second pass not needed
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -452,6 +457,16 @@ __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, con
     a += __shfl_xor(a, 32, 64);
     acc[ii] = a;
   }
+  const bool norm = pass == 0 && blockIdx.y == 0;
+  if (norm) {
+    double a = 0.0;
+#pragma unroll
+    for (int t = 0; t < TILE / 32; ++t) a += w[t] * w[t];
+    a += __shfl_xor(a, 8, 64);
+    a += __shfl_xor(a, 16, 64);
+    a += __shfl_xor(a, 32, 64);
+    if (lane < 8) rw[wid][lane] = a;
+  }
   if (lane < 8)
 #pragma unroll
     for (int ii = 0; ii < ICH; ++ii) red[wid][ii][lane] = acc[ii];
@@ -461,6 +476,31 @@ __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, con
     part[((size_t)blockIdx.x * mstride + i0 + ii) * NB + kk] =
         red[0][ii][kk] + red[1][ii][kk] + red[2][ii][kk] + red[3][ii][kk];
   }
+  if (norm && threadIdx.x < NB)
+    part[((size_t)blockIdx.x * mstride + mstride - 1) * NB + threadIdx.x] =
+        rw[0][threadIdx.x] + rw[1][threadIdx.x] + rw[2][threadIdx.x] + rw[3][threadIdx.x];
+}
+
+// thread per system: the second Gram-Schmidt pass is needed unless, for every live rhs, the
+// projection kept at least 1/sqrt(2) of the norm (Daniel-Gragg-Kaufman-Stewart criterion).
+// small[6] of rhs 0 carries the decision (1: skip the second pass).
+__global__ void kb_reorth_test(const GSys *__restrict__ sys, int nsys, int mstride, const double *__restrict__ part,
+                               const double *__restrict__ part2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const GSys &S = sys[s];
+  bool skip = S.ntiles > 0;
+  for (int k = 0; k < NB && skip; ++k) {
+    const double *sm = S.small + k * 8;
+    if (!(sm[4] > 0.0 && sm[5] < 0.0)) continue;  // rhs not advancing
+    double before = 0.0, after = 0.0;
+    for (int tl = 0; tl < S.ntiles; ++tl) {
+      before += part[((size_t)(S.tile0 + tl) * mstride + mstride - 1) * NB + k];
+      after += part2[(size_t)(S.tile0 + tl) * NB + k];
+    }
+    if (!(after >= 0.5 * before)) skip = false;
+  }
+  S.small[6] = skip ? 1.0 : 0.0;
 }
 
 // thread per (system, i, k): coef = sum over the system's tiles; H(i, j) += coef
@@ -471,6 +511,7 @@ __global__ void kb_dots_final(const GSys *__restrict__ sys, int nsys, int j, int
   if (t >= nsys * per) return;
   const int s = t / per, rem = t % per, i = rem / NB, k = rem % NB;
   const GSys &S = sys[s];
+  if (!first_pass && S.small[6] > 0.5) return;
   double c = 0.0;
   for (int tl = 0; tl < S.ntiles; ++tl) c += part[((size_t)(S.tile0 + tl) * mstride + i) * NB + k];
   S.coef[k * (m + 1) + i] = c;
@@ -480,10 +521,11 @@ __global__ void kb_dots_final(const GSys *__restrict__ sys, int nsys, int j, int
 
 // per tile: W -= sum_i Q_i coef_i ; part2[tile][k] = sum W^2
 __global__ __launch_bounds__(256) void kb_sub(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j, int m,
-                                              double *__restrict__ part2) {
+                                              double *__restrict__ part2, int pass) {
   __shared__ double red[4][NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
+  if (pass == 1 && S.small[6] > 0.5) return;  // uniform: part2 keeps the first pass's norms
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -869,7 +911,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       }
     if (structured) {
       const bool krylov = which_x == 1;
-      const int tok = andr_prof_begin(ctx, krylov ? "gmres_gemv_mfma" : "gmres_resid_fp64",
+      const int tok = andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym",
                                       n2 * 4.0 + 2.0 * n1 * NB * 8.0, 2.0 * 1.5 * n2 * nrhs);
       hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
       hipLaunchKernelGGL(kb_sred, dim3(gsd), dim3(256), 0, st, dsys.p, nsys, partS.p);
@@ -939,7 +981,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
       gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
-      // CGS2: two passes of (coefficients Q^T W, W -= Q coef); each pass streams the basis twice
+      // classical Gram-Schmidt, second pass only where the first lost more than 1/sqrt(2) of the
+      // norm (DGKS); each pass streams the basis twice
       double nlive = 0.0;
       for (int s = 0; s < nsys; ++s)
         if (live.ntiles[s]) nlive += in[s].n;
@@ -948,11 +991,14 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
                                         2.0 * 4.0 * (j + 1) * nlive * NB);
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1,
-                           partT.p);
+                           partT.p, pass);
         const int nthr = nsys * (j + 1) * NB;
         hipLaunchKernelGGL(kb_dots_final, dim3((nthr + 255) / 256), dim3(256), 0, st, dsys.p, nsys, j, m, m + 1,
                            partT.p, pass == 0 ? 1 : 0);
-        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p);
+        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p, pass);
+        if (pass == 0)
+          hipLaunchKernelGGL(kb_reorth_test, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys.p, nsys, m + 1, partT.p,
+                             partT2.p);
       }
       andr_prof_end(ctx, tok_o);
       const int tok_g = andr_prof_begin(ctx, "gmres_givens_scale", 2.0 * vbytes, 0.0);

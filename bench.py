@@ -57,6 +57,18 @@ def stratified(bodies, k, n_fibre):
     return sorted(_spread(range(n_super), bodies, k - kf) + _spread(range(n_super, len(bodies)), bodies, kf))
 
 
+def traffic_ratio(kernel):
+    """HBM bytes / algorithmic bytes of a kernel, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+    runs and recorded in profiles/*.md as 'traffic_ratio: <kernel> <ratio>' (None if absent)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_ratio:" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def cpu_baseline(bodies, budget_s=20.0):
     """Oracle (C, FP64, OpenMP assembly + serial LSQR) on the smallest bodies of the batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -177,10 +189,14 @@ def main():
         name, v = dom
         avg_ms = v["ms"] / v["launches"]
         if "gemv" in name:
-            ach = v["bytes"] / v["launches"] / (avg_ms * 1e-3) / 1e9
+            per_launch = v["bytes"] / v["launches"]
+            ach = per_launch / (avg_ms * 1e-3) / 1e9
+            ratio = traffic_ratio(name)
             roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-                        "avg_launch_ms": round(avg_ms, 4)}
+                        "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": None if ratio is None else round(ratio * per_launch),
+                        "kernel": name, "avg_launch_ms": round(avg_ms, 4),
+                        "algorithmic_bytes_per_launch": round(per_launch)}
         else:
             ach = v["flops"] / v["launches"] / (avg_ms * 1e-3) / 1e12
             peak = FP32_PEAK_TFLOPS if "fp32" in name else FP64_PEAK_TFLOPS
