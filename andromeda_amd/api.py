@@ -32,7 +32,7 @@ _L = C.POINTER(C.c_int64)
 class _Mesh(C.Structure):
     _fields_ = [("nnodes", C.c_int32), ("nelem", C.c_int32), ("nwalls", C.c_int32),
                 ("xyz", _D), ("con", _I), ("bcid", _I), ("normmul", _D),
-                ("index_base", C.c_int32), ("itri", C.c_int32)]
+                ("index_base", C.c_int32), ("itri", C.c_int32), ("nvert", C.c_int32), ("iquad", C.c_int32)]
 
 
 class _LsqrOpts(C.Structure):
@@ -121,13 +121,13 @@ def _pi(a):
 class _MeshView:
     """Holds the arrays referenced by the C struct."""
 
-    def __init__(self, mesh: Mesh, normmul=None, itri: int = 5):
+    def __init__(self, mesh: Mesh, normmul=None, itri: int = 5, iquad: int = 4):
         m = mesh.contiguous()
         self.m = m
         self.normmul = np.ascontiguousarray(normmul if normmul is not None else np.ones(max(m.nwalls, 1)),
                                             dtype=np.float64)
         self.s = _Mesh(m.nnodes, m.nelem, max(m.nwalls, 1), _p(m.xyz), _pi(m.con), _pi(m.bcid),
-                       _p(self.normmul), 0, itri)
+                       _p(self.normmul), 0, itri, int(m.con.shape[1]), iquad)
 
 
 @dataclass
@@ -178,8 +178,8 @@ class Context:
         return rc
 
     # ---- seam 1 ---------------------------------------------------------------------------------
-    def assemble_laplace(self, mesh: Mesh, normmul=None, itri: int = 5, layout: int = ROW_MAJOR):
-        mv = _MeshView(mesh, normmul, itri)
+    def assemble_laplace(self, mesh: Mesh, normmul=None, itri: int = 5, layout: int = ROW_MAJOR, iquad: int = 4):
+        mv = _MeshView(mesh, normmul, itri, iquad)
         nn, ne = mv.m.nnodes, mv.m.nelem
         nsp = nn + ne
         shapeH = (nsp, nn) if layout == ROW_MAJOR else (nn, nsp)
@@ -193,8 +193,9 @@ class Context:
             H, G = H.T, G.T
         return H, G, err
 
-    def assemble_stokes(self, mesh: Mesh, normmul=None, itri: int = 5, extra: float = 0.0, pressure: bool = True):
-        mv = _MeshView(mesh, normmul, itri)
+    def assemble_stokes(self, mesh: Mesh, normmul=None, itri: int = 5, extra: float = 0.0, pressure: bool = True,
+                        iquad: int = 4):
+        mv = _MeshView(mesh, normmul, itri, iquad)
         nn, ne = mv.m.nnodes, mv.m.nelem
         nsp = nn + ne
         T = [np.zeros((nsp, nn)) for _ in range(6)]

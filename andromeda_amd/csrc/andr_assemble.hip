@@ -16,6 +16,8 @@
 #include "andr_pairs.h"
 
 #include <algorithm>
+#include <cmath>
+#include <bitset>
 #include <cstdarg>
 #include <chrono>
 #include <cstring>
@@ -42,7 +44,50 @@ static void expand_wx(const andr_wx_orbit *orb, int norb, std::vector<double> &t
   }
 }
 
+// Gauss-Legendre rules of quadrilaterals (GaussPosWeights, src/mod_gauss.f90:33-391): IQUD 1..8 ->
+// N = 2,4,6,8,16,32,40,48 points per direction, stored one after the other (156 nodes), nodes
+// ascending. Computed at start-up by Newton iteration on the Legendre three-term recurrence.
+static const int k_gl_n[9] = {0, 2, 4, 6, 8, 16, 32, 40, 48};
+static const int k_gl_off[9] = {0, 0, 2, 6, 12, 20, 36, 68, 108};
+static __constant__ double c_gl[156][2];  // node, weight
+static __constant__ int c_gl_n[9], c_gl_off[9];
+
+static void gauss_legendre(int N, double *x, double *w) {
+  for (int i = 0; i < N / 2; ++i) {
+    double z = std::cos(ANDR_PI * (i + 0.75) / (N + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;  // P_0, P_1
+      for (int j = 2; j <= N; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = N * (z * p1 - p0) / (z * z - 1.0);  // P_N'(z)
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    x[N - 1 - i] = z;
+    w[i] = w[N - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
 int andr_upload_rules(andr_ctx *ctx) {
+  {
+    double gl[156][2];
+    for (int iq = 1; iq <= 8; ++iq) {
+      double x[48], w[48];
+      gauss_legendre(k_gl_n[iq], x, w);
+      for (int i = 0; i < k_gl_n[iq]; ++i) {
+        gl[k_gl_off[iq] + i][0] = x[i];
+        gl[k_gl_off[iq] + i][1] = w[i];
+      }
+    }
+    ANDR_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_gl), gl, sizeof(gl)));
+    ANDR_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_gl_n), k_gl_n, sizeof(k_gl_n)));
+    ANDR_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_gl_off), k_gl_off, sizeof(k_gl_off)));
+  }
   std::vector<double> tab;
   expand_wx(andr_wx_deg5, 3, tab);
   expand_wx(andr_wx_deg10, 7, tab);
@@ -58,16 +103,40 @@ int andr_upload_rules(andr_ctx *ctx) {
   return andr_upload_rules_learned(ctx, tab.data(), tf.data());
 }
 
+// bilinear quadrilateral at local coordinates (gi, gj) in [-1, 1]^2 (src/lib_intAna.f90:300-325):
+// shape functions, point, and the unnormalised normal x_ksi x x_eta
+__device__ inline void quad_point(const double *x1, const double *x2, const double *x3, const double *x4, double gi,
+                                  double gj, double fig4[4], double f[3], double an[3]) {
+  const double e1m = 1.0 - gi, e1p = 1.0 + gi, e2m = 1.0 - gj, e2p = 1.0 + gj;
+  fig4[0] = 0.25 * e1m * e2m;
+  fig4[1] = 0.25 * e1p * e2m;
+  fig4[2] = 0.25 * e1p * e2p;
+  fig4[3] = 0.25 * e1m * e2p;
+  double ks[3], et[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    f[j] = fig4[0] * x1[j] + fig4[1] * x2[j] + fig4[2] * x3[j] + fig4[3] * x4[j];
+    ks[j] = 0.25 * (-e2m * x1[j] + e2m * x2[j] + e2p * x3[j] - e2p * x4[j]);
+    et[j] = 0.25 * (-e1m * x1[j] - e1p * x2[j] + e1p * x3[j] + e1m * x4[j]);
+  }
+  an[0] = ks[1] * et[2] - et[1] * ks[2];
+  an[1] = et[0] * ks[2] - ks[0] * et[2];
+  an[2] = ks[0] * et[1] - et[0] * ks[1];
+}
+
 // ------------------------------------------------------------------------------------------------
-// geometry: CalElementNormal (src/lib_mesh.f90:716-769), Heron area (src/mod_trikotnik.f90:786-802),
-// CalQMesh centroid (src/lib_mesh.f90:577-629)
+// geometry: CalElementNormal (src/lib_mesh.f90:716-769, first three nodes for either element type),
+// Heron area (src/mod_trikotnik.f90:786-802) or, for quadrilaterals, the integral of |J| with the
+// N = 8 rule (CalElementArea, src/lib_int.f90:606-721), CalQMesh centre (src/lib_mesh.f90:577-629)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_geometry(int ne, const double *__restrict__ xyz, const int *__restrict__ con,
+__global__ void k_geometry(int ne, int nv, const double *__restrict__ xyz, const int *__restrict__ con,
                            const int *__restrict__ bcid, const double *__restrict__ wallmul, double *__restrict__ nrm,
                            double *__restrict__ area, double *__restrict__ cen, double *__restrict__ mul) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
-  const double *t1 = xyz + 3 * con[3 * e], *t2 = xyz + 3 * con[3 * e + 1], *t3 = xyz + 3 * con[3 * e + 2];
+  const int *cn = con + (size_t)nv * e;
+  const bool quad = nv == 4 && cn[3] >= 0;
+  const double *t1 = xyz + 3 * cn[0], *t2 = xyz + 3 * cn[1], *t3 = xyz + 3 * cn[2];
   double v1[3] = {t2[0] - t1[0], t2[1] - t1[1], t2[2] - t1[2]};
   double v2[3] = {t3[0] - t1[0], t3[1] - t1[1], t3[2] - t1[2]};
   double n0 = v1[1] * v2[2] - v1[2] * v2[1];
@@ -77,30 +146,49 @@ __global__ void k_geometry(int ne, const double *__restrict__ xyz, const int *__
   nrm[3 * e] = n0 / d;
   nrm[3 * e + 1] = n1 / d;
   nrm[3 * e + 2] = n2 / d;
-  auto sq = [](double x) { return x * x; };
-  double a = sqrt(sq(t1[0] - t2[0]) + sq(t1[1] - t2[1]) + sq(t1[2] - t2[2]));
-  double b = sqrt(sq(t3[0] - t2[0]) + sq(t3[1] - t2[1]) + sq(t3[2] - t2[2]));
-  double c = sqrt(sq(t1[0] - t3[0]) + sq(t1[1] - t3[1]) + sq(t1[2] - t3[2]));
-  double p = 0.5 * (a + b + c);
-  area[e] = sqrt(p * (p - a) * (p - b) * (p - c));
-  const double third = 1.0 / 3.0;
-  for (int j = 0; j < 3; ++j) cen[3 * e + j] = third * t1[j] + third * t2[j] + third * t3[j];
+  if (!quad) {
+    auto sq = [](double x) { return x * x; };
+    double a = sqrt(sq(t1[0] - t2[0]) + sq(t1[1] - t2[1]) + sq(t1[2] - t2[2]));
+    double b = sqrt(sq(t3[0] - t2[0]) + sq(t3[1] - t2[1]) + sq(t3[2] - t2[2]));
+    double c = sqrt(sq(t1[0] - t3[0]) + sq(t1[1] - t3[1]) + sq(t1[2] - t3[2]));
+    double p = 0.5 * (a + b + c);
+    area[e] = sqrt(p * (p - a) * (p - b) * (p - c));
+    const double third = 1.0 / 3.0;
+    for (int j = 0; j < 3; ++j) cen[3 * e + j] = third * t1[j] + third * t2[j] + third * t3[j];
+  } else {
+    const double *t4 = xyz + 3 * cn[3];
+    const int off = c_gl_off[4], N = c_gl_n[4];
+    double A = 0.0;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        double fig4[4], f[3], an[3];
+        quad_point(t1, t2, t3, t4, c_gl[off + i][0], c_gl[off + j][0], fig4, f, an);
+        const double aj = c_gl[off + i][1] * c_gl[off + j][1] * sqrt(an[0] * an[0] + an[1] * an[1] + an[2] * an[2]);
+        for (int k = 0; k < 4; ++k) A += aj * fig4[k];
+      }
+    area[e] = A;
+    for (int j = 0; j < 3; ++j) cen[3 * e + j] = 0.25 * t1[j] + 0.25 * t2[j] + 0.25 * t3[j] + 0.25 * t4[j];
+  }
   mul[e] = wallmul[bcid[e] - 1];
 }
 
 // Greedy colouring of elements so that no two elements of a colour share a node.
-static void color_elements(int nn, int ne, const int *con, std::vector<int> &order, std::vector<int> &off) {
-  std::vector<uint64_t> used(nn, 0);
+static int color_elements(int nn, int ne, int nv, const int *con, std::vector<int> &order, std::vector<int> &off) {
+  constexpr int MAXC = 256;  // an element sees at most sum over its nodes of (valence - 1) colours
+  std::vector<std::bitset<MAXC>> used(nn);
   std::vector<int> col(ne);
   int ncol = 0;
   for (int e = 0; e < ne; ++e) {
-    uint64_t m = used[con[3 * e]] | used[con[3 * e + 1]] | used[con[3 * e + 2]];
+    const int *cn = con + (size_t)nv * e;
+    const int nno = (nv == 4 && cn[3] >= 0) ? 4 : 3;
+    std::bitset<MAXC> m;
+    for (int v = 0; v < nno; ++v) m |= used[cn[v]];
     int c = 0;
-    while (c < 64 && (m >> c) & 1ull) ++c;
-    if (c >= 64) c = 63;  // cannot happen for valence < 21; kept in range
+    while (c < MAXC && m[c]) ++c;
+    if (c >= MAXC) return -1;
     col[e] = c;
     ncol = std::max(ncol, c + 1);
-    for (int v = 0; v < 3; ++v) used[con[3 * e + v]] |= (1ull << c);
+    for (int v = 0; v < nno; ++v) used[cn[v]].set(c);
   }
   off.assign(ncol + 1, 0);
   for (int e = 0; e < ne; ++e) off[col[e] + 1]++;
@@ -108,6 +196,7 @@ static void color_elements(int nn, int ne, const int *con, std::vector<int> &ord
   order.resize(ne);
   std::vector<int> pos(off.begin(), off.end() - 1);
   for (int e = 0; e < ne; ++e) order[pos[col[e]]++] = e;
+  return 0;
 }
 
 int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors) {
@@ -115,11 +204,19 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
     return andr_fail(ctx, ANDR_ERR_ARG, "invalid mesh");
   const int nn = m->nnodes, ne = m->nelem;
   const int nw = m->nwalls > 0 ? m->nwalls : 1;
-  std::vector<int> con(3 * (size_t)ne), bcid(ne, 1);
-  for (size_t i = 0; i < 3 * (size_t)ne; ++i) {
+  const int nv = m->nvert == 4 ? 4 : 3;
+  if (m->nvert != 0 && m->nvert != 3 && m->nvert != 4) return andr_fail(ctx, ANDR_ERR_ARG, "nvert must be 0, 3 or 4");
+  std::vector<int> con((size_t)nv * ne), bcid(ne, 1);
+  for (size_t i = 0; i < (size_t)nv * ne; ++i) {
     con[i] = m->con[i] - m->index_base;
+    if (nv == 4 && i % 4 == 3 && con[i] < 0) {  // a triangle in a quadrilateral-capable mesh
+      con[i] = -1;
+      continue;
+    }
     if (con[i] < 0 || con[i] >= nn) return andr_fail(ctx, ANDR_ERR_ARG, "connectivity entry %zu out of range", i);
   }
+  b.iquad = m->iquad == 0 ? 4 : m->iquad;
+  if (b.iquad < 1 || b.iquad > 8) return andr_fail(ctx, ANDR_ERR_ARG, "iquad must be 1..8");
   if (m->bcid)
     for (int e = 0; e < ne; ++e) {
       bcid[e] = m->bcid[e];
@@ -131,20 +228,21 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
   b.itri = m->itri == 0 ? 5 : m->itri;
   if (b.itri < 1 || b.itri > 5) return andr_fail(ctx, ANDR_ERR_ARG, "itri must be 1..5");
   DevBuf<double> d_wm;
-  if (b.xyz.alloc(3 * (size_t)nn, ctx) || b.con.alloc(3 * (size_t)ne, ctx) || b.bcid.alloc(ne, ctx) ||
+  if (b.xyz.alloc(3 * (size_t)nn, ctx) || b.con.alloc((size_t)nv * ne, ctx) || b.bcid.alloc(ne, ctx) ||
       b.nrm.alloc(3 * (size_t)ne, ctx) || b.area.alloc(ne, ctx) || b.cen.alloc(3 * (size_t)ne, ctx) ||
       b.mul.alloc(ne, ctx) || d_wm.alloc(nw, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "device allocation for mesh");
   ANDR_HIP(ctx, hipMemcpyAsync(b.xyz.p, m->xyz, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, ctx->stream));
-  ANDR_HIP(ctx, hipMemcpyAsync(b.con.p, con.data(), sizeof(int) * 3 * ne, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(b.con.p, con.data(), sizeof(int) * nv * ne, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(b.bcid.p, bcid.data(), sizeof(int) * ne, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(d_wm.p, wm.data(), sizeof(double) * nw, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_geometry, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, ne, b.xyz.p, b.con.p, b.bcid.p,
-                     d_wm.p, b.nrm.p, b.area.p, b.cen.p, b.mul.p);
+  hipLaunchKernelGGL(k_geometry, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, ne, nv, b.xyz.p, b.con.p,
+                     b.bcid.p, d_wm.p, b.nrm.p, b.area.p, b.cen.p, b.mul.p);
   ANDR_HIP(ctx, hipGetLastError());
   if (with_colors) {
     std::vector<int> order;
-    color_elements(nn, ne, con.data(), order, b.color_off);
+    if (color_elements(nn, ne, nv, con.data(), order, b.color_off))
+      return andr_fail(ctx, ANDR_ERR_ARG, "mesh node valence too high for the element colouring");
     if (b.color_elems.alloc(ne) || b.d_color_off.alloc(b.color_off.size()))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "device allocation for colours");
     ANDR_HIP(ctx, hipMemcpyAsync(b.color_elems.p, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice, ctx->stream));
@@ -155,6 +253,8 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
   b.g.nn = nn;
   b.g.ne = ne;
   b.g.nwalls = nw;
+  b.g.nv = nv;
+  b.g.iquad = b.iquad;
   b.g.xyz = b.xyz.p;
   b.g.con = b.con.p;
   b.g.bcid = b.bcid.p;
@@ -193,6 +293,122 @@ __device__ inline void block_sum(double (&v)[N], double *sh) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Quadrilateral pairs. Regular: ndx x ndy sub-cells of the bilinear element, N x N Gauss-Legendre
+// points each, the normal evaluated pointwise (Quad_BEMIntNum, src/lib_intAna.f90:219-344, with
+// max(2, nint(longest side / 1)) sub-cells per direction, :263-276; Quad_StokesIntNum,
+// src/lib_Stokes.f90:3754-3940, 3 x 3 sub-cells). Singular: the element split into two triangles
+// with the source vertex first, or four about the centre (si_qua_lap / si_qua_stk,
+// src/mod_singInt.f90:172-387); the analytic triangle integrals of andr_singular.h do the rest.
+// ------------------------------------------------------------------------------------------------
+template <class Fn>
+__device__ inline void quad_cells(const double *x1, const double *x2, const double *x3, const double *x4, double mul,
+                                  int iq, int ndx, int ndy, Fn &&fn) {
+  const int off = c_gl_off[iq], N = c_gl_n[iq];
+  for (int ix = 1; ix <= ndx; ++ix)
+    for (int ie = 1; ie <= ndy; ++ie) {
+      const double a = -1.0 + (ix - 1) * 2.0 / ndx, b = -1.0 + ix * 2.0 / ndx;
+      const double c = -1.0 + (ie - 1) * 2.0 / ndy, d = -1.0 + ie * 2.0 / ndy;
+      const double dex = 0.25 * (b - a) * (d - c);
+      for (int i = 0; i < N; ++i) {
+        const double gi = a + 0.5 * (c_gl[off + i][0] + 1.0) * (b - a);
+        for (int j = 0; j < N; ++j) {
+          const double gj = c + 0.5 * (c_gl[off + j][0] + 1.0) * (d - c);
+          double fig4[4], f[3], an[3];
+          quad_point(x1, x2, x3, x4, gi, gj, fig4, f, an);
+          const double len = sqrt(an[0] * an[0] + an[1] * an[1] + an[2] * an[2]);
+          const double n[3] = {mul * an[0] / len, mul * an[1] / len, mul * an[2] / len};
+          fn(fig4, f, n, dex * c_gl[off + i][1] * c_gl[off + j][1] * len);
+        }
+      }
+    }
+}
+
+__device__ inline double dist3(const double *a, const double *b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// Laplace pair of a quadrilateral: out = {G, H at nodes 1..4}; isrc 1..4 vertex, 5 centre source
+__device__ __noinline__ void quad_laplace_pair(const double *x1, const double *x2, const double *x3,
+                                               const double *x4, const double *s, double mul, int iq, int isrc,
+                                               double *out) {
+  for (int k = 0; k < 5; ++k) out[k] = 0.0;
+  if (isrc == 0) {
+    const double d13 = fmax(dist3(x1, x2), dist3(x3, x4)), d24 = fmax(dist3(x2, x3), dist3(x4, x1));
+    const int ndx = max(2, (int)(d13 + 0.5)), ndy = max(2, (int)(d24 + 0.5));  // minedge = 1
+    const double k4pi = 0.25 / ANDR_PI;
+    quad_cells(x1, x2, x3, x4, mul, iq, ndx, ndy, [&](const double *fig4, const double *f, const double *n, double w) {
+      const double rx = s[0] - f[0], ry = s[1] - f[1], rz = s[2] - f[2];  // r = s - f
+      const double ira = rsqrt(rx * rx + ry * ry + rz * rz);
+      const double fg = k4pi * ira;
+      const double fh = (rx * n[0] + ry * n[1] + rz * n[2]) * ira * ira * fg;
+      out[0] += w * fg;
+      for (int k = 0; k < 4; ++k) out[1 + k] += w * fh * fig4[k];
+    });
+    return;
+  }
+  const double *v[4] = {x1, x2, x3, x4};
+  if (isrc <= 4) {
+    const double *a = v[isrc - 1], *b = v[isrc % 4], *c = v[(isrc + 1) % 4], *d = v[(isrc + 2) % 4];
+    out[0] = andr_sing_lap(a, b, c, a, 1) + andr_sing_lap(a, c, d, a, 1);
+  } else {
+    for (int k = 0; k < 4; ++k) out[0] += andr_sing_lap(s, v[k], v[(k + 1) % 4], s, 1);
+  }
+}
+
+// Stokes pair of a quadrilateral: acc = {G6, prG3, T[6][4], prT[3][4]} (T, prT, prG of a singular
+// pair are zero)
+__device__ __noinline__ void quad_stokes_pair(const double *x1, const double *x2, const double *x3, const double *x4,
+                                              const double *s, double mul, int iq, int isrc, double *acc) {
+  for (int k = 0; k < 45; ++k) acc[k] = 0.0;
+  if (isrc == 0) {
+    quad_cells(x1, x2, x3, x4, mul, iq, 3, 3, [&](const double *fig4, const double *f, const double *n, double w) {
+      const double rx = f[0] - s[0], ry = f[1] - s[1], rz = f[2] - s[2];  // r = f - s
+      const double rxx = rx * rx, rxy = rx * ry, rxz = rx * rz, ryy = ry * ry, ryz = ry * rz, rzz = rz * rz;
+      const double ir2 = 1.0 / (rxx + ryy + rzz);
+      const double ir = sqrt(ir2);
+      const double ir3 = ir2 * ir;
+      acc[0] += w * (ir * (1.0 + rxx * ir2));
+      acc[1] += w * (rxy * ir3);
+      acc[2] += w * (rxz * ir3);
+      acc[3] += w * (ir * (1.0 + ryy * ir2));
+      acc[4] += w * (ryz * ir3);
+      acc[5] += w * (ir * (1.0 + rzz * ir2));
+      const double c2 = 2.0 * ir3;
+      acc[6] += w * (c2 * rx);
+      acc[7] += w * (c2 * ry);
+      acc[8] += w * (c2 * rz);
+      const double ct = -6.0 * ir2 * ir3 * (rx * n[0] + ry * n[1] + rz * n[2]);
+      const double tc[6] = {ct * rxx, ct * rxy, ct * rxz, ct * ryy, ct * ryz, ct * rzz};
+      const double c4 = 4.0 * ir3;
+      const double pt[3] = {c4 * ((-1.0 + 3.0 * rxx * ir2) * n[0] + 3.0 * rxy * ir2 * n[1] + 3.0 * rxz * ir2 * n[2]),
+                            c4 * (3.0 * rxy * ir2 * n[0] + (-1.0 + 3.0 * ryy * ir2) * n[1] + 3.0 * ryz * ir2 * n[2]),
+                            c4 * (3.0 * rxz * ir2 * n[0] + 3.0 * ryz * ir2 * n[1] + (-1.0 + 3.0 * rzz * ir2) * n[2])};
+      for (int v = 0; v < 4; ++v) {
+        const double wv = w * fig4[v];
+        for (int k = 0; k < 6; ++k) acc[9 + 4 * k + v] += wv * tc[k];
+        for (int k = 0; k < 3; ++k) acc[33 + 4 * k + v] += wv * pt[k];
+      }
+    });
+    return;
+  }
+  const double *v[4] = {x1, x2, x3, x4};
+  double g6[6];
+  if (isrc <= 4) {
+    const double *a = v[isrc - 1], *b = v[isrc % 4], *c = v[(isrc + 1) % 4], *d = v[(isrc + 2) % 4];
+    andr_sing_stk(a, b, c, a, 1, g6);
+    for (int k = 0; k < 6; ++k) acc[k] += g6[k];
+    andr_sing_stk(a, c, d, a, 1, g6);
+    for (int k = 0; k < 6; ++k) acc[k] += g6[k];
+  } else {
+    for (int q = 0; q < 4; ++q) {
+      andr_sing_stk(s, v[q], v[(q + 1) % 4], s, 1, g6);
+      for (int k = 0; k < 6; ++k) acc[k] += g6[k];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Laplace rows: sdIntRowU / sdIntRowQ (src/lib_sdBem.f90:438-733), regular integrals by
 // Triangle_3DLapInt (:859-905) with BemKernel (:911-936), singular by the analytic formula.
 // ------------------------------------------------------------------------------------------------
@@ -220,8 +436,23 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
   for (int c = 0; c < ncolors; ++c) {
     for (int idx = coff[c] + threadIdx.x; idx < coff[c + 1]; idx += blockDim.x) {
       const int e = celems[idx];
-      const int v0 = g.con[3 * e], v1 = g.con[3 * e + 1], v2 = g.con[3 * e + 2];
+      const int *cn = g.con + (size_t)g.nv * e;
+      const int v0 = cn[0], v1 = cn[1], v2 = cn[2];
       const double *x1 = g.xyz + 3 * v0, *x2 = g.xyz + 3 * v1, *x3 = g.xyz + 3 * v2;
+      if (g.nv == 4 && cn[3] >= 0) {  // quadrilateral (Quad_BEMIntNumAna, src/lib_intAna.f90:153-176)
+        const int v3 = cn[3];
+        int qs = 0;
+        if (!qrow) qs = (v0 == r) ? 1 : (v1 == r) ? 2 : (v2 == r) ? 3 : (v3 == r) ? 4 : 0;
+        else if (e == erow) qs = 5;
+        double out[5];
+        quad_laplace_pair(x1, x2, x3, g.xyz + 3 * v3, s, g.mul[e], g.iquad, qs, out);
+        grow[e] = out[0];
+        hrow[v0] += out[1];
+        hrow[v1] += out[2];
+        hrow[v2] += out[3];
+        hrow[v3] += out[4];
+        continue;
+      }
       int isrc = 0;
       if (!qrow) isrc = (v0 == r) ? 1 : (v1 == r) ? 2 : (v2 == r) ? 3 : 0;
       else if (e == erow) isrc = 4;
@@ -267,9 +498,11 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
   const double cfree = acc[0];
   if (threadIdx.x == 0) {
     if (!qrow) hrow[r] += cfree;
-    else {
-      const int *cn = g.con + 3 * erow;
-      for (int v = 0; v < 3; ++v) hrow[cn[v]] += (1.0 / 3.0) * cfree;
+    else {  // spread 1/3 (triangle) or 1/4 (quadrilateral) on the element's nodes
+      const int *cn = g.con + (size_t)g.nv * erow;
+      const int nno = (g.nv == 4 && cn[3] >= 0) ? 4 : 3;
+      const double shpf = nno == 4 ? 0.25 : 1.0 / 3.0;
+      for (int v = 0; v < nno; ++v) hrow[cn[v]] += shpf * cfree;
     }
   }
   __syncthreads();
@@ -352,8 +585,32 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
   for (int c = 0; c < ncolors; ++c) {
     for (int idx = coff[c] + threadIdx.x; idx < coff[c + 1]; idx += blockDim.x) {
       const int e = celems[idx];
-      const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
+      const int *cn = g.con + (size_t)g.nv * e;
+      const int vv[3] = {cn[0], cn[1], cn[2]};
       const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
+      if (g.nv == 4 && cn[3] >= 0) {  // quadrilateral (Quad_StokesIntNumAna, src/lib_intAna.f90:5-37)
+        const int v4[4] = {cn[0], cn[1], cn[2], cn[3]};
+        int qs = 0;
+        if (!qrow) {
+          for (int j = 0; j < 4; ++j)
+            if (v4[j] == r) qs = j + 1;
+        } else if (e == erow) {
+          qs = 5;
+        }
+        double acc[45];
+        quad_stokes_pair(x1, x2, x3, g.xyz + 3 * v4[3], s, g.mul[e], g.iquad, qs, acc);
+        for (int k = 0; k < 6; ++k) G[k * mstrG + (size_t)r * ne + e] = acc[k];
+        for (int k = 0; k < 3; ++k) prG[k * mstrG + (size_t)r * ne + e] = acc[6 + k];
+        for (int k = 0; k < 6; ++k) {
+          double *p = trow(k);
+          for (int v = 0; v < 4; ++v) p[v4[v]] += acc[9 + 4 * k + v];
+        }
+        for (int k = 0; k < 3; ++k) {
+          double *p = trow(6 + k);
+          for (int v = 0; v < 4; ++v) p[v4[v]] += acc[33 + 4 * k + v];
+        }
+        continue;
+      }
       int isrc = 0;
       if (!qrow) isrc = (vv[0] == r) ? 1 : (vv[1] == r) ? 2 : (vv[2] == r) ? 3 : 0;
       else if (e == erow) isrc = 4;
@@ -466,16 +723,17 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
     err = fmax(err, fabs(cy - 0.5 * osemPi));
     err = fmax(err, fabs(cz - 0.5 * osemPi));
     if (threadIdx.x == 0) {
-      const int *cn = g.con + 3 * erow;
-      const double shpf = 1.0 / 3.0;
-      for (int v = 0; v < 3; ++v) {
+      const int *cn = g.con + (size_t)g.nv * erow;
+      const int nno = (g.nv == 4 && cn[3] >= 0) ? 4 : 3;
+      const double shpf = nno == 4 ? 0.25 : 1.0 / 3.0;
+      for (int v = 0; v < nno; ++v) {
         trow(0)[cn[v]] += shpf * (cx + extra);
         trow(3)[cn[v]] += shpf * (cy + extra);
         trow(5)[cn[v]] += shpf * (cz + extra);
       }
       // pressure correction (src/lib_Stokes.f90:1851-1863)
-      for (int v = 0; v < 3; ++v)
-        for (int k = 0; k < 3; ++k) trow(6 + k)[cn[v]] -= psum[k] / 3;
+      for (int v = 0; v < nno; ++v)
+        for (int k = 0; k < 3; ++k) trow(6 + k)[cn[v]] -= psum[k] / nno;
     }
   }
   __syncthreads();

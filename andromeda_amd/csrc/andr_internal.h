@@ -109,8 +109,10 @@ int andr_upload_rules_learned(andr_ctx *ctx, const double *tab, const float *tab
 // Per-body device geometry -------------------------------------------------------------------
 struct BodyGeom {
   int nn = 0, ne = 0, nwalls = 0;
+  int nv = 3;                    // entries of con per element: 3, or 4 (quadrilaterals; -1 marks a triangle)
+  int iquad = 4;                 // Gauss-Legendre rule of quadrilaterals (IQUD 1..8)
   const double *xyz = nullptr;   // 3*nn
-  const int *con = nullptr;      // 3*ne, 0-based
+  const int *con = nullptr;      // nv*ne, 0-based
   const int *bcid = nullptr;     // ne
   const double *nrm = nullptr;   // 3*ne unit normals (CalElementNormal)
   const double *area = nullptr;  // ne (Heron)
@@ -126,6 +128,7 @@ struct DevBody {
   DevBuf<int> d_color_off;
   BodyGeom g;
   int itri = 5;
+  int iquad = 4;
 };
 int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors);
 

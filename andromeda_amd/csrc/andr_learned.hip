@@ -599,6 +599,7 @@ static int labels_common(andr_ctx *ctx, const andr_mesh *mesh, double near_ratio
                          int32_t *far_table) {
   if (!ctx) return ANDR_ERR_ARG;
   if (!(near_ratio > 0.0) || !(rel_tol > 0.0)) return andr_fail(ctx, ANDR_ERR_ARG, "near_ratio and rel_tol must be > 0");
+  if (mesh && mesh->nvert == 4) return andr_fail(ctx, ANDR_ERR_ARG, "quadrature training takes triangle meshes");
   ANDR_HIP(ctx, hipSetDevice(ctx->device));
   DevBody b;
   int rc = andr_body_upload(ctx, mesh, b, false);

@@ -95,6 +95,9 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
                                      const andr_resistance_opts *opts, double *FT, andr_resistance_report *reports) {
   if (!ctx) return ANDR_ERR_ARG;
   if (nbodies <= 0 || !meshes || !FT) return andr_fail(ctx, ANDR_ERR_ARG, "andr_resistance_batch arguments");
+  for (int b = 0; b < nbodies; ++b)
+    if (meshes[b].nvert == 4)
+      return andr_fail(ctx, ANDR_ERR_ARG, "body %d: resistance tensors take triangle meshes (nvert 0 or 3)", b);
   int rc = check_opts(ctx, opts);
   if (rc) return rc;
   ANDR_HIP(ctx, hipSetDevice(ctx->device));

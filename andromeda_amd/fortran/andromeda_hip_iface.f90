@@ -21,6 +21,8 @@ module andromeda_hip
     integer(c_int32_t) :: nnodes, nelem, nwalls
     type(c_ptr) :: xyz, con, bcid, normmul
     integer(c_int32_t) :: index_base, itri
+    integer(c_int32_t) :: nvert  ! entries of con per element: 3 (triangles) or 4 (quadrilaterals)
+    integer(c_int32_t) :: iquad  ! Gauss-Legendre rule of quadrilaterals, IQUD 1..8 (0 -> 4)
   end type
 
   type, bind(C) :: andr_lsqr_opts

@@ -34,7 +34,7 @@ program resistance_host
   ne = size(con, 2)
   nsp = nn + ne
   normmul = 1.0d0
-  mesh = andr_mesh(nn, ne, maxval(bcid), c_loc(xyz), c_loc(con), c_loc(bcid), c_loc(normmul), 1, 5)
+  mesh = andr_mesh(nn, ne, maxval(bcid), c_loc(xyz), c_loc(con), c_loc(bcid), c_loc(normmul), 1, 5, 3, 4)
   rc = andr_create(ctx, 0)
   if (rc /= ANDR_OK) stop "andr_create failed"
   write(*, '(a, i0, a, i0, a)') "mesh: ", nn, " nodes, ", ne, " triangles"

@@ -1,9 +1,10 @@
 """Surface meshes for the BEM hot path: generators for the benchmark bodies and readers for the
 mesh formats the reference accepts.
 
-Element connectivity is 0-based int32 (nelem, 3); node coordinates are float64 (nnodes, 3);
-`bcid` holds the 1-based wall id of every element, as `element(ie)%bcid` in the reference
-(src/mod_mesh.f90:16-80).
+Element connectivity is 0-based int32 (nelem, 3) for triangle meshes, or (nelem, 4) when the mesh
+has 4-node quadrilaterals (a triangle then has -1 as its fourth entry); node coordinates are
+float64 (nnodes, 3); `bcid` holds the 1-based wall id of every element, as `element(ie)%bcid` in
+the reference (src/mod_mesh.f90:16-80).
 """
 from __future__ import annotations
 
@@ -16,7 +17,7 @@ import numpy as np
 @dataclasses.dataclass
 class Mesh:
     xyz: np.ndarray          # (nnodes, 3) float64
-    con: np.ndarray          # (nelem, 3) int32, 0-based
+    con: np.ndarray          # (nelem, 3) or (nelem, 4) int32, 0-based (-1: no fourth node)
     bcid: np.ndarray         # (nelem,) int32, walls numbered from 1
     name: str = ""
 
@@ -252,9 +253,54 @@ def body_in_sphere(inner: Mesh, outer_radius: float, outer_nu: int) -> Mesh:
 # --------------------------------------------------------------------------------------------
 # readers (surface triangles only)
 # --------------------------------------------------------------------------------------------
+def cubed_sphere(n: int, radius: float = 1.0) -> Mesh:
+    """Sphere of 6 n^2 bilinear quadrilaterals: the faces of a cube divided n x n, nodes projected
+    radially (equiangular spacing), counter-clockwise seen from outside (outward normals)."""
+    t = np.tan(np.linspace(-np.pi / 4, np.pi / 4, n + 1))
+    pts, quads, index = [], [], {}
+
+    def node(p):
+        key = tuple(np.round(p / np.linalg.norm(p), 12))
+        if key not in index:
+            index[key] = len(pts)
+            pts.append(radius * p / np.linalg.norm(p))
+        return index[key]
+
+    axes = [(0, 1, 2), (1, 2, 0), (2, 0, 1)]
+    for (a, b, c) in axes:
+        for sign in (1.0, -1.0):
+            for i in range(n):
+                for j in range(n):
+                    corner = []
+                    for (di, dj) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[a] = sign
+                        p[b] = t[i + di]
+                        p[c] = t[j + dj]
+                        corner.append(node(p))
+                    quads.append(corner if sign > 0 else corner[::-1])
+    xyz = np.array(pts)
+    con = np.array(quads, dtype=np.int32)
+    return Mesh(xyz, con, np.ones(len(con), np.int32), f"cubed_sphere{n}")
+
+
+def split_some_quads(mesh: Mesh, every: int = 3) -> Mesh:
+    """Mixed mesh: every `every`-th quadrilateral replaced by two triangles (same nodes)."""
+    rows, bc = [], []
+    for k, q in enumerate(mesh.con):
+        if k % every == 0 and q[3] >= 0:
+            rows += [[q[0], q[1], q[2], -1], [q[0], q[2], q[3], -1]]
+            bc += [mesh.bcid[k], mesh.bcid[k]]
+        else:
+            rows.append(list(q))
+            bc.append(mesh.bcid[k])
+    return Mesh(mesh.xyz.copy(), np.array(rows, np.int32), np.array(bc, np.int32), mesh.name + "_mixed")
+
+
 def read_gmsh(path: str) -> Mesh:
-    """gmsh ASCII v2 or v4 surface mesh (src/lib_meshRead.f90:244-629): triangles (type 2) are
-    kept, the physical tag becomes the wall id. Nodes are renumbered densely in file order."""
+    """gmsh ASCII v2 or v4 surface mesh (src/lib_meshRead.f90:244-629): triangles (type 2) and
+    4-node quadrilaterals (type 3) are kept, the physical tag becomes the wall id. Nodes are
+    renumbered densely in file order."""
     with open(path, "r") as fh:
         lines = fh.read().split("\n")
     i = 0
@@ -306,10 +352,10 @@ def read_gmsh(path: str) -> Mesh:
                 n = int(lines[i + 1])
                 for k in range(n):
                     t = [int(x) for x in lines[i + 2 + k].split()]
-                    if t[1] == 2:
+                    if t[1] in (2, 3):
                         ntag = t[2]
                         tags.append(t[3] if ntag > 0 else 1)
-                        tris.append(t[3 + ntag:3 + ntag + 3])
+                        tris.append(t[3 + ntag:3 + ntag + (3 if t[1] == 2 else 4)])
                 i += 2 + n
             else:
                 hdr = [int(x) for x in lines[i + 1].split()]
@@ -320,8 +366,8 @@ def read_gmsh(path: str) -> Mesh:
                     edim, etag, etype, nb = b
                     for k in range(nb):
                         t = [int(x) for x in lines[j + 1 + k].split()]
-                        if etype == 2:
-                            tris.append(t[1:4])
+                        if etype in (2, 3):
+                            tris.append(t[1:4] if etype == 2 else t[1:5])
                             tags.append(phys_of_entity.get(etag, etag))
                     j += 1 + nb
                 i = j
@@ -330,7 +376,8 @@ def read_gmsh(path: str) -> Mesh:
     used = sorted({v for t in tris for v in t}, key=lambda v: order.index(v) if len(order) < 50000 else v)
     remap = {v: k for k, v in enumerate(used)}
     xyz = np.array([nodes[v] for v in used], dtype=np.float64)
-    con = np.array([[remap[v] for v in t] for t in tris], dtype=np.int32)
+    width = 4 if any(len(t) == 4 for t in tris) else 3
+    con = np.array([[remap[v] for v in t] + [-1] * (width - len(t)) for t in tris], dtype=np.int32)
     utags = sorted(set(tags))
     wall = {t: k + 1 for k, t in enumerate(utags)}
     bcid = np.array([wall[t] for t in tags], dtype=np.int32)

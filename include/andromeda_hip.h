@@ -32,16 +32,23 @@ extern "C" {
 
 typedef struct andr_ctx andr_ctx;
 
-/* Surface mesh of one body (flat 3-node triangles), the data of module mMesh
- * (src/mod_mesh.f90:16-111) passed explicitly. */
+/* Surface mesh of one body (flat 3-node triangles, or bilinear 4-node quadrilaterals mixed with
+ * triangles), the data of module mMesh (src/mod_mesh.f90:16-111) passed explicitly. */
 typedef struct {
   int32_t nnodes, nelem, nwalls;
   const double *xyz;     /* 3*nnodes: x,y,z of node 0, node 1, ...            */
-  const int32_t *con;    /* 3*nelem: vertex ids of every triangle               */
+  const int32_t *con;    /* nvert*nelem: vertex ids of every element            */
   const int32_t *bcid;   /* nelem: wall id 1..nwalls (NULL: every element wall 1) */
   const double *normmul; /* nwalls: NORMMUL factor per wall (NULL: all +1)    */
   int32_t index_base;    /* 0 for C ids, 1 for Fortran ids in con              */
   int32_t itri;          /* regular triangle rule, ITRI 1..5 (7..126 points); 0 -> 5 */
+  int32_t nvert;         /* entries of con per element: 0 or 3 (triangles), or 4: 4-node
+                            quadrilaterals, where a fourth entry below index_base marks a
+                            triangle (src/mod_mesh.f90: element%type 2 / 3). Quadrilaterals are
+                            accepted by the assembly entry points (seam 1); the resistance and
+                            training entry points take triangle meshes */
+  int32_t iquad;         /* Gauss-Legendre rule of quadrilaterals, IQUD 1..8 (N = 2,4,6,8,16,32,
+                            40,48 per direction); 0 -> 4 (src/mod_par.f90:233-234) */
 } andr_mesh;
 
 /* ---- context -------------------------------------------------------------------------- */

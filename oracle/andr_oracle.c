@@ -48,12 +48,66 @@ static double heron(const double x1[3], const double x2[3], const double x3[3]) 
   return sqrt(p * (p - a) * (p - b) * (p - c));
 }
 
-void orc_geometry(int nnodes, int nelem, const double *xyz, const int32_t *con,
-                  double *normal, double *area, double *centroid) {
+/* ------------------------------------------------------------------------------------------
+ * Gauss-Legendre rules: GaussPosWeights (src/mod_gauss.f90:33-391) holds N = 2,4,6,8,16,32,40,48
+ * as literals; here they are computed (Newton iteration on the three-term Legendre recurrence,
+ * nodes ascending as in the reference tables).
+ * ---------------------------------------------------------------------------------------- */
+int orc_gauss_rule(int iq, double *x, double *w) {
+  static const int Ns[8] = {2, 4, 6, 8, 16, 32, 40, 48};
+  if (iq < 1 || iq > 8) return -1;
+  const int N = Ns[iq - 1];
+  for (int i = 0; i < N / 2; ++i) {
+    double z = cos(ORC_PI * (i + 0.75) / (N + 0.5)), pp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= N; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = N * (z * p1 - p2) / (z * z - 1.0);
+      const double dz = p1 / pp;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    x[N - 1 - i] = z;
+    w[i] = w[N - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+  return N;
+}
+
+/* bilinear quadrilateral at local coordinates (gi, gj) in [-1, 1]^2 (Quad_BEMIntNum,
+ * src/lib_intAna.f90:300-325): shape functions fig4, point f, unnormalised normal an = x_ksi x x_eta */
+static void quad_point(const double *x1, const double *x2, const double *x3, const double *x4, double gi, double gj,
+                       double fig4[4], double f[3], double an[3]) {
+  const double e1m = 1.0 - gi, e1p = 1.0 + gi, e2m = 1.0 - gj, e2p = 1.0 + gj;
+  fig4[0] = 0.25 * e1m * e2m;
+  fig4[1] = 0.25 * e1p * e2m;
+  fig4[2] = 0.25 * e1p * e2p;
+  fig4[3] = 0.25 * e1m * e2p;
+  double ks[3], et[3];
+  for (int j = 0; j < 3; ++j) {
+    f[j] = fig4[0] * x1[j] + fig4[1] * x2[j] + fig4[2] * x3[j] + fig4[3] * x4[j];
+    ks[j] = 0.25 * (-e2m * x1[j] + e2m * x2[j] + e2p * x3[j] - e2p * x4[j]);
+    et[j] = 0.25 * (-e1m * x1[j] - e1p * x2[j] + e1p * x3[j] + e1m * x4[j]);
+  }
+  an[0] = ks[1] * et[2] - et[1] * ks[2];
+  an[1] = et[0] * ks[2] - ks[0] * et[2];
+  an[2] = ks[0] * et[1] - et[0] * ks[1];
+}
+
+void orc_geometry_n(int nnodes, int nelem, const double *xyz, const int32_t *con, int nvert,
+                    double *normal, double *area, double *centroid) {
   (void)nnodes;
+  double gx[8], gw[8];
+  orc_gauss_rule(4, gx, gw); /* CalElementArea uses gaus%ng1(4):ng2(4) (src/lib_int.f90:648-649) */
   for (int e = 0; e < nelem; ++e) {
-    const double *t1 = xyz + 3 * con[3 * e], *t2 = xyz + 3 * con[3 * e + 1], *t3 = xyz + 3 * con[3 * e + 2];
-    /* CalElementNormal (src/lib_mesh.f90:716-769) */
+    const int32_t *cn = con + (size_t)nvert * e;
+    const int quad = nvert == 4 && cn[3] >= 0;
+    const double *t1 = xyz + 3 * cn[0], *t2 = xyz + 3 * cn[1], *t3 = xyz + 3 * cn[2];
+    /* CalElementNormal (src/lib_mesh.f90:716-769): first three nodes for either type */
     double v1[3] = {t2[0] - t1[0], t2[1] - t1[1], t2[2] - t1[2]};
     double v2[3] = {t3[0] - t1[0], t3[1] - t1[1], t3[2] - t1[2]};
     double n0 = v1[1] * v2[2] - v1[2] * v2[1];
@@ -61,20 +115,31 @@ void orc_geometry(int nnodes, int nelem, const double *xyz, const int32_t *con,
     double n2 = v1[0] * v2[1] - v1[1] * v2[0];
     double d = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
     normal[3 * e] = n0 / d; normal[3 * e + 1] = n1 / d; normal[3 * e + 2] = n2 / d;
-    /* CalElementArea -> Triangle_Area (src/lib_int.f90:606-721) */
-    area[e] = heron(t1, t2, t3);
-    /* CalQMesh: sum of shape-function weighted vertices with weights 1/3 (src/lib_mesh.f90:577-629) */
-    for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
-      s += (1.0 / 3.0) * t1[j];
-      s += (1.0 / 3.0) * t2[j];
-      s += (1.0 / 3.0) * t3[j];
-      centroid[3 * e + j] = s;
+    if (!quad) {
+      area[e] = heron(t1, t2, t3);
+      /* CalQMesh (src/lib_mesh.f90:577-629): barycentre */
+      for (int j = 0; j < 3; ++j) centroid[3 * e + j] = (1.0 / 3.0) * t1[j] + (1.0 / 3.0) * t2[j] + (1.0 / 3.0) * t3[j];
+    } else {
+      const double *t4 = xyz + 3 * cn[3];
+      double A = 0.0;
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+          double fig4[4], f[3], an[3];
+          quad_point(t1, t2, t3, t4, gx[i], gx[j], fig4, f, an);
+          const double aj = gw[i] * gw[j] * sqrt(an[0] * an[0] + an[1] * an[1] + an[2] * an[2]);
+          for (int k = 0; k < 4; ++k) A += aj * fig4[k];
+        }
+      area[e] = A;
+      for (int j = 0; j < 3; ++j) centroid[3 * e + j] = 0.25 * t1[j] + 0.25 * t2[j] + 0.25 * t3[j] + 0.25 * t4[j];
     }
   }
 }
 
-/* BemKernel (src/lib_sdBem.f90:911-936): r = s - f, G = 1/(4 pi r), H = (r.n)/r^2 G */
+void orc_geometry(int nnodes, int nelem, const double *xyz, const int32_t *con,
+                  double *normal, double *area, double *centroid) {
+  orc_geometry_n(nnodes, nelem, xyz, con, 3, normal, area, centroid);
+}
+
 void orc_bem_kernel(const double s[3], const double f[3], const double n[3], double *G, double *H) {
   double rx = -f[0] + s[0], ry = -f[1] + s[1], rz = -f[2] + s[2];
   double ra = sqrt(rx * rx + ry * ry + rz * rz);
@@ -348,6 +413,119 @@ void orc_tri_stokes_int(const double x1[3], const double x2[3], const double x3[
 }
 
 /* ------------------------------------------------------------------------------------------
+ * Quadrilaterals: regular integrals over bilinear sub-cells, singular ones split in triangles
+ * ---------------------------------------------------------------------------------------- */
+static double dist3(const double *a, const double *b) {
+  return sqrt(pow(a[0] - b[0], 2.0) + pow(a[1] - b[1], 2.0) + pow(a[2] - b[2], 2.0));
+}
+
+/* ndiv sub-cells x ndiv sub-cells of N x N points; calls fn(fig4, f, n_unit * mul, weight) */
+typedef void (*quad_fn)(void *ctx, const double fig4[4], const double f[3], const double n[3], double w);
+static void quad_loop(const double *x1, const double *x2, const double *x3, const double *x4, double mul, int iq,
+                      int ndiv_xi, int ndiv_et, quad_fn fn, void *ctx) {
+  double gx[48], gw[48];
+  const int N = orc_gauss_rule(iq, gx, gw);
+  for (int ix = 1; ix <= ndiv_xi; ++ix)
+    for (int ie = 1; ie <= ndiv_et; ++ie) {
+      const double a = -1.0 + (ix - 1) * 2.0 / ndiv_xi, b = -1.0 + ix * 2.0 / ndiv_xi;
+      const double c = -1.0 + (ie - 1) * 2.0 / ndiv_et, d = -1.0 + ie * 2.0 / ndiv_et;
+      const double dex = 0.25 * (b - a) * (d - c);
+      for (int i = 0; i < N; ++i) {
+        const double gii = a + 0.5 * (gx[i] + 1.0) * (b - a);
+        for (int j = 0; j < N; ++j) {
+          const double gij = c + 0.5 * (gx[j] + 1.0) * (d - c);
+          double fig4[4], f[3], an[3];
+          quad_point(x1, x2, x3, x4, gii, gij, fig4, f, an);
+          const double inv = 1.0 / sqrt(an[0] * an[0] + an[1] * an[1] + an[2] * an[2]);
+          const double n[3] = {mul * an[0] * inv, mul * an[1] * inv, mul * an[2] * inv};
+          fn(ctx, fig4, f, n, dex * gw[i] * gw[j] / inv);
+        }
+      }
+    }
+}
+
+typedef struct { const double *s; double *out; } lapq_ctx;
+static void lapq_fn(void *vc, const double fig4[4], const double f[3], const double n[3], double w) {
+  lapq_ctx *c = (lapq_ctx *)vc;
+  double G, H;
+  orc_bem_kernel(c->s, f, n, &G, &H);
+  for (int k = 0; k < 4; ++k) c->out[1 + k] += H * w * fig4[k];
+  c->out[0] += G * w;
+}
+
+void orc_quad_lap_int(const double *x1, const double *x2, const double *x3, const double *x4,
+                      const double s[3], double mul, int iq, double out[5]) {
+  for (int k = 0; k < 5; ++k) out[k] = 0.0;
+  const double minedge = 1.0; /* fixed in the reference (src/lib_intAna.f90:263) */
+  const double d13 = fmax(dist3(x1, x2), dist3(x3, x4)), d24 = fmax(dist3(x2, x3), dist3(x4, x1));
+  int nxi = (int)(d13 / minedge + 0.5), net = (int)(d24 / minedge + 0.5);
+  nxi = nxi > 2 ? nxi : 2;
+  net = net > 2 ? net : 2;
+  lapq_ctx c = {s, out};
+  quad_loop(x1, x2, x3, x4, mul, iq, nxi, net, lapq_fn, &c);
+}
+
+/* two (vertex source) or four (centre source) triangles with the source at their first vertex */
+double orc_si_qua_lap(const double *x1, const double *x2, const double *x3, const double *x4, int isrc) {
+  const double *v[4] = {x1, x2, x3, x4};
+  if (isrc >= 1 && isrc <= 4) {
+    const double *a = v[isrc - 1], *b = v[isrc % 4], *c = v[(isrc + 1) % 4], *d = v[(isrc + 2) % 4];
+    return orc_si_tri_lap(a, b, c, heron(a, b, c), 1) + orc_si_tri_lap(a, c, d, heron(a, c, d), 1);
+  }
+  if (isrc == 5) {
+    double x5[3];
+    for (int j = 0; j < 3; ++j) x5[j] = 0.25 * (x1[j] + x2[j] + x3[j] + x4[j]);
+    double G = 0.0;
+    for (int k = 0; k < 4; ++k) G += orc_si_tri_lap(x5, v[k], v[(k + 1) % 4], heron(x5, v[k], v[(k + 1) % 4]), 1);
+    return G;
+  }
+  return 0.0;
+}
+
+typedef struct { const double *s; double *acc; } stkq_ctx;
+static void stkq_fn(void *vc, const double fig4[4], const double f[3], const double n[3], double w) {
+  stkq_ctx *c = (stkq_ctx *)vc;
+  double G[9], T[9], prG[3], prT[3];
+  orc_stokes_kernel(c->s, f, n, G, T, prG, prT);
+  static const int idx[6] = {0, 1, 2, 4, 5, 8};
+  for (int k = 0; k < 6; ++k) c->acc[k] += w * G[idx[k]];
+  for (int k = 0; k < 3; ++k) c->acc[6 + k] += w * prG[k];
+  for (int v = 0; v < 4; ++v) {
+    const double wv = w * fig4[v];
+    for (int k = 0; k < 6; ++k) c->acc[9 + 4 * k + v] += wv * T[idx[k]];
+    for (int k = 0; k < 3; ++k) c->acc[33 + 4 * k + v] += wv * prT[k];
+  }
+}
+
+void orc_quad_stokes_int(const double *x1, const double *x2, const double *x3, const double *x4,
+                         const double s[3], double mul, int iq, double acc[48]) {
+  for (int k = 0; k < 48; ++k) acc[k] = 0.0;
+  stkq_ctx c = {s, acc};
+  quad_loop(x1, x2, x3, x4, mul, iq, 3, 3, stkq_fn, &c); /* ndivXi = ndivEt = 3 (:3845-3846) */
+}
+
+void orc_si_qua_stk(const double *x1, const double *x2, const double *x3, const double *x4, int isrc,
+                    double Gout[6]) {
+  const double *v[4] = {x1, x2, x3, x4};
+  double g[6];
+  for (int k = 0; k < 6; ++k) Gout[k] = 0.0;
+  if (isrc >= 1 && isrc <= 4) {
+    const double *a = v[isrc - 1], *b = v[isrc % 4], *c = v[(isrc + 1) % 4], *d = v[(isrc + 2) % 4];
+    orc_si_tri_stk(a, b, c, heron(a, b, c), 1, g);
+    for (int k = 0; k < 6; ++k) Gout[k] += g[k];
+    orc_si_tri_stk(a, c, d, heron(a, c, d), 1, g);
+    for (int k = 0; k < 6; ++k) Gout[k] += g[k];
+  } else if (isrc == 5) {
+    double x5[3];
+    for (int j = 0; j < 3; ++j) x5[j] = 0.25 * (x1[j] + x2[j] + x3[j] + x4[j]);
+    for (int k = 0; k < 4; ++k) {
+      orc_si_tri_stk(x5, v[k], v[(k + 1) % 4], heron(x5, v[k], v[(k + 1) % 4]), 1, g);
+      for (int q = 0; q < 6; ++q) Gout[q] += g[q];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------
  * Row integration and matrix assembly
  * ---------------------------------------------------------------------------------------- */
 typedef struct {
@@ -358,7 +536,7 @@ static void make_geom(const orc_body *b, geom_t *g) {
   g->normal = (double *)malloc(sizeof(double) * 3 * b->nelem);
   g->area = (double *)malloc(sizeof(double) * b->nelem);
   g->cen = (double *)malloc(sizeof(double) * 3 * b->nelem);
-  orc_geometry(b->nnodes, b->nelem, b->xyz, b->con, g->normal, g->area, g->cen);
+  orc_geometry_n(b->nnodes, b->nelem, b->xyz, b->con, b->nvert == 4 ? 4 : 3, g->normal, g->area, g->cen);
 }
 static void free_geom(geom_t *g) { free(g->normal); free(g->area); free(g->cen); }
 
@@ -375,19 +553,25 @@ static double lap_row(const orc_body *b, const geom_t *g, int node_src, int qrow
     const double mul = b->normmul[wall - 1];
     for (int e = 0; e < ne; ++e) {
       if (b->bcid[e] != wall) continue;
-      const int32_t *cn = b->con + 3 * e;
+      const int nv = b->nvert == 4 ? 4 : 3;
+      const int32_t *cn = b->con + (size_t)nv * e;
+      const int nno = (nv == 4 && cn[3] >= 0) ? 4 : 3;
       int isrc = 0;
-      if (qrow < 0) { for (int j = 0; j < 3; ++j) if (cn[j] == node_src) isrc = j + 1; }
-      else if (e == qrow) isrc = 4;
-      double out[4] = {0, 0, 0, 0};
+      if (qrow < 0) { for (int j = 0; j < nno; ++j) if (cn[j] == node_src) isrc = j + 1; }
+      else if (e == qrow) isrc = nno == 4 ? 5 : 4;
+      double out[5] = {0, 0, 0, 0, 0};
       const double *x1 = b->xyz + 3 * cn[0], *x2 = b->xyz + 3 * cn[1], *x3 = b->xyz + 3 * cn[2];
-      if (isrc == 0) {
+      if (nno == 4) { /* Quad_BEMIntNumAna (src/lib_intAna.f90:153-176) */
+        const double *x4 = b->xyz + 3 * cn[3];
+        if (isrc == 0) orc_quad_lap_int(x1, x2, x3, x4, s, mul, b->iquad, out);
+        else out[0] = orc_si_qua_lap(x1, x2, x3, x4, isrc); /* Quad_BEMIntAna: H = 0 */
+      } else if (isrc == 0) {
         double n[3] = {mul * g->normal[3 * e], mul * g->normal[3 * e + 1], mul * g->normal[3 * e + 2]};
         orc_tri_lap_int(x1, x2, x3, s, n, g->area[e], out);
       } else {
         out[0] = orc_si_tri_lap(x1, x2, x3, g->area[e], isrc); /* Triangle_3DSingLapInt, H = 0 */
       }
-      for (int j = 0; j < 3; ++j) rowH[cn[j]] += out[1 + j];
+      for (int j = 0; j < nno; ++j) rowH[cn[j]] += out[1 + j];
       rowG[e] = out[0];
     }
   }
@@ -404,8 +588,11 @@ static double lap_row(const orc_body *b, const geom_t *g, int node_src, int qrow
     }
   } else {
     err = fabs(0.5 - c);
-    const int32_t *cn = b->con + 3 * qrow;
-    for (int j = 0; j < 3; ++j) rowH[cn[j]] += (1.0 / 3.0) * c;
+    const int nv = b->nvert == 4 ? 4 : 3;
+    const int32_t *cn = b->con + (size_t)nv * qrow;
+    const int nno = (nv == 4 && cn[3] >= 0) ? 4 : 3;
+    const double shpf = nno == 4 ? 0.25 : 1.0 / 3.0;
+    for (int j = 0; j < nno; ++j) rowH[cn[j]] += shpf * c;
   }
   return err;
 }
@@ -442,25 +629,41 @@ static double stokes_row(const orc_body *b, const geom_t *g, int node_src, int q
     const double mul = b->normmul[wall - 1];
     for (int e = 0; e < ne; ++e) {
       if (b->bcid[e] != wall) continue;
-      const int32_t *cn = b->con + 3 * e;
+      const int nv = b->nvert == 4 ? 4 : 3;
+      const int32_t *cn = b->con + (size_t)nv * e;
+      const int nno = (nv == 4 && cn[3] >= 0) ? 4 : 3;
       int isrc = 0;
-      if (qrow < 0) { for (int j = 0; j < 3; ++j) if (cn[j] == node_src) isrc = j + 1; }
-      else if (e == qrow) isrc = 4;
-      double acc[36];
+      if (qrow < 0) { for (int j = 0; j < nno; ++j) if (cn[j] == node_src) isrc = j + 1; }
+      else if (e == qrow) isrc = nno == 4 ? 5 : 4;
+      /* unified layout: G6, prG3, T[6][4], prT[3][4] */
+      double acc[48];
       memset(acc, 0, sizeof(acc));
       const double *x1 = b->xyz + 3 * cn[0], *x2 = b->xyz + 3 * cn[1], *x3 = b->xyz + 3 * cn[2];
-      if (isrc == 0) {
-        double n[3] = {mul * g->normal[3 * e], mul * g->normal[3 * e + 1], mul * g->normal[3 * e + 2]};
-        orc_tri_stokes_int(x1, x2, x3, s, n, g->area[e], acc);
+      if (nno == 4) { /* Quad_StokesIntNumAna (src/lib_intAna.f90:5-37) */
+        const double *x4 = b->xyz + 3 * cn[3];
+        if (isrc == 0) orc_quad_stokes_int(x1, x2, x3, x4, s, mul, b->iquad, acc);
+        else orc_si_qua_stk(x1, x2, x3, x4, isrc, acc); /* Quad_StokesIntAna: T = prT = prG = 0 */
       } else {
-        orc_si_tri_stk(x1, x2, x3, g->area[e], isrc, acc); /* Triangle_StokesIntAna: T = prT = prG = 0 */
+        double t36[36];
+        memset(t36, 0, sizeof(t36));
+        if (isrc == 0) {
+          double n[3] = {mul * g->normal[3 * e], mul * g->normal[3 * e + 1], mul * g->normal[3 * e + 2]};
+          orc_tri_stokes_int(x1, x2, x3, s, n, g->area[e], t36);
+        } else {
+          orc_si_tri_stk(x1, x2, x3, g->area[e], isrc, t36); /* Triangle_StokesIntAna: T = prT = prG = 0 */
+        }
+        for (int k = 0; k < 9; ++k) acc[k] = t36[k];
+        for (int k = 0; k < 6; ++k)
+          for (int v = 0; v < 3; ++v) acc[9 + 4 * k + v] = t36[9 + 3 * k + v];
+        for (int k = 0; k < 3; ++k)
+          for (int v = 0; v < 3; ++v) acc[33 + 4 * k + v] = t36[27 + 3 * k + v];
       }
       for (int k = 0; k < 6; ++k)
-        for (int v = 0; v < 3; ++v) rT[k][cn[v]] += acc[9 + 3 * k + v];
+        for (int v = 0; v < nno; ++v) rT[k][cn[v]] += acc[9 + 4 * k + v];
       for (int k = 0; k < 6; ++k) rG[k][e] = acc[k];
       if (qrow < 0 || isrc == 0) {
         for (int k = 0; k < 3; ++k)
-          for (int v = 0; v < 3; ++v) rprT[k][cn[v]] += acc[27 + 3 * k + v];
+          for (int v = 0; v < nno; ++v) rprT[k][cn[v]] += acc[33 + 4 * k + v];
         for (int k = 0; k < 3; ++k) rprG[k][e] = acc[6 + k];
       }
     }
@@ -491,9 +694,11 @@ static double stokes_row(const orc_body *b, const geom_t *g, int node_src, int q
     err = fmax(err, fabs(cy - 0.5 * osemPi));
     err = fmax(err, fabs(cz - 0.5 * osemPi));
     cx += extra; cy += extra; cz += extra;
-    const int32_t *cn = b->con + 3 * qrow;
-    const double shpf = 1.0 / 3.0;
-    for (int v = 0; v < 3; ++v) {
+    const int nv = b->nvert == 4 ? 4 : 3;
+    const int32_t *cn = b->con + (size_t)nv * qrow;
+    const int nno = (nv == 4 && cn[3] >= 0) ? 4 : 3;
+    const double shpf = nno == 4 ? 0.25 : 1.0 / 3.0;
+    for (int v = 0; v < nno; ++v) {
       rT[0][cn[v]] += shpf * cx;
       rT[3][cn[v]] += shpf * cy;
       rT[5][cn[v]] += shpf * cz;
@@ -501,8 +706,8 @@ static double stokes_row(const orc_body *b, const geom_t *g, int node_src, int q
     /* pressure correction (:1851-1863) */
     double px = 0, py = 0, pz = 0;
     for (int j = 0; j < nn; ++j) { px += rprT[0][j]; py += rprT[1][j]; pz += rprT[2][j]; }
-    for (int v = 0; v < 3; ++v) {
-      rprT[0][cn[v]] -= px / 3; rprT[1][cn[v]] -= py / 3; rprT[2][cn[v]] -= pz / 3;
+    for (int v = 0; v < nno; ++v) {
+      rprT[0][cn[v]] -= px / nno; rprT[1][cn[v]] -= py / nno; rprT[2][cn[v]] -= pz / nno;
     }
   }
   return err;
@@ -765,11 +970,14 @@ int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int m
   orc_lsqr_precon(&A, q);
   double *geo_n = (double *)malloc(sizeof(double) * 3 * ne), *area = (double *)malloc(sizeof(double) * ne);
   double *cen = (double *)malloc(sizeof(double) * 3 * ne);
-  orc_geometry(nn, ne, b->xyz, b->con, geo_n, area, cen);
+  const int nvr = b->nvert == 4 ? 4 : 3;
+  orc_geometry_n(nn, ne, b->xyz, b->con, nvr, geo_n, area, cen);
   int32_t *node_wall = (int32_t *)calloc(nn, sizeof(int32_t));
   for (int e = 0; e < ne; ++e)
-    for (int v = 0; v < 3; ++v)
-      if (b->bcid[e] > node_wall[b->con[3 * e + v]]) node_wall[b->con[3 * e + v]] = b->bcid[e];
+    for (int v = 0; v < nvr; ++v) {
+      const int nd = b->con[(size_t)nvr * e + v];
+      if (nd >= 0 && b->bcid[e] > node_wall[nd]) node_wall[nd] = b->bcid[e];
+    }
   double *ub = (double *)malloc(sizeof(double) * nr3), *rb = (double *)malloc(sizeof(double) * n3);
   double *x = (double *)malloc(sizeof(double) * n3);
   int worst = 0;

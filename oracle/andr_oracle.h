@@ -53,11 +53,40 @@ void orc_tri_stokes_int(const double x1[3], const double x2[3], const double x3[
 typedef struct {
   int nnodes, nelem, nwalls;
   const double *xyz;     /* 3*nnodes */
-  const int32_t *con;    /* 3*nelem, 0-based */
+  const int32_t *con;    /* nvert*nelem, 0-based; with nvert = 4 an entry con[4e+3] < 0 marks a triangle */
   const int32_t *bcid;   /* nelem, 1..nwalls */
   const double *normmul; /* nwalls */
   int itri;              /* 1..5 */
+  int nvert;             /* 3 (triangles) or 4 (quadrilaterals and triangles) */
+  int iquad;             /* Gauss-Legendre rule of quadrilaterals, IQUD 1..8 (N = 2,4,6,8,16,32,40,48) */
 } orc_body;
+
+/* Gauss-Legendre nodes and weights on [-1, 1] for rule iq = 1..8 (N = 2,4,6,8,16,32,40,48), the
+ * tables of GaussPosWeights (src/mod_gauss.f90:33-391), computed here by Newton iteration on
+ * the Legendre recurrence. Returns N, or -1. */
+int orc_gauss_rule(int iq, double *x, double *w);
+
+/* Element geometry for a mesh with nvert entries per element (3 or 4): triangles as
+ * orc_geometry; quadrilaterals (src/lib_mesh.f90:577-629,716-769; src/lib_int.f90:606-721):
+ * normal from the first three nodes, area = integral of |J| with the N = 8 rule, centre =
+ * bilinear centre (mean of the 4 nodes). */
+void orc_geometry_n(int nnodes, int nelem, const double *xyz, const int32_t *con, int nvert,
+                    double *normal, double *area, double *centroid);
+
+/* Quad_BEMIntNum (src/lib_intAna.f90:219-344): out[5] = {G, H1..H4} of a bilinear quadrilateral,
+ * ndiv x ndiv sub-cells (>= 2 each, by edge length with minedge = 1) of N x N Gauss points of
+ * rule iq, the pointwise normal times mul. */
+void orc_quad_lap_int(const double *x1, const double *x2, const double *x3, const double *x4,
+                      const double s[3], double mul, int iq, double out[5]);
+/* si_qua_lap (src/mod_singInt.f90:172-260): isrc 1..4 (vertex) or 5 (centre). */
+double orc_si_qua_lap(const double *x1, const double *x2, const double *x3, const double *x4, int isrc);
+/* Quad_StokesIntNum (src/lib_Stokes.f90:3754-3940): 3 x 3 sub-cells; acc[48] accumulates
+ * {G6, prG3, T[6][4], prT[3][4]}. */
+void orc_quad_stokes_int(const double *x1, const double *x2, const double *x3, const double *x4,
+                         const double s[3], double mul, int iq, double acc[48]);
+/* si_qua_stk (src/mod_singInt.f90:266-387): Gout = {Gxx,Gxy,Gxz,Gyy,Gyz,Gzz}. */
+void orc_si_qua_stk(const double *x1, const double *x2, const double *x3, const double *x4, int isrc,
+                    double Gout[6]);
 
 /* sdFormIntegralMatrices (src/lib_int.f90:225-323) with sdIntRowU/sdIntRowQ
  * (src/lib_sdBem.f90:438-733). H is nsp x nnodes, G is nsp x nelem (row-major),

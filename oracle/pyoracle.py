@@ -24,7 +24,8 @@ def _pi(a):
 
 class _Body(C.Structure):
     _fields_ = [("nnodes", C.c_int), ("nelem", C.c_int), ("nwalls", C.c_int),
-                ("xyz", _D), ("con", _I), ("bcid", _I), ("normmul", _D), ("itri", C.c_int)]
+                ("xyz", _D), ("con", _I), ("bcid", _I), ("normmul", _D), ("itri", C.c_int),
+                ("nvert", C.c_int), ("iquad", C.c_int)]
 
 
 class _Csr(C.Structure):
@@ -43,6 +44,7 @@ def lib():
             raise RuntimeError("oracle not built: run `make -C oracle`")
         _lib = C.CDLL(path)
         _lib.orc_si_tri_lap.restype = C.c_double
+        _lib.orc_si_qua_lap.restype = C.c_double
     return _lib
 
 
@@ -60,11 +62,12 @@ def ref_lib():
 class Body:
     """Keeps the numpy arrays alive for the C struct."""
 
-    def __init__(self, mesh, normmul=None, itri: int = 5):
+    def __init__(self, mesh, normmul=None, itri: int = 5, iquad: int = 4):
         m = mesh.contiguous()
         self.mesh = m
         self.normmul = np.ascontiguousarray(normmul if normmul is not None else np.ones(m.nwalls), dtype=np.float64)
-        self.s = _Body(m.nnodes, m.nelem, m.nwalls, _p(m.xyz), _pi(m.con), _pi(m.bcid), _p(self.normmul), itri)
+        self.s = _Body(m.nnodes, m.nelem, m.nwalls, _p(m.xyz), _pi(m.con), _pi(m.bcid), _p(self.normmul), itri,
+                       m.con.shape[1], iquad)
 
 
 def tri_rule(itri: int):
@@ -76,8 +79,44 @@ def tri_rule(itri: int):
 def geometry(mesh):
     m = mesh.contiguous()
     nrm = np.zeros((m.nelem, 3)); area = np.zeros(m.nelem); cen = np.zeros((m.nelem, 3))
-    lib().orc_geometry(m.nnodes, m.nelem, _p(m.xyz), _pi(m.con), _p(nrm), _p(area), _p(cen))
+    lib().orc_geometry_n(m.nnodes, m.nelem, _p(m.xyz), _pi(m.con), m.con.shape[1], _p(nrm), _p(area), _p(cen))
     return nrm, area, cen
+
+
+def gauss_rule(iq: int):
+    x = np.zeros(48); w = np.zeros(48)
+    n = lib().orc_gauss_rule(iq, _p(x), _p(w))
+    return x[:n].copy(), w[:n].copy()
+
+
+def _q4(x1, x2, x3, x4):
+    return [np.ascontiguousarray(v, dtype=np.float64) for v in (x1, x2, x3, x4)]
+
+
+def quad_lap_int(x1, x2, x3, x4, s, mul=1.0, iq=4):
+    x = _q4(x1, x2, x3, x4)
+    out = np.zeros(5)
+    lib().orc_quad_lap_int(*[_p(v) for v in x], _p(np.ascontiguousarray(s, dtype=np.float64)), C.c_double(mul), iq,
+                           _p(out))
+    return out
+
+
+def si_qua_lap(x1, x2, x3, x4, isrc):
+    return lib().orc_si_qua_lap(*[_p(v) for v in _q4(x1, x2, x3, x4)], isrc)
+
+
+def quad_stokes_int(x1, x2, x3, x4, s, mul=1.0, iq=4):
+    x = _q4(x1, x2, x3, x4)
+    acc = np.zeros(48)
+    lib().orc_quad_stokes_int(*[_p(v) for v in x], _p(np.ascontiguousarray(s, dtype=np.float64)), C.c_double(mul),
+                              iq, _p(acc))
+    return acc
+
+
+def si_qua_stk(x1, x2, x3, x4, isrc):
+    g = np.zeros(6)
+    lib().orc_si_qua_stk(*[_p(v) for v in _q4(x1, x2, x3, x4)], isrc, _p(g))
+    return g
 
 
 def si_tri_lap(x1, x2, x3, area, isrc):
@@ -106,16 +145,16 @@ def tri_stokes_int(x1, x2, x3, s, n, area):
     return out
 
 
-def assemble_laplace(mesh, normmul=None, itri=5):
-    b = Body(mesh, normmul, itri)
+def assemble_laplace(mesh, normmul=None, itri=5, iquad=4):
+    b = Body(mesh, normmul, itri, iquad)
     nsp = b.mesh.nnodes + b.mesh.nelem
     H = np.zeros((nsp, b.mesh.nnodes)); G = np.zeros((nsp, b.mesh.nelem)); err = np.zeros(2)
     lib().orc_assemble_laplace(C.byref(b.s), _p(H), _p(G), _p(err))
     return H, G, err
 
 
-def assemble_stokes(mesh, normmul=None, rows=None, extra=0.0, itri=5, pressure=False):
-    b = Body(mesh, normmul, itri)
+def assemble_stokes(mesh, normmul=None, rows=None, extra=0.0, itri=5, pressure=False, iquad=4):
+    b = Body(mesh, normmul, itri, iquad)
     nn, ne = b.mesh.nnodes, b.mesh.nelem
     nsp = nn + ne
     rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
