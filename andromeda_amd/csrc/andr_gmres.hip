@@ -7,9 +7,8 @@
 // one launch per step covers every system (work lists map waves and row tiles to systems), so the
 // launch count does not grow with the number of bodies.
 //
-// Orthogonalisation is classical Gram-Schmidt with a second pass where the first one lost more
-// than 1/sqrt(2) of the vector's norm (the DGKS test, decided on the device per system): both
-// passes are block products over the basis, which keeps the basis traffic streaming.
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): both passes are block
+// products over the basis, which keeps the basis traffic streaming instead of latency bound.
 #include "andr_internal.h"
 #include "andr_solve.h"
 #include "andr_prof.h"
@@ -24,6 +23,7 @@
 namespace {
 
 constexpr int NB = ANDR_NB;
+static_assert(NB == 8, "the matrix-core product packs two 8-column halves into the 16 B columns");
 constexpr int TILE = 64;   // rows per vector tile (256 threads: 32 rows x 8 right-hand sides per pass)
 constexpr int ICH = 16;   // basis vectors per block of the projection kernel
 
@@ -48,7 +48,7 @@ struct GSys {  // device view of one system
   // structured Stokes operator (fp32 path)
   StokesOp op;
   int nep;       // op.ne rounded up to 32
-  float *Z32;    // 3 x NB x nep: z = M^-1 q_j in FP32, [component][rhs][element]
+  double *ZT;    // 3 x NB x nep: z = M^-1 q_j, [component][rhs][element] (Krylov product operand)
   double *Z64;   // 3 ne x NB: z = M^-1 x in FP64 (residual)
   double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
   int et0, net;  // element tiles (32 elements) of the system in the current map
@@ -127,12 +127,12 @@ __global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, con
   if (part && lane < 8) part[(size_t)wave * NB + lane] = y2;
 }
 
-typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int MAXW = ANDR_OP_MAXW;
 
 // ---- structured Stokes operator --------------------------------------------------------------
 // per tile of 32 elements (256 threads = 32 elements x 8 right-hand sides): z_e = M^-1_e v_e for the
-// three components of element e, v = Q_j (mode 0, z to Z32 in FP32) or v = X (mode 1, z to Z64 in
+// three components of element e, v = Q_j (mode 0, z to ZT) or v = X (mode 1, z to Z64 in
 // FP64); partial sums of n_e . z_e per wall (1..MAXW) and rhs for the null-space term.
 __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, const TileMap *__restrict__ emap, int mode,
                                                int jq, double *__restrict__ part_s) {
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
                          mi[6] * q0 + mi[7] * q1 + mi[8] * q2};
     if (mode == 0) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) S.Z32[((size_t)c * NB + k) * S.nep + e] = (float)z[c];
+      for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = z[c];
     } else {
 #pragma unroll
       for (int c = 0; c < 3; ++c) S.Z64[(size_t)(c * ne + e) * NB + k] = z[c];
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
     dz = O.nrm[3 * e] * z[0] + O.nrm[3 * e + 1] * z[1] + O.nrm[3 * e + 2] * z[2];
   } else if (mode == 0 && e < S.nep) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) S.Z32[((size_t)c * NB + k) * S.nep + e] = 0.f;
+    for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = 0.0;
   }
 #pragma unroll
   for (int ww = 0; ww < MAXW; ++ww) {
@@ -201,14 +201,22 @@ __device__ inline double defl_term(const StokesOp &O, const double *sdef, int r,
   return d != 0.0 ? d * O.area[r] * O.nrm[3 * r + c] * sdef[(w - 1) * NB + k] : 0.0;
 }
 
-// The Krylov product W = A M^-1 Q_j on the matrix cores: y_c = sum_en S_{c,en} z_en (+ the null-space
-// term), S the six symmetric FP32 blocks, z = M^-1 q_j rounded to FP32, v_mfma_f32_16x16x4f32 with
-// the 8 right-hand sides in the 16 B columns (8 unused). A wave owns 32 element rows (two 16-row
-// bands) over a column segment; lane (g = lane / 16, c = lane % 16) feeds row 16 T + c with columns
-// 16 kb + 4 g + t of every block (one float4 per block and band: 1 KB contiguous per instruction)
-// and column c of B with the same columns. Nine block products reuse six blocks. The FP32
-// accumulators are added into FP64 every 64 columns. Output lane layout: D[4 g + v][c].
-__global__ __launch_bounds__(256) void kb_gemv_sym(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
+// The Krylov product W = A M^-1 Q_j on the FP64 matrix cores: y_c = sum_en S_{c,en} z_en (+ the
+// null-space term), S the six symmetric FP32 blocks (converted to FP64 in registers, exactly),
+// z = M^-1 q_j in FP64, products and sums in FP64 (v_mfma_f64_16x16x4f64), so the Krylov relation
+// holds to FP64 rounding and the true residual follows the GMRES estimate.
+//
+// The B operand has 16 columns for 8 right-hand sides: each block product fills both halves.
+// With h = column half, three accumulators per 16-row band take the six blocks:
+//   P: xx [z_x | 0] + xy [z_y | z_x] + yy [0 | z_y]  ->  [y_x part | y_y part]
+//   Q: xz [z_x | z_z] + zz [z_z | 0]                  ->  [y_z part | y_x part]
+//   R: yz [z_z | z_y]                                 ->  [y_y part | y_z part]
+// so y_x = P_lo + Q_hi, y_y = R_lo + P_hi, y_z = Q_lo + R_hi: six block products per column step,
+// each matrix block read once. A wave owns 32 element rows (two bands) over a column segment; lane
+// (g = lane / 16, c = lane % 16) feeds row 16 T + c with columns 16 kb + 4 g + t of every block (one
+// float4 per block and band) and column c of B with the same columns. The FP64 variant returns
+// D[g + 4 v][c] in register v of lane (g, c) (rows interleaved, unlike the FP32 16x16x4 form).
+__global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
                                                    int nwaves) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -218,97 +226,110 @@ __global__ __launch_bounds__(256) void kb_gemv_sym(const GSys *__restrict__ sys,
   const StokesOp &O = S.op;
   const int ne = O.ne, row0 = wm.row0, kend = min(wm.k1, ne), KBe = (ne + 15) / 16;
   const int g = lane >> 4, c = lane & 15;
-  const bool cl = c < NB;
+  const bool hi = c >= NB;
   const float *ab[2];
 #pragma unroll
   for (int T = 0; T < 2; ++T) ab[T] = O.S + (size_t)((row0 >> 4) + T) * KBe * 1536 + 16 * c + 4 * g;
-  const float *zb[3];
+  const double *zb[3];
 #pragma unroll
-  for (int en = 0; en < 3; ++en) zb[en] = S.Z32 + ((size_t)en * NB + (cl ? c : 0)) * S.nep + 4 * g;
-  f4v acc[2][3];
-  double acc64[2][3][4];
+  for (int en = 0; en < 3; ++en) zb[en] = S.ZT + ((size_t)en * NB + (c & (NB - 1))) * S.nep + 4 * g;
+  d4v acc[2][3];
 #pragma unroll
   for (int T = 0; T < 2; ++T)
 #pragma unroll
-    for (int co = 0; co < 3; ++co) {
-      acc[T][co] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < 3; ++a) acc[T][a] = d4v{0.0, 0.0, 0.0, 0.0};
+  // one column step t of a 16-column block: operands a[T][q][t], z[en][t]
+  auto step = [&](const float (&a)[2][6][4], const double (&z)[3][4], int t) {
+    const double z0 = z[0][t], z1 = z[1][t], z2 = z[2][t];
+    const double b0 = hi ? 0.0 : z0, b1 = hi ? z0 : z1, b3 = hi ? z1 : 0.0;
+    const double b2 = hi ? z2 : z0, b5 = hi ? 0.0 : z2, b4 = hi ? z1 : z2;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc64[T][co][v] = 0.0;
+    for (int T = 0; T < 2; ++T) {
+      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][0][t], b0, acc[T][0], 0, 0, 0);
+      acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][2][t], b2, acc[T][1], 0, 0, 0);
+      acc[T][2] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][4][t], b4, acc[T][2], 0, 0, 0);
+      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][1][t], b1, acc[T][0], 0, 0, 0);
+      acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][5][t], b5, acc[T][1], 0, 0, 0);
+      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][3][t], b3, acc[T][0], 0, 0, 0);
     }
-  auto flush = [&]() {
-#pragma unroll
-    for (int T = 0; T < 2; ++T)
-#pragma unroll
-      for (int co = 0; co < 3; ++co) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc64[T][co][v] += (double)acc[T][co][v];
-        acc[T][co] = f4v{0.f, 0.f, 0.f, 0.f};
-      }
   };
-  int kb = wm.k0 >> 4, step = 0;
-  for (; 16 * kb + 16 <= kend; ++kb) {
-    float4 a[2][6], z[3];
+  // full column blocks, software-pipelined: the operands of block kb + 1 are requested before the
+  // products of block kb, so that their memory latency overlaps the matrix-core work
+  auto load = [&](int k, float (&a)[2][6][4], double (&z)[3][4]) {
 #pragma unroll
     for (int T = 0; T < 2; ++T)
 #pragma unroll
-      for (int q = 0; q < 6; ++q) a[T][q] = *reinterpret_cast<const float4 *>(ab[T] + (size_t)kb * 1536 + q * 256);
+      for (int q = 0; q < 6; ++q) {
+        const float4 v = *reinterpret_cast<const float4 *>(ab[T] + (size_t)k * 1536 + q * 256);
+        a[T][q][0] = v.x;
+        a[T][q][1] = v.y;
+        a[T][q][2] = v.z;
+        a[T][q][3] = v.w;
+      }
 #pragma unroll
-    for (int en = 0; en < 3; ++en)
-      z[en] = cl ? *reinterpret_cast<const float4 *>(zb[en] + 16 * kb) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int en = 0; en < 3; ++en) {
+      const double2 u = *reinterpret_cast<const double2 *>(zb[en] + 16 * k);
+      const double2 w = *reinterpret_cast<const double2 *>(zb[en] + 16 * k + 2);
+      z[en][0] = u.x;
+      z[en][1] = u.y;
+      z[en][2] = w.x;
+      z[en][3] = w.y;
+    }
+  };
+  int kb = wm.k0 >> 4;
+  if (16 * kb + 16 <= kend) {
+    float an[2][6][4];
+    double zn[3][4];
+    load(kb, an, zn);
+    for (; 16 * kb + 16 <= kend; ++kb) {
+      float a[2][6][4];
+      double z[3][4];
 #pragma unroll
-    for (int T = 0; T < 2; ++T)
+      for (int T = 0; T < 2; ++T)
 #pragma unroll
-      for (int co = 0; co < 3; ++co)
+        for (int q = 0; q < 6; ++q)
 #pragma unroll
-        for (int en = 0; en < 3; ++en) {
-          const float4 av = a[T][sym3(co, en)];
-          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, z[en].x, acc[T][co], 0, 0, 0);
-          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, z[en].y, acc[T][co], 0, 0, 0);
-          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, z[en].z, acc[T][co], 0, 0, 0);
-          acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, z[en].w, acc[T][co], 0, 0, 0);
-        }
-    if ((++step & 3) == 0) flush();
+          for (int t = 0; t < 4; ++t) a[T][q][t] = an[T][q][t];
+#pragma unroll
+      for (int en = 0; en < 3; ++en)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) z[en][t] = zn[en][t];
+      if (16 * kb + 32 <= kend) load(kb + 1, an, zn);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) step(a, z, t);
+    }
   }
   if (16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
-    float a[2][6][4], z[3][4];
+    float a[2][6][4];
+    double z[3][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const bool in = 16 * kb + 4 * g + t < kend;
 #pragma unroll
-      for (int en = 0; en < 3; ++en) z[en][t] = (in && cl) ? zb[en][16 * kb + t] : 0.f;
+      for (int en = 0; en < 3; ++en) z[en][t] = in ? zb[en][16 * kb + t] : 0.0;
 #pragma unroll
       for (int T = 0; T < 2; ++T)
 #pragma unroll
         for (int q = 0; q < 6; ++q) a[T][q][t] = in ? ab[T][(size_t)kb * 1536 + q * 256 + t] : 0.f;
     }
 #pragma unroll
-    for (int T = 0; T < 2; ++T)
+    for (int t = 0; t < 4; ++t) step(a, z, t);
+  }
+  const bool direct = S.nseg <= 1;
+  double *Y = direct ? S.W : S.P + (size_t)wm.seg * S.n * NB;
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double p = acc[T][0][v], q = acc[T][1][v], rr = acc[T][2][v];
+      const double ph = __shfl_xor(p, NB, 64), qh = __shfl_xor(q, NB, 64), rh = __shfl_xor(rr, NB, 64);
+      const int r = row0 + 16 * T + g + 4 * v;
+      if (hi || r >= ne) continue;
+      const double y[3] = {p + qh, rr + ph, q + rh};
 #pragma unroll
       for (int co = 0; co < 3; ++co)
-#pragma unroll
-        for (int en = 0; en < 3; ++en)
-#pragma unroll
-          for (int t = 0; t < 4; ++t)
-            acc[T][co] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][sym3(co, en)][t], z[en][t], acc[T][co], 0, 0, 0);
-  }
-  flush();
-  if (cl) {
-    const bool direct = S.nseg <= 1;
-    double *Y = direct ? S.W : S.P + (size_t)wm.seg * S.n * NB;
-#pragma unroll
-    for (int T = 0; T < 2; ++T)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = row0 + 16 * T + 4 * g + v;
-        if (r >= ne) continue;
-#pragma unroll
-        for (int co = 0; co < 3; ++co) {
-          double y = acc64[T][co][v];
-          if (wm.seg == 0) y += defl_term(O, S.sdef, r, co, c);
-          Y[(size_t)(co * ne + r) * NB + c] = y;
-        }
-      }
-  }
+        Y[(size_t)(co * ne + r) * NB + c] = y[co] + (wm.seg == 0 ? defl_term(O, S.sdef, r, co, c) : 0.0);
+    }
 }
 
 // R = B - A M^-1 X for the structured operator in FP64 (the residual of the refinement loop), from
@@ -419,18 +440,14 @@ __global__ __launch_bounds__(256) void kb_seg_sum(const GSys *__restrict__ sys, 
 }
 
 // per (tile, chunk of ICH basis vectors): partial dots part[tile][i][k] = sum_rows Q_i[row][k] W[row][k]
-// Pass 0 also stores sum W^2 of the tile in slot mstride - 1 (the norm before projection, for the
-// reorthogonalisation test); pass 1 is skipped for systems whose test passed.
 __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
-                                               int mstride, double *__restrict__ part, int pass) {
+                                               int mstride, double *__restrict__ part) {
   __shared__ double red[4][ICH][NB];
-  __shared__ double rw[4][NB];
   const int i0 = blockIdx.y * ICH;
   if (i0 > j) return;  // uniform over the block
   const int ni = min(ICH, j + 1 - i0);
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
-  if (pass == 1 && S.small[6] > 0.5) return;  // uniform: second pass not needed
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -457,16 +474,6 @@ __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, con
     a += __shfl_xor(a, 32, 64);
     acc[ii] = a;
   }
-  const bool norm = pass == 0 && blockIdx.y == 0;
-  if (norm) {
-    double a = 0.0;
-#pragma unroll
-    for (int t = 0; t < TILE / 32; ++t) a += w[t] * w[t];
-    a += __shfl_xor(a, 8, 64);
-    a += __shfl_xor(a, 16, 64);
-    a += __shfl_xor(a, 32, 64);
-    if (lane < 8) rw[wid][lane] = a;
-  }
   if (lane < 8)
 #pragma unroll
     for (int ii = 0; ii < ICH; ++ii) red[wid][ii][lane] = acc[ii];
@@ -476,67 +483,73 @@ __global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, con
     part[((size_t)blockIdx.x * mstride + i0 + ii) * NB + kk] =
         red[0][ii][kk] + red[1][ii][kk] + red[2][ii][kk] + red[3][ii][kk];
   }
-  if (norm && threadIdx.x < NB)
-    part[((size_t)blockIdx.x * mstride + mstride - 1) * NB + threadIdx.x] =
-        rw[0][threadIdx.x] + rw[1][threadIdx.x] + rw[2][threadIdx.x] + rw[3][threadIdx.x];
 }
 
-// thread per system: the second Gram-Schmidt pass is needed unless, for every live rhs, the
-// projection kept at least 1/sqrt(2) of the norm (Daniel-Gragg-Kaufman-Stewart criterion).
-// small[6] of rhs 0 carries the decision (1: skip the second pass).
-__global__ void kb_reorth_test(const GSys *__restrict__ sys, int nsys, int mstride, const double *__restrict__ part,
-                               const double *__restrict__ part2) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys) return;
-  const GSys &S = sys[s];
-  bool skip = S.ntiles > 0;
-  for (int k = 0; k < NB && skip; ++k) {
-    const double *sm = S.small + k * 8;
-    if (!(sm[4] > 0.0 && sm[5] < 0.0)) continue;  // rhs not advancing
-    double before = 0.0, after = 0.0;
-    for (int tl = 0; tl < S.ntiles; ++tl) {
-      before += part[((size_t)(S.tile0 + tl) * mstride + mstride - 1) * NB + k];
-      after += part2[(size_t)(S.tile0 + tl) * NB + k];
-    }
-    if (!(after >= 0.5 * before)) skip = false;
-  }
-  S.small[6] = skip ? 1.0 : 0.0;
-}
-
-// thread per (system, i, k): coef = sum over the system's tiles; H(i, j) += coef
-__global__ void kb_dots_final(const GSys *__restrict__ sys, int nsys, int j, int m, int mstride,
-                              const double *__restrict__ part, int first_pass) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int per = (j + 1) * NB;
-  if (t >= nsys * per) return;
-  const int s = t / per, rem = t % per, i = rem / NB, k = rem % NB;
-  const GSys &S = sys[s];
-  if (!first_pass && S.small[6] > 0.5) return;
+// block per (system, 8 basis vectors): coef = sum over the system's tiles, the four waves taking
+// every fourth tile; H(i, j) += coef. Systems outside the current map have no tiles and return.
+__global__ __launch_bounds__(256) void kb_dots_final(const GSys *__restrict__ sys, int j, int m, int mstride,
+                                                     const double *__restrict__ part, int first_pass) {
+  __shared__ double red[4][64];
+  const GSys &S = sys[blockIdx.x];
+  if (S.ntiles == 0) return;  // uniform over the block
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = lane & 7, i = blockIdx.y * 8 + (lane >> 3);
   double c = 0.0;
-  for (int tl = 0; tl < S.ntiles; ++tl) c += part[((size_t)(S.tile0 + tl) * mstride + i) * NB + k];
-  S.coef[k * (m + 1) + i] = c;
-  double &h = S.H[(size_t)k * (m + 1) * m + (size_t)j * (m + 1) + i];
-  h = first_pass ? c : h + c;
+  if (i <= j)
+    for (int tl = wv; tl < S.ntiles; tl += 4) c += part[((size_t)(S.tile0 + tl) * mstride + i) * NB + k];
+  red[wv][lane] = c;
+  __syncthreads();
+  if (wv == 0 && i <= j) {
+    c = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    S.coef[k * (m + 1) + i] = c;
+    double &h = S.H[(size_t)k * (m + 1) * m + (size_t)j * (m + 1) + i];
+    h = first_pass ? c : h + c;
+  }
 }
 
-// per tile: W -= sum_i Q_i coef_i ; part2[tile][k] = sum W^2
+// per tile: W -= sum_i Q_i coef_i ; part2[tile][k] = sum W^2. Both rows of a thread and four basis
+// vectors per iteration are in flight together (the loop is bound by load latency otherwise).
 __global__ __launch_bounds__(256) void kb_sub(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j, int m,
-                                              double *__restrict__ part2, int pass) {
+                                              double *__restrict__ part2) {
   __shared__ double red[4][NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
-  if (pass == 1 && S.small[6] > 0.5) return;  // uniform: part2 keeps the first pass's norms
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  static_assert(TILE == 64, "two rows per thread");
+  const int r0 = tm.row0 + rr, r1 = tm.row0 + rr + 32;
+  const bool v0 = r0 < n, v1 = r1 < n;
+  const int q0 = v0 ? r0 : 0, q1 = v1 ? r1 : q0;  // in-range rows for the loads
+  const double *cf = S.coef + k * (m + 1);
+  const size_t vs = (size_t)n * NB;
+  const double *Qa = S.Q + (size_t)q0 * NB + k, *Qb = S.Q + (size_t)q1 * NB + k;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+  int i = 0;
+  for (; i + 4 <= j + 1; i += 4) {
+    const double c0 = cf[i], c1 = cf[i + 1], c2 = cf[i + 2], c3 = cf[i + 3];
+    a0 += Qa[(size_t)i * vs] * c0;
+    a1 += Qa[(size_t)(i + 1) * vs] * c1;
+    a2 += Qa[(size_t)(i + 2) * vs] * c2;
+    a3 += Qa[(size_t)(i + 3) * vs] * c3;
+    b0 += Qb[(size_t)i * vs] * c0;
+    b1 += Qb[(size_t)(i + 1) * vs] * c1;
+    b2 += Qb[(size_t)(i + 2) * vs] * c2;
+    b3 += Qb[(size_t)(i + 3) * vs] * c3;
+  }
+  for (; i <= j; ++i) {
+    a0 += Qa[(size_t)i * vs] * cf[i];
+    b0 += Qb[(size_t)i * vs] * cf[i];
+  }
   double w2 = 0.0;
-#pragma unroll
-  for (int t = 0; t < TILE / 32; ++t) {
-    const int row = tm.row0 + rr + 32 * t;
-    if (row >= n) continue;
-    double w = S.W[(size_t)row * NB + k];
-    for (int i = 0; i <= j; ++i) w -= S.Q[((size_t)i * n + row) * NB + k] * S.coef[k * (m + 1) + i];
-    S.W[(size_t)row * NB + k] = w;
+  if (v0) {
+    const double w = S.W[(size_t)r0 * NB + k] - ((a0 + a1) + (a2 + a3));
+    S.W[(size_t)r0 * NB + k] = w;
+    w2 += w * w;
+  }
+  if (v1) {
+    const double w = S.W[(size_t)r1 * NB + k] - ((b0 + b1) + (b2 + b3));
+    S.W[(size_t)r1 * NB + k] = w;
     w2 += w * w;
   }
   w2 += __shfl_xor(w2, 8, 64);
@@ -712,7 +725,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // full work maps (sizes the partial-sum buffers); subsets are rebuilt as systems converge
   // fp32: the structured Stokes operator of the fast path; otherwise a dense FP64 matrix
   const bool structured = fp32;
-  size_t vec_elems = 0, basis_elems = 0, z32_elems = 0;
+  size_t vec_elems = 0, basis_elems = 0, zt_elems = 0;
   int nw_full = 0, nt_full = 0, net_full = 0;
   for (int s = 0; s < nsys; ++s) {
     const int n = in[s].n, ne = in[s].op.ne;
@@ -723,14 +736,14 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     nw_full += structured ? 4 * ((ne + 63) / 64) : (n + WROWS - 1) / WROWS;
     nt_full += (n + TILE - 1) / TILE;
     net_full += structured ? (ne + 31) / 32 : 0;
-    z32_elems += structured ? (size_t)3 * NB * ((ne + 31) / 32 * 32) : 0;
+    zt_elems += structured ? (size_t)3 * NB * ((ne + 31) / 32 * 32) : 0;
   }
   // per system: Hessenberg, rotations, g, coefficients; the per-rhs state (8 doubles per rhs) of all
   // systems sits in one separate array so that the host reads it with one copy
   const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1));
   const size_t state_per = (size_t)NB * 8;
   DevBuf<double> R, W, Q, SM, ST8, partW, partT, partT2, Z64, SDEF, partS;
-  DevBuf<float> Z32;
+  DevBuf<double> ZT;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
   if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
@@ -738,7 +751,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       partT.alloc((size_t)nt_full * (m + 1) * NB, ctx) || partT2.alloc((size_t)nt_full * NB, ctx) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
-  if (structured && (Z64.alloc(vec_elems, ctx) || Z32.alloc(z32_elems, ctx) ||
+  if (structured && (Z64.alloc(vec_elems, ctx) || ZT.alloc(zt_elems, ctx) ||
                      SDEF.alloc((size_t)nsys * MAXW * NB, ctx) || partS.alloc((size_t)net_full * MAXW * NB, ctx)))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES operator workspace");
   // a map set over a list of systems: waves of WROWS rows (FP64-accumulating product), tiles of
@@ -853,7 +866,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     g.op = in[s].op;
     if (structured) {
       g.nep = (g.op.ne + 31) / 32 * 32;
-      g.Z32 = Z32.p + zo;
+      g.ZT = ZT.p + zo;
       g.Z64 = Z64.p + vo;
       g.sdef = SDEF.p + (size_t)s * MAXW * NB;
       zo += (size_t)3 * NB * g.nep;
@@ -981,8 +994,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
       gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
-      // classical Gram-Schmidt, second pass only where the first lost more than 1/sqrt(2) of the
-      // norm (DGKS); each pass streams the basis twice
+      // CGS2: two passes of (coefficients Q^T W, W -= Q coef); each pass streams the basis twice
       double nlive = 0.0;
       for (int s = 0; s < nsys; ++s)
         if (live.ntiles[s]) nlive += in[s].n;
@@ -991,14 +1003,10 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
                                         2.0 * 4.0 * (j + 1) * nlive * NB);
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1,
-                           partT.p, pass);
-        const int nthr = nsys * (j + 1) * NB;
-        hipLaunchKernelGGL(kb_dots_final, dim3((nthr + 255) / 256), dim3(256), 0, st, dsys.p, nsys, j, m, m + 1,
-                           partT.p, pass == 0 ? 1 : 0);
-        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p, pass);
-        if (pass == 0)
-          hipLaunchKernelGGL(kb_reorth_test, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys.p, nsys, m + 1, partT.p,
-                             partT2.p);
+                           partT.p);
+        hipLaunchKernelGGL(kb_dots_final, dim3(nsys, (j + 8) / 8), dim3(256), 0, st, dsys.p, j, m, m + 1, partT.p,
+                           pass == 0 ? 1 : 0);
+        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p);
       }
       andr_prof_end(ctx, tok_o);
       const int tok_g = andr_prof_begin(ctx, "gmres_givens_scale", 2.0 * vbytes, 0.0);
