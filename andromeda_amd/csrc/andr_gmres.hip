@@ -52,6 +52,7 @@ struct GSys {  // device view of one system
   double *Z64;   // 3 ne x NB: z = M^-1 x in FP64 (residual)
   double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
   int et0, net;  // element tiles (32 elements) of the system in the current map
+  int incyc;     // 1 while the system takes part in the current restart cycle
 };
 
 struct WaveMap {
@@ -562,41 +563,53 @@ __global__ __launch_bounds__(256) void kb_sub(const GSys *__restrict__ sys, cons
         red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
-// thread per (system, k): h(j+1, j) = ||W||, Givens rotations, residual estimate
-__global__ void kb_givens(const GSys *__restrict__ sys, int nsys, int j, int m, const double *__restrict__ part2,
-                          double tol) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nsys * NB) return;
-  const int s = t / NB, k = t % NB;
+// wave per (system, rhs) of the current map: h(j+1, j) = ||W||, Givens rotations, residual
+// estimate. The Hessenberg column and the rotations are staged in LDS (3 m + 1 doubles of dynamic
+// shared memory) so that the serial rotation chain of lane 0 does not wait on global memory.
+__global__ __launch_bounds__(64) void kb_givens(const GSys *__restrict__ sys, int j, int m,
+                                                const double *__restrict__ part2, double tol) {
+  extern __shared__ double lds[];
+  const int s = blockIdx.x / NB, k = blockIdx.x % NB, lane = threadIdx.x;
   const GSys &S = sys[s];
-  double *sm = S.small + k * 8;
+  if (S.ntiles == 0) return;  // uniform over the block
+  double *lh = lds, *lc = lds + (m + 1), *ls = lds + (2 * m + 1);
   double hn2 = 0.0;
-  for (int tl = 0; tl < S.ntiles; ++tl) hn2 += part2[(size_t)(S.tile0 + tl) * NB + k];
-  const double hn = sqrt(hn2);
+  for (int tl = lane; tl < S.ntiles; tl += 64) hn2 += part2[(size_t)(S.tile0 + tl) * NB + k];
+  for (int off = 32; off > 0; off >>= 1) hn2 += __shfl_xor(hn2, off, 64);
   double *Hc = S.H + (size_t)k * (m + 1) * m + (size_t)j * (m + 1);
-  Hc[j + 1] = hn;
-  const bool live = sm[4] > 0.0 && sm[5] < 0.0;  // active and not yet frozen in this cycle
-  // apply previous rotations
   double *cs = S.cs + k * m, *sn = S.sn + k * m, *g = S.g + k * (m + 1);
-  for (int i = 0; i < j; ++i) {
-    const double a = Hc[i], b = Hc[i + 1];
-    Hc[i] = cs[i] * a + sn[i] * b;
-    Hc[i + 1] = -sn[i] * a + cs[i] * b;
+  for (int i = lane; i <= j; i += 64) lh[i] = Hc[i];
+  for (int i = lane; i < j; i += 64) {
+    lc[i] = cs[i];
+    ls[i] = sn[i];
   }
-  const double a = Hc[j], b = Hc[j + 1];
-  const double rr = sqrt(a * a + b * b);
-  const double c = rr > 0.0 ? a / rr : 1.0, sg = rr > 0.0 ? b / rr : 0.0;
-  cs[j] = c;
-  sn[j] = sg;
-  Hc[j] = rr;
-  Hc[j + 1] = 0.0;
-  g[j + 1] = -sg * g[j];
-  g[j] = c * g[j];
-  const double est = fabs(g[j + 1]);
-  sm[2] = est;
-  const double tiny = 1e-300;
-  sm[3] = hn > tiny ? 1.0 / hn : 0.0;
-  if (live && (est <= 0.1 * tol * sm[0] || hn <= 1e-14 * rr)) sm[5] = (double)(j + 1);  // steps to use
+  __syncthreads();
+  if (lane == 0) {
+    double *sm = S.small + k * 8;
+    const double hn = sqrt(hn2);
+    const bool live = sm[4] > 0.0 && sm[5] < 0.0;  // active and not yet frozen in this cycle
+    double cur = lh[0];  // apply the previous rotations
+    for (int i = 0; i < j; ++i) {
+      const double b = lh[i + 1], c = lc[i], sg = ls[i];
+      lh[i] = c * cur + sg * b;
+      cur = -sg * cur + c * b;
+    }
+    const double a = cur, b = hn;
+    const double rr = sqrt(a * a + b * b);
+    const double c = rr > 0.0 ? a / rr : 1.0, sg = rr > 0.0 ? b / rr : 0.0;
+    cs[j] = c;
+    sn[j] = sg;
+    lh[j] = rr;
+    Hc[j + 1] = 0.0;
+    g[j + 1] = -sg * g[j];
+    g[j] = c * g[j];
+    const double est = fabs(g[j + 1]);
+    sm[2] = est;
+    sm[3] = hn > 1e-300 ? 1.0 / hn : 0.0;
+    if (live && (est <= 0.1 * tol * sm[0] || hn <= 1e-14 * rr)) sm[5] = (double)(j + 1);  // steps to use
+  }
+  __syncthreads();
+  for (int i = lane; i <= j; i += 64) Hc[i] = lh[i];
 }
 
 // per tile: Q_{j+1} = W * inv_h
@@ -614,24 +627,28 @@ __global__ __launch_bounds__(256) void kb_scale_next(const GSys *__restrict__ sy
   }
 }
 
-// thread per (system, k): back substitution y = R^{-1} g over the steps used; y stored in coef
-__global__ void kb_solve_y(const GSys *__restrict__ sys, int nsys, int steps, int m) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nsys * NB) return;
-  const int s = t / NB, k = t % NB;
+// wave per (system, rhs): back substitution y = R^{-1} g over the steps used (y stored in coef),
+// column by column: y_i = g_i / R_ii, then the lanes update g_l -= R_li y_i for l < i (g in LDS)
+__global__ __launch_bounds__(64) void kb_solve_y(const GSys *__restrict__ sys, int steps, int m) {
+  extern __shared__ double lg[];
+  const int s = blockIdx.x / NB, k = blockIdx.x % NB, lane = threadIdx.x;
   const GSys &S = sys[s];
-  double *sm = S.small + k * 8;
+  if (!S.incyc) return;  // uniform over the block
+  const double *sm = S.small + k * 8;
   double *y = S.coef + k * (m + 1);
   int kk = sm[5] > 0.0 ? (int)sm[5] : steps;
   if (sm[4] <= 0.0) kk = 0;
   const double *H = S.H + (size_t)k * (m + 1) * m;
   const double *g = S.g + k * (m + 1);
-  for (int i = 0; i <= m; ++i) y[i] = 0.0;
+  for (int i = lane; i <= m; i += 64) y[i] = 0.0;
+  for (int i = lane; i < kk; i += 64) lg[i] = g[i];
+  __syncthreads();
   for (int i = kk - 1; i >= 0; --i) {
-    double v = g[i];
-    for (int c = i + 1; c < kk; ++c) v -= H[(size_t)c * (m + 1) + i] * y[c];
     const double d = H[(size_t)i * (m + 1) + i];
-    y[i] = d != 0.0 ? v / d : 0.0;
+    const double yi = d != 0.0 ? lg[i] / d : 0.0;
+    for (int l = lane; l < i; l += 64) lg[l] -= H[(size_t)i * (m + 1) + l] * yi;
+    if (lane == 0) y[i] = yi;
+    __syncthreads();
   }
 }
 
@@ -666,7 +683,7 @@ __global__ __launch_bounds__(256) void kb_copy_b(const GSys *__restrict__ sys, c
 }
 
 // thread per (system, k): true residual norm from the gemv partials; restart set-up
-__global__ void kb_restart(const GSys *__restrict__ sys, int nsys, int m, const double *__restrict__ part,
+__global__ void kb_restart(const GSys *__restrict__ sys, int nsys, const double *__restrict__ part,
                            const int *__restrict__ wave0, const int *__restrict__ nwv, double tol, int first,
                            int nrhs, int *__restrict__ done) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,11 +711,21 @@ __global__ void kb_restart(const GSys *__restrict__ sys, int nsys, int m, const 
     sm[7] = 1.0;
   }
   sm[5] = -1.0;
-  double *g = S.g + k * (m + 1);
-  for (int i = 0; i <= m; ++i) g[i] = 0.0;
-  g[0] = sm[4] > 0.0 ? rn : 0.0;
   sm[3] = (sm[4] > 0.0 && rn > 0.0) ? 1.0 / rn : 0.0;
   if (sm[4] > 0.0) atomicAdd(done, 1);
+}
+
+// thread per (system, k) of the cycle: g = (||r||, 0, ..., 0) in the cycle's layout
+__global__ void kb_ginit(const GSys *__restrict__ sys, int nsys, int m) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * NB) return;
+  const int s = t / NB, k = t % NB;
+  const GSys &S = sys[s];
+  if (S.ntiles == 0) return;
+  const double *sm = S.small + k * 8;
+  double *g = S.g + k * (m + 1);
+  for (int i = 0; i <= m; ++i) g[i] = 0.0;
+  g[0] = sm[4] > 0.0 ? sm[1] : 0.0;
 }
 
 // per tile: Q_0 = R * (1/||r||)
@@ -859,7 +886,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ANDR_HIP(ctx, hipMemcpyAsync(dsys.p, hs.data(), sizeof(GSys) * nsys, hipMemcpyHostToDevice, st));
     return ANDR_OK;
   };
-  size_t vo = 0, bo = 0, zo = 0;
+  size_t vo = 0, zo = 0;
   for (int s = 0; s < nsys; ++s) {
     GSys &g = hs[s];
     const int n = in[s].n;
@@ -878,20 +905,53 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     g.X = in[s].X;
     g.R = R.p + vo;
     g.W = W.p + vo;
-    g.Q = Q.p + bo;
-    double *sp = SM.p + small_per * s;
-    g.H = sp;
-    sp += (size_t)NB * (m + 1) * m;
-    g.cs = sp;
-    sp += NB * m;
-    g.sn = sp;
-    sp += NB * m;
-    g.g = sp;
-    sp += NB * (m + 1);
-    g.coef = sp;
     g.small = ST8.p + state_per * s;
     vo += (size_t)n * NB;
-    bo += (size_t)(m + 1) * n * NB;
+  }
+  // Basis and Hessenberg storage of the systems of a cycle, for restart length mc. The first cycle
+  // takes every system at length m; the systems left after it (slender bodies whose hard columns
+  // converge slowly under restarts) are laid out again with the longest restart that fits the same
+  // buffers (Q, SM, partT), up to MMAX.
+  constexpr int MMAX = 1024;
+  auto small_of = [&](int mc) { return (size_t)NB * ((size_t)(mc + 1) * mc + 2 * mc + 2 * (mc + 1)); };
+  auto layout = [&](const std::vector<int> &list, int mc) {
+    size_t bo = 0, so = 0;
+    for (int s = 0; s < nsys; ++s) hs[s].incyc = 0;
+    for (int s : list) {
+      GSys &g = hs[s];
+      g.incyc = 1;
+      g.Q = Q.p + bo;
+      bo += (size_t)(mc + 1) * in[s].n * NB;
+      double *sp = SM.p + so;
+      so += small_of(mc);
+      g.H = sp;
+      sp += (size_t)NB * (mc + 1) * mc;
+      g.cs = sp;
+      sp += NB * mc;
+      g.sn = sp;
+      sp += NB * mc;
+      g.g = sp;
+      sp += NB * (mc + 1);
+      g.coef = sp;
+    }
+  };
+  auto restart_for = [&](const std::vector<int> &list) {
+    size_t nsum = 0, ntl = 0;
+    for (int s : list) {
+      nsum += in[s].n;
+      ntl += (in[s].n + TILE - 1) / TILE;
+    }
+    for (int mc = MMAX; mc > m; mc -= 8)
+      if ((size_t)(mc + 1) * nsum * NB <= basis_elems && list.size() * small_of(mc) <= small_per * nsys &&
+          ntl * (mc + 1) <= (size_t)nt_full * (m + 1))
+        return mc;
+    return m;
+  };
+  int mc = m;
+  {
+    std::vector<int> all(nsys);
+    for (int s = 0; s < nsys; ++s) all[s] = s;
+    layout(all, m);
   }
   ANDR_HIP(ctx, hipMemsetAsync(SM.p, 0, sizeof(double) * small_per * nsys, st));
   ANDR_HIP(ctx, hipMemsetAsync(ST8.p, 0, sizeof(double) * state_per * nsys, st));
@@ -956,14 +1016,14 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   if (rc) return rc;
   if ((rc = use_tiles(cyc))) return rc;
   int total = 0;
-  bool first = true;
+  bool first = true, first_cycle = true;
   const bool debug = getenv("ANDR_DEBUG_GMRES") != nullptr;
   while (true) {
     // residual r = b - A x of every system of the last cycle (x = 0 on the first), FP64 accumulation
     if (cyc.n_t) hipLaunchKernelGGL(kb_copy_b, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
     gemv(cyc, 0, 0, 0, d_res.p);
     ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, m, partW.p, cyc.w0.p, cyc.nw.p, tol,
+    hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, partW.p, cyc.w0.p, cyc.nw.p, tol,
                        first ? 1 : 0, nrhs, ddone.p);
     ANDR_HIP(ctx, hipGetLastError());
     if ((rc = read_small())) return rc;
@@ -988,9 +1048,13 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     if (cyc_list.empty() || total >= maxit) break;
     if ((rc = build(cyc_list, cyc))) return rc;
     if ((rc = build(cyc_list, live))) return rc;
+    mc = first_cycle ? m : restart_for(cyc_list);
+    layout(cyc_list, mc);
+    first_cycle = false;
     if ((rc = use_tiles(live))) return rc;
+    hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
     hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
-    const int steps_max = std::min(m, maxit - total);
+    const int steps_max = std::min(mc, maxit - total);
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
       gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
@@ -1002,15 +1066,16 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", 2.0 * (2.0 * (j + 1) + 3.0) * vbytes,
                                         2.0 * 4.0 * (j + 1) * nlive * NB);
       for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, m + 1,
+        hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, mc + 1,
                            partT.p);
-        hipLaunchKernelGGL(kb_dots_final, dim3(nsys, (j + 8) / 8), dim3(256), 0, st, dsys.p, j, m, m + 1, partT.p,
+        hipLaunchKernelGGL(kb_dots_final, dim3(nsys, (j + 8) / 8), dim3(256), 0, st, dsys.p, j, mc, mc + 1, partT.p,
                            pass == 0 ? 1 : 0);
-        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, m, partT2.p);
+        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, mc, partT2.p);
       }
       andr_prof_end(ctx, tok_o);
       const int tok_g = andr_prof_begin(ctx, "gmres_givens_scale", 2.0 * vbytes, 0.0);
-      hipLaunchKernelGGL(kb_givens, dim3(gs), dim3(64), 0, st, dsys.p, nsys, j, m, partT2.p, tol);
+      hipLaunchKernelGGL(kb_givens, dim3(nsys * NB), dim3(64), sizeof(double) * (3 * mc + 1), st, dsys.p, j, mc,
+                         partT2.p, tol);
       hipLaunchKernelGGL(kb_scale_next, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j);
       andr_prof_end(ctx, tok_g);
       steps = j + 1;
@@ -1035,8 +1100,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       }
     }
     total += steps;
-    hipLaunchKernelGGL(kb_solve_y, dim3(gs), dim3(64), 0, st, dsys.p, nsys, steps, m);
-    hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, m);
+    hipLaunchKernelGGL(kb_solve_y, dim3(nsys * NB), dim3(64), sizeof(double) * (mc + 1), st, dsys.p, steps, mc);
+    hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, mc);
     ANDR_HIP(ctx, hipGetLastError());
     if ((rc = read_small())) return rc;
     for (int s : cyc_list)

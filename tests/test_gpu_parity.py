@@ -111,6 +111,19 @@ def test_mixed_vs_plain(ctx):
     assert tensor_error(x.FT, p.FT) < 1e-6
 
 
+def test_slender_body_long_restarts(ctx):
+    """A 496-element fibre of the batched workload whose hardest column needs several hundred
+    Krylov steps: the cycles after the first run with a longer restart in the same buffers, next to
+    a body that converges in the first cycle; both stay within 1e-6 of the plain FP64 path."""
+    allb = [m for m, _ in M.batch_bodies(512, 64)]
+    bodies = [allb[523], allb[29]]
+    plain = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
+    fast = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    for f, p in zip(fast, plain):
+        assert tensor_error(f.FT, p.FT) < 1e-6
+    assert fast[0].nit.max() > 128 >= fast[1].nit.max()
+
+
 def test_batch_equals_single(ctx):
     bodies = [M.random_superellipsoid(np.random.default_rng(s), 4)[0] for s in range(3)]
     for prec in (api.PREC_FP64, api.PREC_MIXED):
