@@ -802,7 +802,8 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   ST *arow0 = A + (tiled ? 0 : (size_t)(0 * ne + r) * J.lda);
   ST *arow1 = A + (tiled ? 0 : (size_t)(1 * ne + r) * J.lda);
   ST *arow2 = A + (tiled ? 0 : (size_t)(2 * ne + r) * J.lda);
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+  // one (row, element) pair: matrix entries and the T moments of the right-hand sides
+  auto pair = [&](int e) {
     const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
     const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
     double Gs[6], Tv[6][3];
@@ -871,6 +872,39 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
 #pragma unroll
       for (int d = 0; d < 3; ++d) acc[6 + 3 * k + d] += Tv[k][0] * rv[0][d] + Tv[k][1] * rv[1][d] + Tv[k][2] * rv[2][d];
       acc[24 + k] += Tv[k][0] + Tv[k][1] + Tv[k][2];
+    }
+  };
+  // Learned form: a pair on the fixed-order rule (class 7, the full table) costs ~20 times a
+  // Dunavant pair, and one such lane would hold its whole wave for that long. Those pairs are
+  // queued per wave (in element order, so the result does not depend on timing) and integrated
+  // after the sweep, all lanes on the same rule. A full queue integrates in place.
+  constexpr int QW = 256;
+  __shared__ int s_q[4][QW];
+  __shared__ int s_qn[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int qn = 0;  // uniform over the wave
+  for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
+    const int e = e0 + threadIdx.x;
+    bool defer = J.cls && e < ne && e != r && J.cls[(size_t)r * ne + e] >= 7;
+    const unsigned long long bm = __ballot(defer);
+    const int cnt = __popcll(bm);
+    if (qn + cnt <= QW) {
+      if (defer) s_q[wid][qn + __popcll(bm & ((1ull << lane) - 1ull))] = e;
+      qn += cnt;
+    } else {
+      defer = false;
+    }
+    if (e < ne && !defer) pair(e);
+  }
+  if (lane == 0) s_qn[wid] = qn;
+  __syncthreads();
+  {
+    const int n0 = s_qn[0], n1 = s_qn[1], n2 = s_qn[2], n3 = s_qn[3];
+    const int total = n0 + n1 + n2 + n3;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+      const int w = i < n0 ? 0 : (i < n0 + n1 ? 1 : (i < n0 + n1 + n2 ? 2 : 3));
+      const int base = w == 0 ? 0 : (w == 1 ? n0 : (w == 2 ? n0 + n1 : n0 + n1 + n2));
+      pair(s_q[w][i - base]);
     }
   }
   block_sum<NACC>(acc, red);
