@@ -785,6 +785,9 @@ template <class CT, class ST>
 __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict__ jobs,
                                                        const int2 *__restrict__ rowmap) {
   constexpr int NACC = 6 + 18 + 6;  // sum T (moving), sum T r (moving), sum T (all nodes)
+  // Per-thread partial sums of the T moments in the quadrature type (a few dozen terms each; the
+  // FP32 form's terms carry FP32 quadrature error already), summed over the block in FP64.
+  using AT = CT;
   __shared__ double red[NACC * 4];
   const int2 jr = rowmap[blockIdx.x];
   const SysJob &J = jobs[jr.x];
@@ -792,9 +795,9 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   const int r = jr.y;
   const int ne = g.ne;
   const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
-  double acc[NACC];
+  AT acc[NACC];
 #pragma unroll
-  for (int i = 0; i < NACC; ++i) acc[i] = 0.0;
+  for (int i = 0; i < NACC; ++i) acc[i] = 0;
   const double inv_mu = 1.0 / J.mu;
   ST *A = J.template Aptr<ST>();
   const bool tiled = J.lda == 0;  // symmetric-block layout: S_q = sc G_q only (no null-space term, no M^-1)
@@ -806,7 +809,8 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   auto pair = [&](int e) {
     const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
     const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
-    double Gs[6], Tv[6][3];
+    double Gs[6];
+    AT Tv[6][3];
     const double m = g.mul[e];
     if (e == r) {
       andr_sing_stk(x1, x2, x3, s, 4, Gs);
@@ -859,12 +863,12 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
     }
     // moments of T over the element's moving nodes
     const double *xv[3] = {x1, x2, x3};
-    double mv[3], rv[3][3];
+    AT mv[3], rv[3][3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      mv[v] = J.moving ? (double)J.moving[vv[v]] : 1.0;
+      mv[v] = J.moving ? (AT)J.moving[vv[v]] : (AT)1;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) rv[v][d] = mv[v] * (xv[v][d] - J.center[d]);
+      for (int d = 0; d < 3; ++d) rv[v][d] = mv[v] * (AT)(xv[v][d] - J.center[d]);
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -907,9 +911,12 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
       pair(s_q[w][i - base]);
     }
   }
-  block_sum<NACC>(acc, red);
+  double accd[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; ++i) accd[i] = (double)acc[i];
+  block_sum<NACC>(accd, red);
   if (threadIdx.x == 0) {
-    const double *S = acc, *M = acc + 6, *tsum = acc + 24;
+    const double *S = accd, *M = accd + 6, *tsum = accd + 24;
     const double cfree[3] = {-tsum[0] + J.extra, -tsum[3] + J.extra, -tsum[5] + J.extra};
     // rigid velocities at the row element's nodes, for the free term spread over its vertices
     const int *cn = g.con + 3 * r;

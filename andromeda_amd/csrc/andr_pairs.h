@@ -14,10 +14,11 @@ static inline int andr_upload_rules_here(andr_ctx *ctx, const double *tab, const
 }
 
 
+// Gs: Stokeslet block (FP64); Tv: stresslet moments per vertex, in the quadrature type CT
 template <class CT>
 __device__ inline void stokes_pair_regular(const double s[3], const double *x1, const double *x2, const double *x3,
                                            double nx, double ny, double nz, double area, int qoff, int nq,
-                                           double Gs[6], double Tv[6][3]) {
+                                           double Gs[6], CT Tv[6][3]) {
   // relative geometry formed in FP64, quadrature in CT
   const CT d0 = (CT)(x3[0] - s[0]), d1 = (CT)(x3[1] - s[1]), d2 = (CT)(x3[2] - s[2]);
   const CT a0 = (CT)(x1[0] - x3[0]), a1 = (CT)(x1[1] - x3[1]), a2 = (CT)(x1[2] - x3[2]);
@@ -55,11 +56,12 @@ __device__ inline void stokes_pair_regular(const double s[3], const double *x1, 
 #pragma unroll
       for (int v = 0; v < 3; ++v) t[k][v] += wc[v] * rr[k];
   }
+  const CT ca = (CT)area;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     Gs[k] = (double)g[k] * area;
 #pragma unroll
-    for (int v = 0; v < 3; ++v) Tv[k][v] = (double)t[k][v] * area;
+    for (int v = 0; v < 3; ++v) Tv[k][v] = t[k][v] * ca;
   }
 }
 
