@@ -36,6 +36,9 @@ int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d
 // null-space term). The assembly then stores A M^-1 and the solve returns y with x = M^-1 y.
 int andr_launch_self_blocks(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
 int andr_launch_unprecondition(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
+// Force and torque of the six problems on wall 1 of every job, and its largest row self-check error:
+// d_out[37 j + 6 k + (0..5)] = (F, T) of problem k, d_out[37 j + 36] = error.
+int andr_launch_force_torque(andr_ctx *ctx, const SysJob *d_jobs, int njobs, double *d_out);
 // algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0

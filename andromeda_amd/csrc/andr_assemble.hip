@@ -249,7 +249,8 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
     ANDR_HIP(ctx, hipMemcpyAsync(b.d_color_off.p, b.color_off.data(), sizeof(int) * b.color_off.size(),
                                  hipMemcpyHostToDevice, ctx->stream));
   }
-  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // keeps d_wm alive until the kernel ran
+  // No synchronisation here: d_wm returns to the context's cache, and any later use of that block
+  // is ordered after k_geometry on the same stream; the host vectors were staged by the copies.
   b.g.nn = nn;
   b.g.ne = ne;
   b.g.nwalls = nw;
@@ -976,6 +977,56 @@ int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d
   else
     return andr_fail(ctx, ANDR_ERR_ARG, "FP32 quadrature with FP64 storage is not offered");
   andr_prof_end(ctx, tok);
+  ANDR_HIP(ctx, hipGetLastError());
+  return ANDR_OK;
+}
+
+// block per body: force and torque of the six problems on wall 1, F = sum_e A_e q_e and
+// T = sum_e A_e (c_e - centre) x q_e (the reductions of src/lib_particle.f90:1751-1831), and the
+// largest row self-check error. out[37 b + 6 k + (0..5)] = (F, T) of problem k, out[37 b + 36] = error.
+__global__ __launch_bounds__(256) void k_force_torque(const SysJob *__restrict__ jobs, double *__restrict__ out) {
+  __shared__ double red[36 * 4];
+  __shared__ double emax[4];
+  const SysJob &J = jobs[blockIdx.x];
+  const BodyGeom &g = J.g;
+  const int ne = g.ne;
+  double v[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) v[i] = 0.0;
+  double err = 0.0;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    if (J.err_rows) err = fmax(err, J.err_rows[e]);
+    if (g.bcid[e] != 1) continue;
+    const double a = g.area[e];
+    const double r0 = g.cen[3 * e] - J.center[0], r1 = g.cen[3 * e + 1] - J.center[1],
+                 r2 = g.cen[3 * e + 2] - J.center[2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double q0 = J.X[(size_t)e * ANDR_NRHS + k], q1 = J.X[(size_t)(ne + e) * ANDR_NRHS + k],
+                   q2 = J.X[(size_t)(2 * ne + e) * ANDR_NRHS + k];
+      v[6 * k] += a * q0;
+      v[6 * k + 1] += a * q1;
+      v[6 * k + 2] += a * q2;
+      v[6 * k + 3] += a * (r1 * q2 - r2 * q1);
+      v[6 * k + 4] += a * (r2 * q0 - r0 * q2);
+      v[6 * k + 5] += a * (r0 * q1 - r1 * q0);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o, 64));
+  if ((threadIdx.x & 63) == 0) emax[threadIdx.x >> 6] = err;
+  block_sum<36>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 36; ++i) out[(size_t)blockIdx.x * 37 + i] = v[i];
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, emax[w]);
+    out[(size_t)blockIdx.x * 37 + 36] = m;
+  }
+}
+
+int andr_launch_force_torque(andr_ctx *ctx, const SysJob *d_jobs, int njobs, double *d_out) {
+  if (njobs <= 0) return ANDR_OK;
+  hipLaunchKernelGGL(k_force_torque, dim3(njobs), dim3(256), 0, ctx->stream, d_jobs, d_out);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;
 }

@@ -244,15 +244,19 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
     const double z0 = z[0][t], z1 = z[1][t], z2 = z[2][t];
     const double b0 = hi ? 0.0 : z0, b1 = hi ? z0 : z1, b3 = hi ? z1 : 0.0;
     const double b2 = hi ? z2 : z0, b5 = hi ? 0.0 : z2, b4 = hi ? z1 : z2;
+    // bands interleaved so that consecutive instructions never share an accumulator
 #pragma unroll
-    for (int T = 0; T < 2; ++T) {
-      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][0][t], b0, acc[T][0], 0, 0, 0);
-      acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][2][t], b2, acc[T][1], 0, 0, 0);
-      acc[T][2] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][4][t], b4, acc[T][2], 0, 0, 0);
-      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][1][t], b1, acc[T][0], 0, 0, 0);
-      acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][5][t], b5, acc[T][1], 0, 0, 0);
-      acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][3][t], b3, acc[T][0], 0, 0, 0);
-    }
+    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][0][t], b0, acc[T][0], 0, 0, 0);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][2][t], b2, acc[T][1], 0, 0, 0);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) acc[T][2] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][4][t], b4, acc[T][2], 0, 0, 0);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][1][t], b1, acc[T][0], 0, 0, 0);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][5][t], b5, acc[T][1], 0, 0, 0);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][3][t], b3, acc[T][0], 0, 0, 0);
   };
   // full column blocks, software-pipelined: the operands of block kb + 1 are requested before the
   // products of block kb, so that their memory latency overlaps the matrix-core work

@@ -128,6 +128,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     s += (size_t)meshes[b].nnodes * 200 + (size_t)meshes[b].nelem * 300;
     return s;
   };
+  std::vector<double> ftbuf;  // per body of a wave: 36 force/torque values and the self-check error
   int b0 = 0;
   while (b0 < nbodies) {
     int b1 = b0;
@@ -179,7 +180,6 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         for (int k = 0; k < p.nn; ++k) mv[k] = p.node_wall[k] == 2 ? 1 : 0;
         if (Mbuf[i].alloc(p.nn, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "moving-node flags");
         ANDR_HIP(ctx, hipMemcpyAsync(Mbuf[i].p, mv.data(), p.nn, hipMemcpyHostToDevice, ctx->stream));
-        ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
       }
       const QuadRule qr = andr_rule(p.body.itri);
       SysJob &J = jobs[i];
@@ -205,7 +205,6 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
         deflation_scales(m, nm.data(), opts->mu, dh);
         if (Dbuf[i].alloc(dh.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "deflation scales");
         ANDR_HIP(ctx, hipMemcpyAsync(Dbuf[i].p, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice, ctx->stream));
-        ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         J.defl = Dbuf[i].p;
       }
       if (learned) {
@@ -262,36 +261,21 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     }
     ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double ts1 = andr_now();
-    // ---- force and torque on wall 1 ---------------------------------------------------------------
+    // ---- force and torque on wall 1 (on the device, one copy back for the wave) ------------------
+    {
+      DevBuf<double> dft;
+      if (dft.alloc((size_t)nb * 37, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "force/torque results");
+      if ((rc = andr_launch_force_torque(ctx, djobs.p, nb, dft.p))) return rc;
+      ftbuf.resize((size_t)nb * 37);
+      ANDR_HIP(ctx, hipMemcpyAsync(ftbuf.data(), dft.p, sizeof(double) * ftbuf.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+      ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     for (int i = 0; i < nb; ++i) {
       Prepared &p = *P[i];
-      const int ne = p.ne, n = 3 * ne;
-      std::vector<double> X((size_t)n * ANDR_NRHS), area(ne), cen(3 * (size_t)ne), err(ne);
-      ANDR_HIP(ctx, hipMemcpyAsync(X.data(), Xbuf[i].p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, ctx->stream));
-      ANDR_HIP(ctx, hipMemcpyAsync(area.data(), p.body.g.area, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
-      ANDR_HIP(ctx, hipMemcpyAsync(cen.data(), p.body.g.cen, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, ctx->stream));
-      ANDR_HIP(ctx, hipMemcpyAsync(err.data(), Ebuf[i].p, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
-      ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      const andr_mesh *m = &meshes[b0 + i];
+      const int ne = p.ne;
       double *ft = FT + (size_t)(b0 + i) * 36;
-      for (int k = 0; k < 6; ++k) {
-        double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
-        for (int e = 0; e < ne; ++e) {
-          if (m->bcid && m->bcid[e] != 1) continue;
-          const double q[3] = {X[(size_t)e * ANDR_NRHS + k], X[(size_t)(ne + e) * ANDR_NRHS + k],
-                               X[(size_t)(2 * ne + e) * ANDR_NRHS + k]};
-          const double r[3] = {cen[3 * e] - opts->center[0], cen[3 * e + 1] - opts->center[1],
-                               cen[3 * e + 2] - opts->center[2]};
-          for (int c = 0; c < 3; ++c) F[c] += area[e] * q[c];
-          T[0] += area[e] * (r[1] * q[2] - r[2] * q[1]);
-          T[1] += area[e] * (r[2] * q[0] - r[0] * q[2]);
-          T[2] += area[e] * (r[0] * q[1] - r[1] * q[0]);
-        }
-        for (int c = 0; c < 3; ++c) {
-          ft[6 * k + c] = F[c];
-          ft[6 * k + 3 + c] = T[c];
-        }
-      }
+      for (int k = 0; k < 36; ++k) ft[k] = ftbuf[(size_t)i * 37 + k];
       if (reports) {
         andr_resistance_report &R = reports[b0 + i];
         for (int k = 0; k < 6; ++k) {
@@ -299,7 +283,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
           R.status = std::max(R.status, stops[(size_t)i * 6 + k] > 3 ? stops[(size_t)i * 6 + k] : 0);
         }
         R.err[0] = 0.0;
-        for (int e = 0; e < ne; ++e) R.err[1] = std::max(R.err[1], err[e]);
+        R.err[1] = ftbuf[(size_t)i * 37 + 36];
         R.t_assemble = (ta1 - ta0) / nb;
         R.t_solve = (ts1 - ta1) / nb;
         const QuadRule qr = andr_rule(p.body.itri);
