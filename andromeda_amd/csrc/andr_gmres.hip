@@ -260,12 +260,19 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   };
   // full column blocks, software-pipelined: the operands of block kb + 1 are requested before the
   // products of block kb, so that their memory latency overlaps the matrix-core work
+  // global-address-space views of the operands (plain global loads instead of flat ones)
+  typedef const float __attribute__((address_space(1))) *gfp;
+  typedef const double __attribute__((address_space(1))) *gdp;
+  typedef float f4n __attribute__((ext_vector_type(4)));
+  typedef double d2n __attribute__((ext_vector_type(2)));
+  const gfp ga[2] = {(gfp)ab[0], (gfp)ab[1]};
+  const gdp gz[3] = {(gdp)zb[0], (gdp)zb[1], (gdp)zb[2]};
   auto load = [&](int k, float (&a)[2][6][4], double (&z)[3][4]) {
 #pragma unroll
     for (int T = 0; T < 2; ++T)
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        const float4 v = *reinterpret_cast<const float4 *>(ab[T] + (size_t)k * 1536 + q * 256);
+        const f4n v = *(const f4n __attribute__((address_space(1))) *)(ga[T] + (size_t)k * 1536 + q * 256);
         a[T][q][0] = v.x;
         a[T][q][1] = v.y;
         a[T][q][2] = v.z;
@@ -273,35 +280,41 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
       }
 #pragma unroll
     for (int en = 0; en < 3; ++en) {
-      const double2 u = *reinterpret_cast<const double2 *>(zb[en] + 16 * k);
-      const double2 w = *reinterpret_cast<const double2 *>(zb[en] + 16 * k + 2);
+      const d2n u = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k);
+      const d2n w = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k + 2);
       z[en][0] = u.x;
       z[en][1] = u.y;
       z[en][2] = w.x;
       z[en][3] = w.y;
     }
   };
+  // full column blocks kb < kb_end, two register sets in turn: the operands of the next block are
+  // requested before the products of the current one. The prefetch is unconditional (the last
+  // one re-reads the final block) so that no branch sits between a load and its use.
   int kb = wm.k0 >> 4;
-  if (16 * kb + 16 <= kend) {
-    float an[2][6][4];
-    double zn[3][4];
-    load(kb, an, zn);
-    for (; 16 * kb + 16 <= kend; ++kb) {
-      float a[2][6][4];
-      double z[3][4];
+  const int kb_end = kend >> 4;
+  if (kb < kb_end) {
+    float a0[2][6][4], a1[2][6][4];
+    double z0[3][4], z1[3][4];
+    load(kb, a0, z0);
+    for (; kb + 1 < kb_end; kb += 2) {
+      // the scheduling barriers keep each group of loads ahead of the products it overlaps (left
+      // to itself the scheduler moves loads next to their uses and the prefetch is lost)
+      load(kb + 1, a1, z1);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int T = 0; T < 2; ++T)
+      for (int t = 0; t < 4; ++t) step(a0, z0, t);
+      __builtin_amdgcn_sched_barrier(0);
+      load(min(kb + 2, kb_end - 1), a0, z0);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
+      for (int t = 0; t < 4; ++t) step(a1, z1, t);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (kb < kb_end) {  // odd count: a0 holds block kb
 #pragma unroll
-          for (int t = 0; t < 4; ++t) a[T][q][t] = an[T][q][t];
-#pragma unroll
-      for (int en = 0; en < 3; ++en)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) z[en][t] = zn[en][t];
-      if (16 * kb + 32 <= kend) load(kb + 1, an, zn);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) step(a, z, t);
+      for (int t = 0; t < 4; ++t) step(a0, z0, t);
+      ++kb;
     }
   }
   if (16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
