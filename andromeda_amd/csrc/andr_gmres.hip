@@ -272,7 +272,8 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
     for (int T = 0; T < 2; ++T)
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        const f4n v = *(const f4n __attribute__((address_space(1))) *)(ga[T] + (size_t)k * 1536 + q * 256);
+        // streamed once: non-temporal, so that the re-read operand z keeps its place in L2
+        const f4n v = __builtin_nontemporal_load((const f4n __attribute__((address_space(1))) *)(ga[T] + (size_t)k * 1536 + q * 256));
         a[T][q][0] = v.x;
         a[T][q][1] = v.y;
         a[T][q][2] = v.z;
