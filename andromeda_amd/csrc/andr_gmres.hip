@@ -7,8 +7,9 @@
 // one launch per step covers every system (work lists map waves and row tiles to systems), so the
 // launch count does not grow with the number of bodies.
 //
-// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2): both passes are block
-// products over the basis, which keeps the basis traffic streaming instead of latency bound.
+// Orthogonalisation is classical Gram-Schmidt applied twice with the second pass delayed to the
+// next step (one sweep of dot products and one of updates over the basis per step instead of two
+// of each); both sweeps stream the basis.
 #include "andr_internal.h"
 #include "andr_solve.h"
 #include "andr_prof.h"
@@ -36,10 +37,11 @@ struct GSys {  // device view of one system
   double *R;     // residual / scratch n x NB
   double *W;     // new Krylov vector n x NB
   double *Q;     // (m+1) x n x NB basis
-  double *H;     // NB x (m+1) x m Hessenberg (column-major per rhs)
+  double *H;     // NB x (m+1) x m Hessenberg after the Givens rotations (column-major per rhs)
+  double *Hu;    // NB x (m+1) x m Hessenberg as built (column-major per rhs)
   double *cs, *sn;  // NB x m
   double *g;     // NB x (m+1)
-  double *coef;  // NB x (m+1) projection coefficients of the current pass
+  double *cw;    // NB x cw_stride(m): step coefficients (see kb_adots), then the solution y
   double *small; // per rhs: [0] bnorm, [1] rnorm (true), [2] est, [3] inv_h, [4] active, [5] used steps
   int tile0;     // first vector tile of the system
   int ntiles;
@@ -458,194 +460,280 @@ __global__ __launch_bounds__(256) void kb_seg_sum(const GSys *__restrict__ sys, 
   }
 }
 
-// per (tile, chunk of ICH basis vectors): partial dots part[tile][i][k] = sum_rows Q_i[row][k] W[row][k]
-__global__ __launch_bounds__(256) void kb_dots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
-                                               int mstride, double *__restrict__ part) {
-  __shared__ double red[4][ICH][NB];
+// ---- Arnoldi step with delayed reorthogonalisation (classical Gram-Schmidt twice, one sweep of
+// dot products and one of updates per step). At step j the basis slot Q_j holds the candidate u_j
+// (last step's vector after one projection, normalised by an estimate of its norm) and W = A u_j.
+// One sweep over the basis gives s = V^T u_j, t = V^T W and u.u, u.W, W.W; then
+//   rho = |u_j - V s|,  v_j = (u_j - V s) / rho   (the second projection of the previous vector),
+//   column j-1 of H gains eta_{j-1} s and its subdiagonal becomes eta_{j-1} rho (it is final now),
+//   A v_j = (W - V_{j+1} H s) / rho, so column j is h = (t - (H s)) / rho with h_j from
+//   delta = v_j . W = (u.W - s.t) / rho, and the next candidate is
+//   u_{j+1} = (W - V t - delta v_j) / (rho eta_j),  eta_j^2 = (W.W - t.t - delta^2) / rho^2.
+// The second sweep writes v_j and u_{j+1}. Per right-hand side, the coefficient array cw holds
+// s (slots 0..m), t (m+1..2m+1), and 1/rho, delta, 1/eta (2m+2..2m+4); the dot sweep leaves
+// u.u, u.W, W.W in those three slots.
+__host__ __device__ constexpr int cw_stride(int m) { return 2 * (m + 1) + 6; }
+__host__ __device__ constexpr int part_stride(int m) { return 2 * (m + 1) + 3; }
+
+// per (tile, chunk of ICH basis vectors): partial s_i = Q_i . u and t_i = Q_i . W over the tile's
+// rows, part[(tile * PS + slot) * NB + k]; the first chunk also adds u.u, u.W, W.W. fin: W = 0
+// (closing a cycle: only the reorthogonalisation of the last candidate).
+__global__ __launch_bounds__(256) void kb_adots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
+                                                int m, double *__restrict__ part, int fin) {
+  __shared__ double red[4][2 * ICH + 3][NB];
   const int i0 = blockIdx.y * ICH;
-  if (i0 > j) return;  // uniform over the block
-  const int ni = min(ICH, j + 1 - i0);
+  const int ni = max(0, min(ICH, j - i0));
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
-  const int n = S.n;
+  const int n = S.n, PS = part_stride(m);
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double w[TILE / 32];
+  const double *U = S.Q + (size_t)j * n * NB;
+  double u[TILE / 32], w[TILE / 32];
   int rows[TILE / 32];
 #pragma unroll
   for (int t = 0; t < TILE / 32; ++t) {
     const int row = tm.row0 + rr + 32 * t;
     rows[t] = row < n ? row : -1;
-    w[t] = row < n ? S.W[(size_t)row * NB + k] : 0.0;
+    u[t] = row < n ? U[(size_t)row * NB + k] : 0.0;
+    w[t] = (row < n && !fin) ? S.W[(size_t)row * NB + k] : 0.0;
   }
-  double acc[ICH];
+  double as[ICH], at[ICH];
 #pragma unroll
   for (int ii = 0; ii < ICH; ++ii) {
-    double a = 0.0;
+    double a = 0.0, b = 0.0;
     if (ii < ni) {
       const double *Qi = S.Q + (size_t)(i0 + ii) * n * NB;
 #pragma unroll
       for (int t = 0; t < TILE / 32; ++t)
-        if (rows[t] >= 0) a += Qi[(size_t)rows[t] * NB + k] * w[t];
+        if (rows[t] >= 0) {
+          const double q = Qi[(size_t)rows[t] * NB + k];
+          a += q * u[t];
+          b += q * w[t];
+        }
     }
-    a += __shfl_xor(a, 8, 64);
-    a += __shfl_xor(a, 16, 64);
-    a += __shfl_xor(a, 32, 64);
-    acc[ii] = a;
+    as[ii] = a;
+    at[ii] = b;
   }
-  if (lane < 8)
+  double sc[3] = {0.0, 0.0, 0.0};
+  if (blockIdx.y == 0)
 #pragma unroll
-    for (int ii = 0; ii < ICH; ++ii) red[wid][ii][lane] = acc[ii];
+    for (int t = 0; t < TILE / 32; ++t) {
+      sc[0] += u[t] * u[t];
+      sc[1] += u[t] * w[t];
+      sc[2] += w[t] * w[t];
+    }
+  auto wsum = [&](double x) {
+    x += __shfl_xor(x, 8, 64);
+    x += __shfl_xor(x, 16, 64);
+    x += __shfl_xor(x, 32, 64);
+    return x;
+  };
+#pragma unroll
+  for (int ii = 0; ii < ICH; ++ii) {
+    as[ii] = wsum(as[ii]);
+    at[ii] = wsum(at[ii]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) sc[c] = wsum(sc[c]);
+  if (lane < 8) {
+#pragma unroll
+    for (int ii = 0; ii < ICH; ++ii) {
+      red[wid][ii][lane] = as[ii];
+      red[wid][ICH + ii][lane] = at[ii];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) red[wid][2 * ICH + c][lane] = sc[c];
+  }
   __syncthreads();
-  if (threadIdx.x < ni * NB) {
-    const int ii = threadIdx.x >> 3, kk = threadIdx.x & 7;
-    part[((size_t)blockIdx.x * mstride + i0 + ii) * NB + kk] =
-        red[0][ii][kk] + red[1][ii][kk] + red[2][ii][kk] + red[3][ii][kk];
+  for (int idx = threadIdx.x; idx < (2 * ICH + 3) * NB; idx += blockDim.x) {
+    const int q = idx / NB, kk = idx % NB;
+    int slot;
+    if (q < ICH) {
+      if (q >= ni) continue;
+      slot = i0 + q;
+    } else if (q < 2 * ICH) {
+      if (q - ICH >= ni) continue;
+      slot = (m + 1) + i0 + (q - ICH);
+    } else {
+      if (blockIdx.y != 0) continue;
+      slot = 2 * (m + 1) + (q - 2 * ICH);
+    }
+    part[((size_t)blockIdx.x * PS + slot) * NB + kk] = red[0][q][kk] + red[1][q][kk] + red[2][q][kk] + red[3][q][kk];
   }
 }
 
-// block per (system, 8 basis vectors): coef = sum over the system's tiles, the four waves taking
-// every fourth tile; H(i, j) += coef. Systems outside the current map have no tiles and return.
-__global__ __launch_bounds__(256) void kb_dots_final(const GSys *__restrict__ sys, int j, int m, int mstride,
-                                                     const double *__restrict__ part, int first_pass) {
+// block per (system, 8 of the 2j + 3 values): sums over the system's tiles (the four waves take
+// every fourth tile) into cw. Systems outside the current map have no tiles and return.
+__global__ __launch_bounds__(256) void kb_adots_final(const GSys *__restrict__ sys, int j, int m,
+                                                      const double *__restrict__ part) {
   __shared__ double red[4][64];
   const GSys &S = sys[blockIdx.x];
   if (S.ntiles == 0) return;  // uniform over the block
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int k = lane & 7, i = blockIdx.y * 8 + (lane >> 3);
+  const int k = lane & 7, q = blockIdx.y * 8 + (lane >> 3);
+  const int nq = 2 * j + 3, PS = part_stride(m);
+  // value q: s_q (q < j), t_{q-j} (q < 2j), then u.u, u.W, W.W
+  const int slot = q < j ? q : (q < 2 * j ? (m + 1) + (q - j) : 2 * (m + 1) + (q - 2 * j));
   double c = 0.0;
-  if (i <= j)
-    for (int tl = wv; tl < S.ntiles; tl += 4) c += part[((size_t)(S.tile0 + tl) * mstride + i) * NB + k];
+  if (q < nq)
+    for (int tl = wv; tl < S.ntiles; tl += 4) c += part[((size_t)(S.tile0 + tl) * PS + slot) * NB + k];
   red[wv][lane] = c;
   __syncthreads();
-  if (wv == 0 && i <= j) {
-    c = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    S.coef[k * (m + 1) + i] = c;
-    double &h = S.H[(size_t)k * (m + 1) * m + (size_t)j * (m + 1) + i];
-    h = first_pass ? c : h + c;
-  }
+  if (wv == 0 && q < nq) S.cw[k * cw_stride(m) + slot] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
-// per tile: W -= sum_i Q_i coef_i ; part2[tile][k] = sum W^2. Both rows of a thread and four basis
-// vectors per iteration are in flight together (the loop is bound by load latency otherwise).
-__global__ __launch_bounds__(256) void kb_sub(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j, int m,
-                                              double *__restrict__ part2) {
-  __shared__ double red[4][NB];
-  const TileMap tm = tmap[blockIdx.x];
-  const GSys &S = sys[tm.sys];
-  const int n = S.n;
-  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  static_assert(TILE == 64, "two rows per thread");
-  const int r0 = tm.row0 + rr, r1 = tm.row0 + rr + 32;
-  const bool v0 = r0 < n, v1 = r1 < n;
-  const int q0 = v0 ? r0 : 0, q1 = v1 ? r1 : q0;  // in-range rows for the loads
-  const double *cf = S.coef + k * (m + 1);
-  const size_t vs = (size_t)n * NB;
-  const double *Qa = S.Q + (size_t)q0 * NB + k, *Qb = S.Q + (size_t)q1 * NB + k;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
-  int i = 0;
-  for (; i + 4 <= j + 1; i += 4) {
-    const double c0 = cf[i], c1 = cf[i + 1], c2 = cf[i + 2], c3 = cf[i + 3];
-    a0 += Qa[(size_t)i * vs] * c0;
-    a1 += Qa[(size_t)(i + 1) * vs] * c1;
-    a2 += Qa[(size_t)(i + 2) * vs] * c2;
-    a3 += Qa[(size_t)(i + 3) * vs] * c3;
-    b0 += Qb[(size_t)i * vs] * c0;
-    b1 += Qb[(size_t)(i + 1) * vs] * c1;
-    b2 += Qb[(size_t)(i + 2) * vs] * c2;
-    b3 += Qb[(size_t)(i + 3) * vs] * c3;
-  }
-  for (; i <= j; ++i) {
-    a0 += Qa[(size_t)i * vs] * cf[i];
-    b0 += Qb[(size_t)i * vs] * cf[i];
-  }
-  double w2 = 0.0;
-  if (v0) {
-    const double w = S.W[(size_t)r0 * NB + k] - ((a0 + a1) + (a2 + a3));
-    S.W[(size_t)r0 * NB + k] = w;
-    w2 += w * w;
-  }
-  if (v1) {
-    const double w = S.W[(size_t)r1 * NB + k] - ((b0 + b1) + (b2 + b3));
-    S.W[(size_t)r1 * NB + k] = w;
-    w2 += w * w;
-  }
-  w2 += __shfl_xor(w2, 8, 64);
-  w2 += __shfl_xor(w2, 16, 64);
-  w2 += __shfl_xor(w2, 32, 64);
-  if (lane < 8) red[wid][lane] = w2;
-  __syncthreads();
-  if (threadIdx.x < 8)
-    part2[(size_t)blockIdx.x * NB + threadIdx.x] =
-        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-}
-
-// wave per (system, rhs) of the current map: h(j+1, j) = ||W||, Givens rotations, residual
-// estimate. The Hessenberg column and the rotations are staged in LDS (3 m + 1 doubles of dynamic
-// shared memory) so that the serial rotation chain of lane 0 does not wait on global memory.
-__global__ __launch_bounds__(64) void kb_givens(const GSys *__restrict__ sys, int j, int m,
-                                                const double *__restrict__ part2, double tol) {
+// wave per (system, rhs) of the current map: the scalar part of step j (see above). Finalises
+// column j - 1 of the Hessenberg matrix (unrotated in Hu), rotates it into H with the earlier Givens
+// rotations, updates g and the residual estimate (convergence of j steps), and, unless fin, forms
+// column j and the coefficients of the update sweep. Dynamic LDS: 5 m + 2 doubles.
+__global__ __launch_bounds__(64) void kb_arnoldi(const GSys *__restrict__ sys, int j, int m, double tol, int fin) {
   extern __shared__ double lds[];
   const int s = blockIdx.x / NB, k = blockIdx.x % NB, lane = threadIdx.x;
   const GSys &S = sys[s];
   if (S.ntiles == 0) return;  // uniform over the block
-  double *lh = lds, *lc = lds + (m + 1), *ls = lds + (2 * m + 1);
-  double hn2 = 0.0;
-  for (int tl = lane; tl < S.ntiles; tl += 64) hn2 += part2[(size_t)(S.tile0 + tl) * NB + k];
-  for (int off = 32; off > 0; off >>= 1) hn2 += __shfl_xor(hn2, off, 64);
-  double *Hc = S.H + (size_t)k * (m + 1) * m + (size_t)j * (m + 1);
+  const int CW = cw_stride(m);
+  double *cw = S.cw + (size_t)k * CW;
+  double *sm = S.small + k * 8;
+  double *Hu = S.Hu + (size_t)k * (m + 1) * m, *Hr = S.H + (size_t)k * (m + 1) * m;
   double *cs = S.cs + k * m, *sn = S.sn + k * m, *g = S.g + k * (m + 1);
-  for (int i = lane; i <= j; i += 64) lh[i] = Hc[i];
+  double *ls = lds, *lt = lds + m, *lh = lds + 2 * m, *lc = lds + 3 * m + 2;  // s, t, column, rotations (cs, sn)
+  const double alpha = cw[2 * m + 2], beta = cw[2 * m + 3], gamma = cw[2 * m + 4];
+  double ss = 0.0, st = 0.0, tt = 0.0;
   for (int i = lane; i < j; i += 64) {
-    lc[i] = cs[i];
-    ls[i] = sn[i];
+    const double si = cw[i], ti = cw[(m + 1) + i];
+    ls[i] = si;
+    lt[i] = ti;
+    ss += si * si;
+    st += si * ti;
+    tt += ti * ti;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    ss += __shfl_xor(ss, o, 64);
+    st += __shfl_xor(st, o, 64);
+    tt += __shfl_xor(tt, o, 64);
+  }
+  const double rho2 = alpha - ss;
+  const bool ok = sm[4] > 0.0 && alpha > 0.0 && rho2 > 1e-24 * alpha;
+  const double rho = ok ? sqrt(rho2) : 0.0;
   __syncthreads();
-  if (lane == 0) {
-    double *sm = S.small + k * 8;
-    const double hn = sqrt(hn2);
-    const bool live = sm[4] > 0.0 && sm[5] < 0.0;  // active and not yet frozen in this cycle
-    double cur = lh[0];  // apply the previous rotations
-    for (int i = 0; i < j; ++i) {
-      const double b = lh[i + 1], c = lc[i], sg = ls[i];
-      lh[i] = c * cur + sg * b;
-      cur = -sg * cur + c * b;
+  if (j >= 1) {
+    // column j - 1 becomes final
+    const double eta = Hu[(size_t)(j - 1) * (m + 1) + j];
+    double *hc = Hu + (size_t)(j - 1) * (m + 1);
+    for (int i = lane; i < j; i += 64) {
+      const double v = hc[i] + eta * ls[i];
+      hc[i] = v;
+      lh[i] = v;
     }
-    const double a = cur, b = hn;
-    const double rr = sqrt(a * a + b * b);
-    const double c = rr > 0.0 ? a / rr : 1.0, sg = rr > 0.0 ? b / rr : 0.0;
-    cs[j] = c;
-    sn[j] = sg;
-    lh[j] = rr;
-    Hc[j + 1] = 0.0;
-    g[j + 1] = -sg * g[j];
-    g[j] = c * g[j];
-    const double est = fabs(g[j + 1]);
-    sm[2] = est;
-    sm[3] = hn > 1e-300 ? 1.0 / hn : 0.0;
-    if (live && (est <= 0.1 * tol * sm[0] || hn <= 1e-14 * rr)) sm[5] = (double)(j + 1);  // steps to use
+    if (lane == 0) {
+      hc[j] = eta * rho;
+      lh[j] = eta * rho;
+    }
+    for (int i = lane; i < j - 1; i += 64) lc[i] = cs[i];
+    for (int i = lane; i < j - 1; i += 64) lc[m + i] = sn[i];
+    __syncthreads();
+    if (lane == 0) {
+      const bool live = sm[4] > 0.0 && sm[5] < 0.0;  // active and not yet frozen in this cycle
+      double cur = lh[0];
+      for (int i = 0; i < j - 1; ++i) {
+        const double b = lh[i + 1], c = lc[i], sg = lc[m + i];
+        lh[i] = c * cur + sg * b;
+        cur = -sg * cur + c * b;
+      }
+      const double a = cur, b = lh[j];
+      const double rr = sqrt(a * a + b * b);
+      const double c = rr > 0.0 ? a / rr : 1.0, sg = rr > 0.0 ? b / rr : 0.0;
+      cs[j - 1] = c;
+      sn[j - 1] = sg;
+      lh[j - 1] = rr;
+      g[j] = -sg * g[j - 1];
+      g[j - 1] = c * g[j - 1];
+      const double est = fabs(g[j]);
+      sm[2] = est;
+      if (live && (est <= 0.1 * tol * sm[0] || fabs(b) <= 1e-14 * rr)) sm[5] = (double)j;  // steps to use
+    }
+    __syncthreads();
+    double *hr = Hr + (size_t)(j - 1) * (m + 1);
+    for (int i = lane; i < j; i += 64) hr[i] = lh[i];
+    if (lane == 0) hr[j] = 0.0;
   }
-  __syncthreads();
-  for (int i = lane; i <= j; i += 64) Hc[i] = lh[i];
+  if (fin) return;
+  // column j: h_i = (t_i - (H s)_i) / rho, i < j;  h_j = (delta - (H s)_j) / rho
+  const double delta = ok ? (beta - st) / rho : 0.0;
+  double *hj = Hu + (size_t)j * (m + 1);
+  for (int i = lane; i <= j; i += 64) {
+    double hs = 0.0;
+    for (int c = (i > 0 ? i - 1 : 0); c < j; ++c) hs += Hu[(size_t)c * (m + 1) + i] * ls[c];
+    const double num = (i < j ? lt[i] : delta) - hs;
+    hj[i] = ok ? num / rho : 0.0;
+  }
+  if (lane == 0) {
+    double eta2 = ok ? (gamma - tt - delta * delta) / rho2 : 0.0;
+    const double floor2 = ok ? 1e-28 * gamma / rho2 : 0.0;
+    const double eta = eta2 > floor2 ? sqrt(eta2) : 0.0;
+    hj[j + 1] = eta;
+    cw[2 * m + 2] = ok ? 1.0 / rho : 0.0;
+    cw[2 * m + 3] = delta;
+    cw[2 * m + 4] = eta > 0.0 ? 1.0 / eta : 0.0;
+  }
 }
 
-// per tile: Q_{j+1} = W * inv_h
-__global__ __launch_bounds__(256) void kb_scale_next(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j) {
+// per tile: v_j = (u_j - V s) / rho into Q_j, u_{j+1} = (W - V t - delta v_j) / (rho eta) into Q_{j+1}.
+// Both rows of a thread and four basis vectors per iteration are in flight together.
+__global__ __launch_bounds__(256) void kb_aupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
+                                                  int m) {
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
-  const double inv = S.small[k * 8 + 3];
-  double *Qn = S.Q + (size_t)(j + 1) * n * NB;
-#pragma unroll
-  for (int t = 0; t < TILE / 32; ++t) {
-    const int row = tm.row0 + rr + 32 * t;
-    if (row < n) Qn[(size_t)row * NB + k] = S.W[(size_t)row * NB + k] * inv;
+  static_assert(TILE == 64, "two rows per thread");
+  const int r0 = tm.row0 + rr, r1 = tm.row0 + rr + 32;
+  const bool v0 = r0 < n, v1 = r1 < n;
+  const int q0 = v0 ? r0 : 0, q1 = v1 ? r1 : q0;  // in-range rows for the loads
+  const double *cw = S.cw + (size_t)k * cw_stride(m);
+  const double *cs = cw, *ct = cw + (m + 1);
+  const double irho = cw[2 * m + 2], delta = cw[2 * m + 3], ieta = cw[2 * m + 4];
+  const size_t vs = (size_t)n * NB;
+  const double *Qa = S.Q + (size_t)q0 * NB + k, *Qb = S.Q + (size_t)q1 * NB + k;
+  double sa0 = 0.0, sa1 = 0.0, ta0 = 0.0, ta1 = 0.0, sb0 = 0.0, sb1 = 0.0, tb0 = 0.0, tb1 = 0.0;
+  int i = 0;
+  for (; i + 2 <= j; i += 2) {
+    const double qa0 = Qa[(size_t)i * vs], qa1 = Qa[(size_t)(i + 1) * vs];
+    const double qb0 = Qb[(size_t)i * vs], qb1 = Qb[(size_t)(i + 1) * vs];
+    const double s0 = cs[i], s1 = cs[i + 1], t0 = ct[i], t1 = ct[i + 1];
+    sa0 += qa0 * s0;
+    sa1 += qa1 * s1;
+    ta0 += qa0 * t0;
+    ta1 += qa1 * t1;
+    sb0 += qb0 * s0;
+    sb1 += qb1 * s1;
+    tb0 += qb0 * t0;
+    tb1 += qb1 * t1;
+  }
+  for (; i < j; ++i) {
+    const double qa = Qa[(size_t)i * vs], qb = Qb[(size_t)i * vs];
+    sa0 += qa * cs[i];
+    ta0 += qa * ct[i];
+    sb0 += qb * cs[i];
+    tb0 += qb * ct[i];
+  }
+  double *U = S.Q + (size_t)j * vs, *Un = S.Q + (size_t)(j + 1) * vs;
+  if (v0) {
+    const size_t o = (size_t)r0 * NB + k;
+    const double v = (U[o] - (sa0 + sa1)) * irho;
+    U[o] = v;
+    Un[o] = (S.W[o] - (ta0 + ta1) - delta * v) * irho * ieta;
+  }
+  if (v1) {
+    const size_t o = (size_t)r1 * NB + k;
+    const double v = (U[o] - (sb0 + sb1)) * irho;
+    U[o] = v;
+    Un[o] = (S.W[o] - (tb0 + tb1) - delta * v) * irho * ieta;
   }
 }
 
-// wave per (system, rhs): back substitution y = R^{-1} g over the steps used (y stored in coef),
+// wave per (system, rhs): back substitution y = R^{-1} g over the steps used (y stored in cw),
 // column by column: y_i = g_i / R_ii, then the lanes update g_l -= R_li y_i for l < i (g in LDS)
 __global__ __launch_bounds__(64) void kb_solve_y(const GSys *__restrict__ sys, int steps, int m) {
   extern __shared__ double lg[];
@@ -653,7 +741,7 @@ __global__ __launch_bounds__(64) void kb_solve_y(const GSys *__restrict__ sys, i
   const GSys &S = sys[s];
   if (!S.incyc) return;  // uniform over the block
   const double *sm = S.small + k * 8;
-  double *y = S.coef + k * (m + 1);
+  double *y = S.cw + (size_t)k * cw_stride(m);
   int kk = sm[5] > 0.0 ? (int)sm[5] : steps;
   if (sm[4] <= 0.0) kk = 0;
   const double *H = S.H + (size_t)k * (m + 1) * m;
@@ -677,7 +765,7 @@ __global__ __launch_bounds__(256) void kb_xupdate(const GSys *__restrict__ sys, 
   const GSys &S = sys[tm.sys];
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
-  const double *y = S.coef + k * (m + 1);
+  const double *y = S.cw + (size_t)k * cw_stride(m);
 #pragma unroll
   for (int t = 0; t < TILE / 32; ++t) {
     const int row = tm.row0 + rr + 32 * t;
@@ -785,15 +873,18 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   }
   // per system: Hessenberg, rotations, g, coefficients; the per-rhs state (8 doubles per rhs) of all
   // systems sits in one separate array so that the host reads it with one copy
-  const size_t small_per = (size_t)NB * ((m + 1) * m + 2 * m + 2 * (m + 1));
+  auto small_of = [&](int mc) {  // H, Hu, cs, sn, g, cw per rhs
+    return (size_t)NB * (2 * (size_t)(mc + 1) * mc + 2 * mc + (mc + 1) + cw_stride(mc));
+  };
+  const size_t small_per = small_of(m);
   const size_t state_per = (size_t)NB * 8;
-  DevBuf<double> R, W, Q, SM, ST8, partW, partT, partT2, Z64, SDEF, partS;
+  DevBuf<double> R, W, Q, SM, ST8, partW, partT, Z64, SDEF, partS;
   DevBuf<double> ZT;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
   if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
       SM.alloc(small_per * nsys, ctx) || ST8.alloc(state_per * nsys, ctx) || partW.alloc((size_t)nw_full * NB, ctx) ||
-      partT.alloc((size_t)nt_full * (m + 1) * NB, ctx) || partT2.alloc((size_t)nt_full * NB, ctx) ||
+      partT.alloc((size_t)nt_full * part_stride(m) * NB, ctx) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
   if (structured && (Z64.alloc(vec_elems, ctx) || ZT.alloc(zt_elems, ctx) ||
@@ -931,7 +1022,6 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // converge slowly under restarts) are laid out again with the longest restart that fits the same
   // buffers (Q, SM, partT), up to MMAX.
   constexpr int MMAX = 1024;
-  auto small_of = [&](int mc) { return (size_t)NB * ((size_t)(mc + 1) * mc + 2 * mc + 2 * (mc + 1)); };
   auto layout = [&](const std::vector<int> &list, int mc) {
     size_t bo = 0, so = 0;
     for (int s = 0; s < nsys; ++s) hs[s].incyc = 0;
@@ -944,13 +1034,15 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       so += small_of(mc);
       g.H = sp;
       sp += (size_t)NB * (mc + 1) * mc;
+      g.Hu = sp;
+      sp += (size_t)NB * (mc + 1) * mc;
       g.cs = sp;
       sp += NB * mc;
       g.sn = sp;
       sp += NB * mc;
       g.g = sp;
       sp += NB * (mc + 1);
-      g.coef = sp;
+      g.cw = sp;
     }
   };
   auto restart_for = [&](const std::vector<int> &list) {
@@ -961,7 +1053,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     }
     for (int mc = MMAX; mc > m; mc -= 8)
       if ((size_t)(mc + 1) * nsum * NB <= basis_elems && list.size() * small_of(mc) <= small_per * nsys &&
-          ntl * (mc + 1) <= (size_t)nt_full * (m + 1))
+          ntl * part_stride(mc) <= (size_t)nt_full * part_stride(m))
         return mc;
     return m;
   };
@@ -1076,26 +1168,20 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
       gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
-      // CGS2: two passes of (coefficients Q^T W, W -= Q coef); each pass streams the basis twice
+      // one sweep of dot products and one of updates over the basis (delayed reorthogonalisation)
       double nlive = 0.0;
       for (int s = 0; s < nsys; ++s)
         if (live.ntiles[s]) nlive += in[s].n;
       const double vbytes = nlive * NB * sizeof(double);
-      const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", 2.0 * (2.0 * (j + 1) + 3.0) * vbytes,
-                                        2.0 * 4.0 * (j + 1) * nlive * NB);
-      for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(kb_dots, dim3(live.n_t, (j + ICH) / ICH), dim3(256), 0, st, dsys.p, live.t.p, j, mc + 1,
-                           partT.p);
-        hipLaunchKernelGGL(kb_dots_final, dim3(nsys, (j + 8) / 8), dim3(256), 0, st, dsys.p, j, mc, mc + 1, partT.p,
-                           pass == 0 ? 1 : 0);
-        hipLaunchKernelGGL(kb_sub, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, mc, partT2.p);
-      }
+      const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (2.0 * j + 6.0) * vbytes,
+                                        2.0 * 4.0 * j * nlive * NB);
+      hipLaunchKernelGGL(kb_adots, dim3(live.n_t, std::max(1, (j + ICH - 1) / ICH)), dim3(256), 0, st, dsys.p,
+                         live.t.p, j, mc, partT.p, 0);
+      hipLaunchKernelGGL(kb_adots_final, dim3(nsys, (2 * j + 3 + 7) / 8), dim3(256), 0, st, dsys.p, j, mc, partT.p);
+      hipLaunchKernelGGL(kb_arnoldi, dim3(nsys * NB), dim3(64), sizeof(double) * (5 * mc + 2), st, dsys.p, j, mc, tol,
+                         0);
+      hipLaunchKernelGGL(kb_aupdate, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, mc);
       andr_prof_end(ctx, tok_o);
-      const int tok_g = andr_prof_begin(ctx, "gmres_givens_scale", 2.0 * vbytes, 0.0);
-      hipLaunchKernelGGL(kb_givens, dim3(nsys * NB), dim3(64), sizeof(double) * (3 * mc + 1), st, dsys.p, j, mc,
-                         partT2.p, tol);
-      hipLaunchKernelGGL(kb_scale_next, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j);
-      andr_prof_end(ctx, tok_g);
       steps = j + 1;
       if (steps % 8 == 0 && steps < steps_max) {  // drop systems whose columns are all resolved
         ANDR_HIP(ctx, hipGetLastError());
@@ -1116,6 +1202,15 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
           if ((rc = use_tiles(live))) return rc;
         }
       }
+    }
+    if (steps == steps_max && live.n_t > 0) {
+      // close the cycle: the last column is final once its candidate is reorthogonalised
+      hipLaunchKernelGGL(kb_adots, dim3(live.n_t, std::max(1, (steps + ICH - 1) / ICH)), dim3(256), 0, st, dsys.p,
+                         live.t.p, steps, mc, partT.p, 1);
+      hipLaunchKernelGGL(kb_adots_final, dim3(nsys, (2 * steps + 3 + 7) / 8), dim3(256), 0, st, dsys.p, steps, mc,
+                         partT.p);
+      hipLaunchKernelGGL(kb_arnoldi, dim3(nsys * NB), dim3(64), sizeof(double) * (5 * mc + 2), st, dsys.p, steps, mc,
+                         tol, 1);
     }
     total += steps;
     hipLaunchKernelGGL(kb_solve_y, dim3(nsys * NB), dim3(64), sizeof(double) * (mc + 1), st, dsys.p, steps, mc);
