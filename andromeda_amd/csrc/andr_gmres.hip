@@ -477,89 +477,74 @@ __host__ __device__ constexpr int part_stride(int m) { return 2 * (m + 1) + 3; }
 
 // per (tile, chunk of ICH basis vectors): partial s_i = Q_i . u and t_i = Q_i . W over the tile's
 // rows, part[(tile * PS + slot) * NB + k]; the first chunk also adds u.u, u.W, W.W. fin: W = 0
-// (closing a cycle: only the reorthogonalisation of the last candidate).
+// (closing a cycle: only the reorthogonalisation of the last candidate). Wave w of the block takes
+// the basis vectors i0 + w, i0 + w + 4, ... over all 64 rows of the tile (lane: row within a group
+// of 8, rhs), so that each dot product is reduced across lanes once per tile.
 __global__ __launch_bounds__(256) void kb_adots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
                                                 int m, double *__restrict__ part, int fin) {
-  __shared__ double red[4][2 * ICH + 3][NB];
+  constexpr int PW = ICH / 4;  // basis vectors per wave
   const int i0 = blockIdx.y * ICH;
-  const int ni = max(0, min(ICH, j - i0));
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n, PS = part_stride(m);
-  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int k = lane & 7, r8 = lane >> 3;
   const double *U = S.Q + (size_t)j * n * NB;
-  double u[TILE / 32], w[TILE / 32];
-  int rows[TILE / 32];
+  double u[TILE / 8], w[TILE / 8];
+  int rows[TILE / 8];
 #pragma unroll
-  for (int t = 0; t < TILE / 32; ++t) {
-    const int row = tm.row0 + rr + 32 * t;
+  for (int t = 0; t < TILE / 8; ++t) {
+    const int row = tm.row0 + 8 * t + r8;
     rows[t] = row < n ? row : -1;
     u[t] = row < n ? U[(size_t)row * NB + k] : 0.0;
     w[t] = (row < n && !fin) ? S.W[(size_t)row * NB + k] : 0.0;
   }
-  double as[ICH], at[ICH];
+  double as[PW], at[PW];
 #pragma unroll
-  for (int ii = 0; ii < ICH; ++ii) {
+  for (int p = 0; p < PW; ++p) {
+    const int i = i0 + wid + 4 * p;
     double a = 0.0, b = 0.0;
-    if (ii < ni) {
-      const double *Qi = S.Q + (size_t)(i0 + ii) * n * NB;
+    if (i < j) {
+      const double *Qi = S.Q + (size_t)i * n * NB + k;
 #pragma unroll
-      for (int t = 0; t < TILE / 32; ++t)
+      for (int t = 0; t < TILE / 8; ++t)
         if (rows[t] >= 0) {
-          const double q = Qi[(size_t)rows[t] * NB + k];
+          const double q = Qi[(size_t)rows[t] * NB];
           a += q * u[t];
           b += q * w[t];
         }
     }
-    as[ii] = a;
-    at[ii] = b;
+    as[p] = a;
+    at[p] = b;
   }
-  double sc[3] = {0.0, 0.0, 0.0};
-  if (blockIdx.y == 0)
-#pragma unroll
-    for (int t = 0; t < TILE / 32; ++t) {
-      sc[0] += u[t] * u[t];
-      sc[1] += u[t] * w[t];
-      sc[2] += w[t] * w[t];
-    }
-  auto wsum = [&](double x) {
+  auto rsum = [&](double x) {  // over the 8 row groups of the wave
     x += __shfl_xor(x, 8, 64);
     x += __shfl_xor(x, 16, 64);
     x += __shfl_xor(x, 32, 64);
     return x;
   };
 #pragma unroll
-  for (int ii = 0; ii < ICH; ++ii) {
-    as[ii] = wsum(as[ii]);
-    at[ii] = wsum(at[ii]);
+  for (int p = 0; p < PW; ++p) {
+    const double a = rsum(as[p]), b = rsum(at[p]);
+    const int i = i0 + wid + 4 * p;
+    if (lane < 8 && i < j) {
+      part[((size_t)blockIdx.x * PS + i) * NB + k] = a;
+      part[((size_t)blockIdx.x * PS + (m + 1) + i) * NB + k] = b;
+    }
   }
+  if (blockIdx.y == 0 && wid == 0) {
+    double sc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) sc[c] = wsum(sc[c]);
-  if (lane < 8) {
-#pragma unroll
-    for (int ii = 0; ii < ICH; ++ii) {
-      red[wid][ii][lane] = as[ii];
-      red[wid][ICH + ii][lane] = at[ii];
+    for (int t = 0; t < TILE / 8; ++t) {
+      sc[0] += u[t] * u[t];
+      sc[1] += u[t] * w[t];
+      sc[2] += w[t] * w[t];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) red[wid][2 * ICH + c][lane] = sc[c];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < (2 * ICH + 3) * NB; idx += blockDim.x) {
-    const int q = idx / NB, kk = idx % NB;
-    int slot;
-    if (q < ICH) {
-      if (q >= ni) continue;
-      slot = i0 + q;
-    } else if (q < 2 * ICH) {
-      if (q - ICH >= ni) continue;
-      slot = (m + 1) + i0 + (q - ICH);
-    } else {
-      if (blockIdx.y != 0) continue;
-      slot = 2 * (m + 1) + (q - 2 * ICH);
+    for (int c = 0; c < 3; ++c) {
+      const double v = rsum(sc[c]);
+      if (lane < 8) part[((size_t)blockIdx.x * PS + 2 * (m + 1) + c) * NB + k] = v;
     }
-    part[((size_t)blockIdx.x * PS + slot) * NB + kk] = red[0][q][kk] + red[1][q][kk] + red[2][q][kk] + red[3][q][kk];
   }
 }
 
