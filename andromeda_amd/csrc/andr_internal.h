@@ -118,11 +118,12 @@ struct BodyGeom {
   const double *area = nullptr;  // ne (Heron)
   const double *cen = nullptr;   // 3*ne centroids (CalQMesh)
   const double *mul = nullptr;   // ne: NORMMUL of the element's wall
+  const double *lmax2 = nullptr; // ne: longest edge squared
 };
 
 // Host-side owner of a body's device arrays.
 struct DevBody {
-  DevBuf<double> xyz, nrm, area, cen, mul;
+  DevBuf<double> xyz, nrm, area, cen, mul, lmax2;
   DevBuf<int> con, bcid, color_elems;
   std::vector<int> color_off;  // host copy of color offsets (size ncolors+1)
   DevBuf<int> d_color_off;

@@ -469,6 +469,7 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   const double s[3] = {g_.cen[3 * r], g_.cen[3 * r + 1], g_.cen[3 * r + 2]};
   const double ns[3] = {g_.nrm[3 * r], g_.nrm[3 * r + 1], g_.nrm[3 * r + 2]};
   uint8_t *crow = J.cls + (size_t)r * ne;
+  const double nr2 = near_ratio * near_ratio;
   unsigned long long pts = 0, nl = 0, nf = 0;
   int qn = 0;  // queue length (wave-uniform)
   // classify the first `cnt` queued pairs (all lanes take part: the matrix cores need a full wave)
@@ -509,18 +510,23 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
     bool near = false;
     float f3[3] = {0.f, 0.f, 0.f};
     if (e < ne) {
+      // far test on squared distances from the stored centroid and longest edge: dh >= near_ratio
+      // is d^2 >= near_ratio^2 L^2, and the half-octave bin floor(2 log2(dh / near_ratio)) is the
+      // binary exponent of d^2 / (near_ratio^2 L^2)
+      const double dx = s[0] - g_.cen[3 * e], dy = s[1] - g_.cen[3 * e + 1], dz = s[2] - g_.cen[3 * e + 2];
+      const double q = (dx * dx + dy * dy + dz * dz) / (nr2 * g_.lmax2[e]);
       if (e == r) {
         crow[e] = NCLS_DUN;
+      } else if (q >= 1.0) {
+        const int cl = sfar[min(ilogb(q), ANDR_FAR_BINS - 1)];
+        crow[e] = (uint8_t)cl;
+        pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
       } else {
         const double *x1 = g_.xyz + 3 * g_.con[3 * e], *x2 = g_.xyz + 3 * g_.con[3 * e + 1],
                      *x3 = g_.xyz + 3 * g_.con[3 * e + 2];
         const double n[3] = {g_.nrm[3 * e], g_.nrm[3 * e + 1], g_.nrm[3 * e + 2]};
         const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g_.area[e]);
-        if (pg.dh >= near_ratio) {
-          const int cl = sfar[far_bin(pg.dh, near_ratio)];
-          crow[e] = (uint8_t)cl;
-          pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
-        } else {
+        {
           near = true;
           float f[NIN];
           features(pg, 0, f);
