@@ -184,15 +184,23 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
   }
 }
 
-// thread per (system, wall, rhs): sdef = sum of the system's tile partials
-__global__ void kb_sred(const GSys *__restrict__ sys, int nsys, const double *__restrict__ part_s) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nsys * MAXW * NB) return;
-  const int s = t / (MAXW * NB), r = t % (MAXW * NB);
-  const GSys &S = sys[s];
+// block per system, thread per (wall, rhs) and eighth of the element tiles: sdef = sum of the
+// system's tile partials
+__global__ __launch_bounds__(256) void kb_sred(const GSys *__restrict__ sys, const double *__restrict__ part_s) {
+  static_assert(MAXW * NB == 32, "eight partial sums per (wall, rhs)");
+  __shared__ double red[8][32];
+  const GSys &S = sys[blockIdx.x];
+  const int r = threadIdx.x & 31, p = threadIdx.x >> 5;
   double a = 0.0;
-  for (int i = 0; i < S.net; ++i) a += part_s[(size_t)(S.et0 + i) * MAXW * NB + r];
-  S.sdef[r] = a;
+  for (int i = p; i < S.net; i += 8) a += part_s[(size_t)(S.et0 + i) * MAXW * NB + r];
+  red[p][r] = a;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v += red[q][threadIdx.x];
+    S.sdef[threadIdx.x] = v;
+  }
 }
 
 // null-space term of row (element r, component c) for rhs k
@@ -1066,7 +1074,6 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
   const int gs = (nsys * NB + 63) / 64;
-  const int gsd = (nsys * MAXW * NB + 255) / 256;
   // which_x 0: residual R = B - A X (R holds B); 1: Krylov product W = A Q_j
   auto gemv = [&](const MapSet &ms, int which_x, int jq, int which_y, const double *scales) {
     if (ms.n_w == 0) return;
@@ -1082,7 +1089,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       const int tok = andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym",
                                       n2 * 4.0 + 2.0 * n1 * NB * 8.0, 2.0 * 1.5 * n2 * nrhs);
       hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
-      hipLaunchKernelGGL(kb_sred, dim3(gsd), dim3(256), 0, st, dsys.p, nsys, partS.p);
+      hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
       if (krylov) {
         hipLaunchKernelGGL(kb_gemv_sym, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
         if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
