@@ -366,9 +366,9 @@ __device__ inline bf16x4 act_tile(f32x4 c, const float *bias, int mt, int g) {
 }
 
 // logits of 16 pairs (column block) for one kernel id; returns in lanes g == 0 the class and the
-// softmax confidence of pair c
+// rule class of pair c (quantile decision, see below)
 __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float *feat /* [16][3] */, int kernel,
-                                 int g, int c, int *cls, float *conf) {
+                                 int g, int c, float min_conf, int *cls, float *conf) {
   bf16x4 x;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -411,19 +411,32 @@ __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float
     zz[v] = g == 0 ? mine : other;
     zz[4 + v] = g == 0 ? other : mine;
   }
+  // The classes are ordered (a rule that is good enough stays good enough for every larger class),
+  // so the decision is a quantile of the predicted label distribution: the smallest class whose
+  // cumulative probability reaches min_conf. *conf is 1 when that happens below the last class.
   float mx = zz[0];
-  int best = 0;
 #pragma unroll
-  for (int o = 1; o < NOUT; ++o)
-    if (zz[o] > mx) {
-      mx = zz[o];
-      best = o;
+  for (int o = 1; o < NOUT; ++o) mx = fmaxf(mx, zz[o]);
+  float e[NOUT], sum = 0.f;
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    e[o] = expf(zz[o] - mx);
+    sum += e[o];
+  }
+  const float target = min_conf * sum;
+  float cum = 0.f;
+  int pick = NOUT - 1;
+  bool found = false;
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    cum += e[o];
+    if (!found && cum >= target) {
+      pick = o;
+      found = true;
     }
-  float sum = 0.f;
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o) sum += expf(zz[o] - mx);
-  *cls = best;
-  *conf = 1.f / sum;
+  }
+  *cls = pick;
+  *conf = found ? 1.f : 0.f;
 }
 
 // one block per (job, row): rule class of every pair of the row
@@ -438,7 +451,6 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   __shared__ int qe[CLS_WAVES][CLS_Q];                // queued near pairs: element
   __shared__ float qf[CLS_WAVES][CLS_Q][3];           // and its three geometric features
   __shared__ int qres[CLS_WAVES][64][2];              // class per kernel of a processed batch
-  __shared__ float qconf[CLS_WAVES][64][2];
   __shared__ unsigned long long red[3];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
@@ -478,22 +490,19 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
       if (16 * blk >= cnt) break;  // uniform
       const float *feat = &qf[wid][16 * blk][0];
       int a, b;
-      float ca, cb;
-      mlp_block(T, sb, feat, 1, g, c, &a, &ca);
-      mlp_block(T, sb, feat, 2, g, c, &b, &cb);
+      float ca, cb;  // whether the quantile was reached below the last class (unused here)
+      mlp_block(T, sb, feat, 1, g, c, min_conf, &a, &ca);
+      mlp_block(T, sb, feat, 2, g, c, min_conf, &b, &cb);
       if (g == 0) {
         qres[wid][16 * blk + c][0] = a;
         qres[wid][16 * blk + c][1] = b;
-        qconf[wid][16 * blk + c][0] = ca;
-        qconf[wid][16 * blk + c][1] = cb;
       }
     }
     __builtin_amdgcn_wave_barrier();
     if (lane < cnt) {
       const int a = qres[wid][lane][0], b = qres[wid][lane][1];
-      const float ca = qconf[wid][lane][0], cb = qconf[wid][lane][1];
       uint8_t cl;
-      if (ca < min_conf || cb < min_conf || a == NCLS_DUN || b == NCLS_DUN) {
+      if (a == NCLS_DUN || b == NCLS_DUN) {
         cl = NCLS_DUN;
         ++nf;
       } else {

@@ -99,9 +99,14 @@ def far_table(ctx, meshes, near_ratio, rel_tol):
 
 
 def decide(probs, conf):
-    """Class chosen per record and whether it is taken (confidence reached, not class 7)."""
-    cls = probs.argmax(axis=1)
-    taken = (probs.max(axis=1) >= conf) & (cls < NCLS - 1)
+    """Class chosen per record and whether it is taken (not class 7). The classes are ordered (a
+    rule good enough for a pair stays good enough for every larger class), so the choice is the
+    smallest class whose cumulative probability reaches conf: the conf-quantile of the predicted
+    label."""
+    cum = np.cumsum(probs, axis=1)
+    reached = cum >= conf * cum[:, -1:]
+    cls = np.where(reached.any(axis=1), reached.argmax(axis=1), NCLS - 1)
+    taken = cls < NCLS - 1
     return cls, taken
 
 
@@ -115,8 +120,42 @@ def validation_stats(probs, labels, conf, nq_fixed=126):
             "accuracy": float((cls == labels).mean())}
 
 
+CONFIDENCES = (0.9, 0.95, 0.98, 0.99, 0.995, 0.998, 0.999, 0.9995)
+
+
+def select_confidence(probs, labels, max_unsafe, log=print):
+    """The quantile level with the fewest points whose validation unsafe rate (chosen class below the
+    label) stays within max_unsafe."""
+    stats = [validation_stats(probs, labels, c) for c in CONFIDENCES]
+    ok = [s for s in stats if s["unsafe"] <= max_unsafe]
+    chosen = min(ok, key=lambda s: s["mean_points"]) if ok else stats[-1]
+    for s in stats:
+        log(f"[quad] validation conf {s['confidence']:.4f}: taken {s['taken']:.3f} unsafe {s['unsafe']:.2e} "
+            f"points {s['mean_points']:.1f} (labels {s['oracle_points']:.1f}) accuracy {s['accuracy']:.3f}")
+    log(f"[quad] chosen confidence {chosen['confidence']}")
+    return chosen
+
+
+def reselect(ctx, path, max_unsafe=1e-4, log=print):
+    """Chooses the decision level of a trained model again on its validation bodies (same seeds and
+    sampling as training) and rewrites the model file; the network parameters are kept."""
+    z = dict(np.load(path))
+    near_ratio, rel_tol = float(z["near_ratio"]), float(z["rel_tol"])
+    lo, hi = (int(v) for v in z["val_seeds"])
+    rng = np.random.default_rng(0)
+    va = bodies_for(range(lo, hi + 1), 10)
+    Xva, Yva = collect_pairs(ctx, va, near_ratio, rel_tol, 20000 // 4, rng, log)
+    chosen = select_confidence(ctx.mlp_infer(Xva, z["params"]), Yva, max_unsafe, log)
+    z["confidence"] = np.float32(chosen["confidence"])
+    z["val_unsafe"] = np.float64(chosen["unsafe"])
+    z["val_points"] = np.float64(chosen["mean_points"])
+    np.savez(path, **z)
+    log(f"[quad] wrote {path}")
+    return z
+
+
 def train(ctx, near_ratio=4.0, rel_tol=1e-7, train_seeds=range(0, 100), val_seeds=range(100, 200),
-          n_large=10, per_mesh=20000, epochs=30, batch=4096, lr=2e-3, max_unsafe=1e-3, seed=0, log=print):
+          n_large=10, per_mesh=20000, epochs=30, batch=4096, lr=2e-3, max_unsafe=1e-4, seed=0, log=print):
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     tr = bodies_for(train_seeds, n_large)
@@ -131,14 +170,7 @@ def train(ctx, near_ratio=4.0, rel_tol=1e-7, train_seeds=range(0, 100), val_seed
     t2 = time.perf_counter()
     params, losses = ctx.mlp_train(Xtr, Ytr, init_params(seed), epochs=epochs, batch=batch, lr=lr, seed=seed)
     log(f"[quad] trained {epochs} epochs in {time.perf_counter() - t2:.1f} s; loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    probs = ctx.mlp_infer(Xva, params)
-    stats = [validation_stats(probs, Yva, c) for c in (0.5, 0.6, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99)]
-    ok = [s for s in stats if s["unsafe"] <= max_unsafe]
-    chosen = min(ok, key=lambda s: s["mean_points"]) if ok else stats[-1]
-    for s in stats:
-        log(f"[quad] validation conf {s['confidence']:.2f}: taken {s['taken']:.3f} unsafe {s['unsafe']:.2e} "
-            f"points {s['mean_points']:.1f} (labels {s['oracle_points']:.1f}) accuracy {s['accuracy']:.3f}")
-    log(f"[quad] chosen confidence {chosen['confidence']}")
+    chosen = select_confidence(ctx.mlp_infer(Xva, params), Yva, max_unsafe, log)
     return {"params": params, "confidence": np.float32(chosen["confidence"]), "near_ratio": np.float64(near_ratio),
             "far_table": far, "rel_tol": np.float64(rel_tol), "losses": losses,
             "train_seeds": np.array([min(train_seeds), max(train_seeds)]),
@@ -149,7 +181,8 @@ def train(ctx, near_ratio=4.0, rel_tol=1e-7, train_seeds=range(0, 100), val_seed
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="train a model and write it here")
+    ap.add_argument("--reselect", help="choose the decision level of this model file again")
     ap.add_argument("--near-ratio", type=float, default=4.0)
     ap.add_argument("--rel-tol", type=float, default=1e-7)
     ap.add_argument("--epochs", type=int, default=30)
@@ -158,6 +191,11 @@ def main(argv=None):
     args = ap.parse_args(argv)
     ctx = api.Context(0)
     log = lambda s: print(s, flush=True)  # noqa: E731
+    if args.reselect:
+        reselect(ctx, args.reselect, log=log)
+        return 0
+    if not args.out:
+        ap.error("--out or --reselect is required")
     model = train(ctx, near_ratio=args.near_ratio, rel_tol=args.rel_tol, epochs=args.epochs,
                   per_mesh=args.per_mesh, train_seeds=range(0, args.train_seeds),
                   val_seeds=range(100, 100 + args.train_seeds), log=log)

@@ -169,8 +169,9 @@ int andr_mlp_infer(andr_ctx *ctx, int64_t npairs, const float *features, const f
 int andr_mlp_loss_grad(andr_ctx *ctx, int64_t npairs, const float *features, const int32_t *labels,
                        const float *params, float *loss, float *grad);
 /* Installs the rule selection used by ANDR_PREC_LEARNED. Pairs with distance/size below near_ratio
- * go through the network; confidence is the minimum softmax probability of the chosen class, below
- * it the pair falls back to the fixed-order rule. Pairs farther out take far_table[b], the class
+ * go through the network; each of its two kernels (Stokeslet, stresslet) takes the smallest class
+ * whose cumulative softmax probability reaches `confidence` (a quantile of the predicted label),
+ * the pair the larger of the two; class 7 is the fixed-order rule. Pairs farther out take far_table[b], the class
  * calibrated on training pairs for bin b = floor(2 log2(d / (h near_ratio))), b = 0..15
  * (beyond the last bin: far_table[15]). */
 #define ANDR_FAR_BINS 16

@@ -121,12 +121,14 @@ def test_far_table_fill_and_monotone():
 
 def test_validation_stats():
     from andromeda_amd import quadrature_model as QM
-    probs = np.array([[0.95, 0.05, 0, 0, 0, 0, 0, 0],   # class 0 taken, label 1: unsafe
-                      [0, 0, 0.6, 0.4, 0, 0, 0, 0],     # class 2 below confidence: fixed rule
-                      [0, 0, 0, 0, 0, 0, 0, 1.0]])      # class 7: fixed rule
+    probs = np.array([[0.95, 0.05, 0, 0, 0, 0, 0, 0],   # 0.9-quantile class 0, label 1: unsafe
+                      [0, 0, 0.6, 0.4, 0, 0, 0, 0],     # quantile reached at class 3 (7 points), label 2
+                      [0, 0, 0, 0, 0, 0, 0.5, 0.5]])    # quantile at class 7: fixed rule
     s = QM.validation_stats(probs, np.array([1, 2, 7]), 0.9)
-    assert s["taken"] == pytest.approx(1 / 3) and s["unsafe"] == pytest.approx(1 / 3)
-    assert s["mean_points"] == pytest.approx((3 + 126 + 126) / 3)
+    assert s["taken"] == pytest.approx(2 / 3) and s["unsafe"] == pytest.approx(1 / 3)
+    assert s["mean_points"] == pytest.approx((3 + 7 + 126) / 3)
+    cls, taken = QM.decide(probs, 0.5)  # the median: classes 0, 2 and 6
+    assert cls.tolist() == [0, 2, 6] and taken.tolist() == [True, True, True]
 
 
 # ---- GPU: kernels against the oracle -------------------------------------------------------------
@@ -196,6 +198,7 @@ def test_gpu_labels_match_oracle(ctx, orc, shape):
 
 def _expected_classes(m, orc, model, mesh_nq=126):
     """Oracle decisions of the in-assembly classifier for every centroid-row pair of a body."""
+    from andromeda_amd import quadrature_model as QM
     x, nrm, area, cen = QL.geometry(m.xyz, m.con)
     near, conf, far = float(model["near_ratio"]), float(model["confidence"]), model["far_table"]
     pts = learned = fallback = 0
@@ -214,12 +217,10 @@ def _expected_classes(m, orc, model, mesh_nq=126):
     if feats:
         # BF16 inputs, weights and activations as on the matrix cores; biases FP32
         Z = MO.forward_bf16(np.asarray(model["params"], np.float64), np.array(feats, dtype=np.float32))
-        P = MO.softmax(Z)
-        c = P.argmax(1)
-        pmax = P.max(1)
+        c, _ = QM.decide(MO.softmax(Z), conf)
         for i, (r, e) in enumerate(where):
             a, b = c[2 * i], c[2 * i + 1]
-            if pmax[2 * i] < conf or pmax[2 * i + 1] < conf or a == 7 or b == 7:
+            if a == 7 or b == 7:
                 cls[(r, e)] = 7
                 fallback += 1
             else:
