@@ -1,8 +1,9 @@
-// Restarted GMRES on a batch of dense square systems with up to ANDR_NB right-hand sides each:
-// the solver of the mixed-precision resistance path.
+// Restarted GMRES on a batch of structured Stokes systems (the six symmetric FP32 Stokeslet blocks
+// of a body, right block-Jacobi preconditioner, null-space term) with up to ANDR_NB right-hand sides
+// each: the solver of the mixed-precision and learned resistance forms.
 //
-// Storage of A may be FP32; products accumulate in FP64 and every vector (Krylov basis, Hessenberg,
-// solution) is FP64. Each restart cycle starts from the FP64 residual r = b - A x, so the cycles are
+// The blocks are stored in FP32; products are formed in FP64 and every vector (Krylov basis,
+// Hessenberg, solution) is FP64. Each restart cycle starts from the FP64 residual r = b - A x, so the cycles are
 // the iterative-refinement loop around the Krylov solve. All systems of the batch advance together:
 // one launch per step covers every system (work lists map waves and row tiles to systems), so the
 // launch count does not grow with the number of bodies.
@@ -67,68 +68,6 @@ struct TileMap {
   int sys, row0;
 };
 
-constexpr int WROWS = 64;  // rows of A per wave in the products
-
-// Y = sx .* (A X) + sy .* Y for every system; sx/sy per (system, rhs). part[wave][k] = sum Y^2 over
-// the wave's rows. FP64 accumulation (the residual product of the refinement loop): the wave
-// works through its 64 rows in blocks of 8, each lane covering every 64th column.
-template <class ST>
-__global__ __launch_bounds__(256) void kb_gemv(const GSys *__restrict__ sys, const WaveMap *__restrict__ wmap, int nwaves,
-                                               int which_x, int jq, int which_y, const double *__restrict__ sxy,
-                                               double *__restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wave >= nwaves) return;
-  const WaveMap wm = wmap[wave];
-  const GSys &S = sys[wm.sys];
-  const int m = S.n, n = S.n;
-  const ST *A = (const ST *)S.A;
-  const double *X = which_x == 0 ? S.X : S.Q + (size_t)jq * S.n * NB;
-  double *Y = which_y == 0 ? S.R : S.W;
-  const int64_t lda = S.lda;
-  double y2 = 0.0;
-  for (int row0 = wm.row0; row0 < min(wm.row0 + WROWS, m); row0 += 8) {
-    double acc[64];
-#pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] = 0.0;
-    const ST *arow[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) arow[r] = A + (size_t)min(row0 + r, m - 1) * lda;
-    for (int j = lane; j < n; j += 64) {
-      const double4 x0 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB);
-      const double4 x1 = *reinterpret_cast<const double4 *>(X + (size_t)j * NB + 4);
-      const double xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const double a = (double)arow[r][j];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc[r * 8 + k] += a * xv[k];
-      }
-    }
-#pragma unroll
-    for (int half = 32; half >= 1; half >>= 1) {
-      const bool hi = (lane & half) != 0;
-#pragma unroll
-      for (int i = 0; i < half; ++i) {
-        const double keep = hi ? acc[i + half] : acc[i];
-        const double send = hi ? acc[i] : acc[i + half];
-        acc[i] = keep + __shfl_xor(send, half, 64);
-      }
-    }
-    const int r = lane >> 3, k = lane & 7;
-    if (row0 + r < m) {
-      double *py = Y + (size_t)(row0 + r) * NB + k;
-      const double sx = sxy[(size_t)wm.sys * 2 * NB + k], sy = sxy[(size_t)wm.sys * 2 * NB + NB + k];
-      const double y = sx * acc[0] + (sy != 0.0 ? sy * (*py) : 0.0);
-      *py = y;
-      y2 += y * y;
-    }
-  }
-  y2 += __shfl_xor(y2, 8, 64);
-  y2 += __shfl_xor(y2, 16, 64);
-  y2 += __shfl_xor(y2, 32, 64);
-  if (part && lane < 8) part[(size_t)wave * NB + lane] = y2;
-}
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int MAXW = ANDR_OP_MAXW;
@@ -849,20 +788,20 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   const int m = std::max(4, restart);
   hipStream_t st = ctx->stream;
   // full work maps (sizes the partial-sum buffers); subsets are rebuilt as systems converge
-  // fp32: the structured Stokes operator of the fast path; otherwise a dense FP64 matrix
-  const bool structured = fp32;
+  // the structured Stokes operator of the fast path (FP32 symmetric blocks) is the only operator
+  if (!fp32) return andr_fail(ctx, ANDR_ERR_ARG, "GMRES takes the structured Stokes operator only");
   size_t vec_elems = 0, basis_elems = 0, zt_elems = 0;
   int nw_full = 0, nt_full = 0, net_full = 0;
   for (int s = 0; s < nsys; ++s) {
     const int n = in[s].n, ne = in[s].op.ne;
-    if (structured && (!in[s].op.S || !in[s].op.minv || 3 * ne != n))
+    if (!in[s].op.S || !in[s].op.minv || 3 * ne != n)
       return andr_fail(ctx, ANDR_ERR_ARG, "structured GMRES system %d is incomplete", s);
     vec_elems += (size_t)n * NB;
     basis_elems += (size_t)(m + 1) * n * NB;
-    nw_full += structured ? 4 * ((ne + 63) / 64) : (n + WROWS - 1) / WROWS;
+    nw_full += 4 * ((ne + 63) / 64);
     nt_full += (n + TILE - 1) / TILE;
-    net_full += structured ? (ne + 31) / 32 : 0;
-    zt_elems += structured ? (size_t)3 * NB * ((ne + 31) / 32 * 32) : 0;
+    net_full += (ne + 31) / 32;
+    zt_elems += (size_t)3 * NB * ((ne + 31) / 32 * 32);
   }
   // per system: Hessenberg, rotations, g, coefficients; the per-rhs state (8 doubles per rhs) of all
   // systems sits in one separate array so that the host reads it with one copy
@@ -880,10 +819,10 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       partT.alloc((size_t)nt_full * part_stride(m) * NB, ctx) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
-  if (structured && (Z64.alloc(vec_elems, ctx) || ZT.alloc(zt_elems, ctx) ||
+  if ((Z64.alloc(vec_elems, ctx) || ZT.alloc(zt_elems, ctx) ||
                      SDEF.alloc((size_t)nsys * MAXW * NB, ctx) || partS.alloc((size_t)net_full * MAXW * NB, ctx)))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES operator workspace");
-  // a map set over a list of systems: waves of WROWS rows (FP64-accumulating product), tiles of
+  // a map set over a list of systems: residual blocks of 64 element rows, tiles of
   // TILE rows (vector kernels) and matrix-core waves (64 rows x a column segment). When the list
   // has few rows (the last, slowest systems of a batch) the columns are split into segments so
   // that the product still fills the chip; the segments are summed by kb_seg_sum.
@@ -904,7 +843,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   MapSet cyc, live;
   for (MapSet *ms : {&cyc, &live})
     if (ms->w.alloc(nw_full, ctx) || ms->t.alloc(nt_full, ctx) || ms->w0.alloc(nsys, ctx) || ms->nw.alloc(nsys, ctx) ||
-        (structured && ms->et.alloc(net_full, ctx)))
+        ms->et.alloc(net_full, ctx))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES work maps");
   std::vector<GSys> hs(nsys);
   DevBuf<double> Pseg;
@@ -923,23 +862,16 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ms.p_elems = 0;
     ms.max_seg = 1;
     int row_waves = 0;
-    for (int s : list) row_waves += structured ? (in[s].op.ne + 31) / 32 : 0;
+    for (int s : list) row_waves += (in[s].op.ne + 31) / 32;
     const int want = row_waves > 0 ? std::min(MF_MAX_SEG, (MF_TARGET_WAVES + row_waves - 1) / row_waves) : 1;
     for (int s : list) {
       const int n = in[s].n, ne = in[s].op.ne;
-      if (structured) {
-        w0[s] = 4 * (int)wm.size();
-        for (int r = 0; r < ne; r += 64) wm.push_back({s, r});
-        nw[s] = 4 * (int)wm.size() - w0[s];
-      } else {
-        w0[s] = (int)wm.size();
-        for (int r = 0; r < n; r += WROWS) wm.push_back({s, r});
-        nw[s] = (int)wm.size() - w0[s];
-      }
+      w0[s] = 4 * (int)wm.size();
+      for (int r = 0; r < ne; r += 64) wm.push_back({s, r});
+      nw[s] = 4 * (int)wm.size() - w0[s];
       ms.tile0[s] = (int)tm.size();
       for (int r = 0; r < n; r += TILE) tm.push_back({s, r});
       ms.ntiles[s] = (int)tm.size() - ms.tile0[s];
-      if (!structured) continue;
       ms.et0[s] = (int)em.size();
       for (int e = 0; e < ne; e += 32) em.push_back({s, e});
       ms.net[s] = (int)em.size() - ms.et0[s];
@@ -993,13 +925,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     GSys &g = hs[s];
     const int n = in[s].n;
     g.op = in[s].op;
-    if (structured) {
-      g.nep = (g.op.ne + 31) / 32 * 32;
-      g.ZT = ZT.p + zo;
-      g.Z64 = Z64.p + vo;
-      g.sdef = SDEF.p + (size_t)s * MAXW * NB;
-      zo += (size_t)3 * NB * g.nep;
-    }
+    g.nep = (g.op.ne + 31) / 32 * 32;
+    g.ZT = ZT.p + zo;
+    g.Z64 = Z64.p + vo;
+    g.sdef = SDEF.p + (size_t)s * MAXW * NB;
+    zo += (size_t)3 * NB * g.nep;
     g.A = in[s].A;
     g.lda = in[s].lda;
     g.n = n;
@@ -1075,33 +1005,26 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
 
   const int gs = (nsys * NB + 63) / 64;
   // which_x 0: residual R = B - A X (R holds B); 1: Krylov product W = A Q_j
-  auto gemv = [&](const MapSet &ms, int which_x, int jq, int which_y, const double *scales) {
+  auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales) {
     if (ms.n_w == 0) return;
     double n2 = 0.0, n1 = 0.0;
     for (int s = 0; s < nsys; ++s)
       if (ms.ntiles[s]) {
         const double ne = in[s].op.ne;
-        n2 += structured ? 6.0 * ne * ne : (double)in[s].n * in[s].n;
+        n2 += 6.0 * ne * ne;
         n1 += in[s].n;
       }
-    if (structured) {
-      const bool krylov = which_x == 1;
-      const int tok = andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym",
-                                      n2 * 4.0 + 2.0 * n1 * NB * 8.0, 2.0 * 1.5 * n2 * nrhs);
-      hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
-      hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
-      if (krylov) {
-        hipLaunchKernelGGL(kb_gemv_sym, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
-        if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
-      } else {
-        hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
-      }
-      andr_prof_end(ctx, tok);
-      return;
+    const bool krylov = which_x == 1;
+    const int tok = andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
+                                    2.0 * 1.5 * n2 * nrhs);
+    hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
+    hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
+    if (krylov) {
+      hipLaunchKernelGGL(kb_gemv_sym, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
+      if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
+    } else {
+      hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
     }
-    const int tok = andr_prof_begin(ctx, "gmres_gemv_fp64", n2 * 8.0 + 3.0 * n1 * NB * 8.0, 2.0 * n2 * nrhs);
-    hipLaunchKernelGGL(kb_gemv<double>, dim3((ms.n_w + 3) / 4), dim3(256), 0, st, dsys.p, ms.w.p, ms.n_w, which_x, jq,
-                       which_y, scales, partW.p);
     andr_prof_end(ctx, tok);
   };
   std::vector<int> it_count((size_t)nsys * NB, 0);
@@ -1123,7 +1046,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   while (true) {
     // residual r = b - A x of every system of the last cycle (x = 0 on the first), FP64 accumulation
     if (cyc.n_t) hipLaunchKernelGGL(kb_copy_b, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
-    gemv(cyc, 0, 0, 0, d_res.p);
+    gemv(cyc, 0, 0, d_res.p);
     ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, partW.p, cyc.w0.p, cyc.nw.p, tol,
                        first ? 1 : 0, nrhs, ddone.p);
@@ -1159,7 +1082,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     const int steps_max = std::min(mc, maxit - total);
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
-      gemv(live, 1, j, 1, d_arn.p);  // W = A Q_j
+      gemv(live, 1, j, d_arn.p);  // W = A Q_j
       // one sweep of dot products and one of updates over the basis (delayed reorthogonalisation)
       double nlive = 0.0;
       for (int s = 0; s < nsys; ++s)

@@ -61,12 +61,14 @@ struct StokesOp {
 #define ANDR_OP_MAXW 4
 
 struct GmresSys {
-  const void *A;    // FP64 row-major with leading dimension lda (fp32 = false)
-  int64_t lda;
+  const void *A;    // unused (the operator is op)
+  int64_t lda;      // unused
   int n;
-  StokesOp op;      // fp32 = true: the structured Stokes operator, n = 3 op.ne
+  StokesOp op;      // the structured Stokes operator, n = 3 op.ne
   const double *B;  // n x ANDR_NB
   double *X;        // n x ANDR_NB
 };
+// Batched restarted GMRES (right-preconditioned, FP64 vectors) on structured Stokes systems; fp32
+// must be true (the FP32 symmetric-block operator). iters: per system and rhs, the Krylov steps.
 int andr_gmres_batch(andr_ctx *ctx, const GmresSys *sys, int nsys, bool fp32, int nrhs, int restart, int maxit,
                      double tol, int *iters);
