@@ -132,7 +132,7 @@ __device__ inline void quad_point(const double *x1, const double *x2, const doub
 __global__ void k_geometry(int ne, int nv, const double *__restrict__ xyz, const int *__restrict__ con,
                            const int *__restrict__ bcid, const double *__restrict__ wallmul, double *__restrict__ nrm,
                            double *__restrict__ area, double *__restrict__ cen, double *__restrict__ mul,
-                           double *__restrict__ lmax2) {
+                           float4 *__restrict__ cenl) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int *cn = con + (size_t)nv * e;
@@ -171,7 +171,7 @@ __global__ void k_geometry(int ne, int nv, const double *__restrict__ xyz, const
     for (int j = 0; j < 3; ++j) cen[3 * e + j] = 0.25 * t1[j] + 0.25 * t2[j] + 0.25 * t3[j] + 0.25 * t4[j];
   }
   mul[e] = wallmul[bcid[e] - 1];
-  {  // longest edge squared (the pair-distance scale of the learned rule selection)
+  {  // centroid and longest edge (the pair-distance scale of the learned rule selection), FP32
     const int nno = quad ? 4 : 3;
     double l2 = 0.0;
     for (int v = 0; v < nno; ++v) {
@@ -179,7 +179,7 @@ __global__ void k_geometry(int ne, int nv, const double *__restrict__ xyz, const
       const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
       l2 = fmax(l2, dx * dx + dy * dy + dz * dz);
     }
-    lmax2[e] = l2;
+    cenl[e] = make_float4((float)cen[3 * e], (float)cen[3 * e + 1], (float)cen[3 * e + 2], (float)(1.0 / l2));
   }
 }
 
@@ -241,14 +241,14 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
   DevBuf<double> d_wm;
   if (b.xyz.alloc(3 * (size_t)nn, ctx) || b.con.alloc((size_t)nv * ne, ctx) || b.bcid.alloc(ne, ctx) ||
       b.nrm.alloc(3 * (size_t)ne, ctx) || b.area.alloc(ne, ctx) || b.cen.alloc(3 * (size_t)ne, ctx) ||
-      b.mul.alloc(ne, ctx) || b.lmax2.alloc(ne, ctx) || d_wm.alloc(nw, ctx))
+      b.mul.alloc(ne, ctx) || b.cenl.alloc(ne, ctx) || d_wm.alloc(nw, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "device allocation for mesh");
   ANDR_HIP(ctx, hipMemcpyAsync(b.xyz.p, m->xyz, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(b.con.p, con.data(), sizeof(int) * nv * ne, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(b.bcid.p, bcid.data(), sizeof(int) * ne, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(d_wm.p, wm.data(), sizeof(double) * nw, hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_geometry, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, ne, nv, b.xyz.p, b.con.p,
-                     b.bcid.p, d_wm.p, b.nrm.p, b.area.p, b.cen.p, b.mul.p, b.lmax2.p);
+                     b.bcid.p, d_wm.p, b.nrm.p, b.area.p, b.cen.p, b.mul.p, b.cenl.p);
   ANDR_HIP(ctx, hipGetLastError());
   if (with_colors) {
     std::vector<int> order;
@@ -274,7 +274,7 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
   b.g.area = b.area.p;
   b.g.cen = b.cen.p;
   b.g.mul = b.mul.p;
-  b.g.lmax2 = b.lmax2.p;
+  b.g.cenl = b.cenl.p;
   return ANDR_OK;
 }
 

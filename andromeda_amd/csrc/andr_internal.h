@@ -118,12 +118,14 @@ struct BodyGeom {
   const double *area = nullptr;  // ne (Heron)
   const double *cen = nullptr;   // 3*ne centroids (CalQMesh)
   const double *mul = nullptr;   // ne: NORMMUL of the element's wall
-  const double *lmax2 = nullptr; // ne: longest edge squared
+  const float4 *cenl = nullptr;  // ne: centroid (x, y, z) and 1 / (longest edge)^2 in FP32, for the
+                                 // distance classes of the learned rule selection
 };
 
 // Host-side owner of a body's device arrays.
 struct DevBody {
-  DevBuf<double> xyz, nrm, area, cen, mul, lmax2;
+  DevBuf<double> xyz, nrm, area, cen, mul;
+  DevBuf<float4> cenl;
   DevBuf<int> con, bcid, color_elems;
   std::vector<int> color_off;  // host copy of color offsets (size ncolors+1)
   DevBuf<int> d_color_off;

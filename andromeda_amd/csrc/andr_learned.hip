@@ -52,8 +52,8 @@ __device__ inline PairGeo pair_geo(const double s[3], const double ns[3], const 
 }
 
 __device__ inline void features(const PairGeo &p, int kernel, float f[NIN]) {
-  f[0] = (float)log(p.dh);
-  f[1] = (float)log(p.aspect);
+  f[0] = logf((float)p.dh);  // the network takes FP32 inputs; FP32 logarithms are as good here
+  f[1] = logf((float)p.aspect);
   f[2] = (float)p.cosn;
   f[3] = kernel == 0 ? 1.f : 0.f;
   f[4] = kernel == 1 ? 1.f : 0.f;
@@ -481,7 +481,8 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   const double s[3] = {g_.cen[3 * r], g_.cen[3 * r + 1], g_.cen[3 * r + 2]};
   const double ns[3] = {g_.nrm[3 * r], g_.nrm[3 * r + 1], g_.nrm[3 * r + 2]};
   uint8_t *crow = J.cls + (size_t)r * ne;
-  const double nr2 = near_ratio * near_ratio;
+  const float inv_nr2 = (float)(1.0 / (near_ratio * near_ratio));
+  const float4 sr = g_.cenl[r];
   unsigned long long pts = 0, nl = 0, nf = 0;
   int qn = 0;  // queue length (wave-uniform)
   // classify the first `cnt` queued pairs (all lanes take part: the matrix cores need a full wave)
@@ -519,15 +520,16 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
     bool near = false;
     float f3[3] = {0.f, 0.f, 0.f};
     if (e < ne) {
-      // far test on squared distances from the stored centroid and longest edge: dh >= near_ratio
-      // is d^2 >= near_ratio^2 L^2, and the half-octave bin floor(2 log2(dh / near_ratio)) is the
-      // binary exponent of d^2 / (near_ratio^2 L^2)
-      const double dx = s[0] - g_.cen[3 * e], dy = s[1] - g_.cen[3 * e + 1], dz = s[2] - g_.cen[3 * e + 2];
-      const double q = (dx * dx + dy * dy + dz * dz) / (nr2 * g_.lmax2[e]);
+      // far test on squared distances from the stored centroid and longest edge (FP32): dh >=
+      // near_ratio is d^2 >= near_ratio^2 L^2, and the half-octave bin floor(2 log2(dh / near_ratio))
+      // is the binary exponent of d^2 / (near_ratio^2 L^2)
+      const float4 ce = g_.cenl[e];
+      const float dx = sr.x - ce.x, dy = sr.y - ce.y, dz = sr.z - ce.z;
+      const float q = (dx * dx + dy * dy + dz * dz) * ce.w * inv_nr2;
       if (e == r) {
         crow[e] = NCLS_DUN;
-      } else if (q >= 1.0) {
-        const int cl = sfar[min(ilogb(q), ANDR_FAR_BINS - 1)];
+      } else if (q >= 1.f) {
+        const int cl = sfar[min(ilogbf(q), ANDR_FAR_BINS - 1)];
         crow[e] = (uint8_t)cl;
         pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
       } else {
