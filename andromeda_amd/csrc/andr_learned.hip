@@ -5,7 +5,7 @@
 // element) pair, the cheapest triangle rule that reproduces the fixed-order integral: classes
 // 0..6 are the Dunavant rules with 3, 4, 6, 7, 12, 13 and 19 points, class 7 the mesh's own
 // fixed-order rule. Far pairs take a rule from a distance table calibrated on the same labels.
-// Labels are produced here from FP64 integrals (the fixed-order rule is the judge), the network
+// Labels are produced here from FP64 integrals (the fixed-order rule is the reference), the network
 // is trained here (forward, backward and Adam kernels), and inference runs in a classification
 // pass before the assembly with BF16 weights in LDS and FP32 accumulation.
 #include "andr_learned.h"

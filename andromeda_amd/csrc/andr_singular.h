@@ -12,7 +12,7 @@
 //   Laplace:  G = (1/4pi) sum h I0                                   (H = 0 since r.n = 0)
 //   Stokeslet (delta_ij/r + r_i r_j/r^3, no 1/8pi):  G_ij = sum [ h I0 delta_ij + int e_i e_j/r ]
 //
-// All arithmetic in FP64 (the north-star precision rule keeps singular integrals FP64).
+// All arithmetic in FP64 (singular integrals stay FP64 in every precision form).
 #pragma once
 
 #include <math.h>
