@@ -124,6 +124,21 @@ def test_slender_body_long_restarts(ctx):
     assert fast[0].nit.max() > 128 >= fast[1].nit.max()
 
 
+def test_tiny_bodies_krylov_space_exhausted(ctx):
+    """20- and 80-element spheres (60 and 240 unknowns): GMRES can span the whole space, so the
+    Arnoldi process meets a (near) breakdown; the fast forms still agree with the plain path, and a
+    batch with one tiny and one ordinary body gives the same tensors as the bodies alone."""
+    tiny = [M.icosphere(0), M.icosphere(1)]
+    for m in tiny:
+        p = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+        f = ctx.resistance(m, precision=api.PREC_MIXED)
+        assert np.all(np.isfinite(f.FT)) and tensor_error(f.FT, p.FT) < 1e-6
+    mixed = [tiny[0], M.icosphere(3)]
+    rb = ctx.resistance_batch(mixed, precision=api.PREC_MIXED)
+    for b, r in zip(mixed, rb):
+        assert _rel(r.FT, ctx.resistance(b, precision=api.PREC_MIXED).FT) < 1e-12
+
+
 def test_batch_equals_single(ctx):
     bodies = [M.random_superellipsoid(np.random.default_rng(s), 4)[0] for s in range(3)]
     for prec in (api.PREC_FP64, api.PREC_MIXED):
