@@ -13,7 +13,9 @@
 #include "andr_prof.h"
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace {
 
@@ -422,124 +424,218 @@ int andr_colscale(andr_ctx *ctx, void *A, bool fp32, int64_t lda, int m, int n, 
   return ANDR_OK;
 }
 
-int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m, int n, const double *q,
-                     const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop) {
-  if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
-  if (m < 1 || n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
-  hipStream_t st = ctx->stream;
-  const int wavesN = (m + 7) / 8;
-  const int blocksN = (wavesN + 3) / 4;
-  const int ns = splits_for(m, n);
-  const int rps = (m + ns - 1) / ns;
-  const int blocksC = (n * NB + 255) / 256;
-  const int blocksM = (m * NB + 255) / 256;
+namespace {
+
+// One LSQR solve, advanced in blocks of iterations on a stream of its own (several solves of a batch
+// then overlap on the device); the profiling tokens are taken only on the context's stream.
+struct LsqrRun {
+  andr_ctx *ctx = nullptr;
+  hipStream_t st = nullptr;
+  const void *A = nullptr;
+  bool fp32 = false;
+  int64_t lda = 0;
+  int m = 0, n = 0, nrhs = 0, itnlim = 0, it = 0;
+  const double *q = nullptr;
+  double *X = nullptr;
+  bool done = false;
+  int wavesN = 0, blocksN = 0, ns = 0, rps = 0, blocksC = 0, blocksM = 0;
   DevBuf<double> U, V, W, Xl, X0, Xs, P, partU, partV, partW, partB, sc;
   DevBuf<LsqrState> S;
   DevBuf<int> act;
-  if (U.alloc((size_t)m * NB) || V.alloc((size_t)n * NB) || W.alloc((size_t)n * NB) || Xl.alloc((size_t)n * NB) ||
-      X0.alloc((size_t)n * NB) || Xs.alloc((size_t)n * NB) || P.alloc((size_t)ns * n * NB) ||
-      partU.alloc((size_t)wavesN * NB) || partV.alloc((size_t)blocksC * NB) || partW.alloc((size_t)blocksC * NB) ||
-      partB.alloc((size_t)blocksM * NB) || sc.alloc(8 * NB) || S.alloc(NB) || act.alloc(NB))
-    return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR workspace (m=%d n=%d)", m, n);
-  double *sxn = sc.p, *syn = sc.p + NB, *sxt = sc.p + 2 * NB, *syt = sc.p + 3 * NB;
-  double *t1 = sc.p + 4 * NB, *t2 = sc.p + 5 * NB, *ia = sc.p + 6 * NB, *one = sc.p + 7 * NB;
-  // host-side initial state
   LsqrState hs[NB];
-  std::memset(hs, 0, sizeof(hs));
-  const int itnlim = maxit > 0 ? maxit : 10 * (m + n + 50);
-  for (int k = 0; k < NB; ++k) {
-    hs[k].active = k < nrhs ? 1 : 0;
-    hs[k].itnlim = itnlim;
+  double *sxn = nullptr, *syn = nullptr, *sxt = nullptr, *syt = nullptr, *t1 = nullptr, *t2 = nullptr, *ia = nullptr;
+  double abytes = 0, vbytes = 0, aflops = 0;
+
+  int tok(const char *name, double bytes) {
+    return st == ctx->stream ? andr_prof_begin(ctx, name, bytes, aflops) : -1;
   }
-  double hsc[8 * NB];
-  for (int k = 0; k < NB; ++k) {
-    hsc[k] = -1.0;       // sxn: U = -A x_s + U
-    hsc[NB + k] = 1.0;   // syn
-    hsc[7 * NB + k] = 1.0;
-  }
-  ANDR_HIP(ctx, hipMemcpyAsync(S.p, hs, sizeof(hs), hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(sc.p, hsc, sizeof(hsc), hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(X0.p, X, sizeof(double) * n * NB, hipMemcpyDeviceToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(U.p, B, sizeof(double) * m * NB, hipMemcpyDeviceToDevice, st));
-  ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
-  ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
-  ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
-  const double abytes = (double)m * n * (fp32 ? 4.0 : 8.0);
-  const double vbytes = ((double)m + n) * NB * 8.0;
-  const double aflops = 2.0 * m * n * nrhs;
-  auto gemv_n = [&](const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
-    const int tok = andr_prof_begin(ctx, "lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0, aflops);
+  void gemv_n(const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
+    const int t = tok("lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0);
     if (fp32)
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
     else
       hipLaunchKernelGGL(k_gemv_n<double>, dim3(blocksN), dim3(256), 0, st, (const double *)A, lda, m, n, Xin, Y, sx,
                          sy, part);
-    andr_prof_end(ctx, tok);
-  };
-  auto gemv_t = [&](const double *Y) {
-    const int tok = andr_prof_begin(ctx, "lsqr_gemv_t", abytes + vbytes + (double)ns * n * NB * 8.0, aflops);
+    andr_prof_end(ctx, t);
+  }
+  void gemv_t(const double *Y) {
+    const int t = tok("lsqr_gemv_t", abytes + vbytes + (double)ns * n * NB * 8.0);
     dim3 g((n + 255) / 256, ns);
     if (fp32)
       hipLaunchKernelGGL(k_gemv_t<float>, g, dim3(256), 0, st, (const float *)A, lda, m, n, Y, rps, P.p);
     else
       hipLaunchKernelGGL(k_gemv_t<double>, g, dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps, P.p);
-    andr_prof_end(ctx, tok);
-  };
-  // x_s = x0 / q ; U = b - A x_s ; ||U||, ||b||
-  hipLaunchKernelGGL(k_scale_div, dim3(blocksC), dim3(256), 0, st, n, X0.p, q, Xs.p);
-  gemv_n(Xs.p, U.p, sxn, syn, partU.p);
-  hipLaunchKernelGGL(k_sumsq_parts, dim3(blocksM), dim3(256), 0, st, m, B, partB.p);
-  hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, partB.p, blocksM, eps, sxt, syt);
-  // V = A^T u
-  gemv_t(U.p);
-  hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, (const double *)nullptr, sxt, syt,
-                     partV.p, partW.p);
-  hipLaunchKernelGGL(k_lsqr_init2, dim3(1), dim3(64), 0, st, S.p, partV.p, blocksC, sxn, syn, ia);
-  // w = v / alpha  (X update with t1 = 0, t2 = 0, W = V * inv_alpha)
-  {
-    double z[2 * NB];
+    andr_prof_end(ctx, t);
+  }
+
+  // workspace, initial state, x_s = x0 / q, U = b - A x_s, V = A^T U, W = V / alpha
+  int begin(andr_ctx *c, hipStream_t stream, const void *A_, bool fp32_, int64_t lda_, int m_, int n_,
+            const double *q_, const double *B, double *X_, int nrhs_, int maxit, double eps) {
+    ctx = c;
+    st = stream;
+    A = A_;
+    fp32 = fp32_;
+    lda = lda_;
+    m = m_;
+    n = n_;
+    q = q_;
+    X = X_;
+    nrhs = nrhs_;
+    if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
+    if (m < 1 || n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
+    wavesN = (m + 7) / 8;
+    blocksN = (wavesN + 3) / 4;
+    ns = splits_for(m, n);
+    rps = (m + ns - 1) / ns;
+    blocksC = (n * NB + 255) / 256;
+    blocksM = (m * NB + 255) / 256;
+    if (U.alloc((size_t)m * NB, ctx) || V.alloc((size_t)n * NB, ctx) || W.alloc((size_t)n * NB, ctx) ||
+        Xl.alloc((size_t)n * NB, ctx) || X0.alloc((size_t)n * NB, ctx) || Xs.alloc((size_t)n * NB, ctx) ||
+        P.alloc((size_t)ns * n * NB, ctx) || partU.alloc((size_t)wavesN * NB, ctx) ||
+        partV.alloc((size_t)blocksC * NB, ctx) || partW.alloc((size_t)blocksC * NB, ctx) ||
+        partB.alloc((size_t)blocksM * NB, ctx) || sc.alloc(8 * NB, ctx) || S.alloc(NB, ctx) || act.alloc(NB, ctx))
+      return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR workspace (m=%d n=%d)", m, n);
+    sxn = sc.p;
+    syn = sc.p + NB;
+    sxt = sc.p + 2 * NB;
+    syt = sc.p + 3 * NB;
+    t1 = sc.p + 4 * NB;
+    t2 = sc.p + 5 * NB;
+    ia = sc.p + 6 * NB;
+    std::memset(hs, 0, sizeof(hs));
+    itnlim = maxit > 0 ? maxit : 10 * (m + n + 50);
     for (int k = 0; k < NB; ++k) {
-      z[k] = 0.0;
-      z[NB + k] = 0.0;
+      hs[k].active = k < nrhs ? 1 : 0;
+      hs[k].itnlim = itnlim;
     }
-    ANDR_HIP(ctx, hipMemcpyAsync(t1, z, sizeof(double) * 2 * NB, hipMemcpyHostToDevice, st));
+    double hsc[8 * NB];
+    for (int k = 0; k < NB; ++k) {
+      hsc[k] = -1.0;       // sxn: U = -A x_s + U
+      hsc[NB + k] = 1.0;   // syn
+      hsc[2 * NB + k] = hsc[3 * NB + k] = hsc[4 * NB + k] = hsc[5 * NB + k] = hsc[6 * NB + k] = 0.0;
+      hsc[7 * NB + k] = 1.0;
+    }
+    ANDR_HIP(ctx, hipMemcpyAsync(S.p, hs, sizeof(hs), hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(sc.p, hsc, sizeof(hsc), hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(X0.p, X, sizeof(double) * n * NB, hipMemcpyDeviceToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(U.p, B, sizeof(double) * m * NB, hipMemcpyDeviceToDevice, st));
+    ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
+    ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
+    ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
+    abytes = (double)m * n * (fp32 ? 4.0 : 8.0);
+    vbytes = ((double)m + n) * NB * 8.0;
+    aflops = 2.0 * m * n * nrhs;
+    // x_s = x0 / q ; U = b - A x_s ; ||U||, ||b||
+    hipLaunchKernelGGL(k_scale_div, dim3(blocksC), dim3(256), 0, st, n, X0.p, q, Xs.p);
+    gemv_n(Xs.p, U.p, sxn, syn, partU.p);
+    hipLaunchKernelGGL(k_sumsq_parts, dim3(blocksM), dim3(256), 0, st, m, B, partB.p);
+    hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, partB.p, blocksM, eps, sxt, syt);
+    // V = A^T u
+    gemv_t(U.p);
+    hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, (const double *)nullptr, sxt, syt,
+                       partV.p, partW.p);
+    hipLaunchKernelGGL(k_lsqr_init2, dim3(1), dim3(64), 0, st, S.p, partV.p, blocksC, sxn, syn, ia);
+    // w = v / alpha  (X update with t1 = 0, t2 = 0, W = V * inv_alpha)
     int allon[NB];
     for (int k = 0; k < NB; ++k) allon[k] = 1;
     ANDR_HIP(ctx, hipMemcpyAsync(act.p, allon, sizeof(allon), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
+    ANDR_HIP(ctx, hipGetLastError());
+    return ANDR_OK;
   }
-  ANDR_HIP(ctx, hipGetLastError());
-  // main loop
-  const int poll = 8;
-  int hact[NB];
-  for (int it = 0;; ++it) {
-    gemv_n(V.p, U.p, sxn, syn, partU.p);
-    hipLaunchKernelGGL(k_lsqr_mid, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, sxt, syt);
-    gemv_t(U.p);
-    hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, W.p, sxt, syt, partV.p, partW.p);
-    hipLaunchKernelGGL(k_lsqr_end, dim3(1), dim3(64), 0, st, S.p, partV.p, partW.p, blocksC, sxn, syn, t1, t2, ia,
-                       act.p);
-    hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
-    if ((it + 1) % poll == 0 || it + 1 >= itnlim) {
-      ANDR_HIP(ctx, hipGetLastError());
-      ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
-      ANDR_HIP(ctx, hipStreamSynchronize(st));
-      bool any = false;
-      for (int k = 0; k < nrhs; ++k) any = any || hs[k].active;
-      if (!any || it + 1 >= itnlim) break;
+
+  // enqueues iterations up to the next multiple of `poll` (or the iteration limit); no host wait
+  void advance(int poll) {
+    do {
+      gemv_n(V.p, U.p, sxn, syn, partU.p);
+      hipLaunchKernelGGL(k_lsqr_mid, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, sxt, syt);
+      gemv_t(U.p);
+      hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, W.p, sxt, syt, partV.p,
+                         partW.p);
+      hipLaunchKernelGGL(k_lsqr_end, dim3(1), dim3(64), 0, st, S.p, partV.p, partW.p, blocksC, sxn, syn, t1, t2, ia,
+                         act.p);
+      hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
+      ++it;
+    } while (it % poll != 0 && it < itnlim);
+  }
+
+  // reads the per-column states; done when no column is active or the limit is reached
+  int poll_state() {
+    ANDR_HIP(ctx, hipGetLastError());
+    ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    bool any = false;
+    for (int k = 0; k < nrhs; ++k) any = any || hs[k].active;
+    done = !any || it >= itnlim;
+    return ANDR_OK;
+  }
+
+  // x = q .* x + x0, iteration counts and stop codes
+  int finish(int *nit, int *istop) {
+    hipLaunchKernelGGL(k_scale_mul_add, dim3(blocksC), dim3(256), 0, st, n, q, Xl.p, X0.p, X);
+    ANDR_HIP(ctx, hipGetLastError());
+    ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    for (int k = 0; k < nrhs; ++k) {
+      if (nit) nit[k] = hs[k].itn;
+      if (istop) istop[k] = hs[k].istop;
     }
+    return ANDR_OK;
   }
-  (void)hact;
-  // x = q .* x + x0
-  hipLaunchKernelGGL(k_scale_mul_add, dim3(blocksC), dim3(256), 0, st, n, q, Xl.p, X0.p, X);
-  ANDR_HIP(ctx, hipGetLastError());
-  ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
-  ANDR_HIP(ctx, hipStreamSynchronize(st));
-  for (int k = 0; k < nrhs; ++k) {
-    if (nit) nit[k] = hs[k].itn;
-    if (istop) istop[k] = hs[k].istop;
+};
+
+constexpr int LSQR_POLL = 8;  // iterations between reads of the convergence state
+
+}  // namespace
+
+int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m, int n, const double *q,
+                     const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop) {
+  LsqrRun R;
+  int rc = R.begin(ctx, ctx->stream, A, fp32, lda, m, n, q, B, X, nrhs, maxit, eps);
+  if (rc) return rc;
+  while (!R.done) {
+    R.advance(LSQR_POLL);
+    if ((rc = R.poll_state())) return rc;
   }
-  (void)one;
-  return ANDR_OK;
+  return R.finish(nit, istop);
+}
+
+int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs) {
+  if (count <= 0) return ANDR_OK;
+  if (count == 1) {
+    const LsqrJob &J = jobs[0];
+    return andr_lsqr_device(ctx, J.A, J.fp32, J.lda, J.m, J.n, J.q, J.B, J.X, J.nrhs, J.maxit, J.eps, J.nit, J.istop);
+  }
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the systems are ready
+  const int nst = std::min(count, LSQR_STREAMS);
+  std::vector<hipStream_t> streams(nst, nullptr);
+  int rc = ANDR_OK;
+  for (int i = 0; i < nst && !rc; ++i)
+    if (hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking) != hipSuccess)
+      rc = andr_fail(ctx, ANDR_ERR_HIP, "stream creation for the LSQR batch");
+  {
+    std::vector<LsqrRun> runs(count);
+    for (int i = 0; i < count && !rc; ++i) {
+      const LsqrJob &J = jobs[i];
+      rc = runs[i].begin(ctx, streams[i % nst], J.A, J.fp32, J.lda, J.m, J.n, J.q, J.B, J.X, J.nrhs, J.maxit, J.eps);
+    }
+    // all solves advance together, a block of iterations at a time, each on its stream
+    for (bool left = !rc; left && !rc;) {
+      left = false;
+      for (auto &R : runs)
+        if (!R.done) R.advance(LSQR_POLL);
+      for (auto &R : runs)
+        if (!R.done && !rc) {
+          rc = R.poll_state();
+          left = left || !R.done;
+        }
+    }
+    for (int i = 0; i < count && !rc; ++i) rc = runs[i].finish(jobs[i].nit, jobs[i].istop);
+    for (auto s : streams)
+      if (s) (void)hipStreamSynchronize(s);  // the workspaces go back to the cache after this
+  }
+  for (auto s : streams)
+    if (s) (void)hipStreamDestroy(s);
+  return rc;
 }

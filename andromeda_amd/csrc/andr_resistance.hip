@@ -260,17 +260,19 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     // ---- solve ----------------------------------------------------------------------------------
     std::vector<int> iters((size_t)nb * 6, 0), stops((size_t)nb * 6, 0);
     if (plain) {
+      // column scaling of every body, then the LSQR solves of the wave together on several streams
+      std::vector<DevBuf<double>> qs(nb);
+      std::vector<LsqrJob> lj(nb);
       for (int i = 0; i < nb; ++i) {
         const int n = 3 * P[i]->ne;
-        DevBuf<double> q;
-        if (q.alloc(n, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scale");
-        rc = andr_colscale(ctx, Abuf[i].p, false, jobs[i].lda, n, n, q.p);
+        if (qs[i].alloc(n, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scale");
+        rc = andr_colscale(ctx, Abuf[i].p, false, jobs[i].lda, n, n, qs[i].p);
         if (rc) return rc;
         ANDR_HIP(ctx, hipMemsetAsync(Xbuf[i].p, 0, sizeof(double) * n * ANDR_NRHS, ctx->stream));
-        rc = andr_lsqr_device(ctx, Abuf[i].p, false, jobs[i].lda, n, n, q.p, Bbuf[i].p, Xbuf[i].p, 6, opts->maxit, eps,
-                              &iters[(size_t)i * 6], &stops[(size_t)i * 6]);
-        if (rc) return rc;
+        lj[i] = {Abuf[i].p, false, jobs[i].lda, n, n, qs[i].p, Bbuf[i].p, Xbuf[i].p, 6, opts->maxit, eps,
+                 &iters[(size_t)i * 6], &stops[(size_t)i * 6]};
       }
+      if ((rc = andr_lsqr_device_batch(ctx, nb, lj.data()))) return rc;
     } else {
       std::vector<GmresSys> gs(nb);
       for (int i = 0; i < nb; ++i) {

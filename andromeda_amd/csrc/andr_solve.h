@@ -30,6 +30,22 @@ int andr_colscale(andr_ctx *ctx, void *A, bool fp32, int64_t lda, int m, int n, 
 // (0 -> 10(m+n+50)); nit/istop (host, nrhs entries) receive the results.
 int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m, int n, const double *q,
                      const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop);
+// Several independent LSQR solves (as andr_lsqr_device each), advanced together on up to
+// LSQR_STREAMS streams so that the small ones overlap on the device. Waits for the context's stream
+// first; returns when every solve has finished.
+struct LsqrJob {
+  const void *A;
+  bool fp32;
+  int64_t lda;
+  int m, n;
+  const double *q, *B;
+  double *X;
+  int nrhs, maxit;
+  double eps;
+  int *nit, *istop;
+};
+#define LSQR_STREAMS 4
+int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs);
 
 // Restarted GMRES on a batch of square systems (FP32 or FP64 storage, FP64 vectors); restart
 // cycles recompute the FP64 residual b - A x. Stops per system when every active column reaches
