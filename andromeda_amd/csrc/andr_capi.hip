@@ -191,10 +191,10 @@ extern "C" int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_
 }
 
 // ---- seam 2 ----------------------------------------------------------------------------------
-__global__ void k_csr_to_dense(int m, int64_t n, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
+__global__ void k_csr_to_dense(int m, int64_t ld, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
                                const double *__restrict__ a, double *__restrict__ D) {
   const int i = blockIdx.x;
-  for (int64_t p = ia[i] - 1 + threadIdx.x; p < ia[i + 1] - 1; p += blockDim.x) D[(size_t)i * n + (ja[p] - 1)] = a[p];
+  for (int64_t p = ia[i] - 1 + threadIdx.x; p < ia[i + 1] - 1; p += blockDim.x) D[(size_t)i * ld + (ja[p] - 1)] = a[p];
 }
 
 __global__ void k_pack_col(int n, const double *__restrict__ v, double *__restrict__ V) {
@@ -213,7 +213,7 @@ struct LsqrCache {
   const double *key_a = nullptr;
   int64_t nnz = 0;
   int m = 0, n = 0;
-  DevBuf<double> D, q;
+  DevBuf<double> D, q;  // D: dense m x n, leading dimension dense_ld(n)
   std::vector<std::pair<int64_t, double>> probe;
 };
 std::mutex g_mu;
@@ -233,6 +233,9 @@ bool probe_ok(const LsqrCache &c, const double *a) {
     if (a[pr.first] != pr.second) return false;
   return true;
 }
+// leading dimension of the dense device copy: rows start on 128-byte lines
+int64_t dense_ld(int n) { return ((int64_t)n + 15) / 16 * 16; }
+
 int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
                DevBuf<double> &D) {
   // the device scatter trusts these indices: check the 1-based CSR on the host first
@@ -245,13 +248,14 @@ int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, cons
   DevBuf<int64_t> dia;
   DevBuf<int32_t> dja;
   DevBuf<double> da;
-  if (D.alloc((size_t)m * n) || dia.alloc(m + 1) || dja.alloc(nnz) || da.alloc(nnz))
+  const int64_t ld = dense_ld(n);
+  if (D.alloc((size_t)m * ld) || dia.alloc(m + 1) || dja.alloc(nnz) || da.alloc(nnz))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR matrix %d x %d", m, n);
-  ANDR_HIP(ctx, hipMemsetAsync(D.p, 0, sizeof(double) * (size_t)m * n, ctx->stream));
+  ANDR_HIP(ctx, hipMemsetAsync(D.p, 0, sizeof(double) * (size_t)m * ld, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(dia.p, ia, sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(dja.p, ja, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
   ANDR_HIP(ctx, hipMemcpyAsync(da.p, a, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, ctx->stream, m, (int64_t)n, dia.p, dja.p, da.p, D.p);
+  hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, ctx->stream, m, ld, dia.p, dja.p, da.p, D.p);
   ANDR_HIP(ctx, hipGetLastError());
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ANDR_OK;
@@ -285,7 +289,7 @@ extern "C" void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *
   (void)hipSetDevice(ctx->device);
   LsqrCache &c = g_cache;
   if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D) || c.q.alloc(*n) ||
-      andr_colscale(ctx, c.D.p, false, *n, *m, *n, c.q.p)) {
+      andr_colscale(ctx, c.D.p, false, dense_ld(*n), *m, *n, c.q.p)) {
     fprintf(stderr, "andromeda_hip: lsqr_precon_: %s\n", ctx->err.c_str());
     c.key_a = nullptr;
     return;
@@ -337,7 +341,7 @@ extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *n
   (void)hipMemcpyAsync(tmp.p, x, sizeof(double) * N, hipMemcpyHostToDevice, st);
   hipLaunchKernelGGL(k_pack_col, dim3((N * ANDR_NB + 255) / 256), dim3(256), 0, st, N, tmp.p, dX.p);
   int it = 0, is = 0;
-  int rc = andr_lsqr_device(ctx, c.D.p, false, N, M, N, dq.p, dB.p, dX.p, 1, *maxit, *eps, &it, &is);
+  int rc = andr_lsqr_device(ctx, c.D.p, false, dense_ld(N), M, N, dq.p, dB.p, dX.p, 1, *maxit, *eps, &it, &is);
   if (rc) {
     fprintf(stderr, "andromeda_hip: lsqr_solve_: %s\n", ctx->err.c_str());
     *ierr = rc;
@@ -357,11 +361,13 @@ extern "C" int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double
   const double t0 = andr_now();
   andr_lsqr_opts o = opts ? *opts : andr_lsqr_opts{0, 1e-15, 1};
   DevBuf<double> dA, dq, dB, dX;
-  if (dA.alloc((size_t)m * n) || dq.alloc(n) || dB.alloc((size_t)m * ANDR_NB) || dX.alloc((size_t)n * ANDR_NB))
+  const int64_t lda = (n + 15) / 16 * 16;  // rows start on 128-byte lines
+  if (dA.alloc((size_t)m * lda) || dq.alloc(n) || dB.alloc((size_t)m * ANDR_NB) || dX.alloc((size_t)n * ANDR_NB))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "andr_lsqr_dense workspace");
-  ANDR_HIP(ctx, hipMemcpyAsync(dA.p, A, sizeof(double) * m * n, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpy2DAsync(dA.p, sizeof(double) * lda, A, sizeof(double) * n, sizeof(double) * n, m,
+                                 hipMemcpyHostToDevice, ctx->stream));
   if (o.precon) {
-    int rc = andr_colscale(ctx, dA.p, false, n, (int)m, (int)n, dq.p);
+    int rc = andr_colscale(ctx, dA.p, false, lda, (int)m, (int)n, dq.p);
     if (rc) return rc;
   }
   std::vector<double> hb((size_t)m * ANDR_NB), hx((size_t)n * ANDR_NB);
@@ -376,7 +382,7 @@ extern "C" int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double
     ANDR_HIP(ctx, hipMemcpyAsync(dB.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, ctx->stream));
     ANDR_HIP(ctx, hipMemcpyAsync(dX.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, ctx->stream));
     int it[ANDR_NB], is[ANDR_NB];
-    int rc = andr_lsqr_device(ctx, dA.p, false, n, (int)m, (int)n, o.precon ? dq.p : nullptr, dB.p, dX.p, nr, o.maxit,
+    int rc = andr_lsqr_device(ctx, dA.p, false, lda, (int)m, (int)n, o.precon ? dq.p : nullptr, dB.p, dX.p, nr, o.maxit,
                               o.eps, it, is);
     if (rc) return rc;
     ANDR_HIP(ctx, hipMemcpyAsync(hx.data(), dX.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, ctx->stream));

@@ -87,6 +87,154 @@ __global__ __launch_bounds__(256) void k_gemv_n(const ST *__restrict__ A, int64_
   if (lane < 8) part[(size_t)wave * NB + lane] = y2;
 }
 
+// The same product for FP64 A on the matrix cores (v_mfma_f64_16x16x4f64). A wave owns GN_ROWS = 64
+// rows (four 16-row tiles) over the column segment blockIdx.y; lane (g = lane / 16, c = lane % 16)
+// feeds row 16 T + c with columns 16 kb + 4 g + t (t = 0..3, one instruction each) and column c of
+// B = X (right-hand side c; the upper half c >= NB is zero). The result D[g + 4 v][c] sits in
+// register v of lane (g, c). With one segment the update Y = sx .* D + sy .* Y and
+// out[wave][k] = sum of the new Y^2 over the wave's rows are done here; with several the raw
+// segment products go to out = P[segment][row][k] and k_combine_n finishes.
+constexpr int GN_T = 4, GN_ROWS = 16 * GN_T;
+constexpr int GN_TARGET_WAVES = 8192;  // measured: 2048 -> 3.9, 8192 -> 4.6 TB/s on a 24000^2 matrix
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <bool SEG, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A, int64_t lda, int m, int n,
+                                                   const double *__restrict__ X, int seglen, double *__restrict__ Y,
+                                                   const double *__restrict__ sx, const double *__restrict__ sy,
+                                                   double *__restrict__ out) {
+  typedef const double __attribute__((address_space(1))) *gdp;
+  typedef double d2n __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int row0 = wave * GN_ROWS;
+  if (row0 >= m) return;
+  const int g = lane >> 4, c = lane & 15;
+  const bool hi = c >= NB;
+  const int k0 = blockIdx.y * seglen, kend = min(n, k0 + seglen);
+  gdp arow[GN_T];
+#pragma unroll
+  for (int T = 0; T < GN_T; ++T) arow[T] = (gdp)(A + (size_t)min(row0 + 16 * T + c, m - 1) * lda + 4 * g);
+  const gdp xb = (gdp)(X + (size_t)(4 * g) * NB + (c & (NB - 1)));
+  d4v acc[GN_T];
+#pragma unroll
+  for (int T = 0; T < GN_T; ++T) acc[T] = d4v{0.0, 0.0, 0.0, 0.0};
+  auto load = [&](int kb, double (&a)[GN_T][4], double (&x)[4]) {
+#pragma unroll
+    for (int T = 0; T < GN_T; ++T) {
+      if (VEC) {  // the matrix is streamed once: non-temporal, so that X keeps its place in L2
+        const d2n u = __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(arow[T] + 16 * kb));
+        const d2n w =
+            __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(arow[T] + 16 * kb + 2));
+        a[T][0] = u.x;
+        a[T][1] = u.y;
+        a[T][2] = w.x;
+        a[T][3] = w.y;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a[T][t] = __builtin_nontemporal_load(arow[T] + 16 * kb + t);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = xb[(size_t)(16 * kb + t) * NB];
+  };
+  auto step = [&](const double (&a)[GN_T][4], const double (&x)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double b = hi ? 0.0 : x[t];
+#pragma unroll
+      for (int T = 0; T < GN_T; ++T) acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[T][t], b, acc[T], 0, 0, 0);
+    }
+  };
+  // full 16-column blocks, two register sets in turn: the operands of the next block are requested
+  // before the products of the current one (the last prefetch re-reads the final block)
+  int kb = k0 >> 4;
+  const int kb_end = kend >> 4;
+  if (kb < kb_end) {
+    double a0[GN_T][4], a1[GN_T][4], x0[4], x1[4];
+    load(kb, a0, x0);
+    for (; kb + 1 < kb_end; kb += 2) {
+      load(kb + 1, a1, x1);
+      __builtin_amdgcn_sched_barrier(0);
+      step(a0, x0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(min(kb + 2, kb_end - 1), a0, x0);
+      __builtin_amdgcn_sched_barrier(0);
+      step(a1, x1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (kb < kb_end) {  // odd count: a0 holds block kb
+      step(a0, x0);
+      ++kb;
+    }
+  }
+  if (16 * kb < kend) {  // last, partial column block (last segment only): masked
+    double a[GN_T][4], x[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int col = 16 * kb + 4 * g + t;
+      const bool in = col < kend;
+      x[t] = in ? X[(size_t)col * NB + (c & (NB - 1))] : 0.0;
+#pragma unroll
+      for (int T = 0; T < GN_T; ++T) a[T][t] = in ? A[(size_t)min(row0 + 16 * T + c, m - 1) * lda + col] : 0.0;
+    }
+    step(a, x);
+  }
+  if (SEG) {
+    double *P = out + (size_t)blockIdx.y * m * NB;
+#pragma unroll
+    for (int T = 0; T < GN_T; ++T)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 16 * T + g + 4 * v;
+        if (!hi && r < m) P[(size_t)r * NB + c] = acc[T][v];
+      }
+    return;
+  }
+  double ss = 0.0;
+#pragma unroll
+  for (int T = 0; T < GN_T; ++T)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = row0 + 16 * T + g + 4 * v;
+      if (!hi && r < m) {
+        double *py = Y + (size_t)r * NB + c;
+        const double y = sx[c] * acc[T][v] + sy[c] * (*py);
+        *py = y;
+        ss += y * y;
+      }
+    }
+  ss += __shfl_xor(ss, 16, 64);  // lanes of equal c
+  ss += __shfl_xor(ss, 32, 64);
+  if (lane < NB) out[(size_t)wave * NB + lane] = ss;
+}
+
+// Y = sx .* sum_s P[s] + sy .* Y for the segmented product; part[block][k] = sum of the new Y^2.
+__global__ __launch_bounds__(256) void k_combine_n(int m, int nseg, const double *__restrict__ P,
+                                                   double *__restrict__ Y, const double *__restrict__ sx,
+                                                   const double *__restrict__ sy, double *__restrict__ part) {
+  __shared__ double red[4 * NB];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // one (row, k) pair per thread
+  const int k = t & 7;
+  double y2 = 0.0;
+  if (t < m * NB) {
+    double s = 0.0;
+    for (int sg = 0; sg < nseg; ++sg) s += P[(size_t)sg * m * NB + t];
+    const double y = sx[k] * s + sy[k] * Y[t];
+    Y[t] = y;
+    y2 = y * y;
+  }
+  y2 += __shfl_xor(y2, 8, 64);
+  y2 += __shfl_xor(y2, 16, 64);
+  y2 += __shfl_xor(y2, 32, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < 8) red[wid * NB + lane] = y2;
+  __syncthreads();
+  if (threadIdx.x < 8)
+    part[(size_t)blockIdx.x * NB + threadIdx.x] =
+        (red[threadIdx.x] + red[NB + threadIdx.x]) + (red[2 * NB + threadIdx.x] + red[3 * NB + threadIdx.x]);
+}
+
 // P[s][j][k] = sum_{i in split s} A[i][j] Y[i][k]; one column per thread, splits over blockIdx.y.
 template <class ST>
 __global__ __launch_bounds__(256) void k_gemv_t(const ST *__restrict__ A, int64_t lda, int m, int n,
@@ -163,21 +311,34 @@ __global__ __launch_bounds__(256) void k_combine_t(int n, int nsplit, const doub
   }
 }
 
-__device__ inline double sum_parts(const double *part, int np, int k) {
+// Column sums of the partials part[i][k], i < np, by a block of 256 threads (32 partial rows per
+// column in flight); thread k < NB receives the sum of column k. Every thread of the block calls it.
+__device__ double block_sum_parts(const double *__restrict__ part, int np, double *red) {
+  const int t = threadIdx.x, k = t & 7;
   double s = 0.0;
-  for (int i = 0; i < np; ++i) s += part[(size_t)i * NB + k];
-  return s;
+  for (int i = t >> 3; i < np; i += 32) s += part[(size_t)i * NB + k];
+  s += __shfl_xor(s, 8, 64);
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  __syncthreads();  // red may still be read from the previous call
+  if ((t & 63) < 8) red[(t >> 6) * NB + (t & 63)] = s;
+  __syncthreads();
+  return t < NB ? (red[t] + red[NB + t]) + (red[2 * NB + t] + red[3 * NB + t]) : 0.0;
 }
+#define LSQR_SCALAR_THREADS 256
 
 // ---- scalar kernels (one thread per column) -------------------------------------------------
 // start: U = b - A x_s is formed; beta = ||U||, btol = eps ||b|| / ||r0||; set scales for A^T u.
 __global__ void k_lsqr_start(LsqrState *st, const double *partU, int npU, const double *partB, int npB, double eps,
                              double *sxt, double *syt) {
+  __shared__ double red[4 * NB];
+  const double sr = block_sum_parts(partU, npU, red);
+  const double sb = block_sum_parts(partB, npB, red);
   const int k = threadIdx.x;
   if (k >= NB) return;
   LsqrState &S = st[k];
-  const double nrm_r = sqrt(sum_parts(partU, npU, k));
-  const double nrm_b = sqrt(sum_parts(partB, npB, k));
+  const double nrm_r = sqrt(sr);
+  const double nrm_b = sqrt(sb);
   S.atol = eps;
   S.btol = eps * nrm_b / nrm_r;
   S.ctol = 1.0 / 1.0e8;
@@ -199,10 +360,12 @@ __global__ void k_lsqr_start(LsqrState *st, const double *partU, int npU, const 
 
 // after the first A^T u: alpha, w = v, the initial recurrence values (src/lib_lsqr.f:952-1008)
 __global__ void k_lsqr_init2(LsqrState *st, const double *partV, int npV, double *sxn, double *syn, double *inv_alpha) {
+  __shared__ double red[4 * NB];
+  const double sv = block_sum_parts(partV, npV, red);
   const int k = threadIdx.x;
   if (k >= NB) return;
   LsqrState &S = st[k];
-  S.alpha = S.beta > 0.0 ? sqrt(sum_parts(partV, npV, k)) : 0.0;
+  S.alpha = S.beta > 0.0 ? sqrt(sv) : 0.0;
   inv_alpha[k] = S.alpha > 0.0 ? 1.0 / S.alpha : 0.0;
   const double arnorm = S.alpha * S.beta;
   S.rhobar = S.alpha;
@@ -221,6 +384,8 @@ __global__ void k_lsqr_init2(LsqrState *st, const double *partV, int npV, double
 
 // after U = A v - alpha u: beta, anorm; scales of the A^T product
 __global__ void k_lsqr_mid(LsqrState *st, const double *partU, int npU, double *sxt, double *syt) {
+  __shared__ double red[4 * NB];
+  const double su = block_sum_parts(partU, npU, red);
   const int k = threadIdx.x;
   if (k >= NB) return;
   LsqrState &S = st[k];
@@ -230,7 +395,7 @@ __global__ void k_lsqr_mid(LsqrState *st, const double *partU, int npU, double *
     return;
   }
   S.itn += 1;
-  const double beta = sqrt(sum_parts(partU, npU, k));
+  const double beta = sqrt(su);
   double temp = d2norm(S.alpha, beta);
   temp = d2norm(temp, 0.0);
   S.anorm = d2norm(S.anorm, temp);
@@ -249,6 +414,9 @@ __global__ void k_lsqr_mid(LsqrState *st, const double *partU, int npU, double *
 // (src/lib_lsqr.f:1015-1152)
 __global__ void k_lsqr_end(LsqrState *st, const double *partV, const double *partW, int np, double *sxn, double *syn,
                            double *t1o, double *t2o, double *inv_alpha, int *active_out) {
+  __shared__ double red[4 * NB];
+  const double sv = block_sum_parts(partV, np, red);
+  const double sw = block_sum_parts(partW, np, red);
   const int k = threadIdx.x;
   if (k >= NB) return;
   LsqrState &S = st[k];
@@ -261,9 +429,9 @@ __global__ void k_lsqr_end(LsqrState *st, const double *partV, const double *par
     active_out[k] = 0;
     return;
   }
-  if (S.beta > 0.0) S.alpha = sqrt(sum_parts(partV, np, k));
+  if (S.beta > 0.0) S.alpha = sqrt(sv);
   const double alpha = S.alpha, beta = S.beta;
-  const double wnorm = sqrt(sum_parts(partW, np, k));
+  const double wnorm = sqrt(sw);
   const double rhbar1 = S.rhobar;
   const double rho = d2norm(rhbar1, beta);
   const double cs = rhbar1 / rho, sn = beta / rho;
@@ -439,7 +607,9 @@ struct LsqrRun {
   double *X = nullptr;
   bool done = false;
   int wavesN = 0, blocksN = 0, ns = 0, rps = 0, blocksC = 0, blocksM = 0;
-  DevBuf<double> U, V, W, Xl, X0, Xs, P, partU, partV, partW, partB, sc;
+  int nsegN = 1, seglenN = 0, partsN = 0;  // A X: column segments and the partial sums it leaves
+  bool vecN = false;
+  DevBuf<double> U, V, W, Xl, X0, Xs, P, PN, partU, partV, partW, partB, sc;
   DevBuf<LsqrState> S;
   DevBuf<int> act;
   LsqrState hs[NB];
@@ -451,12 +621,29 @@ struct LsqrRun {
   }
   void gemv_n(const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
     const int t = tok("lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0);
-    if (fp32)
+    if (fp32) {
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
-    else
-      hipLaunchKernelGGL(k_gemv_n<double>, dim3(blocksN), dim3(256), 0, st, (const double *)A, lda, m, n, Xin, Y, sx,
-                         sy, part);
+    } else {
+      const double *Ad = (const double *)A;
+      const dim3 grid((wavesN + 3) / 4, nsegN);
+      if (nsegN == 1) {
+        if (vecN)
+          hipLaunchKernelGGL((k_gemv_n_mf<false, true>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
+                             sy, part);
+        else
+          hipLaunchKernelGGL((k_gemv_n_mf<false, false>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
+                             sy, part);
+      } else {
+        if (vecN)
+          hipLaunchKernelGGL((k_gemv_n_mf<true, true>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
+                             sy, PN.p);
+        else
+          hipLaunchKernelGGL((k_gemv_n_mf<true, false>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
+                             sy, PN.p);
+        hipLaunchKernelGGL(k_combine_n, dim3(blocksM), dim3(256), 0, st, m, nsegN, PN.p, Y, sx, sy, part);
+      }
+    }
     andr_prof_end(ctx, t);
   }
   void gemv_t(const double *Y) {
@@ -484,15 +671,26 @@ struct LsqrRun {
     nrhs = nrhs_;
     if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
     if (m < 1 || n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
-    wavesN = (m + 7) / 8;
-    blocksN = (wavesN + 3) / 4;
+    if (fp32) {  // VALU product: one wave per 8 rows
+      wavesN = (m + 7) / 8;
+      blocksN = (wavesN + 3) / 4;
+      partsN = wavesN;
+    } else {  // matrix-core product: 64 rows per wave, columns split until ~8192 waves are launched
+      wavesN = (m + GN_ROWS - 1) / GN_ROWS;
+      nsegN = std::max(1, std::min((GN_TARGET_WAVES + wavesN - 1) / wavesN, (n + 255) / 256));
+      seglenN = ((n + nsegN - 1) / nsegN + 15) / 16 * 16;
+      nsegN = (n + seglenN - 1) / seglenN;
+      partsN = nsegN == 1 ? wavesN : (m * NB + 255) / 256;
+      vecN = lda % 2 == 0 && ((uintptr_t)A % 16) == 0;
+    }
     ns = splits_for(m, n);
     rps = (m + ns - 1) / ns;
     blocksC = (n * NB + 255) / 256;
     blocksM = (m * NB + 255) / 256;
     if (U.alloc((size_t)m * NB, ctx) || V.alloc((size_t)n * NB, ctx) || W.alloc((size_t)n * NB, ctx) ||
         Xl.alloc((size_t)n * NB, ctx) || X0.alloc((size_t)n * NB, ctx) || Xs.alloc((size_t)n * NB, ctx) ||
-        P.alloc((size_t)ns * n * NB, ctx) || partU.alloc((size_t)wavesN * NB, ctx) ||
+        P.alloc((size_t)ns * n * NB, ctx) || PN.alloc(nsegN > 1 ? (size_t)nsegN * m * NB : 0, ctx) ||
+        partU.alloc((size_t)partsN * NB, ctx) ||
         partV.alloc((size_t)blocksC * NB, ctx) || partW.alloc((size_t)blocksC * NB, ctx) ||
         partB.alloc((size_t)blocksM * NB, ctx) || sc.alloc(8 * NB, ctx) || S.alloc(NB, ctx) || act.alloc(NB, ctx))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR workspace (m=%d n=%d)", m, n);
@@ -530,12 +728,12 @@ struct LsqrRun {
     hipLaunchKernelGGL(k_scale_div, dim3(blocksC), dim3(256), 0, st, n, X0.p, q, Xs.p);
     gemv_n(Xs.p, U.p, sxn, syn, partU.p);
     hipLaunchKernelGGL(k_sumsq_parts, dim3(blocksM), dim3(256), 0, st, m, B, partB.p);
-    hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, partB.p, blocksM, eps, sxt, syt);
+    hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, partU.p, partsN, partB.p, blocksM, eps, sxt, syt);
     // V = A^T u
     gemv_t(U.p);
     hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, (const double *)nullptr, sxt, syt,
                        partV.p, partW.p);
-    hipLaunchKernelGGL(k_lsqr_init2, dim3(1), dim3(64), 0, st, S.p, partV.p, blocksC, sxn, syn, ia);
+    hipLaunchKernelGGL(k_lsqr_init2, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, partV.p, blocksC, sxn, syn, ia);
     // w = v / alpha  (X update with t1 = 0, t2 = 0, W = V * inv_alpha)
     int allon[NB];
     for (int k = 0; k < NB; ++k) allon[k] = 1;
@@ -549,11 +747,11 @@ struct LsqrRun {
   void advance(int poll) {
     do {
       gemv_n(V.p, U.p, sxn, syn, partU.p);
-      hipLaunchKernelGGL(k_lsqr_mid, dim3(1), dim3(64), 0, st, S.p, partU.p, wavesN, sxt, syt);
+      hipLaunchKernelGGL(k_lsqr_mid, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, partU.p, partsN, sxt, syt);
       gemv_t(U.p);
       hipLaunchKernelGGL(k_combine_t, dim3(blocksC), dim3(256), 0, st, n, ns, P.p, V.p, W.p, sxt, syt, partV.p,
                          partW.p);
-      hipLaunchKernelGGL(k_lsqr_end, dim3(1), dim3(64), 0, st, S.p, partV.p, partW.p, blocksC, sxn, syn, t1, t2, ia,
+      hipLaunchKernelGGL(k_lsqr_end, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, partV.p, partW.p, blocksC, sxn, syn, t1, t2, ia,
                          act.p);
       hipLaunchKernelGGL(k_lsqr_update, dim3(blocksC), dim3(256), 0, st, n, Xl.p, W.p, V.p, t1, t2, ia, act.p);
       ++it;
@@ -603,9 +801,14 @@ int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m
 
 int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs) {
   if (count <= 0) return ANDR_OK;
-  if (count == 1) {
-    const LsqrJob &J = jobs[0];
-    return andr_lsqr_device(ctx, J.A, J.fp32, J.lda, J.m, J.n, J.q, J.B, J.X, J.nrhs, J.maxit, J.eps, J.nit, J.istop);
+  if (count == 1 || andr_prof(ctx).on) {  // profiled: one after another, so that each event pair times one launch
+    for (int i = 0; i < count; ++i) {
+      const LsqrJob &J = jobs[i];
+      const int rc =
+          andr_lsqr_device(ctx, J.A, J.fp32, J.lda, J.m, J.n, J.q, J.B, J.X, J.nrhs, J.maxit, J.eps, J.nit, J.istop);
+      if (rc) return rc;
+    }
+    return ANDR_OK;
   }
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the systems are ready
   const int nst = std::min(count, LSQR_STREAMS);

@@ -69,6 +69,39 @@ def traffic_ratio(kernel):
     return None
 
 
+def kernel_roofline(name, v):
+    """Roofline entry of one profiled kernel: HBM-bound for the matrix products (algorithmic bytes
+    over the mean launch time), VALU-bound otherwise (algorithmic flops)."""
+    avg_ms = v["ms"] / v["launches"]
+    if "gemv" in name:
+        per_launch = v["bytes"] / v["launches"]
+        ach = per_launch / (avg_ms * 1e-3) / 1e9
+        ratio = traffic_ratio(name)
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": None if ratio is None else round(ratio * per_launch),
+                "kernel": name, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": round(per_launch)}
+    ach = v["flops"] / v["launches"] / (avg_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if "fp32" in name else FP64_PEAK_TFLOPS
+    return {"bound": "valu", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4), "traffic": None, "kernel": name, "avg_launch_ms": round(avg_ms, 4)}
+
+
+def kernel_table(prof):
+    """Per-kernel totals of a profiled region and the roofline of its dominant kernel."""
+    kernels = {}
+    for name, v in prof.items():
+        sec = v["ms"] * 1e-3
+        kernels[name] = {"launches": v["launches"], "ms_total": round(v["ms"], 3),
+                         "GBps": round(v["bytes"] / sec / 1e9, 1) if sec > 0 else None,
+                         "TFLOPs": round(v["flops"] / sec / 1e12, 2) if sec > 0 else None}
+    if not prof:
+        return kernels, None
+    name, v = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    return kernels, kernel_roofline(name, v)
+
+
 def cpu_baseline(bodies, budget_s=20.0):
     """Oracle (C, FP64, OpenMP assembly + serial LSQR) on the smallest bodies of the batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -176,33 +209,16 @@ def main():
     log("[bench] sample (body, F, fast its, err): " + ", ".join(
         f"({i}, {bodies[i].nelem}, {int(f.nit.max())}, {e:.1e})" for i, f, e in zip(idx, fast_s, errs)))
 
-    # dominant kernel of the timed region and its roofline
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None
-    roofline = None
-    kernels = {}
-    for name, v in prof.items():
-        sec = v["ms"] * 1e-3
-        kernels[name] = {"launches": v["launches"], "ms_total": round(v["ms"], 3),
-                         "GBps": round(v["bytes"] / sec / 1e9, 1) if sec > 0 else None,
-                         "TFLOPs": round(v["flops"] / sec / 1e12, 2) if sec > 0 else None}
-    if dom:
-        name, v = dom
-        avg_ms = v["ms"] / v["launches"]
-        if "gemv" in name:
-            per_launch = v["bytes"] / v["launches"]
-            ach = per_launch / (avg_ms * 1e-3) / 1e9
-            ratio = traffic_ratio(name)
-            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": None if ratio is None else round(ratio * per_launch),
-                        "kernel": name, "avg_launch_ms": round(avg_ms, 4),
-                        "algorithmic_bytes_per_launch": round(per_launch)}
-        else:
-            ach = v["flops"] / v["launches"] / (avg_ms * 1e-3) / 1e12
-            peak = FP32_PEAK_TFLOPS if "fp32" in name else FP64_PEAK_TFLOPS
-            roofline = {"bound": "valu", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                        "frac": round(ach / peak, 4), "traffic": None, "kernel": name,
-                        "avg_launch_ms": round(avg_ms, 4)}
+    kernels, roofline = kernel_table(prof)
+
+    # kernel rooflines of the plain path on a quarter of the sample (profiled on its own: with
+    # profiling on, the plain solves run one after another so that each launch is timed alone)
+    ctx.profile(True)
+    ctx.resistance_batch(sample[::4], precision=api.PREC_FP64, eps=1e-12)
+    plain_kernels, plain_roofline = kernel_table(ctx.profile_read())
+    ctx.profile(False)
+    log("[bench] plain FP64 kernels: " + ", ".join(
+        f"{k} {v['GBps']} GB/s {v['TFLOPs']} TFLOP/s" for k, v in plain_kernels.items()))
 
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     if cpu:
@@ -227,7 +243,8 @@ def main():
                    "form": prec_name},
         "plain_fp64": {"sample_bodies": len(sample), "tensors_per_s": round(plain_rate, 3),
                        "fast_tensors_per_s_same_sample": round(len(sample) / t_fast, 3),
-                       "speedup_same_sample": round(t_plain / t_fast, 2)},
+                       "speedup_same_sample": round(t_plain / t_fast, 2),
+                       "roofline": plain_roofline, "kernels": plain_kernels},
         "accuracy": {"max_tensor_err_vs_fp64": float(f"{max(errs):.3e}"), "bodies": len(sample)},
         "roofline": roofline,
         "kernels": kernels,

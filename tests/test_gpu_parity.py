@@ -66,6 +66,22 @@ def test_lsqr_dense_laplace_system(ctx, orc):
     assert np.linalg.norm(X[:, 0] - xo) / np.linalg.norm(xo) < 1e-9
 
 
+@pytest.mark.parametrize("m,n", [(301, 77), (1000, 999), (2050, 640), (64, 64)])
+def test_lsqr_dense_shapes(ctx, m, n):
+    """Least-squares solutions of six right-hand sides against numpy's lstsq. The shapes cover an odd
+    leading dimension (scalar matrix loads), one and several column segments of the matrix-core
+    product A V, a partial last column block (n % 16 != 0) and row counts that are not a multiple of
+    the 64 rows a wave owns."""
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.normal(size=(m, n))
+    A[:n, :n] += 4.0 * np.sqrt(m) * np.eye(n)
+    B = rng.normal(size=(m, 6))
+    X, st = ctx.lsqr_dense(A, B, eps=1e-14)
+    Xo = np.linalg.lstsq(A, B, rcond=None)[0]
+    assert all(s in (1, 2) for s in st["istop"]), st
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
+
+
 def test_fortran_lsqr_shim(ctx, orc):
     rng = np.random.default_rng(11)
     n = 300
