@@ -94,6 +94,8 @@ def load_library(path: str = LIB_PATH):
     lib.andr_quad_far_calibration.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_double, C.c_double, _I]
     lib.andr_profile.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_profile_read.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _L, _D, _D, _D]
+    lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
+    lib.andr_autotune_read.argtypes = [C.c_void_p, C.c_int32, _I, _I, _I, _D]
     _lib = lib
     return lib
 
@@ -102,7 +104,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
-            "andr_profile_read", "andr_quad_far_calibration"]
+            "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read"]
 FAR_BINS = 16
 
 
@@ -258,6 +260,20 @@ class Context:
         n, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
         while self.lib.andr_profile_read(self.h, i, name, 128, C.byref(n), C.byref(ms), C.byref(by), C.byref(fl)) == 1:
             out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": by.value, "flops": fl.value}
+            i += 1
+        return out
+
+    def autotune(self, mode=True):
+        """True/1: tune; False/0: 64 rows per wave; 32, 64 or 128: that many rows per wave."""
+        self._check(self.lib.andr_autotune(self.h, int(mode)), "andr_autotune")
+
+    def autotune_table(self) -> list:
+        """Autotuner choices: rows per wave of the plain solve's FP64 product per size bucket."""
+        out = []
+        i = 0
+        lr, lc, rpw, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        while self.lib.andr_autotune_read(self.h, i, C.byref(lr), C.byref(lc), C.byref(rpw), C.byref(ms)) == 1:
+            out.append({"log2_rows": lr.value, "log2_cols": lc.value, "rows_per_wave": rpw.value, "ms": ms.value})
             i += 1
         return out
 

@@ -42,6 +42,11 @@ struct andr_ctx {
   float model_conf = 0.9f;
   double model_near = 4.0;
   void *d_model = nullptr;  // packed bf16 weights + fp32 biases
+  // autotuner of the plain solve's FP64 product A X: (ceil log2 m, ceil log2 n) -> (16-row tiles
+  // per wave, best time in ms); filled on first use of a bucket while autotune is on
+  bool autotune = true;
+  int gemv_n_tiles = 4;  // with the autotuner off
+  std::map<std::pair<int, int>, std::pair<int, float>> tune_gemv_n;
 };
 
 // status helpers ------------------------------------------------------------------------------

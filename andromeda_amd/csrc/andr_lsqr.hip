@@ -87,18 +87,19 @@ __global__ __launch_bounds__(256) void k_gemv_n(const ST *__restrict__ A, int64_
   if (lane < 8) part[(size_t)wave * NB + lane] = y2;
 }
 
-// The same product for FP64 A on the matrix cores (v_mfma_f64_16x16x4f64). A wave owns GN_ROWS = 64
-// rows (four 16-row tiles) over the column segment blockIdx.y; lane (g = lane / 16, c = lane % 16)
-// feeds row 16 T + c with columns 16 kb + 4 g + t (t = 0..3, one instruction each) and column c of
+// The same product for FP64 A on the matrix cores (v_mfma_f64_16x16x4f64). A wave owns 16 GN_T rows
+// (GN_T 16-row tiles) over the column segment blockIdx.y; lane (g = lane / 16, c = lane % 16) feeds
+// row c of tile T with columns 16 kb + 4 g + t (t = 0..3, one instruction each) and column c of
 // B = X (right-hand side c; the upper half c >= NB is zero). The result D[g + 4 v][c] sits in
 // register v of lane (g, c). With one segment the update Y = sx .* D + sy .* Y and
 // out[wave][k] = sum of the new Y^2 over the wave's rows are done here; with several the raw
-// segment products go to out = P[segment][row][k] and k_combine_n finishes.
-constexpr int GN_T = 4, GN_ROWS = 16 * GN_T;
-constexpr int GN_TARGET_WAVES = 8192;  // measured: 2048 -> 3.9, 8192 -> 4.6 TB/s on a 24000^2 matrix
+// segment products go to out = P[segment][row][k] and k_combine_n finishes. The segment length
+// depends on the matrix shape only, and every row's sum runs over its segment in the same order for
+// every GN_T, so the tile count (chosen by the autotuner) never changes a result bit.
+constexpr int GN_SEGLEN = 1024;  // measured on 13500^2 / 24000^2: 4.5 / 4.8 TB/s (1360 / 1104: 4.3 / 4.6)
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <bool SEG, bool VEC>
+template <int GN_T, bool SEG, bool VEC>
 __global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A, int64_t lda, int m, int n,
                                                    const double *__restrict__ X, int seglen, double *__restrict__ Y,
                                                    const double *__restrict__ sx, const double *__restrict__ sy,
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A,
   typedef double d2n __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int row0 = wave * GN_ROWS;
+  const int row0 = wave * 16 * GN_T;
   if (row0 >= m) return;
   const int g = lane >> 4, c = lane & 15;
   const bool hi = c >= NB;
@@ -608,6 +609,7 @@ struct LsqrRun {
   bool done = false;
   int wavesN = 0, blocksN = 0, ns = 0, rps = 0, blocksC = 0, blocksM = 0;
   int nsegN = 1, seglenN = 0, partsN = 0;  // A X: column segments and the partial sums it leaves
+  int tilesN = 4;                           // A X: 16-row tiles per wave (autotuned)
   bool vecN = false;
   DevBuf<double> U, V, W, Xl, X0, Xs, P, PN, partU, partV, partW, partB, sc;
   DevBuf<LsqrState> S;
@@ -619,30 +621,80 @@ struct LsqrRun {
   int tok(const char *name, double bytes) {
     return st == ctx->stream ? andr_prof_begin(ctx, name, bytes, aflops) : -1;
   }
+  template <int T, bool SEG>
+  void launch_mf_t(const double *Xin, double *Y, const double *sx, const double *sy, double *out) {
+    const dim3 grid((m + 64 * T - 1) / (64 * T), nsegN);  // four waves of 16 T rows per block
+    if (vecN)
+      hipLaunchKernelGGL((k_gemv_n_mf<T, SEG, true>), grid, dim3(256), 0, st, (const double *)A, lda, m, n, Xin,
+                         seglenN, Y, sx, sy, out);
+    else
+      hipLaunchKernelGGL((k_gemv_n_mf<T, SEG, false>), grid, dim3(256), 0, st, (const double *)A, lda, m, n, Xin,
+                         seglenN, Y, sx, sy, out);
+  }
+  void launch_mf(int tiles, bool seg, const double *Xin, double *Y, const double *sx, const double *sy, double *out) {
+    switch (tiles * 2 + (seg ? 1 : 0)) {
+      case 4: launch_mf_t<2, false>(Xin, Y, sx, sy, out); break;
+      case 5: launch_mf_t<2, true>(Xin, Y, sx, sy, out); break;
+      case 16: launch_mf_t<8, false>(Xin, Y, sx, sy, out); break;
+      case 17: launch_mf_t<8, true>(Xin, Y, sx, sy, out); break;
+      case 9: launch_mf_t<4, true>(Xin, Y, sx, sy, out); break;
+      default: launch_mf_t<4, false>(Xin, Y, sx, sy, out); break;
+    }
+  }
+  // Autotuner of the FP64 product: on the first large system of a (log2 m, log2 n) bucket, time the
+  // segmented product with 2, 4 and 8 tiles per wave on the actual matrix (its output goes to the
+  // segment partials only) and keep the fastest for the bucket. The choice does not change results.
+  int tune_tiles() {
+    const std::pair<int, int> key{ilog2c(m), ilog2c(n)};
+    auto it = ctx->tune_gemv_n.find(key);
+    if (it != ctx->tune_gemv_n.end()) {
+      tilesN = it->second.first;
+      return ANDR_OK;
+    }
+    ANDR_HIP(ctx, hipDeviceSynchronize());  // time the candidates alone on the device
+    hipEvent_t e0, e1;
+    ANDR_HIP(ctx, hipEventCreate(&e0));
+    ANDR_HIP(ctx, hipEventCreate(&e1));
+    int best = 4;
+    float best_ms = 1e30f;
+    for (int tiles : {2, 4, 8}) {
+      launch_mf(tiles, true, V.p, U.p, sxn, syn, PN.p);  // warm-up
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        (void)hipEventRecord(e0, st);
+        launch_mf(tiles, true, V.p, U.p, sxn, syn, PN.p);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms = std::min(ms, t);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = tiles;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ANDR_HIP(ctx, hipGetLastError());
+    ctx->tune_gemv_n[key] = {best, best_ms};
+    tilesN = best;
+    return ANDR_OK;
+  }
+  static int ilog2c(int v) {
+    int b = 0;
+    while ((1 << b) < v) ++b;
+    return b;
+  }
+
   void gemv_n(const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
     const int t = tok("lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0);
     if (fp32) {
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
     } else {
-      const double *Ad = (const double *)A;
-      const dim3 grid((wavesN + 3) / 4, nsegN);
-      if (nsegN == 1) {
-        if (vecN)
-          hipLaunchKernelGGL((k_gemv_n_mf<false, true>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
-                             sy, part);
-        else
-          hipLaunchKernelGGL((k_gemv_n_mf<false, false>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
-                             sy, part);
-      } else {
-        if (vecN)
-          hipLaunchKernelGGL((k_gemv_n_mf<true, true>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
-                             sy, PN.p);
-        else
-          hipLaunchKernelGGL((k_gemv_n_mf<true, false>), grid, dim3(256), 0, st, Ad, lda, m, n, Xin, seglenN, Y, sx,
-                             sy, PN.p);
-        hipLaunchKernelGGL(k_combine_n, dim3(blocksM), dim3(256), 0, st, m, nsegN, PN.p, Y, sx, sy, part);
-      }
+      launch_mf(tilesN, nsegN > 1, Xin, Y, sx, sy, nsegN > 1 ? PN.p : part);
+      if (nsegN > 1) hipLaunchKernelGGL(k_combine_n, dim3(blocksM), dim3(256), 0, st, m, nsegN, PN.p, Y, sx, sy, part);
     }
     andr_prof_end(ctx, t);
   }
@@ -675,12 +727,15 @@ struct LsqrRun {
       wavesN = (m + 7) / 8;
       blocksN = (wavesN + 3) / 4;
       partsN = wavesN;
-    } else {  // matrix-core product: 64 rows per wave, columns split until ~8192 waves are launched
-      wavesN = (m + GN_ROWS - 1) / GN_ROWS;
-      nsegN = std::max(1, std::min((GN_TARGET_WAVES + wavesN - 1) / wavesN, (n + 255) / 256));
-      seglenN = ((n + nsegN - 1) / nsegN + 15) / 16 * 16;
+    } else {  // matrix-core product over column segments; room for the most partials
+      // segments of GN_SEGLEN columns, shorter (down to 256) when that leaves fewer than 2048 waves
+      // of 64 rows; a function of the shape only, so that the tile count never changes the sums
+      const int64_t rw = (m + 63) / 64;
+      seglenN = GN_SEGLEN;
+      if (rw * ((n + GN_SEGLEN - 1) / GN_SEGLEN) < 2048)
+        seglenN = (int)std::max<int64_t>(256, ((int64_t)n * rw / 2048 + 15) / 16 * 16);
       nsegN = (n + seglenN - 1) / seglenN;
-      partsN = nsegN == 1 ? wavesN : (m * NB + 255) / 256;
+      partsN = std::max((m + 31) / 32, (m * NB + 255) / 256);
       vecN = lda % 2 == 0 && ((uintptr_t)A % 16) == 0;
     }
     ns = splits_for(m, n);
@@ -721,6 +776,14 @@ struct LsqrRun {
     ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
     ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
     ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
+    if (!fp32) {
+      tilesN = ctx->autotune ? 4 : ctx->gemv_n_tiles;
+      if (nsegN > 1 && ctx->autotune && (double)m * n * 8.0 >= 2.56e8) {
+        const int rc = tune_tiles();
+        if (rc) return rc;
+      }
+      partsN = nsegN == 1 ? (m + 16 * tilesN - 1) / (16 * tilesN) : blocksM;  // one per wave, or per block
+    }
     abytes = (double)m * n * (fp32 ? 4.0 : 8.0);
     vbytes = ((double)m + n) * NB * 8.0;
     aflops = 2.0 * m * n * nrhs;

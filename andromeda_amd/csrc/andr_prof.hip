@@ -1,4 +1,4 @@
-// Per-kernel event timing (see andr_prof.h) and its C-ABI readers.
+// Per-kernel event timing (see andr_prof.h), its C-ABI readers and those of the autotuner table.
 #include "andr_prof.h"
 
 #include <cstring>
@@ -87,5 +87,26 @@ extern "C" int andr_profile_read(andr_ctx *ctx, int32_t idx, char *name, int32_t
   if (ms) *ms = it->second.ms;
   if (bytes) *bytes = it->second.bytes;
   if (flops) *flops = it->second.flops;
+  return 1;
+}
+
+extern "C" int andr_autotune(andr_ctx *ctx, int32_t mode) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (mode != 0 && mode != 1 && mode != 32 && mode != 64 && mode != 128)
+    return andr_fail(ctx, ANDR_ERR_ARG, "andr_autotune: mode %d (0, 1, 32, 64 or 128)", mode);
+  ctx->autotune = mode == 1;
+  ctx->gemv_n_tiles = mode > 1 ? mode / 16 : 4;
+  return ANDR_OK;
+}
+
+extern "C" int andr_autotune_read(andr_ctx *ctx, int32_t idx, int32_t *log2_rows, int32_t *log2_cols,
+                                  int32_t *rows_per_wave, double *ms) {
+  if (!ctx || idx < 0 || idx >= (int)ctx->tune_gemv_n.size()) return 0;
+  auto it = ctx->tune_gemv_n.begin();
+  std::advance(it, idx);
+  if (log2_rows) *log2_rows = it->first.first;
+  if (log2_cols) *log2_cols = it->first.second;
+  if (rows_per_wave) *rows_per_wave = 16 * it->second.first;
+  if (ms) *ms = it->second.second;
   return 1;
 }

@@ -191,6 +191,17 @@ int andr_profile(andr_ctx *ctx, int32_t enable);
 int andr_profile_read(andr_ctx *ctx, int32_t idx, char *name, int32_t namelen, int64_t *launches, double *ms,
                       double *bytes, double *flops);
 
+/* ---- autotuner (no reference counterpart) ---------------------------------------------------
+ * The plain solve's FP64 product A V picks its rows per wave (32, 64 or 128) by timing the
+ * candidates on the first large system (>= 256 MB) of every (ceil log2 rows, ceil log2 columns)
+ * bucket; the choice is kept per context and never changes a result bit (the column segments and
+ * every summation order are fixed). andr_autotune(ctx, mode): mode 1 tunes (the default), 0 uses 64
+ * rows per wave, 32 / 64 / 128 fix that many; andr_autotune_read returns 1 and fills entry idx (0..)
+ * of the table, 0 past the end. */
+int andr_autotune(andr_ctx *ctx, int32_t mode);
+int andr_autotune_read(andr_ctx *ctx, int32_t idx, int32_t *log2_rows, int32_t *log2_cols, int32_t *rows_per_wave,
+                       double *ms);
+
 #ifdef __cplusplus
 }
 #endif
