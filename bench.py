@@ -132,6 +132,9 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "learned", "auto"], default="auto")
     ap.add_argument("--model", default=os.path.join(ROOT, "andromeda_amd", "quad_model.npz"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the plain FP64 path on the sample, one body at a time with per-kernel timing "
+                         "(the workload of the plain path's kernel traces and counters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches): the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -162,6 +165,20 @@ def main():
         f"sum F^2 = {sum(b.nelem ** 2 for b in bodies):.3e} (mesh generation {time.perf_counter() - t0:.1f} s)")
 
     ctx = api.Context(0)
+    if args.plain_only:
+        idx = stratified(bodies, args.sample, n_fibre)
+        ctx.profile(True)
+        tp = time.perf_counter()
+        for i in idx:
+            ctx.resistance_batch([bodies[i]], precision=api.PREC_FP64, eps=1e-12)
+        t_plain = time.perf_counter() - tp
+        kernels, roofline = kernel_table(ctx.profile_read())
+        ctx.profile(False)
+        log(f"[bench] plain FP64 only, bodies {idx} (F = {[bodies[i].nelem for i in idx]}): "
+            f"{len(idx) / t_plain:.3f} tensors/s; autotuner {ctx.autotune_table()}")
+        print(json.dumps({"plain_fp64_only": {"bodies": idx, "tensors_per_s": round(len(idx) / t_plain, 3),
+                                              "roofline": roofline, "kernels": kernels}}), flush=True)
+        return
     prec = api.PREC_MIXED
     prec_name = "mixed"
     if args.precision in ("learned", "auto") and os.path.exists(args.model):
