@@ -187,3 +187,13 @@ def test_batch_equals_single(ctx):
         for b, r in zip(bodies, rb):
             s = ctx.resistance(b, precision=prec)
             assert _rel(r.FT, s.FT) < 1e-12
+
+
+def test_plain_batch_streams_bitwise(ctx):
+    """Six plain FP64 solves of different sizes share four streams (two streams carry two solves):
+    every tensor and iteration count is bit-for-bit that of the body solved alone."""
+    bodies = [M.random_superellipsoid(np.random.default_rng(10 + i), nu)[0] for i, nu in enumerate((3, 5, 4, 6, 3, 4))]
+    rb = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
+    for b, r in zip(bodies, rb):
+        s = ctx.resistance(b, precision=api.PREC_FP64, eps=1e-12)
+        assert np.array_equal(r.FT, s.FT) and np.array_equal(r.nit, s.nit)
