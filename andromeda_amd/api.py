@@ -94,6 +94,12 @@ def load_library(path: str = LIB_PATH):
     lib.andr_quad_far_calibration.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_double, C.c_double, _I]
     lib.andr_profile.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_profile_read.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _L, _D, _D, _D]
+    lib.andr_bodies_upload.argtypes = [C.c_void_p, C.c_int32, C.POINTER(_Mesh), C.POINTER(_ResOpts),
+                                       C.POINTER(C.c_void_p)]
+    lib.andr_bodies_count.argtypes = [C.c_void_p]
+    lib.andr_resistance_batch_resident.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_ResOpts), _D,
+                                                   C.POINTER(_ResReport)]
+    lib.andr_bodies_free.argtypes = [C.c_void_p, C.c_void_p]
     lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_autotune_read.argtypes = [C.c_void_p, C.c_int32, _I, _I, _I, _D]
     _lib = lib
@@ -104,7 +110,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
-            "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read"]
+            "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
+            "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free"]
 FAR_BINS = 16
 
 
@@ -143,6 +150,33 @@ class ResistanceResult:
     quad_points: int
     pairs_learned: int
     pairs_fallback: int
+
+
+def _results(FT, reps):
+    return [ResistanceResult(FT[i].reshape(6, 6).copy(), np.array(list(r.nit)), r.status, (r.err[0], r.err[1]),
+                             r.t_assemble, r.t_solve, r.quad_points, r.pairs_learned, r.pairs_fallback)
+            for i, r in enumerate(reps)]
+
+
+class Bodies:
+    """Device-resident bodies of a context (andr_bodies_upload / andr_bodies_free)."""
+
+    def __init__(self, ctx, h, form, mu):
+        self.ctx, self.h, self.form, self.mu = ctx, h, form, mu
+
+    def __len__(self):
+        return self.ctx.lib.andr_bodies_count(self.h) if self.h else 0
+
+    def free(self):
+        if self.h and getattr(self.ctx, "h", None):
+            self.ctx.lib.andr_bodies_free(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -239,12 +273,29 @@ class Context:
         reps = (_ResReport * len(views))()
         self._check(self.lib.andr_resistance_batch(self.h, len(views), arr, C.byref(opts), _p(FT), reps),
                     "andr_resistance_batch")
-        out = []
-        for i, r in enumerate(reps):
-            out.append(ResistanceResult(FT[i].reshape(6, 6).copy(), np.array(list(r.nit)), r.status,
-                                        (r.err[0], r.err[1]), r.t_assemble, r.t_solve, r.quad_points,
-                                        r.pairs_learned, r.pairs_fallback))
-        return out
+        return _results(FT, reps)
+
+    def upload_bodies(self, meshes, form: int = FORM_EXTERIOR, mu: float = 1.0, itri: int = 5,
+                      normmul=None) -> "Bodies":
+        """Device-resident bodies for repeated resistance batches (andr_bodies_upload)."""
+        views = [_MeshView(m, normmul, itri) for m in meshes]
+        arr = (_Mesh * len(views))(*[v.s for v in views])
+        opts = _ResOpts(form, PREC_FP64, mu, 0.0, 0, (C.c_double * 3)(0.0, 0.0, 0.0))
+        h = C.c_void_p()
+        self._check(self.lib.andr_bodies_upload(self.h, len(views), arr, C.byref(opts), C.byref(h)),
+                    "andr_bodies_upload")
+        return Bodies(self, h, form, mu)
+
+    def resistance_batch_resident(self, bodies: "Bodies", precision: int = PREC_FP64, eps: float = 0.0,
+                                  maxit: int = 0, center=(0.0, 0.0, 0.0)):
+        """andr_resistance_batch on device-resident bodies (form and mu of the upload)."""
+        n = self.lib.andr_bodies_count(bodies.h)
+        opts = _ResOpts(bodies.form, precision, bodies.mu, eps, maxit, (C.c_double * 3)(*center))
+        FT = np.zeros((n, 36))
+        reps = (_ResReport * n)()
+        self._check(self.lib.andr_resistance_batch_resident(self.h, bodies.h, C.byref(opts), _p(FT), reps),
+                    "andr_resistance_batch_resident")
+        return _results(FT, reps)
 
     def resistance(self, mesh, **kw) -> ResistanceResult:
         return self.resistance_batch([mesh], **kw)[0]

@@ -145,6 +145,22 @@ int andr_resistance(andr_ctx *ctx, const andr_mesh *mesh, const andr_resistance_
 int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes, const andr_resistance_opts *opts,
                           double *FT, andr_resistance_report *reports);
 
+/* Device-resident bodies (no reference counterpart: the reference reads a mesh and solves it once
+ * per run). For callers that compute the tensors of the same bodies more than once, e.g. at several
+ * precisions: andr_bodies_upload copies the geometry to the device and runs the closure tests of
+ * the walls for opts->form and opts->mu (the other option fields are not used); the host meshes are
+ * not referenced afterwards. andr_resistance_batch_resident is andr_resistance_batch on those bodies
+ * (same form and mu, any precision), FT and reports in the order of the uploaded meshes.
+ * andr_bodies_count returns the number of bodies; andr_bodies_free releases them and must come
+ * before andr_destroy of the context. */
+typedef struct andr_bodies andr_bodies;
+int andr_bodies_upload(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes, const andr_resistance_opts *opts,
+                       andr_bodies **bodies);
+int andr_bodies_count(const andr_bodies *bodies);
+int andr_resistance_batch_resident(andr_ctx *ctx, const andr_bodies *bodies, const andr_resistance_opts *opts,
+                                   double *FT, andr_resistance_report *reports);
+int andr_bodies_free(andr_ctx *ctx, andr_bodies *bodies);
+
 /* ---- learned quadrature (no reference counterpart: the reference uses one fixed rule) ----- */
 /* Network: 6 inputs -> 3 hidden layers of 64 (ReLU) -> 8 classes; classes 0..6 are the Dunavant
  * rules with 3, 4, 6, 7, 12, 13, 19 points, class 7 the fixed-order rule of the mesh (ITRI).

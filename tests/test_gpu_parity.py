@@ -197,3 +197,22 @@ def test_plain_batch_streams_bitwise(ctx):
     for b, r in zip(bodies, rb):
         s = ctx.resistance(b, precision=api.PREC_FP64, eps=1e-12)
         assert np.array_equal(r.FT, s.FT) and np.array_equal(r.nit, s.nit)
+
+
+def test_resident_bodies_match_host_meshes(ctx):
+    """Bodies uploaded once give, at every form, the tensors and iteration counts of the call that
+    takes host meshes, bit for bit; a viscosity other than the upload's is refused."""
+    bodies = [M.random_superellipsoid(np.random.default_rng(20 + i), nu)[0] for i, nu in enumerate((3, 5, 4))]
+    res = ctx.upload_bodies(bodies)
+    try:
+        assert len(res) == 3
+        for prec in (api.PREC_FP64, api.PREC_MIXED):
+            a = ctx.resistance_batch(bodies, precision=prec)
+            b = ctx.resistance_batch_resident(res, precision=prec)
+            for x, y in zip(a, b):
+                assert np.array_equal(x.FT, y.FT) and np.array_equal(x.nit, y.nit)
+        res.mu = 2.0
+        with pytest.raises(RuntimeError):
+            ctx.resistance_batch_resident(res)
+    finally:
+        res.free()
