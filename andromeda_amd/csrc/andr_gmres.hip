@@ -388,6 +388,40 @@ __global__ __launch_bounds__(256) void kb_resid_sym(const GSys *__restrict__ sys
   }
 }
 
+// The residual of the first cycle, x = 0: R = B (already copied) and the same partial sums of R^2
+// as kb_resid_sym, over the same bands, without the product (A M^-1 0 contributes exact zeros).
+__global__ __launch_bounds__(256) void kb_resid_zero(const GSys *__restrict__ sys, const WaveMap *__restrict__ bmap,
+                                                     double *__restrict__ part) {
+  const WaveMap bm = bmap[blockIdx.x];
+  const GSys &S = sys[bm.sys];
+  const int ne = S.op.ne;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int band = (bm.row0 >> 4) + wid;
+  const int r = 16 * band + c;
+  double r2[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) r2[k] = 0.0;
+  if (16 * band < ne && g == 0 && r < ne) {
+#pragma unroll
+    for (int co = 0; co < 3; ++co)
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const double v = S.R[(size_t)(co * ne + r) * NB + k];
+        r2[k] += v * v;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    for (int off = 1; off < 64; off <<= 1) r2[k] += __shfl_xor(r2[k], off, 64);
+  if (lane < NB) {
+    double v = r2[0];
+#pragma unroll
+    for (int k = 1; k < NB; ++k) v = lane == k ? r2[k] : v;
+    part[((size_t)blockIdx.x * 4 + wid) * NB + lane] = v;
+  }
+}
+
 // per tile of a system with nseg > 1: W = sx .* sum over segments of the partial products
 __global__ __launch_bounds__(256) void kb_seg_sum(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap,
                                                   const double *__restrict__ sxy) {
@@ -1046,7 +1080,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   while (true) {
     // residual r = b - A x of every system of the last cycle (x = 0 on the first), FP64 accumulation
     if (cyc.n_t) hipLaunchKernelGGL(kb_copy_b, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
-    gemv(cyc, 0, 0, d_res.p);
+    if (first) {
+      if (cyc.n_w) hipLaunchKernelGGL(kb_resid_zero, dim3(cyc.n_w), dim3(256), 0, st, dsys.p, cyc.w.p, partW.p);
+    } else {
+      gemv(cyc, 0, 0, d_res.p);
+    }
     ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, partW.p, cyc.w0.p, cyc.nw.p, tol,
                        first ? 1 : 0, nrhs, ddone.p);
