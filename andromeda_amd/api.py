@@ -77,6 +77,8 @@ def load_library(path: str = LIB_PATH):
     lib.andr_assemble_laplace.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, _D, _D, _D]
     lib.andr_assemble_stokes.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.c_double,
                                          C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), _D]
+    lib.andr_stokes_domain_values.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, _D, _D, _D, C.c_double,
+                                              _D, _D, _D]
     lib.andr_lsqr_dense.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _D, C.c_int32, _D, _D,
                                     C.POINTER(_LsqrOpts), C.POINTER(_LsqrStats)]
     lib.andr_resistance.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.POINTER(_ResOpts), _D, C.POINTER(_ResReport)]
@@ -111,7 +113,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
             "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
-            "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free"]
+            "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
+            "andr_stokes_domain_values"]
 FAR_BINS = 16
 
 
@@ -246,6 +249,20 @@ class Context:
         self._check(self.lib.andr_assemble_stokes(self.h, C.byref(mv.s), ROW_MAJOR, extra, Tp, Gp, pTp, pGp, _p(err)),
                     "andr_assemble_stokes")
         return T, G, prT, prG, err
+
+    def stokes_domain_values(self, mesh: Mesh, pts, u, q, diff: float = 1.0, normmul=None, itri: int = 5):
+        """Velocity (npts, 3), pressure (npts,) and closure error (npts,) at points of the flow domain
+        from nodal velocities u (nnodes, 3) and element tractions q (nelem, 3)."""
+        mv = _MeshView(mesh, normmul, itri)
+        P = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        U = np.ascontiguousarray(u, dtype=np.float64).reshape(mv.m.nnodes, 3)
+        Q = np.ascontiguousarray(q, dtype=np.float64).reshape(mv.m.nelem, 3)
+        vel = np.zeros((P.shape[0], 3))
+        p = np.zeros(P.shape[0])
+        err = np.zeros(P.shape[0])
+        self._check(self.lib.andr_stokes_domain_values(self.h, C.byref(mv.s), P.shape[0], _p(P), _p(U), _p(Q), diff,
+                                                       _p(vel), _p(p), _p(err)), "andr_stokes_domain_values")
+        return vel, p, err
 
     # ---- seam 2 ---------------------------------------------------------------------------------
     def lsqr_dense(self, A, B, X0=None, maxit: int = 0, eps: float = 1e-15, precon: bool = True):

@@ -39,6 +39,10 @@ int andr_launch_unprecondition(andr_ctx *ctx, const SysJob *d_jobs, const int2 *
 // Force and torque of the six problems on wall 1 of every job, and its largest row self-check error:
 // d_out[37 j + 6 k + (0..5)] = (F, T) of problem k, d_out[37 j + 36] = error.
 int andr_launch_force_torque(andr_ctx *ctx, const SysJob *d_jobs, int njobs, double *d_out);
+// Velocity, pressure and closure error at npts domain points (device arrays pts[3 npts], u[3 nn],
+// q[3 ne]; out[5 npts]); triangle meshes.
+int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double *pts, const double *u, const double *q,
+                              double diff, double *out);
 // algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0

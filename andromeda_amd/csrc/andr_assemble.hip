@@ -631,52 +631,8 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
       double Tv[6][3] = {{0}}, pT[3][3] = {{0}};
       if (isrc == 0) {
         const double m = g.mul[e];
-        const double nx = m * g.nrm[3 * e], ny = m * g.nrm[3 * e + 1], nz = m * g.nrm[3 * e + 2];
-        for (int q = 0; q < nq; ++q) {
-          const double l1 = c_qd[qoff + q][0], l2 = c_qd[qoff + q][1], l3 = c_qd[qoff + q][2];
-          const double w = c_qd[qoff + q][3];
-          const double rx = (l1 * x1[0] + l2 * x2[0] + l3 * x3[0]) - s[0];
-          const double ry = (l1 * x1[1] + l2 * x2[1] + l3 * x3[1]) - s[1];
-          const double rz = (l1 * x1[2] + l2 * x2[2] + l3 * x3[2]) - s[2];
-          const double rxx = rx * rx, rxy = rx * ry, rxz = rx * rz, ryy = ry * ry, ryz = ry * rz, rzz = rz * rz;
-          const double ir2 = 1.0 / (rxx + ryy + rzz);
-          const double ir = sqrt(ir2);
-          const double ir3 = ir2 * ir;
-          Gs[0] += w * (ir * (1.0 + rxx * ir2));
-          Gs[1] += w * (rxy * ir3);
-          Gs[2] += w * (rxz * ir3);
-          Gs[3] += w * (ir * (1.0 + ryy * ir2));
-          Gs[4] += w * (ryz * ir3);
-          Gs[5] += w * (ir * (1.0 + rzz * ir2));
-          const double c2 = 2.0 * ir3;
-          pG[0] += w * (c2 * rx);
-          pG[1] += w * (c2 * ry);
-          pG[2] += w * (c2 * rz);
-          const double ct = -6.0 * ir2 * ir3 * (rx * nx + ry * ny + rz * nz);
-          const double tc[6] = {ct * rxx, ct * rxy, ct * rxz, ct * ryy, ct * ryz, ct * rzz};
-          const double c4 = 4.0 * ir3;
-          const double pt[3] = {c4 * ((-1.0 + 3.0 * rxx * ir2) * nx + 3.0 * rxy * ir2 * ny + 3.0 * rxz * ir2 * nz),
-                                c4 * (3.0 * rxy * ir2 * nx + (-1.0 + 3.0 * ryy * ir2) * ny + 3.0 * ryz * ir2 * nz),
-                                c4 * (3.0 * rxz * ir2 * nx + 3.0 * ryz * ir2 * ny + (-1.0 + 3.0 * rzz * ir2) * nz)};
-          const double wl[3] = {w * l1, w * l2, w * l3};
-#pragma unroll
-          for (int k = 0; k < 6; ++k)
-#pragma unroll
-            for (int v = 0; v < 3; ++v) Tv[k][v] += wl[v] * tc[k];
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int v = 0; v < 3; ++v) pT[k][v] += wl[v] * pt[k];
-        }
-        const double A = g.area[e];
-        for (int k = 0; k < 6; ++k) {
-          Gs[k] *= A;
-          for (int v = 0; v < 3; ++v) Tv[k][v] *= A;
-        }
-        for (int k = 0; k < 3; ++k) {
-          pG[k] *= A;
-          for (int v = 0; v < 3; ++v) pT[k][v] *= A;
-        }
+        stokes_tri_full(s, x1, x2, x3, m * g.nrm[3 * e], m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2], g.area[e], qoff,
+                        nq, Gs, pG, Tv, pT);
       } else {
         andr_sing_stk(x1, x2, x3, s, isrc, Gs);  // T, prT, prG of a singular pair are zero
       }
@@ -758,6 +714,88 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
     }
   }
   if (threadIdx.x == 0) err_rows[r] = err;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Velocity and pressure at points of the flow domain: GetStokesDomainValue (src/lib_Stokes.f90:
+// 3043-3390). One workgroup per point; the threads take elements, integrate each with the mesh's
+// rule (the point is off the boundary, so every pair is regular) and contract the integrals with
+// the nodal velocities u (3 per node) and the element tractions q (3 per element) at once, so that
+// no integral row is stored. out[5 p + 0..2] = velocity, out[5 p + 3] = pressure, out[5 p + 4] =
+// the closure error of the point's integrals (:3255-3300): the T closure -sum_i T_ii = 8 pi and
+// sum_e G n_e = 0 hold inside the domain. osem = 8 pi as the reference forms it.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stokes_domain(BodyGeom g, int qoff, int nq, const double *__restrict__ pts,
+                                                       const double *__restrict__ u, const double *__restrict__ q,
+                                                       double diff, double osem, double *__restrict__ out) {
+  constexpr int NV = 14;  // T u (3), G q (3), prT u, prG q, T closure (3), G n (3)
+  __shared__ double red[NV][4];
+  const int ip = blockIdx.x;
+  const double s[3] = {pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2]};
+  double a[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) a[i] = 0.0;
+  for (int e = threadIdx.x; e < g.ne; e += blockDim.x) {
+    const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
+    const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
+    const double m = g.mul[e];
+    const double nx = m * g.nrm[3 * e], ny = m * g.nrm[3 * e + 1], nz = m * g.nrm[3 * e + 2];
+    double Gs[6], pG[3], Tv[6][3], pT[3][3];
+    stokes_tri_full(s, x1, x2, x3, nx, ny, nz, g.area[e], qoff, nq, Gs, pG, Tv, pT);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const double ux = u[3 * vv[v]], uy = u[3 * vv[v] + 1], uz = u[3 * vv[v] + 2];
+      a[0] += Tv[0][v] * ux + Tv[1][v] * uy + Tv[2][v] * uz;
+      a[1] += Tv[1][v] * ux + Tv[3][v] * uy + Tv[4][v] * uz;
+      a[2] += Tv[2][v] * ux + Tv[4][v] * uy + Tv[5][v] * uz;
+      a[6] += pT[0][v] * ux + pT[1][v] * uy + pT[2][v] * uz;
+      a[8] += Tv[0][v];
+      a[9] += Tv[3][v];
+      a[10] += Tv[5][v];
+    }
+    const double qx = q[3 * e], qy = q[3 * e + 1], qz = q[3 * e + 2];
+    a[3] += Gs[0] * qx + Gs[1] * qy + Gs[2] * qz;
+    a[4] += Gs[1] * qx + Gs[3] * qy + Gs[4] * qz;
+    a[5] += Gs[2] * qx + Gs[4] * qy + Gs[5] * qz;
+    a[7] += pG[0] * qx + pG[1] * qy + pG[2] * qz;
+    a[11] += Gs[0] * nx + Gs[1] * ny + Gs[2] * nz;
+    a[12] += Gs[1] * nx + Gs[3] * ny + Gs[4] * nz;
+    a[13] += Gs[2] * nx + Gs[4] * ny + Gs[5] * nz;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double v = a[i];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[i][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
+    double *o = out + 5 * (size_t)ip;
+    for (int c = 0; c < 3; ++c) o[c] = -(t[c] + t[3 + c] / diff) / osem;
+    o[3] = -(diff * t[6] - t[7]) / osem;
+    double err = -9.99;
+    for (int c = 0; c < 3; ++c) err = fmax(err, fabs(-t[8 + c] / osem - 1.0));
+    for (int c = 0; c < 3; ++c) err = fmax(err, fabs(t[11 + c]));
+    o[4] = err;
+  }
+}
+
+int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double *pts, const double *u, const double *q,
+                              double diff, double *out) {
+  if (npts <= 0) return ANDR_OK;
+  const QuadRule qr = andr_rule(b.itri);
+  const double osem = (double)atanf(1.0f) * 4.0 * 8.0;  // REAL(4) ATAN(1.0) (src/lib_Stokes.f90:3089)
+  const double evals = (double)npts * b.g.ne * qr.npts;
+  const int tok = andr_prof_begin(ctx, "stokes_domain", 8.0 * (3.0 * npts + 5.0 * npts), evals * 175.0);
+  hipLaunchKernelGGL(k_stokes_domain, dim3(npts), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts, pts, u, q, diff,
+                     osem, out);
+  andr_prof_end(ctx, tok);
+  ANDR_HIP(ctx, hipGetLastError());
+  return ANDR_OK;
 }
 
 int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, double *G, double *prT, double *prG,

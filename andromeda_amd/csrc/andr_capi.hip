@@ -190,6 +190,38 @@ extern "C" int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_
   return max_err(ctx, dE.p, nn, ne, err);
 }
 
+extern "C" int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, int32_t npts, const double *pts,
+                                         const double *u, const double *q, double diff, double *vel, double *p,
+                                         double *err) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (!mesh || npts < 0 || (npts > 0 && (!pts || !u || !q || !vel || !p)) || !(diff > 0.0))
+    return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_domain_values arguments");
+  if (mesh->nvert == 4) return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_domain_values takes triangle meshes");
+  if (npts == 0) return ANDR_OK;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  int rc = andr_body_upload(ctx, mesh, b, false);
+  if (rc) return rc;
+  const int nn = b.g.nn, ne = b.g.ne;
+  DevBuf<double> dpts, du, dq, dout;
+  if (dpts.alloc(3 * (size_t)npts, ctx) || du.alloc(3 * (size_t)nn, ctx) || dq.alloc(3 * (size_t)ne, ctx) ||
+      dout.alloc(5 * (size_t)npts, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "domain values (%d points)", npts);
+  ANDR_HIP(ctx, hipMemcpyAsync(dpts.p, pts, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(du.p, u, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(dq.p, q, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = andr_launch_stokes_domain(ctx, b, npts, dpts.p, du.p, dq.p, diff, dout.p))) return rc;
+  std::vector<double> h(5 * (size_t)npts);
+  ANDR_HIP(ctx, hipMemcpyAsync(h.data(), dout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < npts; ++i) {
+    for (int c = 0; c < 3; ++c) vel[3 * i + c] = h[5 * (size_t)i + c];
+    p[i] = h[5 * (size_t)i + 3];
+    if (err) err[i] = h[5 * (size_t)i + 4];
+  }
+  return ANDR_OK;
+}
+
 // ---- seam 2 ----------------------------------------------------------------------------------
 __global__ void k_csr_to_dense(int m, int64_t ld, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
                                const double *__restrict__ a, double *__restrict__ D) {

@@ -14,6 +14,74 @@ static inline int andr_upload_rules_here(andr_ctx *ctx, const double *tab, const
 }
 
 
+// The regular integrals of one triangle seen from s, in FP64 with pressure (Triangle_3DStokesInt,
+// src/lib_Stokes.f90:2341-2446, with StokesKernel :2450-2544): Stokeslet Gs, pressure vector pG,
+// stresslet moments Tv and pressure moments pT per vertex, for the normal (nx, ny, nz) and area A.
+__device__ inline void stokes_tri_full(const double s[3], const double *x1, const double *x2, const double *x3,
+                                       double nx, double ny, double nz, double A, int qoff, int nq, double Gs[6],
+                                       double pG[3], double Tv[6][3], double pT[3][3]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    Gs[k] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) Tv[k][v] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pG[k] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) pT[k][v] = 0.0;
+  }
+  for (int q = 0; q < nq; ++q) {
+    const double l1 = c_qd[qoff + q][0], l2 = c_qd[qoff + q][1], l3 = c_qd[qoff + q][2];
+    const double w = c_qd[qoff + q][3];
+    const double rx = (l1 * x1[0] + l2 * x2[0] + l3 * x3[0]) - s[0];
+    const double ry = (l1 * x1[1] + l2 * x2[1] + l3 * x3[1]) - s[1];
+    const double rz = (l1 * x1[2] + l2 * x2[2] + l3 * x3[2]) - s[2];
+    const double rxx = rx * rx, rxy = rx * ry, rxz = rx * rz, ryy = ry * ry, ryz = ry * rz, rzz = rz * rz;
+    const double ir2 = 1.0 / (rxx + ryy + rzz);
+    const double ir = sqrt(ir2);
+    const double ir3 = ir2 * ir;
+    Gs[0] += w * (ir * (1.0 + rxx * ir2));
+    Gs[1] += w * (rxy * ir3);
+    Gs[2] += w * (rxz * ir3);
+    Gs[3] += w * (ir * (1.0 + ryy * ir2));
+    Gs[4] += w * (ryz * ir3);
+    Gs[5] += w * (ir * (1.0 + rzz * ir2));
+    const double c2 = 2.0 * ir3;
+    pG[0] += w * (c2 * rx);
+    pG[1] += w * (c2 * ry);
+    pG[2] += w * (c2 * rz);
+    const double ct = -6.0 * ir2 * ir3 * (rx * nx + ry * ny + rz * nz);
+    const double tc[6] = {ct * rxx, ct * rxy, ct * rxz, ct * ryy, ct * ryz, ct * rzz};
+    const double c4 = 4.0 * ir3;
+    const double pt[3] = {c4 * ((-1.0 + 3.0 * rxx * ir2) * nx + 3.0 * rxy * ir2 * ny + 3.0 * rxz * ir2 * nz),
+                          c4 * (3.0 * rxy * ir2 * nx + (-1.0 + 3.0 * ryy * ir2) * ny + 3.0 * ryz * ir2 * nz),
+                          c4 * (3.0 * rxz * ir2 * nx + 3.0 * ryz * ir2 * ny + (-1.0 + 3.0 * rzz * ir2) * nz)};
+    const double wl[3] = {w * l1, w * l2, w * l3};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) Tv[k][v] += wl[v] * tc[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) pT[k][v] += wl[v] * pt[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    Gs[k] *= A;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) Tv[k][v] *= A;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pG[k] *= A;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) pT[k][v] *= A;
+  }
+}
+
 // Gs: Stokeslet block (FP64); Tv: stresslet moments per vertex, in the quadrature type CT
 template <class CT>
 __device__ inline void stokes_pair_regular(const double s[3], const double *x1, const double *x2, const double *x3,

@@ -145,6 +145,17 @@ int andr_resistance(andr_ctx *ctx, const andr_mesh *mesh, const andr_resistance_
 int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes, const andr_resistance_opts *opts,
                           double *FT, andr_resistance_report *reports);
 
+/* Replaces GetStokesDomainValue (src/lib_Stokes.f90:3043-3390) as calculateDomainValues
+ * (src/lib_domain.f90:463-527) calls it for every domain point: velocity and pressure at npts points
+ * pts[3*npts] of the flow domain from the boundary solution, u[3*nnodes] the nodal velocities (x, y,
+ * z per node) and q[3*nelem] the element tractions, viscosity diff:
+ * vel = -(sum T u + sum G q / diff) / 8 pi, p = -(diff sum prT.u - sum prG.q) / 8 pi, with every
+ * element integrated by the mesh's rule (ITRI). err[npts] (may be NULL) is the reference's closure
+ * error of the point's integrals; above 1e-2 the point is not inside the domain (getStokesDV,
+ * src/lib_Stokes.f90:3020-3037). Triangle meshes. */
+int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, int32_t npts, const double *pts, const double *u,
+                              const double *q, double diff, double *vel, double *p, double *err);
+
 /* Device-resident bodies (no reference counterpart: the reference reads a mesh and solves it once
  * per run). For callers that compute the tensors of the same bodies more than once, e.g. at several
  * precisions: andr_bodies_upload copies the geometry to the device and runs the closure tests of

@@ -749,6 +749,85 @@ void orc_assemble_stokes(const orc_body *b, int nrows, const int32_t *rows, doub
   free_geom(&g);
 }
 
+void orc_stokes_domain_values(const orc_body *b, int npts, const double *pts, const double *u, const double *q,
+                              double diff, double *vel, double *p, double *err) {
+  ensure_rule(b->itri);
+  geom_t g;
+  make_geom(b, &g);
+  const int nn = b->nnodes, ne = b->nelem;
+  const double osemPi = (double)(atanf(1.0f)) * 4.0 * 8.0;
+#pragma omp parallel
+  {
+    double *buf = (double *)malloc(sizeof(double) * (size_t)(9 * nn + 9 * ne));
+    double *rT[6], *rG[6], *rprT[3], *rprG[3];
+    for (int k = 0; k < 6; ++k) { rT[k] = buf + k * nn; rG[k] = buf + 9 * nn + k * ne; }
+    for (int k = 0; k < 3; ++k) { rprT[k] = buf + (6 + k) * nn; rprG[k] = buf + 9 * nn + (6 + k) * ne; }
+#pragma omp for schedule(dynamic, 2)
+    for (int ip = 0; ip < npts; ++ip) {
+      memset(buf, 0, sizeof(double) * (size_t)(9 * nn + 9 * ne));
+      const double *s = pts + 3 * (size_t)ip;
+      /* walls of the subdomain, elements of each wall (:3123-3216) */
+      for (int wall = 1; wall <= b->nwalls; ++wall) {
+        const double mul = b->normmul[wall - 1];
+        for (int e = 0; e < ne; ++e) {
+          if (b->bcid[e] != wall) continue;
+          const int32_t *cn = b->con + 3 * (size_t)e;
+          const double n[3] = {mul * g.normal[3 * e], mul * g.normal[3 * e + 1], mul * g.normal[3 * e + 2]};
+          double acc[36];
+          memset(acc, 0, sizeof(acc));
+          orc_tri_stokes_int(b->xyz + 3 * cn[0], b->xyz + 3 * cn[1], b->xyz + 3 * cn[2], s, n, g.area[e], acc);
+          for (int v = 0; v < 3; ++v) {
+            for (int k = 0; k < 6; ++k) rT[k][cn[v]] += acc[9 + 3 * k + v];
+            for (int k = 0; k < 3; ++k) rprT[k][cn[v]] += acc[27 + 3 * k + v];
+          }
+          for (int k = 0; k < 6; ++k) rG[k][e] = acc[k];
+          for (int k = 0; k < 3; ++k) rprG[k][e] = acc[6 + k];
+        }
+      }
+      /* closure checks (:3255-3300) */
+      double er = -9.99, cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int i = 0; i < nn; ++i) { cx -= rT[0][i]; cy -= rT[3][i]; cz -= rT[5][i]; }
+      er = fmax(er, fabs(cx / osemPi - 1.0));
+      er = fmax(er, fabs(cy / osemPi - 1.0));
+      er = fmax(er, fabs(cz / osemPi - 1.0));
+      cx = cy = cz = 0.0;
+      for (int i = 0; i < ne; ++i) {
+        const double multi = b->normmul[b->bcid[i] - 1];
+        const double *nr = g.normal + 3 * (size_t)i;
+        cx = cx - rG[0][i] * nr[0] * multi - rG[1][i] * nr[1] * multi - rG[2][i] * nr[2] * multi;
+        cy = cy - rG[1][i] * nr[0] * multi - rG[3][i] * nr[1] * multi - rG[4][i] * nr[2] * multi;
+        cz = cz - rG[2][i] * nr[0] * multi - rG[4][i] * nr[1] * multi - rG[5][i] * nr[2] * multi;
+      }
+      er = fmax(er, fabs(cx));
+      er = fmax(er, fabs(cy));
+      er = fmax(er, fabs(cz));
+      /* velocity and pressure (:3337-3381) */
+      double rx = 0.0, ry = 0.0, rz = 0.0, pp = 0.0;
+      for (int i = 0; i < nn; ++i) {
+        const double ux = u[3 * i], uy = u[3 * i + 1], uz = u[3 * i + 2];
+        rx = rx + rT[0][i] * ux + rT[1][i] * uy + rT[2][i] * uz;
+        ry = ry + rT[1][i] * ux + rT[3][i] * uy + rT[4][i] * uz;
+        rz = rz + rT[2][i] * ux + rT[4][i] * uy + rT[5][i] * uz;
+        pp = pp + diff * (rprT[0][i] * ux + rprT[1][i] * uy + rprT[2][i] * uz);
+      }
+      for (int i = 0; i < ne; ++i) {
+        const double qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+        rx = rx + rG[0][i] * qx / diff + rG[1][i] * qy / diff + rG[2][i] * qz / diff;
+        ry = ry + rG[1][i] * qx / diff + rG[3][i] * qy / diff + rG[4][i] * qz / diff;
+        rz = rz + rG[2][i] * qx / diff + rG[4][i] * qy / diff + rG[5][i] * qz / diff;
+        pp = pp - (rprG[0][i] * qx + rprG[1][i] * qy + rprG[2][i] * qz);
+      }
+      vel[3 * (size_t)ip] = -rx / osemPi;
+      vel[3 * (size_t)ip + 1] = -ry / osemPi;
+      vel[3 * (size_t)ip + 2] = -rz / osemPi;
+      p[ip] = -pp / osemPi;
+      if (err) err[ip] = er;
+    }
+    free(buf);
+  }
+  free_geom(&g);
+}
+
 /* ------------------------------------------------------------------------------------------
  * LSQR (src/lib_lsqr.f)
  * ---------------------------------------------------------------------------------------- */

@@ -103,6 +103,15 @@ void orc_assemble_stokes(const orc_body *b, int nrows, const int32_t *rows, doub
                          double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]);
 
 /* LSQR, restating src/lib_lsqr.f. CSR with int64 row pointers, 0-based. */
+/* GetStokesDomainValue (src/lib_Stokes.f90:3043-3390) at npts points pts[3*npts] of the flow
+ * domain, triangle meshes: every element of every wall integrated with the mesh rule (isrc = 0,
+ * Triangle_3DStokesInt), the rows contracted with u[3*nnodes] (nodal velocity) and q[3*nelem]
+ * (element traction): vel[3*i..] = -(sum T u + sum G q / diff) / 8pi (:3337-3375),
+ * p[i] = -(diff sum prT.u - sum prG.q) / 8pi, err[i] = the closure checks (:3255-3300), 8pi from a
+ * REAL(4) ATAN(1.0) (:3089). */
+void orc_stokes_domain_values(const orc_body *b, int npts, const double *pts, const double *u, const double *q,
+                              double diff, double *vel, double *p, double *err);
+
 typedef struct { int64_t m, n, nnz; const int64_t *ia; const int32_t *ja; double *a; } orc_csr;
 /* lsqr_precon (src/lib_lsqr.f:131-167,267-329): q = 1/||col||, A <- A diag(q) in place. */
 void orc_lsqr_precon(orc_csr *A, double *q);

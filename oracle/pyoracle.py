@@ -172,6 +172,18 @@ def assemble_stokes(mesh, normmul=None, rows=None, extra=0.0, itri=5, pressure=F
     return T, G, prT, prG, err
 
 
+def stokes_domain_values(mesh, pts, u, q, diff=1.0, normmul=None, itri=5):
+    """Velocity (npts, 3), pressure (npts,) and closure error (npts,) at domain points."""
+    b = Body(mesh, normmul, itri)
+    P = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    U = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
+    Q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    vel = np.zeros((P.shape[0], 3)); p = np.zeros(P.shape[0]); err = np.zeros(P.shape[0])
+    lib().orc_stokes_domain_values(C.byref(b.s), P.shape[0], _p(P), _p(U), _p(Q), C.c_double(diff), _p(vel), _p(p),
+                                   _p(err))
+    return vel, p, err
+
+
 def csr_from_dense(A):
     A = np.ascontiguousarray(A, dtype=np.float64)
     m, n = A.shape
