@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH):
                                          C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), _D]
     lib.andr_stokes_domain_values.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, _D, _D, _D, C.c_double,
                                               _D, _D, _D]
+    lib.andr_stokes_surface_pressure.argtypes = [C.c_void_p, C.POINTER(_Mesh), _D, _D, C.c_double, _D]
     lib.andr_lsqr_dense.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _D, C.c_int32, _D, _D,
                                     C.POINTER(_LsqrOpts), C.POINTER(_LsqrStats)]
     lib.andr_resistance.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.POINTER(_ResOpts), _D, C.POINTER(_ResReport)]
@@ -114,7 +115,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
             "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
-            "andr_stokes_domain_values"]
+            "andr_stokes_domain_values", "andr_stokes_surface_pressure"]
 FAR_BINS = 16
 
 
@@ -263,6 +264,16 @@ class Context:
         self._check(self.lib.andr_stokes_domain_values(self.h, C.byref(mv.s), P.shape[0], _p(P), _p(U), _p(Q), diff,
                                                        _p(vel), _p(p), _p(err)), "andr_stokes_domain_values")
         return vel, p, err
+
+    def stokes_surface_pressure(self, mesh: Mesh, u, q, diff: float = 1.0, normmul=None, itri: int = 5):
+        """Pressure at the element centres (nelem,) from nodal velocities u and element tractions q."""
+        mv = _MeshView(mesh, normmul, itri)
+        U = np.ascontiguousarray(u, dtype=np.float64).reshape(mv.m.nnodes, 3)
+        Q = np.ascontiguousarray(q, dtype=np.float64).reshape(mv.m.nelem, 3)
+        p = np.zeros(mv.m.nelem)
+        self._check(self.lib.andr_stokes_surface_pressure(self.h, C.byref(mv.s), _p(U), _p(Q), diff, _p(p)),
+                    "andr_stokes_surface_pressure")
+        return p
 
     # ---- seam 2 ---------------------------------------------------------------------------------
     def lsqr_dense(self, A, B, X0=None, maxit: int = 0, eps: float = 1e-15, precon: bool = True):

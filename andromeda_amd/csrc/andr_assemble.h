@@ -43,6 +43,9 @@ int andr_launch_force_torque(andr_ctx *ctx, const SysJob *d_jobs, int njobs, dou
 // q[3 ne]; out[5 npts]); triangle meshes.
 int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double *pts, const double *u, const double *q,
                               double diff, double *out);
+// Surface pressure at the element centres (device arrays u[3 nn], q[3 ne], p[ne]); triangle meshes.
+int andr_launch_stokes_surface_pressure(andr_ctx *ctx, DevBody &b, const double *u, const double *q, double diff,
+                                        double *p);
 // algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0

@@ -784,6 +784,70 @@ __global__ __launch_bounds__(256) void k_stokes_domain(BodyGeom g, int qoff, int
   }
 }
 
+// Surface pressure at the element centres: pressureStokes (src/lib_Stokes.f90:4-94) with the centroid
+// rows of prT and prG (sdIntRowQStokes, :1562-1897). One workgroup per element r; the pair (r, r)
+// is singular and has no pressure integrals. The row-sum correction of prT (:1851-1863) moves the
+// row sum onto the three nodes of element r; with u_c = the mean of those nodes' velocities
+// (InterpolateToQMesh) the row contracts to sum prT u - rowsum . u_c, so it is not formed:
+// p_r = -2 (diff (sum prT u - rowsum . u_c) - sum prG q) / 8 pi.
+__global__ __launch_bounds__(256) void k_stokes_surface_pressure(BodyGeom g, int qoff, int nq,
+                                                                 const double *__restrict__ u,
+                                                                 const double *__restrict__ q, double diff,
+                                                                 double osem, double *__restrict__ p) {
+  constexpr int NV = 5;  // prT u, prT row sums (3), prG q
+  __shared__ double red[NV][4];
+  const int r = blockIdx.x;
+  const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
+  double a[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < g.ne; e += blockDim.x) {
+    if (e == r) continue;
+    const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
+    const double m = g.mul[e];
+    double Gs[6], pG[3], Tv[6][3], pT[3][3];
+    stokes_tri_full(s, g.xyz + 3 * vv[0], g.xyz + 3 * vv[1], g.xyz + 3 * vv[2], m * g.nrm[3 * e],
+                    m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2], g.area[e], qoff, nq, Gs, pG, Tv, pT);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const double *uv = u + 3 * vv[v];
+      a[0] += pT[0][v] * uv[0] + pT[1][v] * uv[1] + pT[2][v] * uv[2];
+      a[1] += pT[0][v];
+      a[2] += pT[1][v];
+      a[3] += pT[2][v];
+    }
+    a[4] += pG[0] * q[3 * e] + pG[1] * q[3 * e + 1] + pG[2] * q[3 * e + 2];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double v = a[i];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[i][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
+    const int c0 = g.con[3 * r], c1 = g.con[3 * r + 1], c2 = g.con[3 * r + 2];
+    double pu = t[0];
+    for (int k = 0; k < 3; ++k) pu -= t[1 + k] * ((u[3 * c0 + k] + u[3 * c1 + k] + u[3 * c2 + k]) / 3.0);
+    p[r] = -(diff * pu - t[4]) / osem * 2.0;
+  }
+}
+
+int andr_launch_stokes_surface_pressure(andr_ctx *ctx, DevBody &b, const double *u, const double *q, double diff,
+                                        double *p) {
+  const QuadRule qr = andr_rule(b.itri);
+  const double osem = (double)atanf(1.0f) * 4.0 * 8.0;  // REAL(4) ATAN(1.0) (src/lib_Stokes.f90:17)
+  const double evals = (double)b.g.ne * (b.g.ne - 1.0) * qr.npts;
+  const int tok = andr_prof_begin(ctx, "stokes_surface_pressure", 8.0 * (3.0 * b.g.nn + 4.0 * b.g.ne), evals * 175.0);
+  hipLaunchKernelGGL(k_stokes_surface_pressure, dim3(b.g.ne), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts, u,
+                     q, diff, osem, p);
+  andr_prof_end(ctx, tok);
+  ANDR_HIP(ctx, hipGetLastError());
+  return ANDR_OK;
+}
+
 int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double *pts, const double *u, const double *q,
                               double diff, double *out) {
   if (npts <= 0) return ANDR_OK;

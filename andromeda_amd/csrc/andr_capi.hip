@@ -222,6 +222,27 @@ extern "C" int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, i
   return ANDR_OK;
 }
 
+extern "C" int andr_stokes_surface_pressure(andr_ctx *ctx, const andr_mesh *mesh, const double *u, const double *q,
+                                            double diff, double *p) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (!mesh || !u || !q || !p || !(diff > 0.0)) return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_surface_pressure arguments");
+  if (mesh->nvert == 4) return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_surface_pressure takes triangle meshes");
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  int rc = andr_body_upload(ctx, mesh, b, false);
+  if (rc) return rc;
+  const int nn = b.g.nn, ne = b.g.ne;
+  DevBuf<double> du, dq, dp;
+  if (du.alloc(3 * (size_t)nn, ctx) || dq.alloc(3 * (size_t)ne, ctx) || dp.alloc(ne, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "surface pressure");
+  ANDR_HIP(ctx, hipMemcpyAsync(du.p, u, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, ctx->stream));
+  ANDR_HIP(ctx, hipMemcpyAsync(dq.p, q, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = andr_launch_stokes_surface_pressure(ctx, b, du.p, dq.p, diff, dp.p))) return rc;
+  ANDR_HIP(ctx, hipMemcpyAsync(p, dp.p, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ANDR_OK;
+}
+
 // ---- seam 2 ----------------------------------------------------------------------------------
 __global__ void k_csr_to_dense(int m, int64_t ld, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
                                const double *__restrict__ a, double *__restrict__ D) {

@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "learned", "auto"], default="auto")
     ap.add_argument("--model", default=os.path.join(ROOT, "andromeda_amd", "quad_model.npz"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--domain-only", type=int, default=0, metavar="NPTS",
+                    help="only the domain evaluation: velocity and pressure at NPTS points inside a "
+                         "20480-triangle sphere (the kernel's trace and roofline)")
     ap.add_argument("--plain-only", action="store_true",
                     help="only the plain FP64 path on the sample, one body at a time with per-kernel timing "
                          "(the workload of the plain path's kernel traces and counters)")
@@ -159,6 +162,28 @@ def main():
 
     from andromeda_amd import api
     from andromeda_amd.metrics import tensor_error
+
+    if args.domain_only:
+        from andromeda_amd import mesh as M
+        m = M.icosphere(5)
+        rng = np.random.default_rng(0)
+        d = rng.normal(size=(args.domain_only, 3))
+        pts = 0.8 * d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0, 1, size=(args.domain_only, 1)) ** (1 / 3)
+        u, q = rng.normal(size=(m.nnodes, 3)), rng.normal(size=(m.nelem, 3))
+        ctx = api.Context(0)
+        ctx.stokes_domain_values(m, pts[:64], u, q)  # warm-up
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        ctx.stokes_domain_values(m, pts, u, q)
+        el = time.perf_counter() - t0
+        kernels, roofline = kernel_table(ctx.profile_read())
+        ctx.profile(False)
+        log(f"[bench] domain values: {args.domain_only} points x {m.nelem} triangles in {el:.3f} s "
+            f"({args.domain_only / el:.0f} points/s); {kernels}")
+        print(json.dumps({"domain_values": {"points": args.domain_only, "triangles": m.nelem,
+                                            "points_per_s": round(args.domain_only / el, 1),
+                                            "roofline": roofline, "kernels": kernels}}), flush=True)
+        return
 
     n_fibre = args.bodies // 9
     n_super = args.bodies - n_fibre

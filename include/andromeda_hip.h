@@ -156,6 +156,14 @@ int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshe
 int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, int32_t npts, const double *pts, const double *u,
                               const double *q, double diff, double *vel, double *p, double *err);
 
+/* Replaces pressureStokes (src/lib_Stokes.f90:4-94): the surface pressure at the element centres,
+ * p[nelem], from the centroid rows of the pressure integrals (prT with its row-sum correction, prG)
+ * and the boundary solution u[3*nnodes], q[3*nelem], viscosity diff:
+ * p = -2 (diff prT.(u - u_c) - prG.q) / 8 pi, u_c the mean velocity of the element's nodes.
+ * Triangle meshes. */
+int andr_stokes_surface_pressure(andr_ctx *ctx, const andr_mesh *mesh, const double *u, const double *q, double diff,
+                                 double *p);
+
 /* Device-resident bodies (no reference counterpart: the reference reads a mesh and solves it once
  * per run). For callers that compute the tensors of the same bodies more than once, e.g. at several
  * precisions: andr_bodies_upload copies the geometry to the device and runs the closure tests of

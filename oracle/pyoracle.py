@@ -184,6 +184,25 @@ def stokes_domain_values(mesh, pts, u, q, diff=1.0, normmul=None, itri=5):
     return vel, p, err
 
 
+def stokes_surface_pressure(mesh, u, q, diff=1.0, normmul=None, itri=5):
+    """pressureStokes (src/lib_Stokes.f90:4-94): pressure at the element centres from the centroid
+    rows of prT (with the row-sum correction) and prG, p = -2 (diff prT.(u - u_c) - prG.q) / 8pi,
+    u_c the velocity interpolated to the centre (InterpolateToQMesh, src/lib_mesh.f90:635-690:
+    the mean of the three nodes), 8pi from a REAL(4) ATAN(1.0)."""
+    m = mesh.contiguous()
+    nn, ne = m.nnodes, m.nelem
+    _, _, prT, prG, _ = assemble_stokes(m, normmul, rows=np.arange(nn, nn + ne), itri=itri, pressure=True)
+    U = np.asarray(u, dtype=np.float64).reshape(nn, 3)
+    Q = np.asarray(q, dtype=np.float64).reshape(ne, 3)
+    uc = U[m.con].mean(axis=1)
+    osem = float(np.arctan(np.float32(1.0))) * 4.0 * 8.0
+    p = np.zeros(ne)
+    for k in range(3):
+        p += diff * (prT[k] @ U[:, k] - prT[k].sum(axis=1) * uc[:, k])
+        p -= prG[k] @ Q[:, k]
+    return -p / osem * 2.0
+
+
 def csr_from_dense(A):
     A = np.ascontiguousarray(A, dtype=np.float64)
     m, n = A.shape

@@ -1,9 +1,10 @@
 """Velocity and pressure at points of the flow domain (GetStokesDomainValue,
-src/lib_Stokes.f90:3043-3390). CPU: the oracle pinned by exact Stokes flows inside a sphere — a
+src/lib_Stokes.f90:3043-3390) and on the surface (pressureStokes, :4-94). CPU: the oracle pinned by exact Stokes flows inside a sphere — a
 uniform flow (reproduced up to the reference's single-precision 8 pi) and u = (x^2, -2xy, 0),
 p = 2 mu x (second-order convergence of velocity and pressure under refinement), with the
-closure error telling points inside from points outside. GPU: the HIP kernel against the oracle
-on the same inputs."""
+closure error telling points inside from points outside; the surface pressure is zero for a
+uniform flow and converges at first order to 2 mu x for the quadratic one. GPU: the HIP kernels
+against the oracle on the same inputs."""
 import numpy as np
 import pytest
 
@@ -85,3 +86,29 @@ def test_gpu_domain_quadratic_flow(ctx, orc):
     u, q, ex, pe = _quadratic_flow(orc, m)
     vel, p, err = ctx.stokes_domain_values(m, INSIDE, u, q)
     assert np.abs(vel - ex).max() < 1e-3 and np.abs(p - pe).max() < 1e-3 and err.max() < 1e-7
+
+
+def test_oracle_surface_pressure_pins(orc):
+    U = np.array([1.0, -2.0, 0.5])
+    rms = []
+    for sub in (2, 3):
+        m = M.icosphere(sub)
+        assert np.abs(orc.stokes_surface_pressure(m, np.tile(U, (m.nnodes, 1)), np.zeros((m.nelem, 3)))).max() < 1e-12
+        u, q, _, _ = _quadratic_flow(orc, m)
+        _, _, cen = orc.geometry(m)
+        p = orc.stokes_surface_pressure(m, u, q)
+        rms.append(np.sqrt(np.mean((p - 2 * cen[:, 0]) ** 2)))
+    # first order in the element size (the flat-element factor 2 of the reference)
+    assert rms[0] / rms[1] > 1.8 and rms[1] < 0.035, rms
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["sphere", "superellipsoid"])
+def test_gpu_surface_pressure_vs_oracle(ctx, orc, kind):
+    rng = np.random.default_rng(6)
+    m = M.icosphere(3) if kind == "sphere" else M.random_superellipsoid(np.random.default_rng(9), 6)[0]
+    u = rng.normal(size=(m.nnodes, 3))
+    q = rng.normal(size=(m.nelem, 3))
+    p = ctx.stokes_surface_pressure(m, u, q, diff=0.8)
+    po = orc.stokes_surface_pressure(m, u, q, diff=0.8)
+    assert np.abs(p - po).max() < 1e-10 * np.abs(po).max()
