@@ -79,6 +79,11 @@ def load_library(path: str = LIB_PATH):
                                          C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), _D]
     lib.andr_stokes_domain_values.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, _D, _D, _D, C.c_double,
                                               _D, _D, _D]
+    lib.andr_integrals_write_stokes.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, C.POINTER(_D),
+                                                C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]
+    lib.andr_integrals_read_stokes.argtypes = lib.andr_integrals_write_stokes.argtypes
+    lib.andr_integrals_write_laplace.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]
+    lib.andr_integrals_read_laplace.argtypes = lib.andr_integrals_write_laplace.argtypes
     lib.andr_stokes_surface_pressure.argtypes = [C.c_void_p, C.POINTER(_Mesh), _D, _D, C.c_double, _D]
     lib.andr_lsqr_dense.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _D, C.c_int32, _D, _D,
                                     C.POINTER(_LsqrOpts), C.POINTER(_LsqrStats)]
@@ -115,7 +120,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
             "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
-            "andr_stokes_domain_values", "andr_stokes_surface_pressure"]
+            "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
+            "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace"]
 FAR_BINS = 16
 
 
@@ -154,6 +160,56 @@ class ResistanceResult:
     quad_points: int
     pairs_learned: int
     pairs_fallback: int
+
+
+def _rows(mats):
+    return [np.ascontiguousarray(a, dtype=np.float64) for a in mats]
+
+
+def integrals_write_stokes(path: str, T, G, prT, prG, err=(0.0, 0.0)):
+    """The reference's integrals file (Stokes) from row-major nsp x nnodes / nsp x nqnodes matrices,
+    T and G in the order xx, xy, xz, yy, yz, zz as Context.assemble_stokes returns them."""
+    T, G, prT, prG = _rows(T), _rows(G), _rows(prT), _rows(prG)
+    nsp, nn = T[0].shape
+    nq = G[0].shape[1]
+    e = np.asarray(err, dtype=np.float64)
+    rc = load_library().andr_integrals_write_stokes(
+        path.encode(), nsp, nn, nq, _p(e), (_D * 6)(*[_p(a) for a in T]), (_D * 6)(*[_p(a) for a in G]),
+        (_D * 3)(*[_p(a) for a in prT]), (_D * 3)(*[_p(a) for a in prG]))
+    if rc != 0:
+        raise AndromedaError(f"andr_integrals_write_stokes failed ({rc}): {path}")
+
+
+def integrals_read_stokes(path: str, nsp: int, nnodes: int, nqnodes: int):
+    """(T, G, prT, prG, err) from the reference's integrals file (Stokes); sizes must match."""
+    T = [np.zeros((nsp, nnodes)) for _ in range(6)]
+    G = [np.zeros((nsp, nqnodes)) for _ in range(6)]
+    prT = [np.zeros((nsp, nnodes)) for _ in range(3)]
+    prG = [np.zeros((nsp, nqnodes)) for _ in range(3)]
+    e = np.zeros(2)
+    rc = load_library().andr_integrals_read_stokes(
+        path.encode(), nsp, nnodes, nqnodes, _p(e), (_D * 6)(*[_p(a) for a in T]), (_D * 6)(*[_p(a) for a in G]),
+        (_D * 3)(*[_p(a) for a in prT]), (_D * 3)(*[_p(a) for a in prG]))
+    if rc != 0:
+        raise AndromedaError(f"andr_integrals_read_stokes failed ({rc}): {path}")
+    return T, G, prT, prG, e
+
+
+def integrals_write_laplace(path: str, H, G, err=(0.0, 0.0)):
+    H, G = _rows([H, G])
+    e = np.asarray(err, dtype=np.float64)
+    rc = load_library().andr_integrals_write_laplace(path.encode(), H.shape[0], H.shape[1], G.shape[1], _p(e), _p(H),
+                                                     _p(G))
+    if rc != 0:
+        raise AndromedaError(f"andr_integrals_write_laplace failed ({rc}): {path}")
+
+
+def integrals_read_laplace(path: str, nsp: int, nnodes: int, nqnodes: int):
+    H, G, e = np.zeros((nsp, nnodes)), np.zeros((nsp, nqnodes)), np.zeros(2)
+    rc = load_library().andr_integrals_read_laplace(path.encode(), nsp, nnodes, nqnodes, _p(e), _p(H), _p(G))
+    if rc != 0:
+        raise AndromedaError(f"andr_integrals_read_laplace failed ({rc}): {path}")
+    return H, G, e
 
 
 def _results(FT, reps):

@@ -164,6 +164,27 @@ int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, int32_t npts
 int andr_stokes_surface_pressure(andr_ctx *ctx, const andr_mesh *mesh, const double *u, const double *q, double diff,
                                  double *p);
 
+/* ---- integrals file (and.integrals.bin) -----------------------------------------------------
+ * Replace the reference's writers and readers of its integrals file for one subdomain, so that
+ * matrices assembled here are consumed by the unmodified host and the other way round:
+ * WriteIntegralsToDiskStokes / ReadIntegralsFileStokes (src/lib_Stokes.f90:2742-2872) and
+ * WriteIntegralsToDisk / ReadIntegralsFile (src/lib_int.f90:136-221). Fortran sequential unformatted
+ * records with 4-byte length markers: (Stokes only) the id 1407; (nsp, nnodes, nqnodes); the two
+ * self-check errors; then every matrix column by column, each column as records of 8192 values and a
+ * last record with the rest (WrSMat / wrvec, src/lib_util.f90:201-287). Stokes matrices are passed
+ * as andr_assemble_stokes returns them in ANDR_ROW_MAJOR layout: T and G in the order xx, xy, xz, yy,
+ * yz, zz (the file holds xx, yy, zz, xy, xz, yz), prT and prG x, y, z. A reader checks the id and the
+ * sizes as the reference does and returns ANDR_ERR_ARG on a mismatch or a short file; a NULL matrix
+ * pointer skips that matrix. Host code: no context, no device. */
+int andr_integrals_write_stokes(const char *path, int32_t nsp, int32_t nnodes, int32_t nqnodes, const double err[2],
+                                double *const T[6], double *const G[6], double *const prT[3], double *const prG[3]);
+int andr_integrals_read_stokes(const char *path, int32_t nsp, int32_t nnodes, int32_t nqnodes, double err[2],
+                               double *T[6], double *G[6], double *prT[3], double *prG[3]);
+int andr_integrals_write_laplace(const char *path, int32_t nsp, int32_t nnodes, int32_t nqnodes, const double err[2],
+                                 const double *H, const double *G);
+int andr_integrals_read_laplace(const char *path, int32_t nsp, int32_t nnodes, int32_t nqnodes, double err[2],
+                                double *H, double *G);
+
 /* Device-resident bodies (no reference counterpart: the reference reads a mesh and solves it once
  * per run). For callers that compute the tensors of the same bodies more than once, e.g. at several
  * precisions: andr_bodies_upload copies the geometry to the device and runs the closure tests of
