@@ -84,6 +84,8 @@ def load_library(path: str = LIB_PATH):
     lib.andr_integrals_read_stokes.argtypes = lib.andr_integrals_write_stokes.argtypes
     lib.andr_integrals_write_laplace.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]
     lib.andr_integrals_read_laplace.argtypes = lib.andr_integrals_write_laplace.argtypes
+    lib.andr_bicgstab_dense.argtypes = [C.c_void_p, C.c_int64, _D, C.c_int32, _D, _D, C.c_double, C.c_int32,
+                                        C.POINTER(_LsqrStats)]
     lib.andr_stokes_surface_pressure.argtypes = [C.c_void_p, C.POINTER(_Mesh), _D, _D, C.c_double, _D]
     lib.andr_lsqr_dense.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _D, C.c_int32, _D, _D,
                                     C.POINTER(_LsqrOpts), C.POINTER(_LsqrStats)]
@@ -121,7 +123,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
-            "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace"]
+            "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
+            "andr_bicgstab_dense"]
 FAR_BINS = 16
 
 
@@ -346,6 +349,21 @@ class Context:
         self._check(self.lib.andr_lsqr_dense(self.h, m, n, _p(A), nrhs, _p(Bc), _p(Xc), C.byref(opts), C.byref(st)),
                     "andr_lsqr_dense")
         return Xc.T.copy(), {"nit": list(st.nit)[:nrhs], "istop": list(st.istop)[:nrhs], "seconds": st.seconds}
+
+    def bicgstab_dense(self, A, B, X0=None, tol: float = 1e-8, maxit: int = 1000):
+        """Jacobi-preconditioned BiCGSTAB on a dense square system (the LIS path's default solver)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        n = A.shape[0]
+        B = np.asarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)
+        Xc = np.zeros((nrhs, n)) if X0 is None else np.ascontiguousarray(np.asarray(X0, dtype=np.float64).reshape(n, nrhs).T)
+        st = _LsqrStats()
+        self._check(self.lib.andr_bicgstab_dense(self.h, n, _p(A), nrhs, _p(Bc), _p(Xc), tol, maxit, C.byref(st)),
+                    "andr_bicgstab_dense")
+        return Xc.T.copy(), {"nit": list(st.nit)[:nrhs], "status": list(st.istop)[:nrhs], "seconds": st.seconds}
 
     # ---- seam 3 ---------------------------------------------------------------------------------
     def resistance_batch(self, meshes, form: int = FORM_EXTERIOR, precision: int = PREC_FP64, mu: float = 1.0,

@@ -407,6 +407,47 @@ extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *n
   *ierr = is > 3 ? is : 0;
 }
 
+extern "C" int andr_bicgstab_dense(andr_ctx *ctx, int64_t n, const double *A, int32_t nrhs, const double *B, double *X,
+                                   double tol, int32_t maxit, andr_lsqr_stats *stats) {
+  if (!ctx || !A || !B || !X || n <= 0 || nrhs <= 0 || nrhs > 16)
+    return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_bicgstab_dense arguments") : ANDR_ERR_ARG;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  const double t0 = andr_now();
+  const int64_t lda = (n + 15) / 16 * 16;  // rows start on 128-byte lines
+  DevBuf<double> dA, dB, dX;
+  if (dA.alloc((size_t)n * lda, ctx) || dB.alloc((size_t)n * ANDR_NB, ctx) || dX.alloc((size_t)n * ANDR_NB, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "andr_bicgstab_dense workspace");
+  ANDR_HIP(ctx, hipMemcpy2DAsync(dA.p, sizeof(double) * lda, A, sizeof(double) * n, sizeof(double) * n, n,
+                                 hipMemcpyHostToDevice, ctx->stream));
+  std::vector<double> hb((size_t)n * ANDR_NB), hx((size_t)n * ANDR_NB);
+  for (int g0 = 0; g0 < nrhs; g0 += ANDR_NB) {
+    const int nr = std::min(ANDR_NB, nrhs - g0);
+    std::fill(hb.begin(), hb.end(), 0.0);
+    std::fill(hx.begin(), hx.end(), 0.0);
+    for (int k = 0; k < nr; ++k)
+      for (int64_t i = 0; i < n; ++i) {
+        hb[i * ANDR_NB + k] = B[(g0 + k) * n + i];
+        hx[i * ANDR_NB + k] = X[(g0 + k) * n + i];
+      }
+    ANDR_HIP(ctx, hipMemcpyAsync(dB.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, ctx->stream));
+    ANDR_HIP(ctx, hipMemcpyAsync(dX.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, ctx->stream));
+    int it[ANDR_NB], is[ANDR_NB];
+    int rc = andr_bicgstab_device(ctx, dA.p, lda, (int)n, dB.p, dX.p, nr, maxit, tol > 0.0 ? tol : 1e-8, it, is);
+    if (rc) return rc;
+    ANDR_HIP(ctx, hipMemcpyAsync(hx.data(), dX.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, ctx->stream));
+    ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nr; ++k) {
+      for (int64_t i = 0; i < n; ++i) X[(g0 + k) * n + i] = hx[i * ANDR_NB + k];
+      if (stats) {
+        stats->nit[g0 + k] = it[k];
+        stats->istop[g0 + k] = is[k];
+      }
+    }
+  }
+  if (stats) stats->seconds = andr_now() - t0;
+  return ANDR_OK;
+}
+
 extern "C" int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double *A, int32_t nrhs, const double *B,
                                double *X, const andr_lsqr_opts *opts, andr_lsqr_stats *stats) {
   if (!ctx || !A || !B || !X || m <= 0 || n <= 0 || nrhs <= 0 || nrhs > 16) return ctx ? andr_fail(ctx, ANDR_ERR_ARG, "andr_lsqr_dense arguments") : ANDR_ERR_ARG;

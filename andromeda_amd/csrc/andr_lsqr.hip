@@ -905,3 +905,300 @@ int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs) {
     if (s) (void)hipStreamDestroy(s);
   return rc;
 }
+
+// ---- BiCGSTAB with a Jacobi preconditioner ---------------------------------------------------
+// The Krylov alternative of the reference's LIS path for square systems (parLISslvSet
+// "-i bicgstab -p jacobi -tol 1.0e-8 -maxiter 1000", src/mod_par.f90:288). LIS is not part of the
+// reference's sources; the algorithm is the published preconditioned BiCGSTAB (van der Vorst 1992,
+// in the form of Barrett et al., Templates, 1994, fig. 2.10): the preconditioner M = diag(A) acts
+// on the search directions, the residual is the true one, and a column stops when
+// ||r||_2 <= tol ||b||_2 (LIS's default criterion). NB columns advance together, each with its own
+// scalars; the products are the matrix-core A X above.
+namespace {
+
+struct BicgState {
+  double rho, alpha, omega, bnorm2, tol2;
+  int active, itn, status;  // status: 0 converged, 1 iteration limit, 2 breakdown
+};
+
+// per-block partial sums of X.Y (and of Z.W when Z is given) over n x NB column-interleaved arrays
+__global__ __launch_bounds__(256) void k_bicg_dots(int n, const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const double *__restrict__ W,
+                                                   double *__restrict__ p1, double *__restrict__ p2) {
+  __shared__ double red[2][4 * NB];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (t < n * NB) {
+    a = X[t] * Y[t];
+    if (Z) b = Z[t] * W[t];
+  }
+#pragma unroll
+  for (int off = 8; off < 64; off <<= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < NB) {
+    red[0][wid * NB + lane] = a;
+    red[1][wid * NB + lane] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) {
+    const int k = threadIdx.x;
+    p1[(size_t)blockIdx.x * NB + k] = (red[0][k] + red[0][NB + k]) + (red[0][2 * NB + k] + red[0][3 * NB + k]);
+    if (Z) p2[(size_t)blockIdx.x * NB + k] = (red[1][k] + red[1][NB + k]) + (red[1][2 * NB + k] + red[1][3 * NB + k]);
+  }
+}
+
+// Jacobi: dinv[j] = 1 / A[j][j] (1 where the diagonal is zero, as a Jacobi preconditioner leaves it)
+__global__ void k_bicg_diag(const double *__restrict__ A, int64_t lda, int n, double *__restrict__ dinv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double d = A[(size_t)j * lda + j];
+  dinv[j] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// r = b - A x from Y = A x; r_hat = r; p = v = 0
+__global__ void k_bicg_init(int n, const double *__restrict__ B, const double *__restrict__ AX, double *__restrict__ R,
+                            double *__restrict__ Rh, double *__restrict__ P, double *__restrict__ V) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  const double r = B[t] - AX[t];
+  R[t] = r;
+  Rh[t] = r;
+  P[t] = 0.0;
+  V[t] = 0.0;
+}
+
+// start: ||b||^2 and ||r0||^2; columns with r0 already small are done
+__global__ void k_bicg_start(BicgState *st, const double *pb, const double *pr, int np, double tol, int nrhs,
+                             double *sc) {
+  __shared__ double red[4 * NB];
+  const double b2 = block_sum_parts(pb, np, red);
+  const double r2 = block_sum_parts(pr, np, red);
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  BicgState &S = st[k];
+  S.rho = S.alpha = S.omega = 1.0;
+  S.bnorm2 = b2;
+  S.tol2 = tol * tol * b2;
+  S.itn = 0;
+  S.status = 0;
+  S.active = k < nrhs && r2 > S.tol2 ? 1 : 0;
+  for (int i = 0; i < 6; ++i) sc[i * NB + k] = 0.0;
+}
+
+// scalar slots sc[i * NB + k]: 0 beta, 1 omega (for p), 2 alpha, 3 omega (for x, r)
+// rho_new = r_hat . r; beta = (rho_new / rho) (alpha / omega)
+__global__ void k_bicg_beta(BicgState *st, const double *part, int np, double *sc) {
+  __shared__ double red[4 * NB];
+  const double rho = block_sum_parts(part, np, red);
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  BicgState &S = st[k];
+  if (!S.active) {
+    sc[k] = 0.0;
+    sc[NB + k] = 0.0;
+    return;
+  }
+  if (rho == 0.0) {  // breakdown: r_hat orthogonal to r
+    S.active = 0;
+    S.status = 2;
+    sc[k] = sc[NB + k] = 0.0;
+    return;
+  }
+  S.itn += 1;
+  const double beta = S.itn == 1 ? 0.0 : (rho / S.rho) * (S.alpha / S.omega);
+  S.rho = rho;
+  sc[k] = beta;
+  sc[NB + k] = S.omega;
+}
+
+// p = r + beta (p - omega v); y = dinv .* p
+__global__ void k_bicg_p(int n, const double *__restrict__ R, const double *__restrict__ V, double *__restrict__ P,
+                         double *__restrict__ Yv, const double *__restrict__ dinv, const double *__restrict__ sc,
+                         const BicgState *__restrict__ st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  const int k = t & 7;
+  if (!st[k].active) {
+    Yv[t] = 0.0;
+    return;
+  }
+  const double p = R[t] + sc[k] * (P[t] - sc[NB + k] * V[t]);
+  P[t] = p;
+  Yv[t] = dinv[t >> 3] * p;
+}
+
+// alpha = rho / (r_hat . v)
+__global__ void k_bicg_alpha(BicgState *st, const double *part, int np, double *sc) {
+  __shared__ double red[4 * NB];
+  const double rv = block_sum_parts(part, np, red);
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  BicgState &S = st[k];
+  if (!S.active || rv == 0.0) {
+    if (S.active) {
+      S.active = 0;
+      S.status = 2;
+    }
+    sc[2 * NB + k] = 0.0;
+    return;
+  }
+  S.alpha = S.rho / rv;
+  sc[2 * NB + k] = S.alpha;
+}
+
+// x += alpha y; s = r - alpha v (into R); z = dinv .* s
+__global__ void k_bicg_s(int n, double *__restrict__ X, const double *__restrict__ Yv, double *__restrict__ R,
+                         const double *__restrict__ V, double *__restrict__ Z, const double *__restrict__ dinv,
+                         const double *__restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  const int k = t & 7;
+  const double a = sc[2 * NB + k];
+  X[t] += a * Yv[t];
+  const double s = R[t] - a * V[t];
+  R[t] = s;
+  Z[t] = dinv[t >> 3] * s;
+}
+
+// omega = (t . s) / (t . t)
+__global__ void k_bicg_omega(BicgState *st, const double *pts, const double *ptt, int np, double *sc) {
+  __shared__ double red[4 * NB];
+  const double ts = block_sum_parts(pts, np, red);
+  const double tt = block_sum_parts(ptt, np, red);
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  BicgState &S = st[k];
+  if (!S.active || tt == 0.0) {
+    sc[3 * NB + k] = 0.0;
+    return;
+  }
+  S.omega = ts / tt;
+  sc[3 * NB + k] = S.omega;
+}
+
+// x += omega z; r = s - omega t
+__global__ void k_bicg_x(int n, double *__restrict__ X, const double *__restrict__ Z, double *__restrict__ R,
+                         const double *__restrict__ Tv, const double *__restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * NB) return;
+  const int k = t & 7;
+  const double w = sc[3 * NB + k];
+  X[t] += w * Z[t];
+  R[t] -= w * Tv[t];
+}
+
+// convergence: ||r||^2 <= tol^2 ||b||^2; omega = 0 (with r not small) is a breakdown
+__global__ void k_bicg_check(BicgState *st, const double *part, int np, int itnlim) {
+  __shared__ double red[4 * NB];
+  const double r2 = block_sum_parts(part, np, red);
+  const int k = threadIdx.x;
+  if (k >= NB) return;
+  BicgState &S = st[k];
+  if (!S.active) return;
+  if (r2 <= S.tol2) {
+    S.active = 0;
+    S.status = 0;
+  } else if (S.omega == 0.0) {
+    S.active = 0;
+    S.status = 2;
+  } else if (S.itn >= itnlim) {
+    S.active = 0;
+    S.status = 1;
+  }
+}
+
+}  // namespace
+
+int andr_bicgstab_device(andr_ctx *ctx, const double *A, int64_t lda, int n, const double *B, double *X, int nrhs,
+                         int maxit, double tol, int *nit, int *status) {
+  if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
+  if (n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
+  hipStream_t st = ctx->stream;
+  // the matrix-core product of the LSQR run object, with its own workspaces (q unused)
+  LsqrRun R;
+  R.ctx = ctx;
+  R.st = st;
+  R.A = A;
+  R.fp32 = false;
+  R.lda = lda;
+  R.m = R.n = n;
+  R.seglenN = 1024;
+  {
+    const int64_t rw = (n + 63) / 64;
+    if (rw * ((n + 1023) / 1024) < 2048) R.seglenN = (int)std::max<int64_t>(256, ((int64_t)n * rw / 2048 + 15) / 16 * 16);
+  }
+  R.nsegN = (n + R.seglenN - 1) / R.seglenN;
+  R.tilesN = 4;
+  R.vecN = lda % 2 == 0 && ((uintptr_t)A % 16) == 0;
+  R.blocksM = (n * NB + 255) / 256;
+  R.partsN = R.nsegN == 1 ? (n + 63) / 64 : R.blocksM;
+  const int nb = R.blocksM;
+  DevBuf<double> Rr, Rh, P, V, Yv, Z, Tt, dinv, p1, p2, one, zero, sc;
+  DevBuf<BicgState> S;
+  if (Rr.alloc((size_t)n * NB, ctx) || Rh.alloc((size_t)n * NB, ctx) || P.alloc((size_t)n * NB, ctx) ||
+      V.alloc((size_t)n * NB, ctx) || Yv.alloc((size_t)n * NB, ctx) || Z.alloc((size_t)n * NB, ctx) ||
+      Tt.alloc((size_t)n * NB, ctx) || dinv.alloc(n, ctx) || p1.alloc((size_t)std::max(nb, R.partsN) * NB, ctx) ||
+      p2.alloc((size_t)nb * NB, ctx) || one.alloc(NB, ctx) || zero.alloc(NB, ctx) || sc.alloc(6 * NB, ctx) ||
+      S.alloc(NB, ctx) || R.PN.alloc(R.nsegN > 1 ? (size_t)R.nsegN * n * NB : 0, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "BiCGSTAB workspace (n=%d)", n);
+  double h1[NB], h0[NB];
+  for (int k = 0; k < NB; ++k) {
+    h1[k] = 1.0;
+    h0[k] = 0.0;
+  }
+  ANDR_HIP(ctx, hipMemcpyAsync(one.p, h1, sizeof(h1), hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(zero.p, h0, sizeof(h0), hipMemcpyHostToDevice, st));
+  // the products write out = 1 * (A in) + 0 * out: their outputs start finite
+  ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
+  ANDR_HIP(ctx, hipMemsetAsync(Tt.p, 0, sizeof(double) * n * NB, st));
+  R.abytes = (double)n * n * 8.0;
+  R.vbytes = 2.0 * n * NB * 8.0;
+  R.aflops = 2.0 * n * n * nrhs;
+  const int gb = (n * NB + 255) / 256;
+  auto product = [&](const double *in, double *out) {  // out = A in (its partial squared norms are not used)
+    R.gemv_n(in, out, one.p, zero.p, p1.p);
+  };
+  hipLaunchKernelGGL(k_bicg_diag, dim3((n + 255) / 256), dim3(256), 0, st, A, lda, n, dinv.p);
+  product(X, V.p);  // A x0
+  hipLaunchKernelGGL(k_bicg_init, dim3(gb), dim3(256), 0, st, n, B, V.p, Rr.p, Rh.p, P.p, V.p);
+  hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, B, B, Rr.p, Rr.p, p1.p, p2.p);
+  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, p2.p, nb, tol, nrhs, sc.p);
+  const int itnlim = maxit > 0 ? maxit : 1000;
+  BicgState hs[NB];
+  for (int it = 0; it < itnlim;) {
+    for (int inner = 0; inner < 8 && it < itnlim; ++inner, ++it) {
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh.p, Rr.p, (const double *)nullptr,
+                         (const double *)nullptr, p1.p, p2.p);
+      hipLaunchKernelGGL(k_bicg_beta, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, sc.p);
+      hipLaunchKernelGGL(k_bicg_p, dim3(gb), dim3(256), 0, st, n, Rr.p, V.p, P.p, Yv.p, dinv.p, sc.p, S.p);
+      product(Yv.p, V.p);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh.p, V.p, (const double *)nullptr,
+                         (const double *)nullptr, p1.p, p2.p);
+      hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, sc.p);
+      hipLaunchKernelGGL(k_bicg_s, dim3(gb), dim3(256), 0, st, n, X, Yv.p, Rr.p, V.p, Z.p, dinv.p, sc.p);
+      product(Z.p, Tt.p);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Tt.p, Rr.p, Tt.p, Tt.p, p1.p, p2.p);
+      hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, p2.p, nb, sc.p);
+      hipLaunchKernelGGL(k_bicg_x, dim3(gb), dim3(256), 0, st, n, X, Z.p, Rr.p, Tt.p, sc.p);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rr.p, Rr.p, (const double *)nullptr,
+                         (const double *)nullptr, p1.p, p2.p);
+      hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, itnlim);
+    }
+    ANDR_HIP(ctx, hipGetLastError());
+    ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipStreamSynchronize(st));
+    bool any = false;
+    for (int k = 0; k < nrhs; ++k) any = any || hs[k].active;
+    if (!any) break;
+  }
+  ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  for (int k = 0; k < nrhs; ++k) {
+    if (nit) nit[k] = hs[k].itn;
+    if (status) status[k] = hs[k].active ? 1 : hs[k].status;
+  }
+  return ANDR_OK;
+}

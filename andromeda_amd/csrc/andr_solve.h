@@ -45,6 +45,12 @@ struct LsqrJob {
   int *nit, *istop;
 };
 #define LSQR_STREAMS 4
+// BiCGSTAB with a Jacobi preconditioner on a square FP64 system (device arrays, A row-major with
+// leading dimension lda, B and X n x ANDR_NB interleaved, X the initial guess and the result):
+// per column nit = iterations, status 0 converged (||r|| <= tol ||b||), 1 iteration limit,
+// 2 breakdown. maxit <= 0: 1000.
+int andr_bicgstab_device(andr_ctx *ctx, const double *A, int64_t lda, int n, const double *B, double *X, int nrhs,
+                         int maxit, double tol, int *nit, int *status);
 int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs);
 
 // Restarted GMRES on a batch of square systems (FP32 or FP64 storage, FP64 vectors); restart

@@ -103,6 +103,15 @@ typedef struct {
 int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double *A, int32_t nrhs, const double *B,
                     double *X, const andr_lsqr_opts *opts, andr_lsqr_stats *stats);
 
+/* The Krylov alternative of the reference's LIS path for square systems (parLISslvSet
+ * "-i bicgstab -p jacobi -tol 1.0e-8 -maxiter 1000", src/mod_par.f90:288; LIS itself is an external
+ * library): Jacobi-preconditioned BiCGSTAB on a dense row-major n x n host matrix, nrhs <= 16
+ * right-hand sides (B and X n x nrhs, column k contiguous, X the initial guess and the result).
+ * tol <= 0: 1e-8 (||r||_2 <= tol ||b||_2); maxit <= 0: 1000. stats->istop per column: 0 converged,
+ * 1 iteration limit, 2 breakdown; stats->nit the iterations. */
+int andr_bicgstab_dense(andr_ctx *ctx, int64_t n, const double *A, int32_t nrhs, const double *B, double *X,
+                        double tol, int32_t maxit, andr_lsqr_stats *stats);
+
 /* ---- seam 3: resistance ------------------------------------------------------------------- */
 #define ANDR_FORM_EXTERIOR 0 /* single surface, rigid-body velocity on the body                */
 #define ANDR_FORM_BOUNDED 1  /* body (wall 1) at rest inside an outer wall 2 carrying the flow */
