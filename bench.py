@@ -233,6 +233,10 @@ def main():
         log(f"[bench] step {s}: {time.perf_counter() - ts:.2f} s (assembly "
             f"{sum(r.t_assemble for r in results):.2f} s, solve {sum(r.t_solve for r in results):.2f} s)")
     elapsed = time.perf_counter() - t0
+    pairs = sum(b.nelem * (b.nelem - 1) for b in bodies)
+    log(f"[bench] quadrature: {sum(r.quad_points for r in results) / pairs:.2f} points per pair over {pairs:.3e} "
+        f"pairs; near pairs on a learned rule {sum(r.pairs_learned for r in results):.3e}, on the fixed rule "
+        f"{sum(r.pairs_fallback for r in results):.3e}")
     worst = sorted(range(len(results)), key=lambda i: -int(results[i].nit.max()))[:4]
     log("[bench] most iterations: " + ", ".join(f"body {i} F={bodies[i].nelem} its={int(results[i].nit.max())}"
                                                 for i in worst))
