@@ -125,6 +125,55 @@ module andromeda_hip
       type(andr_resistance_report), intent(out) :: reports(*)
     end function
 
+    ! GetStokesDomainValue (src/lib_Stokes.f90:3043-3390) for npts points: pts(3,npts), u(3,nnodes),
+    ! q(3,nelem) -> vel(3,npts), p(npts), err(npts)
+    integer(c_int) function andr_stokes_domain_values(ctx, mesh, npts, pts, u, q, diff, vel, p, err) &
+        bind(C, name="andr_stokes_domain_values")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: npts
+      real(c_double), intent(in) :: pts(*), u(*), q(*)
+      real(c_double), value :: diff
+      real(c_double), intent(out) :: vel(*), p(*), err(*)
+    end function
+
+    ! pressureStokes (src/lib_Stokes.f90:4-94): p(nelem) at the element centres
+    integer(c_int) function andr_stokes_surface_pressure(ctx, mesh, u, q, diff, p) &
+        bind(C, name="andr_stokes_surface_pressure")
+      import :: c_int, c_ptr, c_double, andr_mesh
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      real(c_double), intent(in) :: u(*), q(*)
+      real(c_double), value :: diff
+      real(c_double), intent(out) :: p(*)
+    end function
+
+    ! the LIS path's solve (parLISslvSet, src/mod_par.f90:288) on one GPU: BiCGSTAB + Jacobi
+    integer(c_int) function andr_bicgstab_dense(ctx, n, A, nrhs, B, X, tol, maxit, stats) &
+        bind(C, name="andr_bicgstab_dense")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, andr_lsqr_stats
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: A(*)
+      integer(c_int32_t), value :: nrhs
+      real(c_double), intent(in) :: B(*)
+      real(c_double), intent(inout) :: X(*)
+      real(c_double), value :: tol
+      integer(c_int32_t), value :: maxit
+      type(andr_lsqr_stats), intent(out) :: stats
+    end function
+
+    ! WriteIntegralsToDisk / ReadIntegralsFile (src/lib_int.f90:136-221): H(nsp,nnodes) and
+    ! G(nsp,nqnodes) passed row-major (ANDR_ROW_MAJOR), i.e. the transposes of Hmat / Gmat
+    integer(c_int) function andr_integrals_write_laplace(path, nsp, nnodes, nqnodes, err, H, G) &
+        bind(C, name="andr_integrals_write_laplace")
+      import :: c_int, c_char, c_int32_t, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nsp, nnodes, nqnodes
+      real(c_double), intent(in) :: err(2), H(*), G(*)
+    end function
+
     integer(c_int) function andr_set_quadrature_model(ctx, params, confidence, near_ratio, far_table) &
         bind(C, name="andr_set_quadrature_model")
       import :: c_int, c_ptr, c_float, c_double, c_int32_t

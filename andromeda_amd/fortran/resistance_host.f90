@@ -6,7 +6,8 @@
 !    1-based CSR, and solves it with lsqr_precon/lsqr_solve called through the reference's
 !    implicit F77 interface (src/lib_CRS.f90:77-85); prints max |q + n_x| (exact flux q = -n_x),
 ! 3. computes the 6x6 resistance tensor of wall 1 (StokesInlet3sidesForceCRS /
-!    TorqueOmegaCRS, src/lib_particle.f90:176-476) in the exterior form and prints it.
+!    TorqueOmegaCRS, src/lib_particle.f90:176-476) in the exterior form and prints it;
+! 4. the velocity at the mesh's centre for a uniform boundary flow (domain values).
 program resistance_host
   use iso_c_binding
   use andromeda_hip
@@ -17,6 +18,8 @@ program resistance_host
   integer(c_int64_t), allocatable :: ia(:)
   real(c_double), target :: normmul(1)
   real(c_double) :: err(2), FT(36), eps, nrm(3), e1(3), e2(3), worst
+  real(c_double), allocatable :: ub(:, :), qb(:, :)
+  real(c_double) :: pt(3), vel(3), pd(1), ed(1)
   type(andr_mesh) :: mesh
   type(andr_resistance_opts) :: opts
   type(andr_resistance_report) :: rep
@@ -96,6 +99,21 @@ program resistance_host
   do k = 0, 5
     write(*, '(a, i0, a, 6es16.8)') "FT ", k, ": ", FT(6 * k + 1:6 * k + 6)
   end do
+  ! 4. domain value at the centre of the mesh for the uniform flow u = (1, 0, 0), q = 0: the closure
+  !    of the integrals returns the flow (GetStokesDomainValue, src/lib_Stokes.f90:3043-3390)
+  allocate(ub(3, nn), qb(3, ne))
+  ub = 0.0d0
+  ub(1, :) = 1.0d0
+  qb = 0.0d0
+  do i = 1, 3
+    pt(i) = sum(xyz(i, :)) / nn
+  end do
+  rc = andr_stokes_domain_values(ctx, mesh, 1_c_int32_t, pt, ub, qb, 1.0d0, vel, pd, ed)
+  if (rc /= ANDR_OK) then
+    print *, "andr_stokes_domain_values: ", andr_error_message(ctx)
+    stop 1
+  end if
+  write(*, '(a, 3es16.8, a, es12.4)') "domain velocity: ", vel, " closure error: ", ed(1)
   rc = andr_destroy(ctx)
 
 contains

@@ -43,3 +43,13 @@ def test_fortran_host_matches_python(ctx):
     assert np.max(np.abs(FT - ref.FT)) <= 1e-9 * np.abs(ref.FT).max()
     # sphere of radius 0.5: drag 6 pi mu a U (coarse mesh: a few per cent)
     assert abs(FT[0, 0] / (6 * np.pi * 0.5) - 1) < 0.05
+    # domain values through the Fortran interface: a uniform boundary flow is returned at the centre
+    # (up to the reference's single-precision 8 pi), the same numbers as the Python binding's
+    m = re.search(r"domain velocity:\s+(\S+)\s+(\S+)\s+(\S+)\s+closure error:\s+(\S+)", out)
+    assert m, out
+    vel = np.array([float(m.group(k)) for k in (1, 2, 3)])
+    u = np.zeros((mesh.nnodes, 3))
+    u[:, 0] = 1.0
+    vp, _, ep = ctx.stokes_domain_values(mesh, mesh.xyz.mean(axis=0)[None, :], u, np.zeros((mesh.nelem, 3)))
+    assert np.abs(vel - vp[0]).max() < 1e-7 and abs(abs(vel[0]) - 1.0) < 1e-6 and np.abs(vel[1:]).max() < 1e-6
+    assert float(m.group(4)) < 1e-6 and ep[0] < 1e-6
