@@ -295,8 +295,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 quadrature+storage, fp64 solve" if prec_name == "mixed" else
-                 "fp32 learned-order quadrature+storage, fp64 solve",
+        "dtype": "f32+f64",
+        "precision": {"quadrature": "f32" if prec_name == "mixed" else "f32, learned rule order",
+                      "singular_integrals": "f64", "matrix_storage": "f32",
+                      "krylov_product_and_solve": "f64", "refinement_residual": "f64"},
         "data": "synthetic (seeded superellipsoids and fibres)",
         "config": {"workload": f"{len(bodies)} bodies ({n_super} superellipsoids + {n_fibre} fibres), "
                                f"F log-uniform [500, 8000], 6 rigid-body RHS, exterior Stokes",
