@@ -156,7 +156,7 @@ def main():
         log(f"[bench] copy calibration: 5 copies of {n * 4} bytes")
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "one GPU only: runs on several GPUs are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -3,7 +3,7 @@
 ! (WriteIntegralsToDiskStokes / ReadIntegralsFileStokes, src/lib_Stokes.f90:2742-2872;
 ! WriteIntegralsToDisk / ReadIntegralsFile, src/lib_int.f90:136-221; WrSMat / RdSMat and
 ! wrvec / rdvec with 8192-value records, src/lib_util.f90:201-287), so that the Fortran runtime's
-! own unformatted records are the judge of the library's files.
+! own unformatted records decide whether the library writes and reads the layout right.
 !   integrals_check write stokes|laplace FILE NSP NN NQ   writes the test pattern
 !   integrals_check read  stokes|laplace FILE NSP NN NQ   reads a file, prints OK or MISMATCH
 ! Pattern: matrix k (1-based in the file's order), entry (i, j) = 1e6 k + 1e3 i + j; errors (0.25, 0.5).
