@@ -64,16 +64,27 @@ def test_oracle_domain_viscosity_scaling(orc):
     assert np.abs(v3 - v1).max() < 1e-13 and np.abs(p3 - 3.0 * p1).max() < 1e-12
 
 
+def _mesh(kind):
+    if kind == "sphere":
+        return M.icosphere(3), None
+    if kind == "superellipsoid":
+        return M.random_superellipsoid(np.random.default_rng(9), 6)[0], None
+    # a particle in a far sphere, two walls, the outer one's normals turned (NORMMUL -1)
+    return M.body_in_sphere(M.geodesic_sphere(3, 0.5), 4.0, 2), np.array([1.0, -1.0])
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["sphere", "superellipsoid"])
+@pytest.mark.parametrize("kind", ["sphere", "superellipsoid", "two_walls"])
 def test_gpu_domain_values_vs_oracle(ctx, orc, kind):
     rng = np.random.default_rng(4)
-    m = M.icosphere(3) if kind == "sphere" else M.random_superellipsoid(np.random.default_rng(9), 6)[0]
+    m, nm = _mesh(kind)
     u = rng.normal(size=(m.nnodes, 3))
     q = rng.normal(size=(m.nelem, 3))
     pts = np.vstack([INSIDE * 0.5, OUTSIDE * 2.0, rng.uniform(-0.3, 0.3, size=(57, 3))])
-    vel, p, err = ctx.stokes_domain_values(m, pts, u, q, diff=1.7)
-    vo, po, eo = orc.stokes_domain_values(m, pts, u, q, diff=1.7)
+    if kind == "two_walls":  # between the particle and the outer sphere
+        pts = np.vstack([pts[:6] * 1.5 + np.array([0.0, 0.0, 1.2]), rng.uniform(0.8, 2.5, size=(20, 3))])
+    vel, p, err = ctx.stokes_domain_values(m, pts, u, q, diff=1.7, normmul=nm)
+    vo, po, eo = orc.stokes_domain_values(m, pts, u, q, diff=1.7, normmul=nm)
     scale = np.abs(vo).max()
     assert np.abs(vel - vo).max() < 1e-10 * scale
     assert np.abs(p - po).max() < 1e-10 * np.abs(po).max()
@@ -103,12 +114,12 @@ def test_oracle_surface_pressure_pins(orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["sphere", "superellipsoid"])
+@pytest.mark.parametrize("kind", ["sphere", "superellipsoid", "two_walls"])
 def test_gpu_surface_pressure_vs_oracle(ctx, orc, kind):
     rng = np.random.default_rng(6)
-    m = M.icosphere(3) if kind == "sphere" else M.random_superellipsoid(np.random.default_rng(9), 6)[0]
+    m, nm = _mesh(kind)
     u = rng.normal(size=(m.nnodes, 3))
     q = rng.normal(size=(m.nelem, 3))
-    p = ctx.stokes_surface_pressure(m, u, q, diff=0.8)
-    po = orc.stokes_surface_pressure(m, u, q, diff=0.8)
+    p = ctx.stokes_surface_pressure(m, u, q, diff=0.8, normmul=nm)
+    po = orc.stokes_surface_pressure(m, u, q, diff=0.8, normmul=nm)
     assert np.abs(p - po).max() < 1e-10 * np.abs(po).max()
