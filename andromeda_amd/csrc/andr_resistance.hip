@@ -161,13 +161,17 @@ size_t body_bytes(int ne, int nn, bool plain, bool learned, int restart) {
   return s;
 }
 
-// long cycles: slender bodies need 150-400 Krylov steps and short restarts stagnate on them
-constexpr int RESTART = 128;
+// The first cycle's length: 128 Krylov steps, longer (up to 192, in steps of 8) while the bodies of
+// the call still fit one wave. Slender bodies then converge without a restart: on the 576-body
+// workload a 160-step cycle took the fibres from ~290 steps (128 + a long second cycle) to <= 160
+// and the step from 2.44 to 2.38 s, while 96 steps made it 2.49 s. Later cycles take the longest
+// restart that fits the same buffers.
+constexpr int RESTART = 128, RESTART_MAX = 192;
 
 // The resistance tensors of one wave of prepared bodies: assembly, solve, force and torque.
 // FT and reports point at the wave's first body; t_up is the wave's upload time (reported).
-int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_opts *opts, bool timing, double t_up,
-             double *FT, andr_resistance_report *reports) {
+int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_opts *opts, int restart, bool timing,
+             double t_up, double *FT, andr_resistance_report *reports) {
   int rc = ANDR_OK;
   const bool exterior = opts->form == ANDR_FORM_EXTERIOR;
   const bool plain = opts->precision == ANDR_PREC_FP64;
@@ -274,7 +278,7 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
       StokesOp op{(const float *)Abuf[i].p, P[i]->ne, jobs[i].minv, g.area, g.nrm, g.bcid, jobs[i].defl};
       gs[i] = {nullptr, 0, 3 * P[i]->ne, op, Bbuf[i].p, Xbuf[i].p};
     }
-    rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, RESTART, opts->maxit > 0 ? opts->maxit : 2000, eps,
+    rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
                           iters.data());
     if (rc) return rc;
     if ((rc = andr_launch_unprecondition(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
@@ -319,12 +323,32 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
   return ANDR_OK;
 }
 
-// waves of bodies whose systems fit 80 % of the free device memory (with the context's cache)
-int plan_waves(andr_ctx *ctx, const std::vector<size_t> &bytes, std::vector<int> &ends) {
+// device memory the systems of a call may take: 80 % of the free memory (with the context's cache)
+int wave_budget(andr_ctx *ctx, size_t *budget) {
   size_t free_b = 0, total_b = 0;
   ANDR_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
   free_b += ctx->pool.cached;  // the context's cached blocks are reusable
-  const size_t budget = (size_t)(0.80 * (double)free_b);
+  *budget = (size_t)(0.80 * (double)free_b);
+  return ANDR_OK;
+}
+
+// the first cycle's length for bodies (ne, nn): RESTART, or longer while they fit one wave
+int first_restart(size_t budget, const std::vector<int> &ne, const std::vector<int> &nn, bool plain, bool learned,
+                  int asked) {
+  if (plain) return RESTART;
+  if (asked > 0) return std::min(asked, 1024);
+  int best = RESTART;
+  for (int r = RESTART + 8; r <= RESTART_MAX; r += 8) {
+    size_t total = 0;
+    for (size_t b = 0; b < ne.size(); ++b) total += body_bytes(ne[b], nn[b], plain, learned, r);
+    if (total > budget) break;
+    best = r;
+  }
+  return best;
+}
+
+// waves of bodies whose systems fit the budget
+int plan_waves(andr_ctx *ctx, size_t budget, const std::vector<size_t> &bytes, std::vector<int> &ends) {
   ends.clear();
   const int nb = (int)bytes.size();
   int b0 = 0;
@@ -366,10 +390,18 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
   const bool plain = opts->precision == ANDR_PREC_FP64, learned = opts->precision == ANDR_PREC_LEARNED;
   if (reports) std::memset(reports, 0, sizeof(andr_resistance_report) * nbodies);
   const bool timing = getenv("ANDR_DEBUG_TIMING") != nullptr;  // per-wave host timing on stderr
+  std::vector<int> ne(nbodies), nn(nbodies);
+  for (int b = 0; b < nbodies; ++b) {
+    ne[b] = meshes[b].nelem;
+    nn[b] = meshes[b].nnodes;
+  }
+  size_t budget = 0;
+  if ((rc = wave_budget(ctx, &budget))) return rc;
+  const int restart = first_restart(budget, ne, nn, plain, learned, opts->restart);
   std::vector<size_t> bytes(nbodies);
-  for (int b = 0; b < nbodies; ++b) bytes[b] = body_bytes(meshes[b].nelem, meshes[b].nnodes, plain, learned, RESTART);
+  for (int b = 0; b < nbodies; ++b) bytes[b] = body_bytes(ne[b], nn[b], plain, learned, restart);
   std::vector<int> ends;
-  if ((rc = plan_waves(ctx, bytes, ends))) return rc;
+  if ((rc = plan_waves(ctx, budget, bytes, ends))) return rc;
   int b0 = 0;
   for (int b1 : ends) {
     const double t0 = andr_now();
@@ -377,7 +409,7 @@ extern "C" int andr_resistance_batch(andr_ctx *ctx, int32_t nbodies, const andr_
     if ((rc = prepare_bodies(ctx, meshes, b0, b1, opts->form == ANDR_FORM_EXTERIOR, opts->mu, !plain, P))) return rc;
     std::vector<Prepared *> pp(P.size());
     for (size_t i = 0; i < P.size(); ++i) pp[i] = P[i].get();
-    if ((rc = run_wave(ctx, pp.data(), b1 - b0, opts, timing, andr_now() - t0, FT + (size_t)b0 * 36,
+    if ((rc = run_wave(ctx, pp.data(), b1 - b0, opts, restart, timing, andr_now() - t0, FT + (size_t)b0 * 36,
                        reports ? reports + b0 : nullptr)))
       return rc;
     b0 = b1;
@@ -418,15 +450,23 @@ extern "C" int andr_resistance_batch_resident(andr_ctx *ctx, const andr_bodies *
   const bool plain = opts->precision == ANDR_PREC_FP64, learned = opts->precision == ANDR_PREC_LEARNED;
   if (reports) std::memset(reports, 0, sizeof(andr_resistance_report) * nbodies);
   const bool timing = getenv("ANDR_DEBUG_TIMING") != nullptr;
+  std::vector<int> ne(nbodies), nn(nbodies);
+  for (int b = 0; b < nbodies; ++b) {
+    ne[b] = bodies->P[b]->ne;
+    nn[b] = bodies->P[b]->nn;
+  }
+  size_t budget = 0;
+  if ((rc = wave_budget(ctx, &budget))) return rc;
+  const int restart = first_restart(budget, ne, nn, plain, learned, opts->restart);
   std::vector<size_t> bytes(nbodies);
-  for (int b = 0; b < nbodies; ++b) bytes[b] = body_bytes(bodies->P[b]->ne, bodies->P[b]->nn, plain, learned, RESTART);
+  for (int b = 0; b < nbodies; ++b) bytes[b] = body_bytes(ne[b], nn[b], plain, learned, restart);
   std::vector<int> ends;
-  if ((rc = plan_waves(ctx, bytes, ends))) return rc;
+  if ((rc = plan_waves(ctx, budget, bytes, ends))) return rc;
   std::vector<Prepared *> pp(nbodies);
   for (int b = 0; b < nbodies; ++b) pp[b] = bodies->P[b].get();
   int b0 = 0;
   for (int b1 : ends) {
-    if ((rc = run_wave(ctx, pp.data() + b0, b1 - b0, opts, timing, 0.0, FT + (size_t)b0 * 36,
+    if ((rc = run_wave(ctx, pp.data() + b0, b1 - b0, opts, restart, timing, 0.0, FT + (size_t)b0 * 36,
                        reports ? reports + b0 : nullptr)))
       return rc;
     b0 = b1;

@@ -41,6 +41,7 @@ module andromeda_hip
     real(c_double) :: mu, eps
     integer(c_int32_t) :: maxit
     real(c_double) :: center(3)
+    integer(c_int32_t) :: restart  ! fast forms: first GMRES cycle; 0 -> automatic
   end type
 
   type, bind(C) :: andr_resistance_report

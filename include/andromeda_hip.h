@@ -129,6 +129,8 @@ typedef struct {
                         residual, mixed and learned) */
   int32_t maxit;     /* 0 -> default */
   double center[3];  /* torque and rotation reference point */
+  int32_t restart;   /* fast forms: steps of the first GMRES cycle; 0 -> 128, longer (up to 192) while
+                        the bodies of the call fit one wave of device memory */
 } andr_resistance_opts;
 
 typedef struct {

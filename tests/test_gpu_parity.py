@@ -154,15 +154,21 @@ def test_mixed_vs_plain(ctx):
 
 def test_slender_body_long_restarts(ctx):
     """A 496-element fibre of the batched workload whose hardest column needs several hundred
-    Krylov steps: the cycles after the first run with a longer restart in the same buffers, next to
-    a body that converges in the first cycle; both stay within 1e-6 of the plain FP64 path."""
+    Krylov steps with a first cycle of 128: the cycles after the first run with a longer restart in
+    the same buffers, next to a body that converges in the first cycle; both stay within 1e-6 of the
+    plain FP64 path. With the automatic first cycle (longer while the bodies fit one wave) the fibre
+    converges without a restart, in fewer steps, to the same tensor."""
     allb = [m for m, _ in M.batch_bodies(512, 64)]
     bodies = [allb[523], allb[29]]
     plain = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
-    fast = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    fast = ctx.resistance_batch(bodies, precision=api.PREC_MIXED, restart=128)
     for f, p in zip(fast, plain):
         assert tensor_error(f.FT, p.FT) < 1e-6
     assert fast[0].nit.max() > 128 >= fast[1].nit.max()
+    auto = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    assert auto[0].nit.max() <= 192 and auto[0].nit.max() < fast[0].nit.max()
+    for f, p in zip(auto, plain):
+        assert tensor_error(f.FT, p.FT) < 1e-6
 
 
 def test_tiny_bodies_krylov_space_exhausted(ctx):
