@@ -272,10 +272,12 @@ def main():
 
     kernels, roofline = kernel_table(prof)
 
-    # kernel rooflines of the plain path on a quarter of the sample (profiled on its own: with
-    # profiling on, the plain solves run one after another so that each launch is timed alone)
+    # kernel rooflines of the plain path on one large body of the sample (about 5000 triangles: a
+    # system of ~2 GB that solves in about a second), profiled on its own
+    big = max((b for b in sample if b.nelem <= 5200), key=lambda b: b.nelem,
+              default=min(sample, key=lambda b: b.nelem))
     ctx.profile(True)
-    ctx.resistance_batch(sample[::4], precision=api.PREC_FP64, eps=1e-12)
+    ctx.resistance_batch([big], precision=api.PREC_FP64, eps=1e-12)
     plain_kernels, plain_roofline = kernel_table(ctx.profile_read())
     ctx.profile(False)
     log("[bench] plain FP64 kernels: " + ", ".join(
@@ -305,6 +307,7 @@ def main():
                    "bodies": len(bodies), "triangles": int(sum(b.nelem for b in bodies)),
                    "form": prec_name},
         "plain_fp64": {"sample_bodies": len(sample), "tensors_per_s": round(plain_rate, 3),
+                       "kernels_profiled_on": f"one body of {big.nelem} triangles",
                        "fast_tensors_per_s_same_sample": round(len(sample) / t_fast, 3),
                        "speedup_same_sample": round(t_plain / t_fast, 2),
                        "roofline": plain_roofline, "kernels": plain_kernels},
