@@ -646,9 +646,13 @@ __global__ __launch_bounds__(64) void kb_arnoldi(const GSys *__restrict__ sys, i
 }
 
 // per tile: v_j = (u_j - V s) / rho into Q_j, u_{j+1} = (W - V t - delta v_j) / (rho eta) into Q_{j+1}.
-// Both rows of a thread and four basis vectors per iteration are in flight together.
+// The basis vectors are taken in chunks of CCH: the block stages the chunk's step coefficients s, t of
+// its system in LDS (lc[i NB + k] = s_(c0+i), lc[(cn + i) NB + k] = t_(c0+i)); both rows of a thread
+// and four basis vectors per iteration (eight loads) are in flight together.
 __global__ __launch_bounds__(256) void kb_aupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
                                                   int m) {
+  constexpr int CCH = 128;
+  __shared__ double lc[2 * CCH * NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n;
@@ -658,31 +662,49 @@ __global__ __launch_bounds__(256) void kb_aupdate(const GSys *__restrict__ sys, 
   const bool v0 = r0 < n, v1 = r1 < n;
   const int q0 = v0 ? r0 : 0, q1 = v1 ? r1 : q0;  // in-range rows for the loads
   const double *cw = S.cw + (size_t)k * cw_stride(m);
-  const double *cs = cw, *ct = cw + (m + 1);
   const double irho = cw[2 * m + 2], delta = cw[2 * m + 3], ieta = cw[2 * m + 4];
   const size_t vs = (size_t)n * NB;
   const double *Qa = S.Q + (size_t)q0 * NB + k, *Qb = S.Q + (size_t)q1 * NB + k;
   double sa0 = 0.0, sa1 = 0.0, ta0 = 0.0, ta1 = 0.0, sb0 = 0.0, sb1 = 0.0, tb0 = 0.0, tb1 = 0.0;
-  int i = 0;
-  for (; i + 2 <= j; i += 2) {
-    const double qa0 = Qa[(size_t)i * vs], qa1 = Qa[(size_t)(i + 1) * vs];
-    const double qb0 = Qb[(size_t)i * vs], qb1 = Qb[(size_t)(i + 1) * vs];
-    const double s0 = cs[i], s1 = cs[i + 1], t0 = ct[i], t1 = ct[i + 1];
-    sa0 += qa0 * s0;
-    sa1 += qa1 * s1;
-    ta0 += qa0 * t0;
-    ta1 += qa1 * t1;
-    sb0 += qb0 * s0;
-    sb1 += qb1 * s1;
-    tb0 += qb0 * t0;
-    tb1 += qb1 * t1;
-  }
-  for (; i < j; ++i) {
-    const double qa = Qa[(size_t)i * vs], qb = Qb[(size_t)i * vs];
-    sa0 += qa * cs[i];
-    ta0 += qa * ct[i];
-    sb0 += qb * cs[i];
-    tb0 += qb * ct[i];
+  for (int c0 = 0; c0 < j; c0 += CCH) {  // j is uniform over the block: every thread meets the barriers
+    const int cn = min(CCH, j - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < 2 * cn * NB; q += 256) {
+      const int kk = q % NB, ii = q / NB;  // ii < cn: s_(c0+ii), else t_(c0+ii-cn)
+      lc[q] = S.cw[(size_t)kk * cw_stride(m) + (ii < cn ? c0 + ii : (m + 1) + c0 + (ii - cn))];
+    }
+    __syncthreads();
+    const double *cs = lc + k, *ct = lc + (size_t)cn * NB + k;
+    const double *Pa = Qa + (size_t)c0 * vs, *Pb = Qb + (size_t)c0 * vs;
+    int i = 0;
+    for (; i + 4 <= cn; i += 4) {
+      double qa[4], qb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        qa[u] = Pa[(size_t)(i + u) * vs];
+        qb[u] = Pb[(size_t)(i + u) * vs];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u += 2) {
+        const double s0 = cs[(i + u) * NB], s1 = cs[(i + u + 1) * NB];
+        const double t0 = ct[(i + u) * NB], t1 = ct[(i + u + 1) * NB];
+        sa0 += qa[u] * s0;
+        sa1 += qa[u + 1] * s1;
+        ta0 += qa[u] * t0;
+        ta1 += qa[u + 1] * t1;
+        sb0 += qb[u] * s0;
+        sb1 += qb[u + 1] * s1;
+        tb0 += qb[u] * t0;
+        tb1 += qb[u + 1] * t1;
+      }
+    }
+    for (; i < cn; ++i) {
+      const double qa = Pa[(size_t)i * vs], qb = Pb[(size_t)i * vs];
+      sa0 += qa * cs[i * NB];
+      ta0 += qa * ct[i * NB];
+      sb0 += qb * cs[i * NB];
+      tb0 += qb * ct[i * NB];
+    }
   }
   double *U = S.Q + (size_t)j * vs, *Un = S.Q + (size_t)(j + 1) * vs;
   if (v0) {
