@@ -110,6 +110,7 @@ def load_library(path: str = LIB_PATH):
     lib.andr_resistance_batch_resident.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_ResOpts), _D,
                                                    C.POINTER(_ResReport)]
     lib.andr_bodies_free.argtypes = [C.c_void_p, C.c_void_p]
+    lib.andr_mesh_validate.argtypes = [C.POINTER(_Mesh), C.c_char_p, C.c_int32]
     lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_autotune_read.argtypes = [C.c_void_p, C.c_int32, _I, _I, _I, _D]
     _lib = lib
@@ -124,7 +125,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
-            "andr_bicgstab_dense"]
+            "andr_bicgstab_dense", "andr_mesh_validate"]
 FAR_BINS = 16
 
 
@@ -150,6 +151,26 @@ class _MeshView:
                                             dtype=np.float64)
         self.s = _Mesh(m.nnodes, m.nelem, max(m.nwalls, 1), _p(m.xyz), _pi(m.con), _pi(m.bcid),
                        _p(self.normmul), 0, itri, int(m.con.shape[1]), iquad)
+
+
+def validate_mesh(mesh: Mesh, nwalls=None, itri: int = 5, iquad: int = 4, con=None, bcid=None):
+    """andr_mesh_validate on the host (no device): (status, message). con and bcid override the
+    mesh's arrays (to check raw inputs), nwalls its wall count."""
+    mv = _MeshView(mesh, None, itri, iquad)
+    keep = []
+    if con is not None:
+        c = np.ascontiguousarray(con, dtype=np.int32)
+        keep.append(c)
+        mv.s.con = _pi(c)
+    if bcid is not None:
+        b = np.ascontiguousarray(bcid, dtype=np.int32)
+        keep.append(b)
+        mv.s.bcid = _pi(b)
+    if nwalls is not None:
+        mv.s.nwalls = nwalls
+    msg = C.create_string_buffer(256)
+    rc = load_library().andr_mesh_validate(C.byref(mv.s), msg, 256)
+    return rc, msg.value.decode()
 
 
 @dataclass

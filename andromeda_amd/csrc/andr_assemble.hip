@@ -210,34 +210,73 @@ static int color_elements(int nn, int ne, int nv, const int *con, std::vector<in
   return 0;
 }
 
+// the mesh checks with the message in msg (host only: no context, no device)
+static int check_mesh_msg(const andr_mesh *m, char *msg, size_t len) {
+  if (!m || m->nnodes <= 0 || m->nelem <= 0 || !m->xyz || !m->con) {
+    snprintf(msg, len, "invalid mesh (sizes or null arrays)");
+    return ANDR_ERR_ARG;
+  }
+  if (m->nvert != 0 && m->nvert != 3 && m->nvert != 4) {
+    snprintf(msg, len, "nvert must be 0, 3 or 4");
+    return ANDR_ERR_ARG;
+  }
+  const int nn = m->nnodes, ne = m->nelem, nw = m->nwalls > 0 ? m->nwalls : 1, nv = m->nvert == 4 ? 4 : 3;
+  for (size_t i = 0; i < (size_t)nv * ne; ++i) {
+    const int c = m->con[i] - m->index_base;
+    if (nv == 4 && i % 4 == 3 && c < 0) continue;  // a triangle in a quadrilateral-capable mesh
+    if (c < 0 || c >= nn) {
+      snprintf(msg, len, "connectivity entry %zu out of range", i);
+      return ANDR_ERR_ARG;
+    }
+  }
+  if (m->bcid)
+    for (int e = 0; e < ne; ++e)
+      if (m->bcid[e] < 1 || m->bcid[e] > nw) {
+        snprintf(msg, len, "bcid of element %d is %d, outside 1..%d", e, m->bcid[e], nw);
+        return ANDR_ERR_ARG;
+      }
+  if (m->iquad < 0 || m->iquad > 8) {
+    snprintf(msg, len, "iquad must be 1..8 (0: default)");
+    return ANDR_ERR_ARG;
+  }
+  if (m->itri < 0 || m->itri > 5) {
+    snprintf(msg, len, "itri must be 1..5 (0: default)");
+    return ANDR_ERR_ARG;
+  }
+  return ANDR_OK;
+}
+
+int andr_check_mesh(andr_ctx *ctx, const andr_mesh *m) {
+  char msg[256];
+  const int rc = check_mesh_msg(m, msg, sizeof(msg));
+  return rc ? andr_fail(ctx, rc, "%s", msg) : ANDR_OK;
+}
+
+extern "C" int andr_mesh_validate(const andr_mesh *mesh, char *msg, int32_t msglen) {
+  char buf[256] = "";
+  const int rc = check_mesh_msg(mesh, buf, sizeof(buf));
+  if (msg && msglen > 0) snprintf(msg, (size_t)msglen, "%s", buf);
+  return rc;
+}
+
 int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors) {
-  if (!m || m->nnodes <= 0 || m->nelem <= 0 || !m->xyz || !m->con)
-    return andr_fail(ctx, ANDR_ERR_ARG, "invalid mesh");
+  int rc = andr_check_mesh(ctx, m);
+  if (rc) return rc;
   const int nn = m->nnodes, ne = m->nelem;
   const int nw = m->nwalls > 0 ? m->nwalls : 1;
   const int nv = m->nvert == 4 ? 4 : 3;
-  if (m->nvert != 0 && m->nvert != 3 && m->nvert != 4) return andr_fail(ctx, ANDR_ERR_ARG, "nvert must be 0, 3 or 4");
   std::vector<int> con((size_t)nv * ne), bcid(ne, 1);
   for (size_t i = 0; i < (size_t)nv * ne; ++i) {
     con[i] = m->con[i] - m->index_base;
-    if (nv == 4 && i % 4 == 3 && con[i] < 0) {  // a triangle in a quadrilateral-capable mesh
-      con[i] = -1;
-      continue;
-    }
-    if (con[i] < 0 || con[i] >= nn) return andr_fail(ctx, ANDR_ERR_ARG, "connectivity entry %zu out of range", i);
+    if (nv == 4 && i % 4 == 3 && con[i] < 0) con[i] = -1;  // a triangle in a quadrilateral-capable mesh
   }
   b.iquad = m->iquad == 0 ? 4 : m->iquad;
-  if (b.iquad < 1 || b.iquad > 8) return andr_fail(ctx, ANDR_ERR_ARG, "iquad must be 1..8");
   if (m->bcid)
-    for (int e = 0; e < ne; ++e) {
-      bcid[e] = m->bcid[e];
-      if (bcid[e] < 1 || bcid[e] > nw) return andr_fail(ctx, ANDR_ERR_ARG, "bcid of element %d out of range", e);
-    }
+    for (int e = 0; e < ne; ++e) bcid[e] = m->bcid[e];
   std::vector<double> wm(nw, 1.0);
   if (m->normmul)
     for (int w = 0; w < nw; ++w) wm[w] = m->normmul[w];
   b.itri = m->itri == 0 ? 5 : m->itri;
-  if (b.itri < 1 || b.itri > 5) return andr_fail(ctx, ANDR_ERR_ARG, "itri must be 1..5");
   DevBuf<double> d_wm;
   if (b.xyz.alloc(3 * (size_t)nn, ctx) || b.con.alloc((size_t)nv * ne, ctx) || b.bcid.alloc(ne, ctx) ||
       b.nrm.alloc(3 * (size_t)ne, ctx) || b.area.alloc(ne, ctx) || b.cen.alloc(3 * (size_t)ne, ctx) ||

@@ -10,7 +10,9 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 double andr_now() {
@@ -247,7 +249,10 @@ extern "C" int andr_stokes_surface_pressure(andr_ctx *ctx, const andr_mesh *mesh
 __global__ void k_csr_to_dense(int m, int64_t ld, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
                                const double *__restrict__ a, double *__restrict__ D) {
   const int i = blockIdx.x;
-  for (int64_t p = ia[i] - 1 + threadIdx.x; p < ia[i + 1] - 1; p += blockDim.x) D[(size_t)i * ld + (ja[p] - 1)] = a[p];
+  // duplicate (i, j) entries add up, as they do in the reference's CSR products (lsqr_matvec1,
+  // src/lib_lsqr.f:172-205); D is zero on entry, so a single entry is stored exactly
+  for (int64_t p = ia[i] - 1 + threadIdx.x; p < ia[i + 1] - 1; p += blockDim.x)
+    atomicAdd(&D[(size_t)i * ld + (ja[p] - 1)], a[p]);
 }
 
 __global__ void k_pack_col(int n, const double *__restrict__ v, double *__restrict__ V) {
@@ -262,29 +267,74 @@ __global__ void k_unpack_col(int n, const double *__restrict__ V, double *__rest
 }
 
 namespace {
+// The device copy of the last matrix, keyed on the array pointers, the sizes and a checksum of every
+// byte of ia, ja and a: a solve on a matrix whose values or pattern changed since the copy was made
+// uploads it again. The checksum is one streaming pass over the host arrays (threads over chunks),
+// cheaper than the upload it saves.
 struct LsqrCache {
   const double *key_a = nullptr;
+  const int64_t *key_ia = nullptr;
+  const int32_t *key_ja = nullptr;
   int64_t nnz = 0;
   int m = 0, n = 0;
+  uint64_t sum = 0;
   DevBuf<double> D, q;  // D: dense m x n, leading dimension dense_ld(n)
-  std::vector<std::pair<int64_t, double>> probe;
 };
 std::mutex g_mu;
 andr_ctx *g_ctx = nullptr;
 LsqrCache g_cache;
+bool g_precon_failed = false;  // the last lsqr_precon_ failed: solves are refused until one succeeds
 
-void probe_fill(LsqrCache &c, const double *a, int64_t nnz) {
-  c.probe.clear();
-  const int64_t np = std::min<int64_t>(nnz, 64);
-  for (int64_t i = 0; i < np; ++i) {
-    int64_t p = (nnz - 1) * i / std::max<int64_t>(np - 1, 1);
-    c.probe.push_back({p, a[p]});
+// 64-bit multiply-xor hash of a byte range, 8 bytes at a time (tail zero-padded)
+uint64_t hash_bytes(const void *p, size_t bytes, uint64_t h) {
+  const unsigned char *c = (const unsigned char *)p;
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t v;
+    std::memcpy(&v, c + i, 8);
+    h = (h ^ v) * 0x100000001b3ull;
+    h ^= h >> 29;
   }
+  uint64_t v = 0;
+  std::memcpy(&v, c + i, bytes - i);
+  return ((h ^ v) * 0x100000001b3ull) ^ bytes;
 }
-bool probe_ok(const LsqrCache &c, const double *a) {
-  for (auto &pr : c.probe)
-    if (a[pr.first] != pr.second) return false;
-  return true;
+// checksum of (ia, ja, a): chunks hashed on up to 16 threads, combined in chunk order
+uint64_t csr_checksum(int m, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a) {
+  struct Part {
+    const void *p;
+    size_t bytes;
+  };
+  std::vector<Part> parts;
+  const size_t chunk = (size_t)1 << 24;
+  auto add = [&](const void *p, size_t bytes) {
+    for (size_t o = 0; o < bytes; o += chunk) parts.push_back({(const char *)p + o, std::min(chunk, bytes - o)});
+  };
+  add(ia, sizeof(int64_t) * ((size_t)m + 1));
+  add(ja, sizeof(int32_t) * (size_t)nnz);
+  add(a, sizeof(double) * (size_t)nnz);
+  std::vector<uint64_t> hs(parts.size());
+  const int nt = (int)std::min<size_t>(parts.size(), 16);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t]() {
+      for (size_t k = t; k < parts.size(); k += nt) hs[k] = hash_bytes(parts[k].p, parts[k].bytes, 0xcbf29ce484222325ull + k);
+    });
+  for (auto &th : pool) th.join();
+  return hash_bytes(hs.data(), sizeof(uint64_t) * hs.size(), 0x84222325cbf29ce4ull);
+}
+void cache_key(LsqrCache &c, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a) {
+  c.key_a = a;
+  c.key_ia = ia;
+  c.key_ja = ja;
+  c.nnz = nnz;
+  c.m = m;
+  c.n = n;
+  c.sum = csr_checksum(m, nnz, ia, ja, a);
+}
+bool cache_hit(const LsqrCache &c, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a) {
+  return c.key_a == a && c.key_ia == ia && c.key_ja == ja && c.nnz == nnz && c.m == m && c.n == n && c.D.p &&
+         c.sum == csr_checksum(m, nnz, ia, ja, a);
 }
 // leading dimension of the dense device copy: rows start on 128-byte lines
 int64_t dense_ld(int n) { return ((int64_t)n + 15) / 16 * 16; }
@@ -329,33 +379,36 @@ extern "C" void andr_lsqr_forget(void) {
   g_cache.D.release();
   g_cache.q.release();
   g_cache.key_a = nullptr;
+  g_precon_failed = false;
 }
 
 extern "C" void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *nnz, double *q, const int64_t *ia,
                              const int32_t *ja, double *a) {
   std::lock_guard<std::mutex> lk(g_mu);
-  andr_ctx *ctx = andr_default_ctx();
-  if (!ctx) {
-    fprintf(stderr, "andromeda_hip: lsqr_precon_: no device\n");
-    return;
-  }
-  (void)hipSetDevice(ctx->device);
+  // The Fortran interface has no status argument: on failure q is set to NaN (a is left as it was)
+  // and every lsqr_solve_ returns ierr = -1 until a later lsqr_precon_ succeeds.
   LsqrCache &c = g_cache;
+  c.key_a = nullptr;
+  g_precon_failed = true;
+  auto fail = [&](const char *msg) {
+    fprintf(stderr, "andromeda_hip: lsqr_precon_: %s\n", msg);
+    if (q && n && *n > 0)
+      for (int j = 0; j < *n; ++j) q[j] = std::numeric_limits<double>::quiet_NaN();
+  };
+  if (!m || !n || !nnz || !q || !ia || !ja || !a) return fail("null argument");
+  andr_ctx *ctx = andr_default_ctx();
+  if (!ctx) return fail("no device");
+  (void)hipSetDevice(ctx->device);
   if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D) || c.q.alloc(*n) ||
-      andr_colscale(ctx, c.D.p, false, dense_ld(*n), *m, *n, c.q.p)) {
-    fprintf(stderr, "andromeda_hip: lsqr_precon_: %s\n", ctx->err.c_str());
-    c.key_a = nullptr;
-    return;
-  }
-  if (hipMemcpy(q, c.q.p, sizeof(double) * (*n), hipMemcpyDeviceToHost) != hipSuccess) return;
+      andr_colscale(ctx, c.D.p, false, dense_ld(*n), *m, *n, c.q.p))
+    return fail(ctx->err.c_str());
+  if (hipMemcpy(q, c.q.p, sizeof(double) * (*n), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail("copy of the column scales");
   // A <- A diag(q) on the host array too (lsqr_matvec4, src/lib_lsqr.f:301-329)
   for (int i = 0; i < *m; ++i)
     for (int64_t p = ia[i] - 1; p < ia[i + 1] - 1; ++p) a[p] = a[p] * q[ja[p] - 1];
-  c.key_a = a;
-  c.nnz = *nnz;
-  c.m = *m;
-  c.n = *n;
-  probe_fill(c, a, *nnz);
+  cache_key(c, *m, *n, *nnz, ia, ja, a);
+  g_precon_failed = false;
 }
 
 extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *nnz, int32_t *maxit, const double *eps,
@@ -363,22 +416,20 @@ extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *n
                             const double *a, const double *b, double *x) {
   std::lock_guard<std::mutex> lk(g_mu);
   andr_ctx *ctx = andr_default_ctx();
-  if (!ctx) {
+  if (!ctx || g_precon_failed) {
+    if (g_precon_failed) fprintf(stderr, "andromeda_hip: lsqr_solve_: the last lsqr_precon_ failed\n");
     *ierr = -1;
     return;
   }
   (void)hipSetDevice(ctx->device);
   LsqrCache &c = g_cache;
-  if (!(c.key_a == a && c.nnz == *nnz && c.m == *m && c.n == *n && probe_ok(c, a))) {
+  if (!cache_hit(c, *m, *n, *nnz, ia, ja, a)) {
     if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D)) {
+      c.key_a = nullptr;
       *ierr = -2;
       return;
     }
-    c.key_a = a;
-    c.nnz = *nnz;
-    c.m = *m;
-    c.n = *n;
-    probe_fill(c, a, *nnz);
+    cache_key(c, *m, *n, *nnz, ia, ja, a);
   }
   if (*maxit == 0) *maxit = 10 * (*m + *n + 50);
   const int M = *m, N = *n;

@@ -838,7 +838,7 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 }  // namespace
 
 int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int nrhs, int restart, int maxit,
-                     double tol, int *iters) {
+                     double tol, int *iters, int *status) {
   if (nsys <= 0) return ANDR_OK;
   if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
   const int m = std::max(4, restart);
@@ -1202,5 +1202,13 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   if (iters)
     for (int s = 0; s < nsys; ++s)
       for (int k = 0; k < nrhs; ++k) iters[s * nrhs + k] = it_count[(size_t)s * NB + k];
+  // hsmall holds the state of the last restart test (or of the last cycle's end): a column still
+  // active stopped at the iteration limit; one marked by the floor rule was accepted above tol
+  if (status)
+    for (int s = 0; s < nsys; ++s)
+      for (int k = 0; k < nrhs; ++k) {
+        const double *sm = sm_of(s, k);
+        status[s * nrhs + k] = sm[4] > 0.0 ? ANDR_GMRES_MAXIT : (sm[7] > 0.0 ? ANDR_GMRES_FLOOR : 0);
+      }
   return ANDR_OK;
 }

@@ -138,6 +138,9 @@ struct DevBody {
   int itri = 5;
   int iquad = 4;
 };
+// Host-side checks of a mesh before anything reads it: sizes, null arrays, nvert, the range of
+// every connectivity entry and wall id, itri and iquad. ANDR_ERR_ARG with a message on failure.
+int andr_check_mesh(andr_ctx *ctx, const andr_mesh *m);
 int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors);
 
 // kernels exposed between translation units ----------------------------------------------------

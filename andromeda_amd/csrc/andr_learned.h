@@ -4,6 +4,9 @@
 #include "andr_internal.h"
 #include "andr_assemble.h"
 
+// algorithmic flops of one pass of the 6 -> 64 -> 64 -> 64 -> 8 network: 2 x (6*64 + 2*64*64 + 64*8)
+#define ANDR_MLP_FLOPS_PER_PASS 18176.0
+
 struct LearnedStats {
   int64_t points;    // regular quadrature points evaluated
   int64_t learned;   // near-field pairs integrated with the rule chosen by the network

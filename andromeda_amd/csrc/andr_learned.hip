@@ -797,6 +797,8 @@ extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *params, flo
     wb[i] = (uint16_t)(u >> 16);
   }
   const size_t bytes = NPARAM * (sizeof(uint16_t) + sizeof(float)) + ANDR_FAR_BINS * sizeof(int) + 8 * sizeof(int);
+  // the previous model is gone from here on: a learned call made after a failure below is refused
+  ctx->has_model = false;
   if (ctx->d_model) (void)hipFree(ctx->d_model);
   ctx->d_model = nullptr;
   ANDR_HIP(ctx, hipMalloc(&ctx->d_model, bytes));
@@ -838,15 +840,25 @@ int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJo
   ANDR_HIP(ctx, hipMemcpyAsync(hs.data(), st.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  double bytes = 0.0, points = 0.0;
+  // fast-form system: the six symmetric FP32 blocks, 6 F^2 entries of 4 bytes
+  double bytes = 0.0, points = 0.0, near = 0.0, cls_bytes = 0.0;
   for (int i = 0; i < njobs; ++i) {
-    bytes += 9.0 * h_jobs[i].g.ne * (double)h_jobs[i].g.ne * 4.0;
+    bytes += 6.0 * h_jobs[i].g.ne * (double)h_jobs[i].g.ne * 4.0;
+    cls_bytes += h_jobs[i].g.ne * (double)h_jobs[i].g.ne;  // one class byte per pair
+    near += (double)(hs[3 * (size_t)i + 1] + hs[3 * (size_t)i + 2]);
     points += (double)hs[3 * (size_t)i];
     if (stats) {
       stats[i].points = (int64_t)hs[3 * (size_t)i];
       stats[i].learned = (int64_t)hs[3 * (size_t)i + 1];
       stats[i].fallback = (int64_t)hs[3 * (size_t)i + 2];
     }
+  }
+  if (tok >= 0) {  // the classification's work is known once the counts are back
+    // two network passes per near pair (Stokeslet and stresslet), 2 x (6*64 + 64*64 + 64*64 + 64*8)
+    // flops each on the BF16 matrix cores; the class bytes written
+    ProfRec &r = andr_prof(ctx).pending[tok];
+    r.flops = near * 2.0 * ANDR_MLP_FLOPS_PER_PASS;
+    r.bytes = cls_bytes;
   }
   return andr_launch_stokes_system(ctx, d_jobs, d_rowmap, nrows, true, true, bytes, points * ANDR_FLOPS_PER_STOKES_EVAL);
 }

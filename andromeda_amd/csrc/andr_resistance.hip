@@ -245,7 +245,8 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
   double alg_bytes = 0.0, alg_flops = 0.0;
   for (int i = 0; i < nb; ++i) {
     const double ne = P[i]->ne;
-    alg_bytes += 9.0 * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
+    // plain: the full 3F x 3F FP64 matrix; fast forms: the six symmetric FP32 blocks (6 F^2)
+    alg_bytes += (plain ? 9.0 : 6.0) * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
     alg_flops += ne * (ne - 1.0) * jobs[i].nq * ANDR_FLOPS_PER_STOKES_EVAL;
   }
   if (!plain && (rc = andr_launch_self_blocks(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
@@ -279,7 +280,7 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
       gs[i] = {nullptr, 0, 3 * P[i]->ne, op, Bbuf[i].p, Xbuf[i].p};
     }
     rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
-                          iters.data());
+                          iters.data(), stops.data());
     if (rc) return rc;
     if ((rc = andr_launch_unprecondition(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
   }
@@ -374,9 +375,13 @@ struct andr_bodies {
 
 static int check_meshes(andr_ctx *ctx, int32_t nbodies, const andr_mesh *meshes) {
   if (nbodies <= 0 || !meshes) return andr_fail(ctx, ANDR_ERR_ARG, "no bodies");
-  for (int b = 0; b < nbodies; ++b)
+  for (int b = 0; b < nbodies; ++b) {
+    // every array is checked here, before the host-side closure tests read the meshes
+    int rc = andr_check_mesh(ctx, &meshes[b]);
+    if (rc) return andr_fail(ctx, rc, "body %d: %s", b, std::string(ctx->err).c_str());
     if (meshes[b].nvert == 4)
       return andr_fail(ctx, ANDR_ERR_ARG, "body %d: resistance tensors take triangle meshes (nvert 0 or 3)", b);
+  }
   return ANDR_OK;
 }
 

@@ -91,6 +91,12 @@ struct GmresSys {
   double *X;        // n x ANDR_NB
 };
 // Batched restarted GMRES (right-preconditioned, FP64 vectors) on structured Stokes systems; fp32
-// must be true (the FP32 symmetric-block operator). iters: per system and rhs, the Krylov steps.
+// must be true (the FP32 symmetric-block operator). iters: per system and rhs, the Krylov steps;
+// status (may be null): per system and rhs, 0 converged (||r|| <= tol ||b||), ANDR_GMRES_FLOOR
+// stagnated at a relative residual <= 1e-6 above tol (the FP32 floor of the near-null direction;
+// accepted), ANDR_GMRES_MAXIT stopped at the iteration limit. The codes follow LSQR's istop
+// (4: ill-conditioned, 5: iteration limit), so a resistance report's status is > 3 for both.
+#define ANDR_GMRES_FLOOR 4
+#define ANDR_GMRES_MAXIT 5
 int andr_gmres_batch(andr_ctx *ctx, const GmresSys *sys, int nsys, bool fp32, int nrhs, int restart, int maxit,
-                     double tol, int *iters);
+                     double tol, int *iters, int *status);

@@ -51,6 +51,13 @@ typedef struct {
                             40,48 per direction); 0 -> 4 (src/mod_par.f90:233-234) */
 } andr_mesh;
 
+/* Host-side checks of a mesh, the ones every entry point taking a mesh makes before it reads the
+ * arrays: positive sizes, non-null xyz and con, nvert 0/3/4, every connectivity entry in range
+ * (after index_base), every bcid in 1..nwalls, itri 0..5, iquad 0..8. Returns ANDR_OK or
+ * ANDR_ERR_ARG with the reason in msg (msglen bytes, may be NULL). No context, no device (the
+ * reference checks its input in its mesh readers, src/lib_meshRead.f90). */
+int andr_mesh_validate(const andr_mesh *mesh, char *msg, int32_t msglen);
+
 /* ---- context -------------------------------------------------------------------------- */
 int andr_create(andr_ctx **ctx, int32_t device);
 int andr_destroy(andr_ctx *ctx);
@@ -78,8 +85,12 @@ int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, d
  * lsqr_precon_: q = 1/||column||_2, a <- a diag(q) in place (src/lib_lsqr.f:131-167).
  * lsqr_solve_: hot start from x, maxit == 0 -> 10(m+n+50) written back, ierr = istop if > 3
  * (src/lib_lsqr.f:33-127). Runs on the device of the process default context (ANDR_DEVICE env,
- * default 0). The device copy made by lsqr_precon_ is reused by later lsqr_solve_ calls on the
- * same array; andr_lsqr_forget() drops it. */
+ * default 0). Duplicate (i, j) entries of the CSR add up, as in the reference's products. The
+ * device copy of the matrix is reused by later lsqr_solve_ calls while the array pointers, the
+ * sizes and a checksum of every byte of ia, ja and a are unchanged (else the matrix is uploaded
+ * again); andr_lsqr_forget() drops it. The Fortran interface has no status argument for
+ * lsqr_precon_: when it fails, q is set to NaN, a is left unscaled and every lsqr_solve_ returns
+ * ierr = -1 until an lsqr_precon_ succeeds. */
 void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *nnz, double *q, const int64_t *ia,
                   const int32_t *ja, double *a);
 void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *nnz, int32_t *maxit, const double *eps,
@@ -135,7 +146,10 @@ typedef struct {
 
 typedef struct {
   int32_t nit[6];       /* iterations per rigid-body problem */
-  int32_t status;       /* max solver status */
+  int32_t status;       /* max solver status over the six problems: 0 converged; LSQR (plain path):
+                           its istop when > 3 (4 ill-conditioned, 5 iteration limit); GMRES (mixed
+                           and learned): 4 stagnated at a relative residual <= 1e-6 above eps (the
+                           FP32 floor of the near-null direction, accepted), 5 iteration limit */
   double err[2];        /* integral self-checks (fspCerr-like, QspCerr-like) */
   double t_assemble;    /* seconds */
   double t_solve;       /* seconds */

@@ -1,0 +1,122 @@
+"""The benchmark's configurations at their full sizes on the GPU, and the solver status reports.
+
+* Single 8000-triangle superellipsoid (geodesic nu = 20, axes U[0.5, 2], p U[2, 8] from a seed):
+  the mixed and learned forms against the plain FP64 path at LSQR eps 1e-12, tensor error <= 1e-6,
+  and the 6 x 6 tensor symmetric and positive definite.
+* The 576-body batch (512 superellipsoids + 64 fibres, F log-uniform in [500, 8000]) in one call:
+  every tensor finite, symmetric within 1e-6 and positive definite; a stratified subset (every size
+  decile, 8 fibres) against the plain FP64 path within 1e-6.
+
+The discrete resistance tensor of the single-layer collocation system is symmetric to rounding
+(the FP64 oracle gives ||R - R^T|| / ||R|| ~ 1e-12 on 500-triangle superellipsoids), so symmetry
+within 1e-6 is a property the fast forms must keep, not only the continuous problem's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from andromeda_amd import api
+from andromeda_amd import mesh as M
+from andromeda_amd.metrics import tensor_error
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "andromeda_amd", "quad_model.npz")
+
+
+def _install_model(ctx):
+    z = np.load(MODEL)
+    ctx.set_quadrature_model(z["params"], float(z["confidence"]), float(z["near_ratio"]), z["far_table"])
+
+
+def _symmetric_pd(FT):
+    """(block-scaled asymmetry, smallest eigenvalue of the symmetric part scaled by the largest)."""
+    R = np.asarray(FT)
+    asym = tensor_error(R, R.T)
+    ev = np.linalg.eigvalsh(0.5 * (R + R.T))
+    return asym, ev.min() / ev.max()
+
+
+@pytest.mark.parametrize("seed", [3, 117])
+def test_superellipsoid_8000_fast_forms_vs_fp64(ctx, seed):
+    m, info = M.random_superellipsoid(np.random.default_rng(seed), 20)
+    assert m.nelem == 8000
+    p = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+    assert p.status == 0, p.nit
+    _install_model(ctx)
+    asym, ev = _symmetric_pd(p.FT)
+    assert asym < 1e-9 and ev > 0, (asym, ev)
+    for prec in (api.PREC_MIXED, api.PREC_LEARNED):
+        f = ctx.resistance(m, precision=prec)
+        assert f.status == 0, (prec, f.nit)
+        err = tensor_error(f.FT, p.FT)
+        print(f"seed {seed} {info} precision {prec}: err {err:.2e}, its {f.nit.tolist()}")
+        assert err < 1e-6, (prec, err)
+        asym, ev = _symmetric_pd(f.FT)
+        assert asym < 1e-6 and ev > 0, (prec, asym, ev)
+
+
+def _stratified(bodies, n_super, k_super, k_fibre):
+    """k_super superellipsoids spread over the size order (so every decile is hit) and k_fibre
+    fibres spread over theirs."""
+    def spread(idx, k):
+        order = sorted(idx, key=lambda i: bodies[i].nelem)
+        return [order[int(round(j * (len(order) - 1) / (k - 1)))] for j in range(k)]
+    return sorted(spread(range(n_super), k_super) + spread(range(n_super, len(bodies)), k_fibre))
+
+
+def test_batch_576_every_tensor_and_stratified_fp64(ctx):
+    bodies = [m for m, _ in M.batch_bodies(512, 64)]
+    _install_model(ctx)
+    res = ctx.upload_bodies(bodies)
+    try:
+        out = ctx.resistance_batch_resident(res, precision=api.PREC_LEARNED)
+    finally:
+        res.free()
+    assert len(out) == 576
+    bad = []
+    for i, r in enumerate(out):
+        asym, ev = _symmetric_pd(r.FT)
+        if not (np.all(np.isfinite(r.FT)) and asym < 1e-6 and ev > 0 and r.status in (0, 4)):
+            bad.append((i, bodies[i].nelem, asym, ev, r.status))
+    assert not bad, bad[:10]
+    idx = _stratified(bodies, 512, 24, 8)
+    deciles = {int(10 * np.searchsorted(np.sort([b.nelem for b in bodies]), bodies[i].nelem) / 576) for i in idx}
+    assert len(idx) == 32 and len(deciles) >= 9
+    sample = ctx.upload_bodies([bodies[i] for i in idx])
+    try:
+        plain = ctx.resistance_batch_resident(sample, precision=api.PREC_FP64, eps=1e-12)
+    finally:
+        sample.free()
+    errs = [tensor_error(out[i].FT, p.FT) for i, p in zip(idx, plain)]
+    print("stratified sample (body, F, err): " +
+          ", ".join(f"({i}, {bodies[i].nelem}, {e:.1e})" for i, e in zip(idx, errs)))
+    assert all(p.status == 0 for p in plain)
+    assert max(errs) < 1e-6, max(errs)
+
+
+def test_gmres_reports_iteration_limit(ctx):
+    """A fast-form solve cut at 4 Krylov steps reports the iteration limit (status 5), LSQR's istop
+    code for the same condition; the converged solve reports 0."""
+    m = M.random_superellipsoid(np.random.default_rng(7), 6)[0]
+    ok = ctx.resistance(m, precision=api.PREC_MIXED)
+    assert ok.status == 0
+    cut = ctx.resistance(m, precision=api.PREC_MIXED, maxit=4)
+    assert cut.status == 5 and cut.nit.max() <= 4
+    lsqr_cut = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12, maxit=3)
+    assert lsqr_cut.status == 5 and lsqr_cut.nit.max() == 3
+
+
+def test_resistance_refuses_bad_wall_ids(ctx):
+    """A wall id outside 1..nwalls is refused with ANDR_ERR_ARG before any host or device code
+    reads the mesh, for every form and for the resident-body upload."""
+    m = M.geodesic_sphere(2)
+    bad = M.Mesh(m.xyz, m.con, np.where(np.arange(m.nelem) == 3, 0, 1).astype(np.int32), "bad")
+    for prec in (api.PREC_FP64, api.PREC_MIXED):
+        with pytest.raises(api.AndromedaError, match=r"\(-1\).*bcid of element 3"):
+            ctx.resistance(bad, precision=prec)
+    with pytest.raises(api.AndromedaError, match=r"\(-1\)"):
+        ctx.upload_bodies([m, bad])
+    assert ctx.resistance(m, precision=api.PREC_MIXED).status == 0  # the context still works

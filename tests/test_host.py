@@ -100,3 +100,29 @@ def test_tensor_error_measure():
     assert abs(tensor_error(X, R) - 1e-7 / np.sqrt(np.linalg.norm(R[:3, :3]) * np.linalg.norm(R[3:, 3:]))) < 1e-15
     X = R * (1 + 1e-6)
     assert abs(tensor_error(X, R) - 1e-6) < 1e-12
+
+
+def test_mesh_validate_rejects_bad_inputs():
+    """andr_mesh_validate (host only): every check an entry point makes before it reads a mesh --
+    wall ids outside 1..nwalls, connectivity out of range, bad rule indices -- is refused with
+    ANDR_ERR_ARG and a reason; a valid mesh passes."""
+    from andromeda_amd import api
+    if not os.path.exists(api.LIB_PATH):
+        pytest.skip("library not built")
+    m = M.geodesic_sphere(2)
+    assert api.validate_mesh(m) == (0, "")
+    bad = m.bcid.copy()
+    bad[5] = 2
+    rc, msg = api.validate_mesh(m, bcid=bad)
+    assert rc == -1 and "bcid of element 5" in msg
+    bad[5] = 0
+    assert api.validate_mesh(m, bcid=bad)[0] == -1
+    assert api.validate_mesh(m, bcid=bad * 0 + 2, nwalls=2)[0] == 0
+    con = m.con.copy()
+    con[3, 1] = m.nnodes
+    rc, msg = api.validate_mesh(m, con=con)
+    assert rc == -1 and "connectivity" in msg
+    con[3, 1] = -1
+    assert api.validate_mesh(m, con=con)[0] == -1
+    assert api.validate_mesh(m, itri=6)[0] == -1
+    assert api.validate_mesh(m, iquad=9)[0] == -1
