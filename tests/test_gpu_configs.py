@@ -7,9 +7,13 @@
   every tensor finite, symmetric within 1e-6 and positive definite; a stratified subset (every size
   decile, 8 fibres) against the plain FP64 path within 1e-6.
 
-The discrete resistance tensor of the single-layer collocation system is symmetric to rounding
-(the FP64 oracle gives ||R - R^T|| / ||R|| ~ 1e-12 on 500-triangle superellipsoids), so symmetry
-within 1e-6 is a property the fast forms must keep, not only the continuous problem's.
+Symmetry: the continuous resistance tensor is symmetric (reciprocal theorem); the collocation
+system's is symmetric only to its discretisation error. On the centrally symmetric superellipsoids
+the coupling blocks vanish and the tensor is symmetric to rounding (the FP64 oracle: 1.6e-12 on a
+720-triangle body); the fibre meshes (rings of nodes, not centrally symmetric) leave couplings of
+~1e-3 whose transposes differ by ~1e-4 of the block scale in the FP64 oracle itself (8.7e-5 on the
+720-triangle fibre of seed 512). So the superellipsoids are held to 1e-6 symmetry, the fibres to
+the 1e-3 level of their discretisation, and the fast forms' distance to FP64 is checked directly.
 """
 import os
 
@@ -79,7 +83,8 @@ def test_batch_576_every_tensor_and_stratified_fp64(ctx):
     bad = []
     for i, r in enumerate(out):
         asym, ev = _symmetric_pd(r.FT)
-        if not (np.all(np.isfinite(r.FT)) and asym < 1e-6 and ev > 0 and r.status in (0, 4)):
+        sym_tol = 1e-6 if i < 512 else 1e-3  # superellipsoids: rounding; fibres: discretisation
+        if not (np.all(np.isfinite(r.FT)) and asym < sym_tol and ev > 0 and r.status in (0, 4)):
             bad.append((i, bodies[i].nelem, asym, ev, r.status))
     assert not bad, bad[:10]
     idx = _stratified(bodies, 512, 24, 8)
@@ -93,7 +98,10 @@ def test_batch_576_every_tensor_and_stratified_fp64(ctx):
     errs = [tensor_error(out[i].FT, p.FT) for i, p in zip(idx, plain)]
     print("stratified sample (body, F, err): " +
           ", ".join(f"({i}, {bodies[i].nelem}, {e:.1e})" for i, e in zip(idx, errs)))
-    assert all(p.status == 0 for p in plain)
+    print("plain FP64 LSQR status (body, status): " + ", ".join(f"({i}, {p.status})" for i, p in zip(idx, plain)))
+    # LSQR's istop 4 (its condition estimate passed conlim = 1e8) is the reference solver's own
+    # outcome on the slenderest fibres; 5 (iteration limit) would leave the reference unconverged
+    assert all(p.status in (0, 4) for p in plain)
     assert max(errs) < 1e-6, max(errs)
 
 
