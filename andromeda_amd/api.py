@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
                                                    C.POINTER(_ResReport)]
     lib.andr_bodies_free.argtypes = [C.c_void_p, C.c_void_p]
     lib.andr_mesh_validate.argtypes = [C.POINTER(_Mesh), C.c_char_p, C.c_int32]
+    lib.andr_bicgstab_dense_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                                C.c_void_p, C.c_double, C.c_int32, C.POINTER(_LsqrStats)]
     lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_autotune_read.argtypes = [C.c_void_p, C.c_int32, _I, _I, _I, _D]
     _lib = lib
@@ -125,7 +127,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
-            "andr_bicgstab_dense", "andr_mesh_validate"]
+            "andr_bicgstab_dense", "andr_mesh_validate", "andr_bicgstab_dense_noalloc"]
 FAR_BINS = 16
 
 
@@ -385,6 +387,16 @@ class Context:
         self._check(self.lib.andr_bicgstab_dense(self.h, n, _p(A), nrhs, _p(Bc), _p(Xc), tol, maxit, C.byref(st)),
                     "andr_bicgstab_dense")
         return Xc.T.copy(), {"nit": list(st.nit)[:nrhs], "status": list(st.istop)[:nrhs], "seconds": st.seconds}
+
+    def bicgstab_dense_noalloc(self, n: int, A_dev: int, lda: int, nrhs: int, B_dev: int, X_dev: int,
+                               tol: float = 1e-8, maxit: int = 1000):
+        """andr_bicgstab_dense_noalloc on caller-owned device arrays (raw device addresses): A n x n
+        row-major with leading dimension lda, B and X n x nrhs with column k contiguous."""
+        st = _LsqrStats()
+        self._check(self.lib.andr_bicgstab_dense_noalloc(self.h, n, C.c_void_p(A_dev), lda, nrhs, C.c_void_p(B_dev),
+                                                         C.c_void_p(X_dev), tol, maxit, C.byref(st)),
+                    "andr_bicgstab_dense_noalloc")
+        return {"nit": list(st.nit)[:nrhs], "status": list(st.istop)[:nrhs], "seconds": st.seconds}
 
     # ---- seam 3 ---------------------------------------------------------------------------------
     def resistance_batch(self, meshes, form: int = FORM_EXTERIOR, precision: int = PREC_FP64, mu: float = 1.0,

@@ -53,7 +53,7 @@ extern "C" int andr_create(andr_ctx **out, int32_t device) {
     delete ctx;
     return ANDR_ERR_HIP;
   }
-  if (andr_upload_rules(ctx) != ANDR_OK) {
+  if (andr_upload_rules(ctx) != ANDR_OK || andr_scratch_setup(ctx) != ANDR_OK) {
     fprintf(stderr, "andromeda_hip: %s\n", ctx->err.c_str());
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

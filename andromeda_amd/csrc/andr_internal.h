@@ -150,6 +150,10 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
 // transposes an m x n row-major device matrix into a column-major host buffer
 int andr_copy_out(andr_ctx *ctx, const double *d, int64_t m, int64_t n, int32_t layout, double *h);
 
+// Device address of the no-allocation work space on the context's device, looked up once per
+// device (andr_noalloc.hip); called by andr_create with the device current.
+int andr_scratch_setup(andr_ctx *ctx);
+
 // Process-wide default context used by the Fortran-style entry points.
 andr_ctx *andr_default_ctx();
 

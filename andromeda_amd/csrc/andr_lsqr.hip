@@ -611,6 +611,8 @@ struct LsqrRun {
   int nsegN = 1, seglenN = 0, partsN = 0;  // A X: column segments and the partial sums it leaves
   int tilesN = 4;                           // A X: 16-row tiles per wave (autotuned)
   bool vecN = false;
+  bool quiet = false;  // no profiling records (the no-allocation path: event creation is skipped)
+  double *pn = nullptr;  // A X segment partials when they come from a caller's workspace
   DevBuf<double> U, V, W, Xl, X0, Xs, P, PN, partU, partV, partW, partB, sc;
   DevBuf<LsqrState> S;
   DevBuf<int> act;
@@ -619,7 +621,7 @@ struct LsqrRun {
   double abytes = 0, vbytes = 0, aflops = 0;
 
   int tok(const char *name, double bytes) {
-    return st == ctx->stream ? andr_prof_begin(ctx, name, bytes, aflops) : -1;
+    return st == ctx->stream && !quiet ? andr_prof_begin(ctx, name, bytes, aflops) : -1;
   }
   template <int T, bool SEG>
   void launch_mf_t(const double *Xin, double *Y, const double *sx, const double *sy, double *out) {
@@ -693,8 +695,9 @@ struct LsqrRun {
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
     } else {
-      launch_mf(tilesN, nsegN > 1, Xin, Y, sx, sy, nsegN > 1 ? PN.p : part);
-      if (nsegN > 1) hipLaunchKernelGGL(k_combine_n, dim3(blocksM), dim3(256), 0, st, m, nsegN, PN.p, Y, sx, sy, part);
+      double *segp = pn ? pn : PN.p;
+      launch_mf(tilesN, nsegN > 1, Xin, Y, sx, sy, nsegN > 1 ? segp : part);
+      if (nsegN > 1) hipLaunchKernelGGL(k_combine_n, dim3(blocksM), dim3(256), 0, st, m, nsegN, segp, Y, sx, sy, part);
     }
     andr_prof_end(ctx, t);
   }
@@ -1112,12 +1115,55 @@ __global__ void k_bicg_check(BicgState *st, const double *part, int np, int itnl
 
 }  // namespace
 
+namespace {
+// the shape of the BiCGSTAB product A X: column segments and the partial sums they leave (a function
+// of n only, so that the workspace size is known before the call)
+struct BicgLayout {
+  int seglen, nseg, blocksM, partsN;
+};
+BicgLayout bicg_layout(int n) {
+  BicgLayout L;
+  L.seglen = 1024;
+  const int64_t rw = (n + 63) / 64;
+  if (rw * ((n + 1023) / 1024) < 2048) L.seglen = (int)std::max<int64_t>(256, ((int64_t)n * rw / 2048 + 15) / 16 * 16);
+  L.nseg = (n + L.seglen - 1) / L.seglen;
+  L.blocksM = (n * NB + 255) / 256;
+  L.partsN = L.nseg == 1 ? (n + 63) / 64 : L.blocksM;
+  return L;
+}
+// doubles of each workspace array, in the order they are carved from a caller's region
+void bicg_sizes(int n, size_t sz[15]) {
+  const BicgLayout L = bicg_layout(n);
+  for (int i = 0; i < 7; ++i) sz[i] = (size_t)n * NB;  // r, r_hat, p, v, y, z, t
+  sz[7] = n;                                           // dinv
+  sz[8] = (size_t)std::max(L.blocksM, L.partsN) * NB;  // p1
+  sz[9] = (size_t)L.blocksM * NB;                      // p2
+  sz[10] = NB;                                         // one
+  sz[11] = NB;                                         // zero
+  sz[12] = 6 * NB;                                     // scalars
+  sz[13] = (sizeof(BicgState) * NB + 7) / 8;           // per-column states
+  sz[14] = L.nseg > 1 ? (size_t)L.nseg * n * NB : 0;   // A X segment partials
+}
+__global__ void k_fill(size_t n, double v, double *__restrict__ p) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+}  // namespace
+
+size_t andr_bicgstab_ws_bytes(int n) {
+  size_t sz[15], tot = 0;
+  bicg_sizes(n, sz);
+  for (size_t v : sz) tot += (v + 31) / 32 * 32;  // every array on a 256-byte boundary
+  return tot * sizeof(double);
+}
+
 int andr_bicgstab_device(andr_ctx *ctx, const double *A, int64_t lda, int n, const double *B, double *X, int nrhs,
-                         int maxit, double tol, int *nit, int *status) {
+                         int maxit, double tol, int *nit, int *status, double *ws, size_t ws_bytes) {
   if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
   if (n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
   hipStream_t st = ctx->stream;
-  // the matrix-core product of the LSQR run object, with its own workspaces (q unused)
+  // the matrix-core product of the LSQR run object (q unused)
+  const BicgLayout L = bicg_layout(n);
   LsqrRun R;
   R.ctx = ctx;
   R.st = st;
@@ -1125,76 +1171,85 @@ int andr_bicgstab_device(andr_ctx *ctx, const double *A, int64_t lda, int n, con
   R.fp32 = false;
   R.lda = lda;
   R.m = R.n = n;
-  R.seglenN = 1024;
-  {
-    const int64_t rw = (n + 63) / 64;
-    if (rw * ((n + 1023) / 1024) < 2048) R.seglenN = (int)std::max<int64_t>(256, ((int64_t)n * rw / 2048 + 15) / 16 * 16);
-  }
-  R.nsegN = (n + R.seglenN - 1) / R.seglenN;
+  R.seglenN = L.seglen;
+  R.nsegN = L.nseg;
   R.tilesN = 4;
   R.vecN = lda % 2 == 0 && ((uintptr_t)A % 16) == 0;
-  R.blocksM = (n * NB + 255) / 256;
-  R.partsN = R.nsegN == 1 ? (n + 63) / 64 : R.blocksM;
+  R.blocksM = L.blocksM;
+  R.partsN = L.partsN;
   const int nb = R.blocksM;
-  DevBuf<double> Rr, Rh, P, V, Yv, Z, Tt, dinv, p1, p2, one, zero, sc;
-  DevBuf<BicgState> S;
-  if (Rr.alloc((size_t)n * NB, ctx) || Rh.alloc((size_t)n * NB, ctx) || P.alloc((size_t)n * NB, ctx) ||
-      V.alloc((size_t)n * NB, ctx) || Yv.alloc((size_t)n * NB, ctx) || Z.alloc((size_t)n * NB, ctx) ||
-      Tt.alloc((size_t)n * NB, ctx) || dinv.alloc(n, ctx) || p1.alloc((size_t)std::max(nb, R.partsN) * NB, ctx) ||
-      p2.alloc((size_t)nb * NB, ctx) || one.alloc(NB, ctx) || zero.alloc(NB, ctx) || sc.alloc(6 * NB, ctx) ||
-      S.alloc(NB, ctx) || R.PN.alloc(R.nsegN > 1 ? (size_t)R.nsegN * n * NB : 0, ctx))
-    return andr_fail(ctx, ANDR_ERR_ALLOC, "BiCGSTAB workspace (n=%d)", n);
-  double h1[NB], h0[NB];
-  for (int k = 0; k < NB; ++k) {
-    h1[k] = 1.0;
-    h0[k] = 0.0;
+  // workspace: the context's memory cache, or carved from the caller's region (no allocation)
+  DevBuf<double> own[15];
+  double *w[15];
+  size_t sz[15];
+  bicg_sizes(n, sz);
+  if (ws) {
+    if (ws_bytes < andr_bicgstab_ws_bytes(n))
+      return andr_fail(ctx, ANDR_ERR_ARG, "BiCGSTAB workspace of %zu bytes, %zu needed (n=%d)", ws_bytes,
+                       andr_bicgstab_ws_bytes(n), n);
+    double *cur = ws;
+    for (int i = 0; i < 15; ++i) {
+      w[i] = sz[i] ? cur : nullptr;
+      cur += (sz[i] + 31) / 32 * 32;
+    }
+    R.quiet = true;
+  } else {
+    for (int i = 0; i < 15; ++i) {
+      if (own[i].alloc(sz[i], ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "BiCGSTAB workspace (n=%d)", n);
+      w[i] = own[i].p;
+    }
   }
-  ANDR_HIP(ctx, hipMemcpyAsync(one.p, h1, sizeof(h1), hipMemcpyHostToDevice, st));
-  ANDR_HIP(ctx, hipMemcpyAsync(zero.p, h0, sizeof(h0), hipMemcpyHostToDevice, st));
-  // the products write out = 1 * (A in) + 0 * out: their outputs start finite
-  ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
-  ANDR_HIP(ctx, hipMemsetAsync(Tt.p, 0, sizeof(double) * n * NB, st));
+  double *Rr = w[0], *Rh = w[1], *P = w[2], *V = w[3], *Yv = w[4], *Z = w[5], *Tt = w[6], *dinv = w[7], *p1 = w[8],
+         *p2 = w[9], *one = w[10], *zero = w[11], *sc = w[12];
+  BicgState *S = (BicgState *)w[13];
+  R.pn = w[14];
+  // constants and cleared outputs by fill kernels (no host copies, no memset nodes): the products
+  // write out = 1 * (A in) + 0 * out, so their outputs start finite
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (size_t)NB, 1.0, one);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (size_t)NB, 0.0, zero);
+  hipLaunchKernelGGL(k_fill, dim3((n * NB + 255) / 256), dim3(256), 0, st, (size_t)n * NB, 0.0, V);
+  hipLaunchKernelGGL(k_fill, dim3((n * NB + 255) / 256), dim3(256), 0, st, (size_t)n * NB, 0.0, Tt);
   R.abytes = (double)n * n * 8.0;
   R.vbytes = 2.0 * n * NB * 8.0;
   R.aflops = 2.0 * n * n * nrhs;
   const int gb = (n * NB + 255) / 256;
   auto product = [&](const double *in, double *out) {  // out = A in (its partial squared norms are not used)
-    R.gemv_n(in, out, one.p, zero.p, p1.p);
+    R.gemv_n(in, out, one, zero, p1);
   };
-  hipLaunchKernelGGL(k_bicg_diag, dim3((n + 255) / 256), dim3(256), 0, st, A, lda, n, dinv.p);
-  product(X, V.p);  // A x0
-  hipLaunchKernelGGL(k_bicg_init, dim3(gb), dim3(256), 0, st, n, B, V.p, Rr.p, Rh.p, P.p, V.p);
-  hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, B, B, Rr.p, Rr.p, p1.p, p2.p);
-  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, p2.p, nb, tol, nrhs, sc.p);
+  hipLaunchKernelGGL(k_bicg_diag, dim3((n + 255) / 256), dim3(256), 0, st, A, lda, n, dinv);
+  product(X, V);  // A x0
+  hipLaunchKernelGGL(k_bicg_init, dim3(gb), dim3(256), 0, st, n, B, V, Rr, Rh, P, V);
+  hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, B, B, Rr, Rr, p1, p2);
+  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S, p1, p2, nb, tol, nrhs, sc);
   const int itnlim = maxit > 0 ? maxit : 1000;
   BicgState hs[NB];
   for (int it = 0; it < itnlim;) {
     for (int inner = 0; inner < 8 && it < itnlim; ++inner, ++it) {
-      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh.p, Rr.p, (const double *)nullptr,
-                         (const double *)nullptr, p1.p, p2.p);
-      hipLaunchKernelGGL(k_bicg_beta, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, sc.p);
-      hipLaunchKernelGGL(k_bicg_p, dim3(gb), dim3(256), 0, st, n, Rr.p, V.p, P.p, Yv.p, dinv.p, sc.p, S.p);
-      product(Yv.p, V.p);
-      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh.p, V.p, (const double *)nullptr,
-                         (const double *)nullptr, p1.p, p2.p);
-      hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, sc.p);
-      hipLaunchKernelGGL(k_bicg_s, dim3(gb), dim3(256), 0, st, n, X, Yv.p, Rr.p, V.p, Z.p, dinv.p, sc.p);
-      product(Z.p, Tt.p);
-      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Tt.p, Rr.p, Tt.p, Tt.p, p1.p, p2.p);
-      hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, p2.p, nb, sc.p);
-      hipLaunchKernelGGL(k_bicg_x, dim3(gb), dim3(256), 0, st, n, X, Z.p, Rr.p, Tt.p, sc.p);
-      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rr.p, Rr.p, (const double *)nullptr,
-                         (const double *)nullptr, p1.p, p2.p);
-      hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S.p, p1.p, nb, itnlim);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh, Rr, (const double *)nullptr,
+                         (const double *)nullptr, p1, p2);
+      hipLaunchKernelGGL(k_bicg_beta, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S, p1, nb, sc);
+      hipLaunchKernelGGL(k_bicg_p, dim3(gb), dim3(256), 0, st, n, Rr, V, P, Yv, dinv, sc, S);
+      product(Yv, V);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rh, V, (const double *)nullptr,
+                         (const double *)nullptr, p1, p2);
+      hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S, p1, nb, sc);
+      hipLaunchKernelGGL(k_bicg_s, dim3(gb), dim3(256), 0, st, n, X, Yv, Rr, V, Z, dinv, sc);
+      product(Z, Tt);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Tt, Rr, Tt, Tt, p1, p2);
+      hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S, p1, p2, nb, sc);
+      hipLaunchKernelGGL(k_bicg_x, dim3(gb), dim3(256), 0, st, n, X, Z, Rr, Tt, sc);
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gb), dim3(256), 0, st, n, Rr, Rr, (const double *)nullptr,
+                         (const double *)nullptr, p1, p2);
+      hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(LSQR_SCALAR_THREADS), 0, st, S, p1, nb, itnlim);
     }
     ANDR_HIP(ctx, hipGetLastError());
-    ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
     ANDR_HIP(ctx, hipStreamSynchronize(st));
     bool any = false;
     for (int k = 0; k < nrhs; ++k) any = any || hs[k].active;
     if (!any) break;
   }
-  ANDR_HIP(ctx, hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
   ANDR_HIP(ctx, hipStreamSynchronize(st));
   for (int k = 0; k < nrhs; ++k) {
     if (nit) nit[k] = hs[k].itn;

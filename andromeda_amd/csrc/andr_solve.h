@@ -49,8 +49,12 @@ struct LsqrJob {
 // leading dimension lda, B and X n x ANDR_NB interleaved, X the initial guess and the result):
 // per column nit = iterations, status 0 converged (||r|| <= tol ||b||), 1 iteration limit,
 // 2 breakdown. maxit <= 0: 1000.
+// ws: null -> the workspace comes from the context's memory cache; else a device region of ws_bytes
+// (at least andr_bicgstab_ws_bytes(n), 256-byte aligned) that the workspace is carved from, with no
+// allocation, no memset and no profiling records in the call.
 int andr_bicgstab_device(andr_ctx *ctx, const double *A, int64_t lda, int n, const double *B, double *X, int nrhs,
-                         int maxit, double tol, int *nit, int *status);
+                         int maxit, double tol, int *nit, int *status, double *ws = nullptr, size_t ws_bytes = 0);
+size_t andr_bicgstab_ws_bytes(int n);
 int andr_lsqr_device_batch(andr_ctx *ctx, int count, const LsqrJob *jobs);
 
 // Restarted GMRES on a batch of square systems (FP32 or FP64 storage, FP64 vectors); restart

@@ -123,6 +123,18 @@ int andr_lsqr_dense(andr_ctx *ctx, int64_t m, int64_t n, const double *A, int32_
 int andr_bicgstab_dense(andr_ctx *ctx, int64_t n, const double *A, int32_t nrhs, const double *B, double *X,
                         double tol, int32_t maxit, andr_lsqr_stats *stats);
 
+/* andr_bicgstab_dense without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). A (n x n, row-major, leading dimension lda >= n),
+ * B and X (n x nrhs, column k contiguous; X the initial guess and the result) are DEVICE pointers
+ * owned by the caller; every work array lives in one fixed 64 MiB device array of the library
+ * (one per device, held until the process ends), so n is limited to what fits it (about 27 000:
+ * ANDR_ERR_ARG above). No hipMalloc, hipFree or memset; the call returns with X final. Calls that
+ * overlap on one device share that array and race: serialize them. Results are those of
+ * andr_bicgstab_dense on the same inputs (same kernels, same summation order). */
+int andr_bicgstab_dense_noalloc(andr_ctx *ctx, int64_t n, const double *A, int64_t lda, int32_t nrhs, const double *B,
+                                double *X, double tol, int32_t maxit, andr_lsqr_stats *stats);
+
 /* ---- seam 3: resistance ------------------------------------------------------------------- */
 #define ANDR_FORM_EXTERIOR 0 /* single surface, rigid-body velocity on the body                */
 #define ANDR_FORM_BOUNDED 1  /* body (wall 1) at rest inside an outer wall 2 carrying the flow */
