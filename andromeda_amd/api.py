@@ -43,6 +43,15 @@ class _LsqrStats(C.Structure):
     _fields_ = [("nit", C.c_int32 * 16), ("istop", C.c_int32 * 16), ("seconds", C.c_double)]
 
 
+class _Subdomain(C.Structure):
+    _fields_ = [("nwalls", C.c_int32), ("walls", _I), ("normmul", _D), ("diff", C.c_double), ("node_wall", _I)]
+
+
+class _LapReport(C.Structure):
+    _fields_ = [("neq", C.c_int32), ("nx", C.c_int32), ("nb", C.c_int32), ("nit", C.c_int32), ("istop", C.c_int32),
+                ("err", C.c_double * 2), ("t_assemble", C.c_double), ("t_solve", C.c_double)]
+
+
 class _ResOpts(C.Structure):
     _fields_ = [("form", C.c_int32), ("precision", C.c_int32), ("mu", C.c_double), ("eps", C.c_double),
                 ("maxit", C.c_int32), ("center", C.c_double * 3), ("restart", C.c_int32)]
@@ -111,6 +120,15 @@ def load_library(path: str = LIB_PATH):
                                                    C.POINTER(_ResReport)]
     lib.andr_bodies_free.argtypes = [C.c_void_p, C.c_void_p]
     lib.andr_mesh_validate.argtypes = [C.POINTER(_Mesh), C.c_char_p, C.c_int32]
+    lib.andr_assemble_laplace_rows.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.c_int32, C.c_int32, _D, _D,
+                                               _D]
+    lib.andr_assemble_stokes_rows.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_double, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D),
+                                              _D]
+    lib.andr_laplace_solve.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), C.c_int32, _I,
+                                       C.POINTER(_LsqrOpts), _D, _D, _D, C.POINTER(_LapReport)]
+    lib.andr_subdomain_node_walls.argtypes = [C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), _I, C.c_char_p,
+                                              C.c_int32]
     lib.andr_bicgstab_dense_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                                 C.c_void_p, C.c_double, C.c_int32, C.POINTER(_LsqrStats)]
     lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
@@ -127,7 +145,9 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
-            "andr_bicgstab_dense", "andr_mesh_validate", "andr_bicgstab_dense_noalloc"]
+            "andr_bicgstab_dense", "andr_mesh_validate", "andr_bicgstab_dense_noalloc", "andr_assemble_laplace_rows",
+            "andr_assemble_stokes_rows", "andr_laplace_solve", "andr_subdomain_node_walls"]
+BC_FUNCTION, BC_FLUX, BC_CONTACT = 1, 2, 3
 FAR_BINS = 16
 
 
@@ -173,6 +193,63 @@ def validate_mesh(mesh: Mesh, nwalls=None, itri: int = 5, iquad: int = 4, con=No
     msg = C.create_string_buffer(256)
     rc = load_library().andr_mesh_validate(C.byref(mv.s), msg, 256)
     return rc, msg.value.decode()
+
+
+@dataclass
+class Subdomain:
+    """Walls (ids 1..nwalls, in the order of the BIC file's SUBDOMAINS line), their NORMMUL factors,
+    the diffusivity, and optional per-node wall overrides (EDGE / CORNER rules: nnodes entries, 0 =
+    the default rule)."""
+    walls: list
+    normmul: list | None = None
+    diff: float = 1.0
+    node_wall: np.ndarray | None = None
+
+
+class _SdView:
+    def __init__(self, sds, nnodes):
+        self.keep = []
+        arr = []
+        for sd in sds:
+            w = np.ascontiguousarray(sd.walls, dtype=np.int32)
+            nm = np.ascontiguousarray(sd.normmul if sd.normmul is not None else np.ones(w.size), dtype=np.float64)
+            nw = None
+            if sd.node_wall is not None:
+                nw = np.ascontiguousarray(sd.node_wall, dtype=np.int32)
+                if nw.size != nnodes:
+                    raise ValueError("node_wall needs one entry per node")
+            self.keep += [w, nm, nw]
+            arr.append(_Subdomain(w.size, _pi(w), _p(nm), float(sd.diff), _pi(nw) if nw is not None else None))
+        self.arr = (_Subdomain * len(arr))(*arr)
+        self.n = len(arr)
+
+
+def subdomain_node_walls(mesh: Mesh, subdomains, itri: int = 5):
+    """(nsd, nnodes) wall id each node of each subdomain takes its condition from (0: not in the
+    subdomain), as andr_laplace_solve uses it (host only)."""
+    mv = _MeshView(mesh, None, itri)
+    sv = _SdView(subdomains, mv.m.nnodes)
+    out = np.zeros((sv.n, mv.m.nnodes), dtype=np.int32)
+    msg = C.create_string_buffer(256)
+    rc = load_library().andr_subdomain_node_walls(C.byref(mv.s), sv.n, sv.arr, _pi(out), msg, 256)
+    if rc != 0:
+        raise AndromedaError(f"andr_subdomain_node_walls failed ({rc}): {msg.value.decode()}")
+    return out
+
+
+@dataclass
+class LaplaceResult:
+    u: np.ndarray          # (nnodes,) function values
+    q: np.ndarray          # (nelem,) fluxes
+    wall_flux: np.ndarray  # (nwalls,) sum of A_e q_e per wall
+    neq: int
+    nx: int
+    nb: int
+    nit: int
+    istop: int
+    err: tuple
+    t_assemble: float
+    t_solve: float
 
 
 @dataclass
@@ -314,6 +391,60 @@ class Context:
         if layout == COL_MAJOR:
             H, G = H.T, G.T
         return H, G, err
+
+    def assemble_laplace_rows(self, mesh: Mesh, row_begin: int, row_end: int, normmul=None, itri: int = 5,
+                              layout: int = ROW_MAJOR, iquad: int = 4):
+        """Rows row_begin .. row_end - 1 of H and G (andr_assemble_laplace_rows)."""
+        mv = _MeshView(mesh, normmul, itri, iquad)
+        nn, ne, nr = mv.m.nnodes, mv.m.nelem, row_end - row_begin
+        H = np.zeros((nr, nn) if layout == ROW_MAJOR else (nn, nr))
+        G = np.zeros((nr, ne) if layout == ROW_MAJOR else (ne, nr))
+        err = np.zeros(2)
+        self._check(self.lib.andr_assemble_laplace_rows(self.h, C.byref(mv.s), row_begin, row_end, layout, _p(H),
+                                                        _p(G), _p(err)), "andr_assemble_laplace_rows")
+        if layout == COL_MAJOR:
+            H, G = H.T, G.T
+        return H, G, err
+
+    def assemble_stokes_rows(self, mesh: Mesh, row_begin: int, row_end: int, normmul=None, itri: int = 5,
+                             extra: float = 0.0, iquad: int = 4):
+        """Rows row_begin .. row_end - 1 of the 18 Stokes matrices (andr_assemble_stokes_rows)."""
+        mv = _MeshView(mesh, normmul, itri, iquad)
+        nn, ne, nr = mv.m.nnodes, mv.m.nelem, row_end - row_begin
+        T = [np.zeros((nr, nn)) for _ in range(6)]
+        G = [np.zeros((nr, ne)) for _ in range(6)]
+        prT = [np.zeros((nr, nn)) for _ in range(3)]
+        prG = [np.zeros((nr, ne)) for _ in range(3)]
+        err = np.zeros(2)
+        self._check(self.lib.andr_assemble_stokes_rows(
+            self.h, C.byref(mv.s), row_begin, row_end, ROW_MAJOR, extra, (_D * 6)(*[_p(x) for x in T]),
+            (_D * 6)(*[_p(x) for x in G]), (_D * 3)(*[_p(x) for x in prT]), (_D * 3)(*[_p(x) for x in prG]),
+            _p(err)), "andr_assemble_stokes_rows")
+        return T, G, prT, prG, err
+
+    def laplace_solve(self, mesh: Mesh, subdomains, wall_bc, u, q, eps=1e-15, maxit=500, itri: int = 5,
+                      iquad: int = 4):
+        """The Laplace boundary-value problem (andr_laplace_solve) for one or more equations:
+        wall_bc (neqn, nwalls) of BC_* kinds, u (neqn, nnodes) and q (neqn, nelem) the known and
+        initial values; eps and maxit scalars or per equation. Returns a LaplaceResult per equation."""
+        mv = _MeshView(mesh, None, itri, iquad)
+        nn, ne, nw = mv.m.nnodes, mv.m.nelem, mv.s.nwalls
+        bc = np.ascontiguousarray(np.atleast_2d(wall_bc), dtype=np.int32)
+        neqn = bc.shape[0]
+        if bc.shape[1] != nw:
+            raise ValueError(f"wall_bc needs {nw} entries per equation")
+        U = np.ascontiguousarray(np.array(u, dtype=np.float64).reshape(neqn, nn))
+        Q = np.ascontiguousarray(np.array(q, dtype=np.float64).reshape(neqn, ne))
+        eps_a = np.broadcast_to(np.asarray(eps, dtype=np.float64), (neqn,))
+        mit_a = np.broadcast_to(np.asarray(maxit, dtype=np.int64), (neqn,))
+        opts = (_LsqrOpts * neqn)(*[_LsqrOpts(int(mit_a[k]), float(eps_a[k]), 1) for k in range(neqn)])
+        flux = np.zeros((neqn, nw))
+        reps = (_LapReport * neqn)()
+        sv = _SdView(subdomains, nn)
+        self._check(self.lib.andr_laplace_solve(self.h, C.byref(mv.s), sv.n, sv.arr, neqn, _pi(bc), opts, _p(U), _p(Q),
+                                                _p(flux), reps), "andr_laplace_solve")
+        return [LaplaceResult(U[k].copy(), Q[k].copy(), flux[k].copy(), r.neq, r.nx, r.nb, r.nit, r.istop,
+                              (r.err[0], r.err[1]), r.t_assemble, r.t_solve) for k, r in enumerate(reps)]
 
     def assemble_stokes(self, mesh: Mesh, normmul=None, itri: int = 5, extra: float = 0.0, pressure: bool = True,
                         iquad: int = 4):

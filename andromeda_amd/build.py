@@ -12,7 +12,7 @@ OUT = os.path.join(HERE, "libandromeda_hip.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["andr_assemble.hip", "andr_lsqr.hip", "andr_gmres.hip", "andr_resistance.hip", "andr_learned.hip", "andr_io.hip",
-           "andr_capi.hip", "andr_prof.hip", "andr_noalloc.hip"]
+           "andr_capi.hip", "andr_prof.hip", "andr_noalloc.hip", "andr_laplace.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 

@@ -468,11 +468,12 @@ template <bool ROW_IN_LDS>
 __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int nq, const int *__restrict__ celems,
                                                       const int *__restrict__ coff, int ncolors,
                                                       double *__restrict__ H, double *__restrict__ G,
-                                                      double *__restrict__ err_rows) {
+                                                      double *__restrict__ err_rows, int r0) {
   extern __shared__ double lds[];
   __shared__ double red[16];
   const int nn = g.nn, ne = g.ne;
-  const int r = blockIdx.x;
+  const int r = r0 + blockIdx.x;  // source point; its outputs go to row blockIdx.x of H, G
+  const int ro = blockIdx.x;
   const bool qrow = r >= nn;
   const int erow = r - nn;
   double s[3];
@@ -480,8 +481,8 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
     for (int j = 0; j < 3; ++j) s[j] = g.xyz[3 * r + j];
   else
     for (int j = 0; j < 3; ++j) s[j] = g.cen[3 * erow + j];
-  double *hrow = ROW_IN_LDS ? lds : H + (size_t)r * nn;
-  double *grow = G + (size_t)r * ne;
+  double *hrow = ROW_IN_LDS ? lds : H + (size_t)ro * nn;
+  double *grow = G + (size_t)ro * ne;
   for (int i = threadIdx.x; i < nn; i += blockDim.x) hrow[i] = 0.0;
   __syncthreads();
   const double k4pi = 0.25 / ANDR_PI;
@@ -572,15 +573,15 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
     err = fabs(0.5 - cfree);
   }
   if (ROW_IN_LDS) {
-    double *out = H + (size_t)r * nn;
+    double *out = H + (size_t)ro * nn;
     for (int i = threadIdx.x; i < nn; i += blockDim.x) out[i] = hrow[i];
   }
-  if (threadIdx.x == 0) err_rows[r] = err;
+  if (threadIdx.x == 0) err_rows[ro] = err;
 }
 
-int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows) {
+int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows, int r0, int nrows) {
   QuadRule qr = andr_rule(b.itri);
-  const int nsp = b.g.nn + b.g.ne;
+  const int nsp = nrows < 0 ? b.g.nn + b.g.ne - r0 : nrows;  // rows r0 .. r0 + nsp - 1
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * b.g.nn;
   const double evals = (double)nsp * b.g.ne * qr.npts;
@@ -590,10 +591,10 @@ int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, do
     ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_laplace_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
     hipLaunchKernelGGL(k_laplace_rows<true>, dim3(nsp), dim3(256), lds, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows);
+                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows, r0);
   } else {
     hipLaunchKernelGGL(k_laplace_rows<false>, dim3(nsp), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows);
+                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows, r0);
   }
   andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
@@ -610,15 +611,15 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
                                                      const int *__restrict__ coff, int ncolors, double extra,
                                                      double *__restrict__ T, double *__restrict__ G,
                                                      double *__restrict__ prT, double *__restrict__ prG,
-                                                     double *__restrict__ err_rows) {
+                                                     double *__restrict__ err_rows, int r0, int nrows) {
   extern __shared__ double lds[];
   __shared__ double red[6 * 4];
   const int nn = g.nn, ne = g.ne;
-  const int nsp = nn + ne;
-  const int r = blockIdx.x;
+  const int r = r0 + blockIdx.x;  // source point; its outputs go to row blockIdx.x of the matrices
+  const int ro = blockIdx.x;
   const bool qrow = r >= nn;
   const int erow = r - nn;
-  const size_t mstrT = (size_t)nsp * nn, mstrG = (size_t)nsp * ne;
+  const size_t mstrT = (size_t)nrows * nn, mstrG = (size_t)nrows * ne;
   double s[3];
   if (!qrow)
     for (int j = 0; j < 3; ++j) s[j] = g.xyz[3 * r + j];
@@ -627,7 +628,7 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
   // node rows: 6 T components then 3 prT components
   auto trow = [&](int k) -> double * {
     if (ROW_IN_LDS) return lds + (size_t)k * nn;
-    return k < 6 ? T + k * mstrT + (size_t)r * nn : prT + (k - 6) * mstrT + (size_t)r * nn;
+    return k < 6 ? T + k * mstrT + (size_t)ro * nn : prT + (k - 6) * mstrT + (size_t)ro * nn;
   };
   for (int k = 0; k < 9; ++k) {
     double *p = trow(k);
@@ -651,8 +652,8 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
         }
         double acc[45];
         quad_stokes_pair(x1, x2, x3, g.xyz + 3 * v4[3], s, g.mul[e], g.iquad, qs, acc);
-        for (int k = 0; k < 6; ++k) G[k * mstrG + (size_t)r * ne + e] = acc[k];
-        for (int k = 0; k < 3; ++k) prG[k * mstrG + (size_t)r * ne + e] = acc[6 + k];
+        for (int k = 0; k < 6; ++k) G[k * mstrG + (size_t)ro * ne + e] = acc[k];
+        for (int k = 0; k < 3; ++k) prG[k * mstrG + (size_t)ro * ne + e] = acc[6 + k];
         for (int k = 0; k < 6; ++k) {
           double *p = trow(k);
           for (int v = 0; v < 4; ++v) p[v4[v]] += acc[9 + 4 * k + v];
@@ -675,8 +676,8 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
       } else {
         andr_sing_stk(x1, x2, x3, s, isrc, Gs);  // T, prT, prG of a singular pair are zero
       }
-      for (int k = 0; k < 6; ++k) G[k * mstrG + (size_t)r * ne + e] = Gs[k];
-      for (int k = 0; k < 3; ++k) prG[k * mstrG + (size_t)r * ne + e] = pG[k];
+      for (int k = 0; k < 6; ++k) G[k * mstrG + (size_t)ro * ne + e] = Gs[k];
+      for (int k = 0; k < 3; ++k) prG[k * mstrG + (size_t)ro * ne + e] = pG[k];
       for (int k = 0; k < 6; ++k) {
         double *p = trow(k);
         for (int v = 0; v < 3; ++v) p[vv[v]] += Tv[k][v];
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
       const double m = g.mul[i];
       const double n0 = g.nrm[3 * i] * m, n1 = g.nrm[3 * i + 1] * m, n2 = g.nrm[3 * i + 2] * m;
-      const size_t o = (size_t)r * ne + i;
+      const size_t o = (size_t)ro * ne + i;
       const double gxx = G[o], gxy = G[mstrG + o], gxz = G[2 * mstrG + o], gyy = G[3 * mstrG + o],
                    gyz = G[4 * mstrG + o], gzz = G[5 * mstrG + o];
       ck[0] -= gxx * n0 + gxy * n1 + gxz * n2;
@@ -747,12 +748,12 @@ __global__ __launch_bounds__(256) void k_stokes_rows(BodyGeom g, int qoff, int n
   __syncthreads();
   if (ROW_IN_LDS) {
     for (int k = 0; k < 9; ++k) {
-      double *out = k < 6 ? T + k * mstrT + (size_t)r * nn : prT + (k - 6) * mstrT + (size_t)r * nn;
+      double *out = k < 6 ? T + k * mstrT + (size_t)ro * nn : prT + (k - 6) * mstrT + (size_t)ro * nn;
       const double *p = lds + (size_t)k * nn;
       for (int i = threadIdx.x; i < nn; i += blockDim.x) out[i] = p[i];
     }
   }
-  if (threadIdx.x == 0) err_rows[r] = err;
+  if (threadIdx.x == 0) err_rows[ro] = err;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -902,9 +903,9 @@ int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double 
 }
 
 int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, double *G, double *prT, double *prG,
-                            double *err_rows) {
+                            double *err_rows, int r0, int nrows) {
   QuadRule qr = andr_rule(b.itri);
-  const int nsp = b.g.nn + b.g.ne;
+  const int nsp = nrows < 0 ? b.g.nn + b.g.ne - r0 : nrows;  // rows r0 .. r0 + nsp - 1
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * 9 * (size_t)b.g.nn;
   const double evals = (double)nsp * b.g.ne * qr.npts;
@@ -914,10 +915,10 @@ int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, 
     ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_stokes_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
     hipLaunchKernelGGL(k_stokes_rows<true>, dim3(nsp), dim3(256), lds, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows);
+                       b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows, r0, nsp);
   } else {
     hipLaunchKernelGGL(k_stokes_rows<false>, dim3(nsp), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows);
+                       b.color_elems.p, b.d_color_off.p, ncol, extra, T, G, prT, prG, err_rows, r0, nsp);
   }
   andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());

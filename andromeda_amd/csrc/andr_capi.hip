@@ -131,13 +131,17 @@ void andr_pool_trim(andr_ctx *ctx) {
 extern "C" const char *andr_last_error(const andr_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
 // ---- seam 1 ----------------------------------------------------------------------------------
-static int max_err(andr_ctx *ctx, const double *d_err, int nn, int ne, double err[2]) {
-  std::vector<double> h(nn + ne);
-  ANDR_HIP(ctx, hipMemcpyAsync(h.data(), d_err, sizeof(double) * (nn + ne), hipMemcpyDeviceToHost, ctx->stream));
+// err[0]: largest U-row (node source point) self-check error of rows r0 .. r0 + nrows - 1, err[1]: the
+// largest of all those rows (fspCerr / QspCerr as CoR_Integrals logs them, src/lib_int.f90:80-87)
+static int max_err(andr_ctx *ctx, const double *d_err, int r0, int nrows, int nn, double err[2]) {
+  std::vector<double> h(nrows);
+  ANDR_HIP(ctx, hipMemcpyAsync(h.data(), d_err, sizeof(double) * nrows, hipMemcpyDeviceToHost, ctx->stream));
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   double eu = 0.0, eq = 0.0;
-  for (int i = 0; i < nn; ++i) eu = std::max(eu, h[i]);
-  for (int i = 0; i < ne; ++i) eq = std::max(eq, h[nn + i]);
+  for (int i = 0; i < nrows; ++i) {
+    if (r0 + i < nn) eu = std::max(eu, h[i]);
+    else eq = std::max(eq, h[i]);
+  }
   if (err) {
     err[0] = eu;
     err[1] = std::max(eu, eq);
@@ -145,51 +149,74 @@ static int max_err(andr_ctx *ctx, const double *d_err, int nn, int ne, double er
   return ANDR_OK;
 }
 
+static int check_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_begin, int32_t row_end, int32_t layout) {
+  if (layout != ANDR_ROW_MAJOR && layout != ANDR_COL_MAJOR) return andr_fail(ctx, ANDR_ERR_ARG, "bad layout");
+  int rc = andr_check_mesh(ctx, mesh);
+  if (rc) return rc;
+  const int nsp = mesh->nnodes + mesh->nelem;
+  if (row_begin < 0 || row_end > nsp || row_begin >= row_end)
+    return andr_fail(ctx, ANDR_ERR_ARG, "row range [%d, %d) outside the %d source points", row_begin, row_end, nsp);
+  return ANDR_OK;
+}
+
+extern "C" int andr_assemble_laplace_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_begin, int32_t row_end,
+                                          int32_t layout, double *H, double *G, double err[2]) {
+  if (!ctx) return ANDR_ERR_ARG;
+  int rc = check_rows(ctx, mesh, row_begin, row_end, layout);
+  if (rc) return rc;
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  DevBody b;
+  if ((rc = andr_body_upload(ctx, mesh, b, true))) return rc;
+  const int nn = b.g.nn, ne = b.g.ne, nr = row_end - row_begin;
+  DevBuf<double> dH, dG, dE;
+  if (dH.alloc((size_t)nr * nn, ctx) || dG.alloc((size_t)nr * ne, ctx) || dE.alloc(nr, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "integral matrices (%d x %d)", nr, nn + ne);
+  if ((rc = andr_launch_laplace_rows(ctx, b, dH.p, dG.p, dE.p, row_begin, nr))) return rc;
+  if (H && (rc = andr_copy_out(ctx, dH.p, nr, nn, layout, H))) return rc;
+  if (G && (rc = andr_copy_out(ctx, dG.p, nr, ne, layout, G))) return rc;
+  return max_err(ctx, dE.p, row_begin, nr, nn, err);
+}
+
 extern "C" int andr_assemble_laplace(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double *H, double *G,
                                      double err[2]) {
   if (!ctx) return ANDR_ERR_ARG;
-  if (layout != ANDR_ROW_MAJOR && layout != ANDR_COL_MAJOR) return andr_fail(ctx, ANDR_ERR_ARG, "bad layout");
+  if (!mesh) return andr_fail(ctx, ANDR_ERR_ARG, "null mesh");
+  return andr_assemble_laplace_rows(ctx, mesh, 0, mesh->nnodes + mesh->nelem, layout, H, G, err);
+}
+
+extern "C" int andr_assemble_stokes_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_begin, int32_t row_end,
+                                         int32_t layout, double extra_free_term, double *T[6], double *G[6],
+                                         double *prT[3], double *prG[3], double err[2]) {
+  if (!ctx) return ANDR_ERR_ARG;
+  int rc = check_rows(ctx, mesh, row_begin, row_end, layout);
+  if (rc) return rc;
   ANDR_HIP(ctx, hipSetDevice(ctx->device));
   DevBody b;
-  int rc = andr_body_upload(ctx, mesh, b, true);
-  if (rc) return rc;
-  const int nn = b.g.nn, ne = b.g.ne, nsp = nn + ne;
-  DevBuf<double> dH, dG, dE;
-  if (dH.alloc((size_t)nsp * nn) || dG.alloc((size_t)nsp * ne) || dE.alloc(nsp))
-    return andr_fail(ctx, ANDR_ERR_ALLOC, "integral matrices (%d x %d)", nsp, nn + ne);
-  rc = andr_launch_laplace_rows(ctx, b, dH.p, dG.p, dE.p);
-  if (rc) return rc;
-  rc = andr_copy_out(ctx, dH.p, nsp, nn, layout, H);
-  if (rc) return rc;
-  rc = andr_copy_out(ctx, dG.p, nsp, ne, layout, G);
-  if (rc) return rc;
-  return max_err(ctx, dE.p, nn, ne, err);
+  if ((rc = andr_body_upload(ctx, mesh, b, true))) return rc;
+  const int nn = b.g.nn, ne = b.g.ne, nr = row_end - row_begin;
+  const size_t sT = (size_t)nr * nn, sG = (size_t)nr * ne;
+  DevBuf<double> dT, dG, dpT, dpG, dE;
+  if (dT.alloc(6 * sT, ctx) || dG.alloc(6 * sG, ctx) || dpT.alloc(3 * sT, ctx) || dpG.alloc(3 * sG, ctx) ||
+      dE.alloc(nr, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "Stokes integral matrices (%zu doubles)", 9 * (sT + sG));
+  if ((rc = andr_launch_stokes_rows(ctx, b, extra_free_term, dT.p, dG.p, dpT.p, dpG.p, dE.p, row_begin, nr))) return rc;
+  for (int k = 0; k < 6; ++k) {
+    if (T && T[k] && (rc = andr_copy_out(ctx, dT.p + k * sT, nr, nn, layout, T[k]))) return rc;
+    if (G && G[k] && (rc = andr_copy_out(ctx, dG.p + k * sG, nr, ne, layout, G[k]))) return rc;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (prT && prT[k] && (rc = andr_copy_out(ctx, dpT.p + k * sT, nr, nn, layout, prT[k]))) return rc;
+    if (prG && prG[k] && (rc = andr_copy_out(ctx, dpG.p + k * sG, nr, ne, layout, prG[k]))) return rc;
+  }
+  return max_err(ctx, dE.p, row_begin, nr, nn, err);
 }
 
 extern "C" int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double extra_free_term,
                                     double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]) {
   if (!ctx) return ANDR_ERR_ARG;
-  if (layout != ANDR_ROW_MAJOR && layout != ANDR_COL_MAJOR) return andr_fail(ctx, ANDR_ERR_ARG, "bad layout");
-  ANDR_HIP(ctx, hipSetDevice(ctx->device));
-  DevBody b;
-  int rc = andr_body_upload(ctx, mesh, b, true);
-  if (rc) return rc;
-  const int nn = b.g.nn, ne = b.g.ne, nsp = nn + ne;
-  const size_t sT = (size_t)nsp * nn, sG = (size_t)nsp * ne;
-  DevBuf<double> dT, dG, dpT, dpG, dE;
-  if (dT.alloc(6 * sT) || dG.alloc(6 * sG) || dpT.alloc(3 * sT) || dpG.alloc(3 * sG) || dE.alloc(nsp))
-    return andr_fail(ctx, ANDR_ERR_ALLOC, "Stokes integral matrices (%zu doubles)", 9 * (sT + sG));
-  rc = andr_launch_stokes_rows(ctx, b, extra_free_term, dT.p, dG.p, dpT.p, dpG.p, dE.p);
-  if (rc) return rc;
-  for (int k = 0; k < 6; ++k) {
-    if (T && T[k] && (rc = andr_copy_out(ctx, dT.p + k * sT, nsp, nn, layout, T[k]))) return rc;
-    if (G && G[k] && (rc = andr_copy_out(ctx, dG.p + k * sG, nsp, ne, layout, G[k]))) return rc;
-  }
-  for (int k = 0; k < 3; ++k) {
-    if (prT && prT[k] && (rc = andr_copy_out(ctx, dpT.p + k * sT, nsp, nn, layout, prT[k]))) return rc;
-    if (prG && prG[k] && (rc = andr_copy_out(ctx, dpG.p + k * sG, nsp, ne, layout, prG[k]))) return rc;
-  }
-  return max_err(ctx, dE.p, nn, ne, err);
+  if (!mesh) return andr_fail(ctx, ANDR_ERR_ARG, "null mesh");
+  return andr_assemble_stokes_rows(ctx, mesh, 0, mesh->nnodes + mesh->nelem, layout, extra_free_term, T, G, prT, prG,
+                                   err);
 }
 
 extern "C" int andr_stokes_domain_values(andr_ctx *ctx, const andr_mesh *mesh, int32_t npts, const double *pts,

@@ -144,9 +144,12 @@ int andr_check_mesh(andr_ctx *ctx, const andr_mesh *m);
 int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_colors);
 
 // kernels exposed between translation units ----------------------------------------------------
-int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows);
+// rows r0 .. r0 + nrows - 1 of the nsp source points (node rows first; nrows < 0: to the end); the
+// outputs hold those rows only (row k of H is source point r0 + k)
+int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows, int r0 = 0,
+                             int nrows = -1);
 int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, double *G, double *prT,
-                            double *prG, double *err_rows);
+                            double *prG, double *err_rows, int r0 = 0, int nrows = -1);
 // transposes an m x n row-major device matrix into a column-major host buffer
 int andr_copy_out(andr_ctx *ctx, const double *d, int64_t m, int64_t n, int32_t layout, double *h);
 

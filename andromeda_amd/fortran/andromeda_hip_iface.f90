@@ -36,6 +36,22 @@ module andromeda_hip
     real(c_double) :: seconds
   end type
 
+  integer(c_int), parameter :: ANDR_BC_FUNCTION = 1, ANDR_BC_FLUX = 2, ANDR_BC_CONTACT = 3
+
+  ! andr_subdomain: a subdomain's walls (loWalls), their NORMMUL factors, the diffusivity, and
+  ! optional per-node wall ids of the EDGE / CORNER rules (nnodes entries, 0 = the default rule)
+  type, bind(C) :: andr_subdomain
+    integer(c_int32_t) :: nwalls
+    type(c_ptr) :: walls, normmul
+    real(c_double) :: diff
+    type(c_ptr) :: node_wall
+  end type
+
+  type, bind(C) :: andr_laplace_report
+    integer(c_int32_t) :: neq, nx, nb, nit, istop
+    real(c_double) :: err(2), t_assemble, t_solve
+  end type
+
   type, bind(C) :: andr_resistance_opts
     integer(c_int32_t) :: form, precision
     real(c_double) :: mu, eps
@@ -173,6 +189,34 @@ module andromeda_hip
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: nsp, nnodes, nqnodes
       real(c_double), intent(in) :: err(2), H(*), G(*)
+    end function
+
+    ! sdIntRowU / sdIntRowQ over a row range (0-based rows row_begin .. row_end - 1 of nsp)
+    integer(c_int) function andr_assemble_laplace_rows(ctx, mesh, row_begin, row_end, layout, H, G, err) &
+        bind(C, name="andr_assemble_laplace_rows")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: row_begin, row_end, layout
+      real(c_double), intent(out) :: H(*), G(*), err(2)
+    end function
+
+    ! solveLaplace from the integrals on (src/laplace.f90): sdFormIntegralMatrices per subdomain,
+    ! sdSetUpXandB, sdSetUpSysMrhsM_CRS, sdSolveCRS and CalFluxIntegralWalls for neqn equations;
+    ! wall_bc(nwalls, neqn), u(nnodes, neqn), q(nelem, neqn), wall_flux(nwalls, neqn)
+    integer(c_int) function andr_laplace_solve(ctx, mesh, nsd, sds, neqn, wall_bc, opts, u, q, wall_flux, &
+        reports) bind(C, name="andr_laplace_solve")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh, andr_subdomain, andr_lsqr_opts, andr_laplace_report
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: nsd
+      type(andr_subdomain), intent(in) :: sds(*)
+      integer(c_int32_t), value :: neqn
+      integer(c_int32_t), intent(in) :: wall_bc(*)
+      type(andr_lsqr_opts), intent(in) :: opts(*)
+      real(c_double), intent(inout) :: u(*), q(*)
+      real(c_double), intent(out) :: wall_flux(*)
+      type(andr_laplace_report), intent(out) :: reports(*)
     end function
 
     integer(c_int) function andr_set_quadrature_model(ctx, params, confidence, near_ratio, far_table) &

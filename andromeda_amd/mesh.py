@@ -20,6 +20,7 @@ class Mesh:
     con: np.ndarray          # (nelem, 3) or (nelem, 4) int32, 0-based (-1: no fourth node)
     bcid: np.ndarray         # (nelem,) int32, walls numbered from 1
     name: str = ""
+    walls: list | None = None  # wall names by wall id (walls[w - 1]), from the mesh file's physical names
 
     @property
     def nnodes(self) -> int:
@@ -36,7 +37,7 @@ class Mesh:
     def contiguous(self) -> "Mesh":
         return Mesh(np.ascontiguousarray(self.xyz, dtype=np.float64),
                     np.ascontiguousarray(self.con, dtype=np.int32),
-                    np.ascontiguousarray(self.bcid, dtype=np.int32), self.name)
+                    np.ascontiguousarray(self.bcid, dtype=np.int32), self.name, self.walls)
 
 
 # --------------------------------------------------------------------------------------------
@@ -299,8 +300,10 @@ def split_some_quads(mesh: Mesh, every: int = 3) -> Mesh:
 
 def read_gmsh(path: str) -> Mesh:
     """gmsh ASCII v2 or v4 surface mesh (src/lib_meshRead.f90:244-629): triangles (type 2) and
-    4-node quadrilaterals (type 3) are kept, the physical tag becomes the wall id. Nodes are
-    renumbered densely in file order."""
+    4-node quadrilaterals (type 3) are kept, the physical tag becomes the wall id (the used tags in
+    ascending order numbered 1, 2, ...: the tag itself when the tags are 1..n, as the reference
+    assumes), and the names of $PhysicalNames become Mesh.walls. Nodes are renumbered densely in file
+    order."""
     with open(path, "r") as fh:
         lines = fh.read().split("\n")
     i = 0
@@ -310,9 +313,16 @@ def read_gmsh(path: str) -> Mesh:
     tris: list = []
     tags: list = []
     phys_of_entity: dict = {}
+    phys_names: dict = {}
     while i < len(lines):
         ln = lines[i].strip()
-        if ln == "$MeshFormat":
+        if ln == "$PhysicalNames":
+            n = int(lines[i + 1])
+            for k in range(n):
+                t = lines[i + 2 + k].split(None, 2)
+                phys_names[int(t[1])] = t[2].strip().strip('"')
+            i += 2 + n
+        elif ln == "$MeshFormat":
             version = float(lines[i + 1].split()[0])
             i += 2
         elif ln == "$Entities" and version >= 4:
@@ -381,4 +391,5 @@ def read_gmsh(path: str) -> Mesh:
     utags = sorted(set(tags))
     wall = {t: k + 1 for k, t in enumerate(utags)}
     bcid = np.array([wall[t] for t in tags], dtype=np.int32)
-    return Mesh(xyz, con, bcid, path.split("/")[-1])
+    names = [phys_names.get(t, str(t)) for t in utags] if phys_names else None
+    return Mesh(xyz, con, bcid, path.split("/")[-1], names)

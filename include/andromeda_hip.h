@@ -79,6 +79,17 @@ int andr_assemble_laplace(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, 
 int andr_assemble_stokes(andr_ctx *ctx, const andr_mesh *mesh, int32_t layout, double extra_free_term,
                          double *T[6], double *G[6], double *prT[3], double *prG[3], double err[2]);
 
+/* Rows row_begin .. row_end - 1 (0-based, of the nsp = nnodes + nelem source points, node rows first)
+ * of the same matrices: H and G hold (row_end - row_begin) rows (T, G, prT, prG likewise), in the
+ * requested layout; err[2] covers those rows only. Replaces the per-row integration that the
+ * reference's distributed path calls for the rows a process owns (sdIntRowU/Q, sdIntRowU/QStokes
+ * from src/lib_StokesLis.f90:695,805). A null matrix pointer skips that matrix. */
+int andr_assemble_laplace_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_begin, int32_t row_end,
+                               int32_t layout, double *H, double *G, double err[2]);
+int andr_assemble_stokes_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_begin, int32_t row_end,
+                              int32_t layout, double extra_free_term, double *T[6], double *G[6], double *prT[3],
+                              double *prG[3], double err[2]);
+
 /* ---- seam 2: least squares --------------------------------------------------------------
  * Link-compatible with the Fortran routines of src/lib_lsqr.f:1-167 (gfortran/flang name mangling,
  * all arguments by reference, CSR with 1-based INTEGER(8) row pointers and INTEGER columns).
@@ -134,6 +145,54 @@ int andr_bicgstab_dense(andr_ctx *ctx, int64_t n, const double *A, int32_t nrhs,
  * andr_bicgstab_dense on the same inputs (same kernels, same summation order). */
 int andr_bicgstab_dense_noalloc(andr_ctx *ctx, int64_t n, const double *A, int64_t lda, int32_t nrhs, const double *B,
                                 double *X, double tol, int32_t maxit, andr_lsqr_stats *stats);
+
+/* ---- the Laplace boundary-value problem ---------------------------------------------------
+ * A subdomain is a list of walls with their NORMMUL factors and a diffusivity (the SUBDOMAINS and
+ * NORMMUL lines of the reference's BIC file, src/lib_sdBem.f90:319-430, src/lib_mesh.f90:1148-1160).
+ * Its elements are those of its walls (wall by wall, in the order given), its nodes those of its
+ * elements; a node takes its boundary condition from the wall of the last of its elements in that
+ * order, unless node_wall (nnodes entries, may be NULL) gives a nonzero wall id for it (the EDGE and
+ * CORNER rules of the BIC file, src/lib_sdBem.f90:112-300). */
+typedef struct {
+  int32_t nwalls;
+  const int32_t *walls;     /* wall ids 1..mesh.nwalls */
+  const double *normmul;    /* per listed wall; NULL: all +1 */
+  double diff;              /* > 0 */
+  const int32_t *node_wall; /* nnodes or NULL */
+} andr_subdomain;
+
+#define ANDR_BC_FUNCTION 1 /* u known (src/mod_eqns.f90 iFunction) */
+#define ANDR_BC_FLUX 2     /* q known (iFlux) */
+#define ANDR_BC_CONTACT 3  /* u and q unknown: a wall shared by two subdomains (iContact) */
+
+typedef struct {
+  int32_t neq, nx, nb;  /* equations (system rows), unknowns, knowns (sdSetUpXandB) */
+  int32_t nit, istop;   /* LSQR iterations and stopping code (src/lib_lsqr.f:1125-1134) */
+  double err[2];        /* largest fSP and qSP self-check errors of the subdomains' integrals */
+  double t_assemble;    /* seconds: integral matrices of all subdomains (shared by the equations) */
+  double t_solve;       /* seconds: this equation's system, solve and scatter */
+} andr_laplace_report;
+
+/* Replaces the reference's Laplace driver from the integrals on (solveLaplace, src/laplace.f90):
+ * the integral matrices of every subdomain (sdFormIntegralMatrices, src/lib_int.f90:225-323), the
+ * unknowns and knowns of every equation (sdSetUpXandB, src/lib_sdBem.f90:4-107), the system and
+ * right-hand side (sdSetUpSysMrhsM_CRS and CRSxV, src/lib_CRS.f90:148-343,764-787), the column-scaled
+ * LSQR solve with the hot start and the scatter back to u and q (sdSolveCRS, src/lib_CRS.f90:4-142),
+ * and the flux integral of every wall (CalFluxIntegralWalls, src/lib_mesh.f90:851-882), all on the
+ * device. neqn equations share the integrals: wall_bc[en * nwalls + w - 1] is the kind (ANDR_BC_*)
+ * of wall w for equation en; u[en * nnodes + i] and q[en * nelem + e] hold the known values and the
+ * initial values of the unknowns on entry and the solution on return; opts[en] (may be NULL) the LSQR
+ * limits (maxit 0: 10(m+n+50); eps 0: 1e-15); wall_flux[en * nwalls + w - 1] (may be NULL) gets
+ * sum_e A_e q_e over the elements of wall w; reports[en] (may be NULL). */
+int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds, int32_t neqn,
+                       const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q, double *wall_flux,
+                       andr_laplace_report *reports);
+
+/* The wall each node of each subdomain takes its condition from, as andr_laplace_solve uses it:
+ * node_wall[s * nnodes + i] (0: node i is not in subdomain s). Host only: no context, no device.
+ * ANDR_ERR_ARG with the reason in msg for an invalid mesh or subdomain. */
+int andr_subdomain_node_walls(const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds, int32_t *node_wall,
+                              char *msg, int32_t msglen);
 
 /* ---- seam 3: resistance ------------------------------------------------------------------- */
 #define ANDR_FORM_EXTERIOR 0 /* single surface, rigid-body velocity on the body                */
