@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libandromeda_hip.so")
 ROW_MAJOR, COL_MAJOR = 0, 1
 FORM_EXTERIOR, FORM_BOUNDED = 0, 1
 PREC_FP64, PREC_MIXED, PREC_LEARNED = 0, 1, 2
+PROBLEMS_RIGID, PROBLEMS_STRAIN = 0, 1
 MLP_NPARAM = 9288
 
 _D = C.POINTER(C.c_double)
@@ -54,7 +55,8 @@ class _LapReport(C.Structure):
 
 class _ResOpts(C.Structure):
     _fields_ = [("form", C.c_int32), ("precision", C.c_int32), ("mu", C.c_double), ("eps", C.c_double),
-                ("maxit", C.c_int32), ("center", C.c_double * 3), ("restart", C.c_int32)]
+                ("maxit", C.c_int32), ("center", C.c_double * 3), ("restart", C.c_int32),
+                ("problems", C.c_int32)]
 
 
 class _ResReport(C.Structure):
@@ -532,10 +534,10 @@ class Context:
     # ---- seam 3 ---------------------------------------------------------------------------------
     def resistance_batch(self, meshes, form: int = FORM_EXTERIOR, precision: int = PREC_FP64, mu: float = 1.0,
                          eps: float = 0.0, maxit: int = 0, center=(0.0, 0.0, 0.0), itri: int = 5, normmul=None,
-                         restart: int = 0):
+                         restart: int = 0, problems: int = PROBLEMS_RIGID):
         views = [_MeshView(m, normmul, itri) for m in meshes]
         arr = (_Mesh * len(views))(*[v.s for v in views])
-        opts = _ResOpts(form, precision, mu, eps, maxit, (C.c_double * 3)(*center), restart)
+        opts = _ResOpts(form, precision, mu, eps, maxit, (C.c_double * 3)(*center), restart, problems)
         FT = np.zeros((len(views), 36))
         reps = (_ResReport * len(views))()
         self._check(self.lib.andr_resistance_batch(self.h, len(views), arr, C.byref(opts), _p(FT), reps),
@@ -547,17 +549,18 @@ class Context:
         """Device-resident bodies for repeated resistance batches (andr_bodies_upload)."""
         views = [_MeshView(m, normmul, itri) for m in meshes]
         arr = (_Mesh * len(views))(*[v.s for v in views])
-        opts = _ResOpts(form, PREC_FP64, mu, 0.0, 0, (C.c_double * 3)(0.0, 0.0, 0.0), 0)
+        opts = _ResOpts(form, PREC_FP64, mu, 0.0, 0, (C.c_double * 3)(0.0, 0.0, 0.0), 0, PROBLEMS_RIGID)
         h = C.c_void_p()
         self._check(self.lib.andr_bodies_upload(self.h, len(views), arr, C.byref(opts), C.byref(h)),
                     "andr_bodies_upload")
         return Bodies(self, h, form, mu)
 
     def resistance_batch_resident(self, bodies: "Bodies", precision: int = PREC_FP64, eps: float = 0.0,
-                                  maxit: int = 0, center=(0.0, 0.0, 0.0), restart: int = 0):
+                                  maxit: int = 0, center=(0.0, 0.0, 0.0), restart: int = 0,
+                                  problems: int = PROBLEMS_RIGID):
         """andr_resistance_batch on device-resident bodies (form and mu of the upload)."""
         n = self.lib.andr_bodies_count(bodies.h)
-        opts = _ResOpts(bodies.form, precision, bodies.mu, eps, maxit, (C.c_double * 3)(*center), restart)
+        opts = _ResOpts(bodies.form, precision, bodies.mu, eps, maxit, (C.c_double * 3)(*center), restart, problems)
         FT = np.zeros((n, 36))
         reps = (_ResReport * n)()
         self._check(self.lib.andr_resistance_batch_resident(self.h, bodies.h, C.byref(opts), _p(FT), reps),

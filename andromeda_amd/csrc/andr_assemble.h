@@ -16,6 +16,8 @@ struct SysJob {
   double *B;                // 3 ne x ANDR_NRHS right-hand sides (problems 0..5, the rest 0)
   const uint8_t *moving;    // per node: 1 if it carries the rigid-body velocity (null: all nodes)
   double center[3];         // rotation centre of problems 3..5
+  int strain;               // 0: the six rigid-body problems; 1: columns 0..2 are the strain fields
+                            // (0, z, y), (z, 0, x), (y, x, 0) about the centre (TorquePiCRS)
   double mu, extra;
   int qoff, nq;        // the fixed-order rule
   double *err_rows;    // ne (may be null)

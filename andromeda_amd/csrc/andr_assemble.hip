@@ -1085,6 +1085,15 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
       const double w = (J.moving ? (double)J.moving[cn[v]] : 1.0) / 3.0;
       const double *p = g.xyz + 3 * cn[v];
       const double q[3] = {p[0] - J.center[0], p[1] - J.center[1], p[2] - J.center[2]};
+      if (J.strain) {  // strain fields (0, qz, qy), (qz, 0, qx), (qy, qx, 0) in columns 0..2
+        ubar[1][0] += w * q[2];
+        ubar[2][0] += w * q[1];
+        ubar[0][1] += w * q[2];
+        ubar[2][1] += w * q[0];
+        ubar[0][2] += w * q[1];
+        ubar[1][2] += w * q[0];
+        continue;
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c) ubar[c][c] += w;
       // e_x x q = (0, -qz, qy); e_y x q = (qz, 0, -qx); e_z x q = (-qy, qx, 0)
@@ -1107,6 +1116,12 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
       b[5] = -Mc[0][1] + Mc[1][0];
 #pragma unroll
       for (int k = 6; k < ANDR_NRHS; ++k) b[k] = 0.0;
+      if (J.strain) {  // sum_v T_{c,i} (A_k q_v)_i for the symmetric strain matrices A_k
+        b[0] = Mc[1][2] + Mc[2][1];
+        b[1] = Mc[0][2] + Mc[2][0];
+        b[2] = Mc[0][1] + Mc[1][0];
+        b[3] = b[4] = b[5] = 0.0;
+      }
 #pragma unroll
       for (int k = 0; k < ANDR_NRHS; ++k) J.B[(size_t)(c * ne + r) * ANDR_NRHS + k] = b[k] + cfree[c] * ubar[c][k];
     }

@@ -91,6 +91,8 @@ static int check_opts(andr_ctx *ctx, const andr_resistance_opts *o) {
   if (o->precision == ANDR_PREC_LEARNED && !ctx->has_model)
     return andr_fail(ctx, ANDR_ERR_STATE, "learned precision needs andr_set_quadrature_model first");
   if (!(o->mu > 0.0)) return andr_fail(ctx, ANDR_ERR_ARG, "mu must be positive");
+  if (o->problems != ANDR_PROBLEMS_RIGID && o->problems != ANDR_PROBLEMS_STRAIN)
+    return andr_fail(ctx, ANDR_ERR_ARG, "problems must be ANDR_PROBLEMS_RIGID or ANDR_PROBLEMS_STRAIN");
   return ANDR_OK;
 }
 
@@ -212,6 +214,7 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
     J.B = Bbuf[i].p;
     J.moving = exterior ? nullptr : Mbuf[i].p;
     for (int d = 0; d < 3; ++d) J.center[d] = opts->center[d];
+    J.strain = opts->problems == ANDR_PROBLEMS_STRAIN ? 1 : 0;
     J.mu = opts->mu;
     J.extra = extra;
     J.qoff = qr.offset;
@@ -258,6 +261,7 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
   const double ta1 = andr_now();
   // ---- solve ----------------------------------------------------------------------------------
   std::vector<int> iters((size_t)nb * 6, 0), stops((size_t)nb * 6, 0);
+  const int nprob = opts->problems == ANDR_PROBLEMS_STRAIN ? 3 : 6;  // right-hand sides per body
   if (plain) {
     // column scaling of every body, then the LSQR solves of the wave together on several streams
     std::vector<DevBuf<double>> qs(nb);
@@ -268,7 +272,7 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
       rc = andr_colscale(ctx, Abuf[i].p, false, jobs[i].lda, n, n, qs[i].p);
       if (rc) return rc;
       ANDR_HIP(ctx, hipMemsetAsync(Xbuf[i].p, 0, sizeof(double) * n * ANDR_NRHS, ctx->stream));
-      lj[i] = {Abuf[i].p, false, jobs[i].lda, n, n, qs[i].p, Bbuf[i].p, Xbuf[i].p, 6, opts->maxit, eps,
+      lj[i] = {Abuf[i].p, false, jobs[i].lda, n, n, qs[i].p, Bbuf[i].p, Xbuf[i].p, nprob, opts->maxit, eps,
                &iters[(size_t)i * 6], &stops[(size_t)i * 6]};
     }
     if ((rc = andr_lsqr_device_batch(ctx, nb, lj.data()))) return rc;
@@ -279,8 +283,14 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
       StokesOp op{(const float *)Abuf[i].p, P[i]->ne, jobs[i].minv, g.area, g.nrm, g.bcid, jobs[i].defl};
       gs[i] = {nullptr, 0, 3 * P[i]->ne, op, Bbuf[i].p, Xbuf[i].p};
     }
-    rc = andr_gmres_batch(ctx, gs.data(), nb, true, 6, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
-                          iters.data(), stops.data());
+    std::vector<int> it_g((size_t)nb * nprob), st_g((size_t)nb * nprob);
+    rc = andr_gmres_batch(ctx, gs.data(), nb, true, nprob, restart, opts->maxit > 0 ? opts->maxit : 2000, eps,
+                          it_g.data(), st_g.data());
+    for (int i = 0; i < nb; ++i)
+      for (int k = 0; k < nprob; ++k) {
+        iters[(size_t)i * 6 + k] = it_g[(size_t)i * nprob + k];
+        stops[(size_t)i * 6 + k] = st_g[(size_t)i * nprob + k];
+      }
     if (rc) return rc;
     if ((rc = andr_launch_unprecondition(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
   }

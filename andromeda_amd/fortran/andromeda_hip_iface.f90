@@ -13,6 +13,7 @@ module andromeda_hip
   integer(c_int), parameter :: ANDR_ROW_MAJOR = 0, ANDR_COL_MAJOR = 1
   integer(c_int), parameter :: ANDR_FORM_EXTERIOR = 0, ANDR_FORM_BOUNDED = 1
   integer(c_int), parameter :: ANDR_PREC_FP64 = 0, ANDR_PREC_MIXED = 1, ANDR_PREC_LEARNED = 2
+  integer(c_int), parameter :: ANDR_PROBLEMS_RIGID = 0, ANDR_PROBLEMS_STRAIN = 1
   integer(c_int), parameter :: ANDR_MLP_NPARAM = 9288, ANDR_FAR_BINS = 16
 
   ! andr_mesh: pointers to the host's arrays (xyz(3,nnodes), con(3,nelem), bcid(nelem),
@@ -58,6 +59,7 @@ module andromeda_hip
     integer(c_int32_t) :: maxit
     real(c_double) :: center(3)
     integer(c_int32_t) :: restart  ! fast forms: first GMRES cycle; 0 -> automatic
+    integer(c_int32_t) :: problems ! 0: six rigid-body problems; 1: three strain fields (TorquePiCRS)
   end type
 
   type, bind(C) :: andr_resistance_report

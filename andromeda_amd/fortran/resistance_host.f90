@@ -91,6 +91,7 @@ program resistance_host
   opts%maxit = 0
   opts%center = 0.0d0
   opts%restart = 0
+  opts%problems = ANDR_PROBLEMS_RIGID
   rc = andr_resistance(ctx, mesh, opts, FT, rep)
   if (rc /= ANDR_OK) then
     print *, "andr_resistance: ", andr_error_message(ctx)

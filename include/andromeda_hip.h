@@ -213,7 +213,13 @@ typedef struct {
   double center[3];  /* torque and rotation reference point */
   int32_t restart;   /* fast forms: steps of the first GMRES cycle; 0 -> 128, longer (up to 192) while
                         the bodies of the call fit one wave of device memory */
+  int32_t problems;  /* ANDR_PROBLEMS_RIGID (0): the six rigid-body problems; ANDR_PROBLEMS_STRAIN (1):
+                        the three strain fields of StokesInlet3sidesTorquePiCRS (src/lib_particle.f90:
+                        6-168, velocities :72-85): (0, z, y), (z, 0, x), (y, x, 0) about `center`, in
+                        FT rows 0..2 (rows 3..5 zero) */
 } andr_resistance_opts;
+#define ANDR_PROBLEMS_RIGID 0
+#define ANDR_PROBLEMS_STRAIN 1
 
 typedef struct {
   int32_t nit[6];       /* iterations per rigid-body problem */
@@ -232,7 +238,12 @@ typedef struct {
 /* Replaces StokesInlet3sidesForceCRS / TorqueOmegaCRS (src/lib_particle.f90:176-476): six
  * rigid-body problems (translations e_x, e_y, e_z, rotations e_k x (r - center)); FT[6*k + i] is
  * component i of (force, torque) for problem k, force = sum_e A_e q_e and torque =
- * sum_e A_e (c_e - center) x q_e over wall 1 (src/lib_particle.f90:1751-1831). */
+ * sum_e A_e (c_e - center) x q_e over wall 1 (src/lib_particle.f90:1751-1831). With
+ * opts->problems = ANDR_PROBLEMS_STRAIN it replaces StokesInlet3sidesTorquePiCRS
+ * (src/lib_particle.f90:6-168): the three strain problems, whose torques are the Pi tensor. The
+ * signs are those of the reference's bounded set-up (the body at rest in the flow): the drag of a
+ * translation and the torque of a rotation are positive, and a strain field turns a body's long
+ * axis toward its stretching direction (Jeffery's torque). */
 int andr_resistance(andr_ctx *ctx, const andr_mesh *mesh, const andr_resistance_opts *opts, double FT[36],
                     andr_resistance_report *report);
 

@@ -1019,7 +1019,12 @@ static void dense_to_csr(int64_t m, int64_t n, double *a, orc_csr *A, int64_t *i
   A->m = m; A->n = n; A->nnz = m * n; A->ia = ia; A->ja = ja; A->a = a;
 }
 
-int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int maxit, double FT[36], int nits[6]) {
+/* StokesInlet3sidesForceCRS / TorqueOmegaCRS (problems = 0: translations e_k, rotations e_k x r) and
+ * StokesInlet3sidesTorquePiCRS (problems = 1: the strain fields (0, z, y), (z, 0, x), (y, x, 0) of
+ * src/lib_particle.f90:72-85), src/lib_particle.f90:6-476; bounded form: the velocity on wall 2, the
+ * body (wall 1) at rest; exterior form: the velocity on the body. FT[6 k + (0..5)] = (F, T) on wall 1. */
+int orc_resistance_problems(const orc_body *b, int exterior, double mu, double eps, int maxit, int problems,
+                            double FT[36], int nits[6]) {
   const int nn = b->nnodes, ne = b->nelem;
   /* exterior: only the body wall (wall 1) with normals into the body */
   double *nm = (double *)malloc(sizeof(double) * b->nwalls);
@@ -1060,13 +1065,19 @@ int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int m
   double *ub = (double *)malloc(sizeof(double) * nr3), *rb = (double *)malloc(sizeof(double) * n3);
   double *x = (double *)malloc(sizeof(double) * n3);
   int worst = 0;
-  for (int k = 0; k < 6; ++k) {
-    /* rigid-body velocity of every node: translation e_k or rotation e_(k-3) x r */
+  const int nprob = problems == 1 ? 3 : 6;
+  for (int k = 0; k < 36; ++k) FT[k] = 0.0;
+  for (int k = 0; k < nprob; ++k) {
+    /* velocity of every node: translation e_k, rotation e_(k-3) x r, or strain field k */
     for (int i = 0; i < nn; ++i) {
       const double *p = b->xyz + 3 * i;
       double u[3] = {0, 0, 0};
       if (!exterior && node_wall[i] != 2) { for (int c = 0; c < 3; ++c) ub[(size_t)c * nn + i] = 0.0; continue; }
-      if (k < 3) u[k] = 1.0;
+      if (problems == 1) {
+        if (k == 0) { u[1] = p[2]; u[2] = p[1]; }
+        if (k == 1) { u[0] = p[2]; u[2] = p[0]; }
+        if (k == 2) { u[1] = p[0]; u[0] = p[1]; }
+      } else if (k < 3) u[k] = 1.0;
       else {
         double om[3] = {0, 0, 0};
         om[k - 3] = 1.0;
@@ -1097,4 +1108,8 @@ int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int m
   free(nm); free(rows); free(sys); free(rhs); free(ia); free(ja); free(q);
   free(geo_n); free(area); free(cen); free(ub); free(rb); free(x); free(node_wall);
   return worst;
+}
+
+int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int maxit, double FT[36], int nits[6]) {
+  return orc_resistance_problems(b, exterior, mu, eps, maxit, 0, FT, nits);
 }

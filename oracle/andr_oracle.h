@@ -140,6 +140,8 @@ void orc_stokes_dirichlet_system(const orc_body *b, double *const T[6], double *
  * free term + 8*pi); exterior == 0 is the reference's bounded set-up: the body (wall 1) at rest
  * inside an outer wall (wall 2) that carries the velocity, as in src/lib_particle.f90:341-476.
  * FT[36]: FT[6*k + i] = component i of (F, T) for problem k. Returns the max LSQR ierr. */
+int orc_resistance_problems(const orc_body *b, int exterior, double mu, double eps, int maxit, int problems,
+                            double FT[36], int nits[6]);
 int orc_resistance(const orc_body *b, int exterior, double mu, double eps, int maxit,
                    double FT[36], int nits[6]);
 

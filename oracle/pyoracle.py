@@ -247,11 +247,12 @@ def stokes_dirichlet_system(mesh, T, G, row_offset=0, diff=1.0, normmul=None):
     return sys, rhs
 
 
-def resistance(mesh, exterior=True, mu=1.0, eps=1e-12, maxit=0, normmul=None, itri=5):
-    """Returns FT (6, 6): row k = (F, T) for problem k, and the LSQR iteration counts."""
+def resistance(mesh, exterior=True, mu=1.0, eps=1e-12, maxit=0, normmul=None, itri=5, problems=0):
+    """Returns FT (6, 6): row k = (F, T) for problem k, and the LSQR iteration counts; problems = 1:
+    the three strain fields (rows 0..2, rows 3..5 zero)."""
     b = Body(mesh, normmul, itri)
     FT = np.zeros(36)
     nits = np.zeros(6, dtype=np.int32)
-    ierr = lib().orc_resistance(C.byref(b.s), 1 if exterior else 0, C.c_double(mu), C.c_double(eps), maxit,
-                                _p(FT), _pi(nits))
+    ierr = lib().orc_resistance_problems(C.byref(b.s), 1 if exterior else 0, C.c_double(mu), C.c_double(eps), maxit,
+                                         problems, _p(FT), _pi(nits))
     return FT.reshape(6, 6), nits, ierr

@@ -239,3 +239,71 @@ def test_sphere_20480_drag_and_fast_form(ctx):
     assert np.all(np.abs(Om / (8 * np.pi) - 1) < 5e-3), Om
     f = ctx.resistance(m, precision=api.PREC_MIXED)
     assert tensor_error(f.FT, p.FT) < 1e-6
+
+
+def _seam_system(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, n)) + 6 * np.sqrt(n) * np.eye(n)
+    return A, rng.normal(size=n), rng.normal(size=n)
+
+
+def test_fortran_lsqr_shim_hot_start_and_iteration_limit(ctx, orc):
+    """lsqr_solve_ as the reference's drivers call it: a hot start x0 != 0 (src/lib_lsqr.f:84-113:
+    x <- x/q, u = b - A x, x = x0 + q dx) and maxit = 3 (istop = ierr = 5, nit = 3) give the
+    oracle's iterates; maxit is not written back when nonzero."""
+    n = 200
+    A, b, x0 = _seam_system(n, 21)
+    ia, ja, a = orc.csr_from_dense(A)
+    qo = orc.lsqr_precon(ia, ja, a, n, n)
+    ia1, ja1, a1 = ia + 1, ja + 1, A.reshape(-1).copy()
+    q = api.lsqr_precon(n, n, ia1, ja1, a1)
+    for maxit, eps in ((0, 1e-14), (3, 1e-15)):
+        xo, info = orc.lsqr_solve(ia, ja, a, n, n, qo, b, x0=x0, maxit=maxit, eps=eps)
+        x = x0.copy()
+        r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, x, maxit=maxit, eps=eps)
+        print(f"maxit {maxit}: nit {r['nit']} (oracle {info['nit']}) ierr {r['ierr']} (oracle {info['ierr']})")
+        if maxit == 3:
+            assert r["nit"] == info["nit"] == 3 and r["ierr"] == info["ierr"] == 5 and r["maxit"] == 3
+            assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
+        else:
+            assert r["ierr"] == 0 and abs(r["nit"] - info["nit"]) <= 2
+            assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_fortran_lsqr_shim_duplicates_cache_and_failure(ctx, orc):
+    """A CSR with duplicate (i, j) entries solves as their sum; a solve after the values change in
+    place (same arrays) uploads the new matrix; a failed lsqr_precon_ sets q to NaN and the next
+    lsqr_solve_ returns ierr = -1 until an lsqr_precon_ succeeds."""
+    n = 120
+    A, b, _ = _seam_system(n, 22)
+    # duplicates: every diagonal entry split in two halves, stored twice in its row
+    ia, ja, vals = [1], [], []
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                ja += [j + 1, j + 1]
+                vals += [0.5 * A[i, j], 0.5 * A[i, j]]
+            else:
+                ja.append(j + 1)
+                vals.append(A[i, j])
+        ia.append(len(ja) + 1)
+    ia1, ja1, a1 = np.array(ia, np.int64), np.array(ja, np.int32), np.array(vals)
+    q = api.lsqr_precon(n, n, ia1, ja1, a1)
+    x = np.zeros(n)
+    r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, x, eps=1e-14)
+    assert r["ierr"] == 0 and np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-10
+    # same arrays, new values: the cached device copy must not be used
+    a1 *= 2.0
+    x2 = np.zeros(n)
+    r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, x2, eps=1e-14)
+    assert r["ierr"] == 0 and np.linalg.norm(x2 - 0.5 * x) / np.linalg.norm(x) < 1e-9
+    # a failed precon (a column index out of range) is visible in q and in the next solve
+    bad = ja1.copy()
+    bad[5] = n + 3
+    qb = api.lsqr_precon(n, n, ia1, bad, a1.copy())
+    assert np.all(np.isnan(qb))
+    r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, np.zeros(n), eps=1e-14)
+    assert r["ierr"] == -1
+    api.lsqr_precon(n, n, ia1, ja1, a1.copy())
+    r = api.lsqr_solve(n, n, ia1, ja1, a1, q, b, np.zeros(n), eps=1e-14)
+    assert r["ierr"] == 0
