@@ -129,6 +129,8 @@ def load_library(path: str = LIB_PATH):
                                               _D]
     lib.andr_laplace_solve.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), C.c_int32, _I,
                                        C.POINTER(_LsqrOpts), _D, _D, _D, C.POINTER(_LapReport)]
+    lib.andr_stokes_solve.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.POINTER(_Subdomain), _I, C.POINTER(_LsqrOpts),
+                                      _D, _D, _D, C.POINTER(_LapReport)]
     lib.andr_subdomain_node_walls.argtypes = [C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), _I, C.c_char_p,
                                               C.c_int32]
     lib.andr_bicgstab_dense_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
@@ -148,7 +150,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
             "andr_bicgstab_dense", "andr_mesh_validate", "andr_bicgstab_dense_noalloc", "andr_assemble_laplace_rows",
-            "andr_assemble_stokes_rows", "andr_laplace_solve", "andr_subdomain_node_walls"]
+            "andr_assemble_stokes_rows", "andr_laplace_solve", "andr_subdomain_node_walls", "andr_stokes_solve"]
 BC_FUNCTION, BC_FLUX, BC_CONTACT = 1, 2, 3
 FAR_BINS = 16
 
@@ -447,6 +449,24 @@ class Context:
                                                 _p(flux), reps), "andr_laplace_solve")
         return [LaplaceResult(U[k].copy(), Q[k].copy(), flux[k].copy(), r.neq, r.nx, r.nb, r.nit, r.istop,
                               (r.err[0], r.err[1]), r.t_assemble, r.t_solve) for k, r in enumerate(reps)]
+
+    def stokes_solve(self, mesh: Mesh, subdomain: "Subdomain", wall_bc, u, q, eps=1e-15, maxit=500, itri: int = 5):
+        """The general Stokes problem (andr_stokes_solve): wall_bc (3, nwalls) of BC_* kinds per velocity
+        component, u (3, nnodes) and q (3, nelem) the known and initial values. Returns a
+        LaplaceResult with u (3, nnodes), q (3, nelem) and wall_flux (3, nwalls) the wall forces."""
+        mv = _MeshView(mesh, None, itri)
+        nn, ne, nw = mv.m.nnodes, mv.m.nelem, mv.s.nwalls
+        bc = np.ascontiguousarray(wall_bc, dtype=np.int32).reshape(3, nw)
+        U = np.ascontiguousarray(np.array(u, dtype=np.float64).reshape(3, nn))
+        Q = np.ascontiguousarray(np.array(q, dtype=np.float64).reshape(3, ne))
+        opts = _LsqrOpts(int(maxit), float(eps), 1)
+        force = np.zeros((3, nw))
+        rep = _LapReport()
+        sv = _SdView([subdomain], nn)
+        self._check(self.lib.andr_stokes_solve(self.h, C.byref(mv.s), sv.arr, _pi(bc), C.byref(opts), _p(U), _p(Q),
+                                               _p(force), C.byref(rep)), "andr_stokes_solve")
+        return LaplaceResult(U, Q, force, rep.neq, rep.nx, rep.nb, rep.nit, rep.istop, (rep.err[0], rep.err[1]),
+                             rep.t_assemble, rep.t_solve)
 
     def assemble_stokes(self, mesh: Mesh, normmul=None, itri: int = 5, extra: float = 0.0, pressure: bool = True,
                         iquad: int = 4):

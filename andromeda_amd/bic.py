@@ -9,6 +9,8 @@ Reference routines restated here (host-side input handling; the solve is andr_la
                     from HandleCornersEdges src/lib_sdBem.f90:112-300
   bic_value         GetBiCValue            src/lib_mesh.f90:1770-1800
   conditions        sdApplyInitalCond      src/lib_mesh.f90:1628-1692
+The Stokes driver's general problem (three equations, the velocity components) runs through
+`solve_stokes_case` (andr_stokes_solve).
 """
 from __future__ import annotations
 
@@ -251,3 +253,16 @@ def solve_laplace_case(ctx: "api.Context", mesh: Mesh, prob: BicProblem, itri: i
     res = ctx.laplace_solve(mesh, prob.subdomains, np.array(bcs), np.array(us), np.array(qs), eps=ep, maxit=mi,
                             itri=itri, iquad=iquad)
     return dict(zip(prob.equations, res))
+
+
+def solve_stokes_case(ctx: "api.Context", mesh: Mesh, prob: BicProblem, itri: int = 5, eps=None, maxit=None):
+    """The Stokes driver's general problem on the device (andr_stokes_solve): the three equations of
+    the BIC file are the velocity components (EQUATIONS 3: X, Y, Z), one subdomain, the solver limits
+    of the first equation (src/lib_Stokes.f90:626-629)."""
+    if len(prob.equations) != 3 or len(prob.subdomains) != 1:
+        raise ValueError("a Stokes case has three equations (the velocity components) and one subdomain")
+    bcs, us, qs = zip(*[conditions(mesh, prob, e) for e in prob.equations])
+    slv = prob.solver[prob.equations[0]]
+    return ctx.stokes_solve(mesh, prob.subdomains[0], np.array(bcs), np.array(us), np.array(qs),
+                            eps=slv.eps if eps is None else eps, maxit=slv.maxit if maxit is None else maxit,
+                            itri=itri)

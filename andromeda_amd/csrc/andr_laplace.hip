@@ -1,5 +1,7 @@
 // The Laplace boundary-value problem of the reference's Laplace driver (solveLaplace,
-// src/laplace.f90) on the device: subdomains of walls, per-wall boundary conditions (function,
+// src/laplace.f90) and the general Stokes problem of its Stokes driver (one subdomain; velocity or
+// traction known per wall and component, src/lib_Stokes.f90:417-637, src/lib_StokesFull.f90) on the
+// device. Laplace: subdomains of walls, per-wall boundary conditions (function,
 // flux or contact), the integral matrices of every subdomain, the system and its right-hand side
 // formed from them, the column-scaled LSQR solve, the unknowns scattered back to the boundary values
 // and the flux integral of every wall.
@@ -150,6 +152,59 @@ __global__ __launch_bounds__(256) void k_wall_flux(int ne, const int *__restrict
   if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// The mesh of one subdomain's walls: nodes in node-list order, elements in element-list order, wall
+// ids renumbered 1..nwalls in the subdomain's order with its NORMMUL factors.
+struct SubMesh {
+  std::vector<double> xyz, nm;
+  std::vector<int32_t> con, bc;
+  andr_mesh m;
+};
+void make_submesh(const andr_mesh *mesh, const andr_subdomain &sd, const SdLists &l, SubMesh &S) {
+  const int nn = mesh->nnodes, nv = mesh->nvert == 4 ? 4 : 3;
+  std::vector<int> remap(nn, -1);
+  for (size_t k = 0; k < l.nodes.size(); ++k) remap[l.nodes[k]] = (int)k;
+  S.xyz.resize(3 * l.nodes.size());
+  for (size_t k = 0; k < l.nodes.size(); ++k)
+    for (int d = 0; d < 3; ++d) S.xyz[3 * k + d] = mesh->xyz[3 * (size_t)l.nodes[k] + d];
+  S.con.resize((size_t)nv * l.elems.size());
+  S.bc.resize(l.elems.size());
+  for (size_t k = 0; k < l.elems.size(); ++k) {
+    for (int v = 0; v < nv; ++v) {
+      const int c = mesh->con[(size_t)nv * l.elems[k] + v] - mesh->index_base;
+      S.con[(size_t)nv * k + v] = (nv == 4 && v == 3 && c < 0) ? -1 : remap[c];
+    }
+    for (int w = 0; w < sd.nwalls; ++w)
+      if (sd.walls[w] == l.qwall[k]) {
+        S.bc[k] = w + 1;
+        break;
+      }
+  }
+  S.nm.assign(sd.nwalls, 1.0);
+  if (sd.normmul)
+    for (int w = 0; w < sd.nwalls; ++w) S.nm[w] = sd.normmul[w];
+  S.m = *mesh;
+  S.m.nnodes = (int)l.nodes.size();
+  S.m.nelem = (int)l.elems.size();
+  S.m.nwalls = sd.nwalls;
+  S.m.xyz = S.xyz.data();
+  S.m.con = S.con.data();
+  S.m.bcid = S.bc.data();
+  S.m.normmul = S.nm.data();
+  S.m.index_base = 0;
+}
+
+// normmul / diff of every element of a subdomain (the G columns' factor in the system)
+std::vector<double> g_factors(const andr_subdomain &sd, const SdLists &l) {
+  std::vector<double> gm(l.elems.size());
+  for (size_t k = 0; k < l.elems.size(); ++k) {
+    double multi = 1.0;
+    for (int w = 0; w < sd.nwalls; ++w)
+      if (sd.walls[w] == l.qwall[k]) multi = sd.normmul ? sd.normmul[w] : 1.0;
+    gm[k] = multi / sd.diff;
+  }
+  return gm;
+}
+
 template <class T>
 int upload(andr_ctx *ctx, DevBuf<T> &d, const std::vector<T> &h) {
   if (d.alloc(std::max<size_t>(h.size(), 1), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "Laplace maps");
@@ -207,37 +262,9 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
   std::vector<SdDev> D(nsd);
   double errU = 0.0, errQ = 0.0;
   for (int s = 0; s < nsd; ++s) {
-    const SdLists &l = L[s];
-    const andr_subdomain &sd = sds[s];
-    std::vector<int> remap(nn, -1);
-    for (size_t k = 0; k < l.nodes.size(); ++k) remap[l.nodes[k]] = (int)k;
-    std::vector<double> xyz(3 * l.nodes.size());
-    for (size_t k = 0; k < l.nodes.size(); ++k)
-      for (int d = 0; d < 3; ++d) xyz[3 * k + d] = mesh->xyz[3 * (size_t)l.nodes[k] + d];
-    std::vector<int32_t> con((size_t)nv * l.elems.size()), bc(l.elems.size());
-    for (size_t k = 0; k < l.elems.size(); ++k) {
-      for (int v = 0; v < nv; ++v) {
-        const int c = mesh->con[(size_t)nv * l.elems[k] + v] - mesh->index_base;
-        con[(size_t)nv * k + v] = (nv == 4 && v == 3 && c < 0) ? -1 : remap[c];
-      }
-      for (int w = 0; w < sd.nwalls; ++w)
-        if (sd.walls[w] == l.qwall[k]) {
-          bc[k] = w + 1;
-          break;
-        }
-    }
-    std::vector<double> nm(sd.nwalls, 1.0);
-    if (sd.normmul)
-      for (int w = 0; w < sd.nwalls; ++w) nm[w] = sd.normmul[w];
-    andr_mesh sm = *mesh;
-    sm.nnodes = (int)l.nodes.size();
-    sm.nelem = (int)l.elems.size();
-    sm.nwalls = sd.nwalls;
-    sm.xyz = xyz.data();
-    sm.con = con.data();
-    sm.bcid = bc.data();
-    sm.normmul = nm.data();
-    sm.index_base = 0;
+    SubMesh sub;
+    make_submesh(mesh, sds[s], L[s], sub);
+    const andr_mesh &sm = sub.m;
     DevBody body;
     if ((rc = andr_body_upload(ctx, &sm, body, true))) return rc;
     SdDev &d = D[s];
@@ -341,7 +368,7 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
         const int nns = (int)l.nodes.size(), nes = (int)l.elems.size();
         std::vector<int> rr, nc(nns), qc(nes);
         std::vector<uint8_t> nk(nns), qk(nes);
-        std::vector<double> gm(nes);
+        const std::vector<double> gm = g_factors(sd, l);
         for (int k = 0; k < nns; ++k) {
           const int bc = bcw[l.nwall[k] - 1];
           nc[k] = col[l.nodes[k]];
@@ -352,10 +379,6 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
           const int bc = bcw[l.qwall[k] - 1];
           qc[k] = qcol[l.elems[k]];
           qk[k] = bc == ANDR_BC_FLUX;
-          double multi = 1.0;
-          for (int w = 0; w < sd.nwalls; ++w)
-            if (sd.walls[w] == l.qwall[k]) multi = sd.normmul ? sd.normmul[w] : 1.0;
-          gm[k] = multi / sd.diff;
           if (bc != ANDR_BC_FLUX) rr.push_back(nns + k);
         }
         // every column must address the unknowns or the knowns it is written to: a point listed by two
@@ -419,6 +442,202 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
       R.t_assemble = t1 - t0;
       R.t_solve = andr_now() - ts;
     }
+  }
+  return ANDR_OK;
+}
+
+// ---- the general Stokes problem (one subdomain) ------------------------------------------------------
+namespace {
+
+// One block per system row (component een of source point r): for every column component en,
+// sys[row][col] = -T_{een,en}(r, j) or -(normmul / diff) G_{een,en}(r, j) where the value is
+// unknown, and rhs[row] = sum of +T and +(normmul / diff) G times the known values (formUrow /
+// formQrow, src/lib_StokesFull.f90:4-160; b = rhsM b, src/lib_Stokes.f90:417-469).
+__global__ __launch_bounds__(256) void k_stk_system(int nn_sd, int ne_sd, int nsp, const double *__restrict__ T,
+                                                    const double *__restrict__ G, const int2 *__restrict__ rows,
+                                                    const int *__restrict__ ncol, const uint8_t *__restrict__ nknown,
+                                                    const int *__restrict__ qcol, const uint8_t *__restrict__ qknown,
+                                                    const double *__restrict__ gmul, const double *__restrict__ b,
+                                                    double *__restrict__ sys, int64_t lda, double *__restrict__ B) {
+  __shared__ double red[4];
+  const int row = blockIdx.x;
+  const int een = rows[row].x, r = rows[row].y;
+  double *srow = sys + (size_t)row * lda;
+  const size_t sT = (size_t)nsp * nn_sd, sG = (size_t)nsp * ne_sd;
+  double acc = 0.0;
+  for (int en = 0; en < 3; ++en) {
+    const double *tr = T + sym3(een, en) * sT + (size_t)r * nn_sd;
+    const double *gr = G + sym3(een, en) * sG + (size_t)r * ne_sd;
+    const int *nc = ncol + (size_t)en * nn_sd, *qc = qcol + (size_t)en * ne_sd;
+    const uint8_t *nk = nknown + (size_t)en * nn_sd, *qk = qknown + (size_t)en * ne_sd;
+    for (int j = threadIdx.x; j < nn_sd; j += blockDim.x) {
+      const double v = tr[j];
+      if (nk[j]) acc += v * b[nc[j]];
+      else srow[nc[j]] = -v;
+    }
+    for (int j = threadIdx.x; j < ne_sd; j += blockDim.x) {
+      const double v = gmul[j] * gr[j];
+      if (qk[j]) acc += v * b[qc[j]];
+      else srow[qc[j]] = -v;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) B[(size_t)row * ANDR_NB] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace
+
+extern "C" int andr_stokes_solve(andr_ctx *ctx, const andr_mesh *mesh, const andr_subdomain *sd,
+                                 const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q,
+                                 double *wall_force, andr_laplace_report *report) {
+  if (!ctx) return ANDR_ERR_ARG;
+  if (!wall_bc || !u || !q) return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_solve arguments");
+  int rc = andr_check_mesh(ctx, mesh);
+  if (rc) return rc;
+  if (mesh->nvert == 4) return andr_fail(ctx, ANDR_ERR_ARG, "andr_stokes_solve takes triangle meshes");
+  char msg[256];
+  std::vector<SdLists> LL;
+  if ((rc = build_lists(mesh, 1, sd, LL, msg, sizeof(msg)))) return andr_fail(ctx, rc, "%s", msg);
+  const SdLists &l = LL[0];
+  const int nn = mesh->nnodes, ne = mesh->nelem, nw = mesh->nwalls > 0 ? mesh->nwalls : 1;
+  for (int k = 0; k < 3 * nw; ++k)
+    if (wall_bc[k] < ANDR_BC_FUNCTION || wall_bc[k] > ANDR_BC_CONTACT)
+      return andr_fail(ctx, ANDR_ERR_ARG, "wall_bc[%d] = %d is not a boundary condition kind", k, wall_bc[k]);
+  ANDR_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (report) std::memset(report, 0, sizeof(andr_laplace_report));
+  const double t0 = andr_now();
+  // ---- the 12 integral matrices of the subdomain (bounded form: no exterior free term) ------------
+  SubMesh sub;
+  make_submesh(mesh, *sd, l, sub);
+  DevBody body;
+  if ((rc = andr_body_upload(ctx, &sub.m, body, true))) return rc;
+  const int nns = sub.m.nnodes, nes = sub.m.nelem, nsp = nns + nes;
+  const size_t sT = (size_t)nsp * nns, sG = (size_t)nsp * nes;
+  DevBuf<double> dT, dG, dpT, dpG, dE;
+  if (dT.alloc(6 * sT, ctx) || dG.alloc(6 * sG, ctx) || dpT.alloc(3 * sT, ctx) || dpG.alloc(3 * sG, ctx) ||
+      dE.alloc(nsp, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "Stokes integral matrices of the subdomain");
+  if ((rc = andr_launch_stokes_rows(ctx, body, 0.0, dT.p, dG.p, dpT.p, dpG.p, dE.p))) return rc;
+  std::vector<double> he(nsp);
+  ANDR_HIP(ctx, hipMemcpyAsync(he.data(), dE.p, sizeof(double) * nsp, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  dpT.release();
+  dpG.release();
+  double errU = 0.0, errQ = 0.0;
+  for (int i = 0; i < nsp; ++i) {
+    if (i < nns) errU = std::max(errU, he[i]);
+    else errQ = std::max(errQ, he[i]);
+  }
+  const double t1 = andr_now();
+  // ---- sdSetUpXandB per component, stacked as StokesBigSystemXB ---------------------------------------
+  std::vector<int> ncol(3 * (size_t)nns), qcol(3 * (size_t)nes);
+  std::vector<uint8_t> nknown(3 * (size_t)nns), qknown(3 * (size_t)nes);
+  int nx = 0, nb = 0;
+  for (int en = 0; en < 3; ++en) {
+    const int32_t *bcw = wall_bc + (size_t)en * nw;
+    int cx = 0, cb = 0;  // this component's unknowns and knowns, in the reference's order
+    for (int k = 0; k < nns; ++k) {
+      const bool kn = bcw[l.nwall[k] - 1] == ANDR_BC_FUNCTION;
+      nknown[(size_t)en * nns + k] = kn;
+      ncol[(size_t)en * nns + k] = kn ? nb + cb++ : nx + cx++;
+    }
+    for (int k = 0; k < nes; ++k) {
+      const bool kn = bcw[l.qwall[k] - 1] == ANDR_BC_FLUX;
+      qknown[(size_t)en * nes + k] = kn;
+      qcol[(size_t)en * nes + k] = kn ? nb + cb++ : nx + cx++;
+    }
+    nx += cx;
+    nb += cb;
+  }
+  // rows in order: nodes, then elements, the components of a point inside (stokesFormCRSsysMrhsM)
+  std::vector<int2> rows;
+  for (int k = 0; k < nns; ++k)
+    for (int een = 0; een < 3; ++een)
+      if (!nknown[(size_t)een * nns + k]) rows.push_back(make_int2(een, k));
+  for (int k = 0; k < nes; ++k)
+    for (int een = 0; een < 3; ++een)
+      if (!qknown[(size_t)een * nes + k]) rows.push_back(make_int2(een, nns + k));
+  const int neq = (int)rows.size();
+  if (nx == 0 || neq == 0) return andr_fail(ctx, ANDR_ERR_ARG, "the Stokes problem has no unknowns");
+  // formStokesLeftRightHandSideVectors: knowns b and the hot start x from u and q
+  std::vector<double> b(std::max(nb, 1), 0.0), xp((size_t)nx * ANDR_NB, 0.0);
+  std::vector<int> src_u(3 * (size_t)nn, -1), src_q(3 * (size_t)ne, -1);
+  for (int en = 0; en < 3; ++en) {
+    for (int k = 0; k < nns; ++k) {
+      const int isp = l.nodes[k], c = ncol[(size_t)en * nns + k];
+      const double v = u[(size_t)en * nn + isp];
+      if (nknown[(size_t)en * nns + k]) b[c] = v;
+      else {
+        xp[(size_t)c * ANDR_NB] = v;
+        src_u[(size_t)en * nn + isp] = c;
+      }
+    }
+    for (int k = 0; k < nes; ++k) {
+      const int isp = l.elems[k], c = qcol[(size_t)en * nes + k];
+      const double v = q[(size_t)en * ne + isp];
+      if (qknown[(size_t)en * nes + k]) b[c] = v;
+      else {
+        xp[(size_t)c * ANDR_NB] = v;
+        src_q[(size_t)en * ne + isp] = c;
+      }
+    }
+  }
+  const int64_t lda = ((int64_t)nx + 15) / 16 * 16;
+  DevBuf<double> sys, B, X, db, qs, dgm, du, dq;
+  DevBuf<int2> drows;
+  DevBuf<int> dnc, dqc, dsu, dsq;
+  DevBuf<uint8_t> dnk, dqk;
+  if (sys.alloc((size_t)neq * lda, ctx) || B.alloc((size_t)neq * ANDR_NB, ctx) || X.alloc((size_t)nx * ANDR_NB, ctx) ||
+      qs.alloc(nx, ctx) || du.alloc(3 * (size_t)nn, ctx) || dq.alloc(3 * (size_t)ne, ctx))
+    return andr_fail(ctx, ANDR_ERR_ALLOC, "Stokes system %d x %d", neq, nx);
+  if ((rc = upload(ctx, db, b)) || (rc = upload(ctx, dgm, g_factors(*sd, l))) || (rc = upload(ctx, drows, rows)) ||
+      (rc = upload(ctx, dnc, ncol)) || (rc = upload(ctx, dqc, qcol)) || (rc = upload(ctx, dnk, nknown)) ||
+      (rc = upload(ctx, dqk, qknown)) || (rc = upload(ctx, dsu, src_u)) || (rc = upload(ctx, dsq, src_q)))
+    return rc;
+  ANDR_HIP(ctx, hipMemcpyAsync(X.p, xp.data(), sizeof(double) * xp.size(), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st, (size_t)neq * lda, sys.p);
+  hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * ANDR_NB + 255) / 256), dim3(256), 0, st, (size_t)neq * ANDR_NB,
+                     B.p);
+  hipLaunchKernelGGL(k_stk_system, dim3(neq), dim3(256), 0, st, nns, nes, nsp, dT.p, dG.p, drows.p, dnc.p, dnk.p,
+                     dqc.p, dqk.p, dgm.p, db.p, sys.p, lda, B.p);
+  ANDR_HIP(ctx, hipGetLastError());
+  const int maxit = opts ? opts->maxit : 0;
+  const double eps = opts && opts->eps > 0.0 ? opts->eps : 1e-15;
+  if ((rc = andr_colscale(ctx, sys.p, false, lda, neq, nx, qs.p))) return rc;
+  int nit = 0, istop = 0;
+  if ((rc = andr_lsqr_device(ctx, sys.p, false, lda, neq, nx, qs.p, B.p, X.p, 1, maxit, eps, &nit, &istop))) return rc;
+  // StokesDistributeUnknowns, then the force on every wall (sum_e A_e q_e per component)
+  ANDR_HIP(ctx, hipMemcpyAsync(du.p, u, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dq.p, q, sizeof(double) * 3 * ne, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_lap_scatter, dim3((3 * nn + 255) / 256), dim3(256), 0, st, 3 * nn, dsu.p, X.p, du.p);
+  hipLaunchKernelGGL(k_lap_scatter, dim3((3 * ne + 255) / 256), dim3(256), 0, st, 3 * ne, dsq.p, X.p, dq.p);
+  DevBody whole;
+  DevBuf<double> dforce;
+  if (wall_force) {
+    if ((rc = andr_body_upload(ctx, mesh, whole, false))) return rc;
+    if (dforce.alloc(3 * (size_t)nw, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "wall forces");
+    for (int en = 0; en < 3; ++en)
+      hipLaunchKernelGGL(k_wall_flux, dim3(nw), dim3(256), 0, st, ne, whole.g.bcid, whole.g.area, dq.p + (size_t)en * ne,
+                         dforce.p + (size_t)en * nw);
+    ANDR_HIP(ctx, hipMemcpyAsync(wall_force, dforce.p, sizeof(double) * 3 * nw, hipMemcpyDeviceToHost, st));
+  }
+  ANDR_HIP(ctx, hipGetLastError());
+  ANDR_HIP(ctx, hipMemcpyAsync(u, du.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(q, dq.p, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, st));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
+  if (report) {
+    report->neq = neq;
+    report->nx = nx;
+    report->nb = nb;
+    report->nit = nit;
+    report->istop = istop;
+    report->err[0] = errU;
+    report->err[1] = std::max(errU, errQ);
+    report->t_assemble = t1 - t0;
+    report->t_solve = andr_now() - t1;
   }
   return ANDR_OK;
 }

@@ -188,6 +188,21 @@ int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const 
                        const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q, double *wall_flux,
                        andr_laplace_report *reports);
 
+/* The general Stokes problem of the reference's Stokes driver on one subdomain (its Stokes path has
+ * one, src/lib_StokesFull.f90:183): the 12 integral matrices (sdFormIntegralMatricesStokes, bounded
+ * form), the unknowns and knowns of the three velocity components (sdSetUpXandB per component,
+ * stacked as StokesBigSystemXB, src/lib_Stokes.f90:1180-1276), the system rows of the unknown values
+ * (stokesFormCRSsysMrhsM with formUrow / formQrow: -T and -(normmul/diff) G for the unknowns, +T and
+ * +(normmul/diff) G times the knowns into b = rhsM b, src/lib_Stokes.f90:475-637), the column-scaled
+ * LSQR solve with the hot start (StokesBigSystemSOLVEcrs, :417-469, with the solver limits of the first
+ * equation, :626-629) and the scatter back (StokesDistributeUnknowns, src/lib_StokesFull.f90:379-501).
+ * wall_bc[k * nwalls + w - 1] is the kind of wall w for velocity component k (x, y, z); u[k * nnodes + i]
+ * and q[k * nelem + e] are the known and initial values on entry and the solution on return;
+ * wall_force[k * nwalls + w - 1] (may be NULL) gets sum_e A_e q_k,e over wall w. Triangle meshes. */
+int andr_stokes_solve(andr_ctx *ctx, const andr_mesh *mesh, const andr_subdomain *sd, const int32_t *wall_bc,
+                      const andr_lsqr_opts *opts, double *u, double *q, double *wall_force,
+                      andr_laplace_report *report);
+
 /* The wall each node of each subdomain takes its condition from, as andr_laplace_solve uses it:
  * node_wall[s * nnodes + i] (0: node i is not in subdomain s). Host only: no context, no device.
  * ANDR_ERR_ARG with the reason in msg for an invalid mesh or subdomain. */
