@@ -31,11 +31,15 @@ struct SysJob {
 #endif
 };
 
+// vertex_sums (FP32 quadrature and storage only): rigid-body problems on bodies whose elements each
+// move as a whole (all three nodes moving or all at rest) take the vertex-summed stresslet moments
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
-                              bool fp32_store, double alg_bytes, double alg_flops);
+                              bool fp32_store, double alg_bytes, double alg_flops, bool vertex_sums = false);
 // Right block-Jacobi preconditioner of the iterative forms: minv[e] = inverse of the 3x3 self block
 // (singular Stokeslet block of element e at its own centroid, scaled as the system, plus the
 // null-space term). The assembly then stores A M^-1 and the solve returns y with x = M^-1 y.
+// In the symmetric-block layout (lda 0) it also writes the blocks' diagonal entries, which the fast
+// assembly (FP32 quadrature and storage) leaves to it: launch it before andr_launch_stokes_system.
 int andr_launch_self_blocks(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
 int andr_launch_unprecondition(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows);
 // Force and torque of the six problems on wall 1 of every job, and its largest row self-check error:
@@ -48,8 +52,11 @@ int andr_launch_stokes_domain(andr_ctx *ctx, DevBody &b, int npts, const double 
 // Surface pressure at the element centres (device arrays u[3 nn], q[3 ne], p[ne]); triangle meshes.
 int andr_launch_stokes_surface_pressure(andr_ctx *ctx, DevBody &b, const double *u, const double *q, double diff,
                                         double *p);
-// algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops, Laplace G + H 50 flops
+// algorithmic work of a pair evaluation: Stokes G + T kernels 107 flops (per-vertex stresslet
+// moments), 60 flops with the vertex-summed moments of the fast form's rigid-body problems
+// (stokes_pair_sum); Laplace G + H 50 flops
 #define ANDR_FLOPS_PER_STOKES_EVAL 107.0
+#define ANDR_FLOPS_PER_STOKES_EVAL_SUM 60.0
 #define ANDR_FLOPS_PER_LAPLACE_EVAL 50.0
 // Dunavant rules (classes 0..6 of the learned form) in the quadrature table
 #define ANDR_DUN_OFF0 297

@@ -1133,14 +1133,245 @@ __global__ __launch_bounds__(256) void k_stokes_system(const SysJob *__restrict_
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Fast form of the resistance system (FP32 quadrature into the six symmetric FP32 blocks; the
+// singular diagonal entries come from k_self_blocks in FP64). The rows are those of
+// k_stokes_system, with two changes to how the work maps onto the wave:
+//  * learned form: the elements of a row are counting-sorted by rule class in LDS before they are
+//    integrated, so that a wave runs one rule at a time (the rule's table loads are then scalar and
+//    no lane idles through a longer rule of a neighbour);
+//  * SUM: rigid-body problems where the three nodes of every element move alike take the
+//    vertex-summed stresslet moments (stokes_pair_sum: 6 sums per quadrature point instead of 18);
+//    otherwise (strain problems, elements with nodes of both kinds) the per-vertex moments.
+// ------------------------------------------------------------------------------------------------
+constexpr int FS_CHUNK = 8192;  // elements sorted at a time (16-bit indices in LDS)
+constexpr int FS_NCLS = 9;      // Dunavant classes 0..6, the fixed-order rule 7, the row's own element 8
+
+template <bool SUM>
+__global__ __launch_bounds__(256) void k_stokes_system_fast(const SysJob *__restrict__ jobs,
+                                                            const int2 *__restrict__ rowmap) {
+  // S: T sums times the node velocity flag; tsum: all T sums; M (per-vertex form): rotation /
+  // strain moments sum_v T_k(e, v) flag_v (x_v - centre)_d
+  constexpr int NACC = SUM ? 12 : 30;
+  __shared__ uint16_t s_list[FS_CHUNK];
+  __shared__ int s_cnt[4][FS_NCLS];
+  __shared__ double red[NACC * 4];
+  const int2 jr = rowmap[blockIdx.x];
+  const SysJob &J = jobs[jr.x];
+  const BodyGeom &g = J.g;
+  const int r = jr.y, ne = g.ne;
+  const double s[3] = {g.cen[3 * r], g.cen[3 * r + 1], g.cen[3 * r + 2]};
+  float acc[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; ++i) acc[i] = 0.f;
+  const double inv_mu = 1.0 / J.mu;
+  float *A = J.template Aptr<float>();
+  const int KB = (ne + 15) / 16;
+  // qoff, nq: one rule for the whole wave (its table entries are then scalar loads)
+  auto pair = [&](int e, int qoff, int nq) {
+    qoff = __builtin_amdgcn_readfirstlane(qoff);
+    nq = __builtin_amdgcn_readfirstlane(nq);
+    const int vv[3] = {g.con[3 * e], g.con[3 * e + 1], g.con[3 * e + 2]};
+    const double *x1 = g.xyz + 3 * vv[0], *x2 = g.xyz + 3 * vv[1], *x3 = g.xyz + 3 * vv[2];
+    const double m = g.mul[e];
+    double Gs[6];
+    if constexpr (SUM) {
+      const float m0 = J.moving ? (float)J.moving[vv[0]] : 1.f;
+      float Ts[6];
+      stokes_pair_sum<float>(s, x1, x2, x3, m * g.nrm[3 * e], m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2], g.area[e],
+                             qoff, nq, Gs, Ts);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        acc[k] += m0 * Ts[k];
+        acc[6 + k] += Ts[k];
+      }
+    } else {
+      const float mv[3] = {J.moving ? (float)J.moving[vv[0]] : 1.f, J.moving ? (float)J.moving[vv[1]] : 1.f,
+                           J.moving ? (float)J.moving[vv[2]] : 1.f};
+      float Tv[6][3];
+      stokes_pair_regular<float>(s, x1, x2, x3, m * g.nrm[3 * e], m * g.nrm[3 * e + 1], m * g.nrm[3 * e + 2],
+                                 g.area[e], qoff, nq, Gs, Tv);
+      const double *xv[3] = {x1, x2, x3};
+      float rv[3][3];
+#pragma unroll
+      for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) rv[v][d] = mv[v] * (float)(xv[v][d] - J.center[d]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        acc[k] += mv[0] * Tv[k][0] + mv[1] * Tv[k][1] + mv[2] * Tv[k][2];
+        acc[6 + k] += Tv[k][0] + Tv[k][1] + Tv[k][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          acc[12 + 3 * k + d] += Tv[k][0] * rv[0][d] + Tv[k][1] * rv[1][d] + Tv[k][2] * rv[2][d];
+      }
+    }
+    const double sc = -m * inv_mu;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) A[andr_sym_off(q, r, e, KB)] = (float)(sc * Gs[q]);
+  };
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (!J.cls) {  // one rule for every pair
+    for (int e = threadIdx.x; e < ne; e += blockDim.x)
+      if (e != r) pair(e, J.qoff, J.nq);
+  } else {
+    const uint8_t *crow = J.cls + (size_t)r * ne;
+    for (int c0 = 0; c0 < ne; c0 += FS_CHUNK) {
+      const int n = min(ne - c0, FS_CHUNK);
+      auto cls_of = [&](int i) { return c0 + i == r ? 8 : min((int)crow[c0 + i], 7); };
+      // count the chunk's elements per class and wave
+      int cnt[FS_NCLS];
+#pragma unroll
+      for (int k = 0; k < FS_NCLS; ++k) cnt[k] = 0;
+      for (int i0 = 0; i0 < n; i0 += blockDim.x) {
+        const int i = i0 + threadIdx.x;
+        const int k = i < n ? cls_of(i) : FS_NCLS;
+#pragma unroll
+        for (int kk = 0; kk < FS_NCLS; ++kk) cnt[kk] += __popcll(__ballot(k == kk));
+      }
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < FS_NCLS; ++k) s_cnt[wid][k] = cnt[k];
+      __syncthreads();
+      // class k occupies [off[k], off[k + 1]) of the sorted list; wave w's elements of class k start
+      // at off[k] + (class-k count of waves < w)
+      int off[FS_NCLS + 1], pos0[FS_NCLS];
+      off[0] = 0;
+#pragma unroll
+      for (int k = 0; k < FS_NCLS; ++k) {
+        int before = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const int c = s_cnt[w][k];
+          before += w < wid ? c : 0;
+          tot += c;
+        }
+        pos0[k] = __builtin_amdgcn_readfirstlane(off[k] + before);  // uniform over the wave
+        off[k + 1] = __builtin_amdgcn_readfirstlane(off[k] + tot);
+      }
+      for (int i0 = 0; i0 < n; i0 += blockDim.x) {
+        const int i = i0 + threadIdx.x;
+        const int k = i < n ? cls_of(i) : FS_NCLS;
+        int pos = 0;
+#pragma unroll
+        for (int kk = 0; kk < FS_NCLS; ++kk) {
+          const unsigned long long bm = __ballot(k == kk);
+          if (k == kk) pos = pos0[kk] + __popcll(bm & below);
+          pos0[kk] += __popcll(bm);
+        }
+        if (i < n) s_list[pos] = (uint16_t)i;
+      }
+      __syncthreads();
+      // integrate in sorted order; a wave whose lanes straddle a class boundary runs one class at a time
+      for (int p0 = 0; p0 < n; p0 += blockDim.x) {
+        const int p = p0 + threadIdx.x;
+        int k = FS_NCLS - 1, e = 0;
+        if (p < n) {
+          e = c0 + s_list[p];
+          k = 0;
+#pragma unroll
+          for (int kk = 1; kk < FS_NCLS; ++kk) k += p >= off[kk] ? 1 : 0;
+        }
+        bool todo = k < 8;  // class 8, the row's own element, is k_self_blocks' singular block
+        for (;;) {
+          const unsigned long long bm = __ballot(todo);
+          if (!bm) break;
+          const int kk = __builtin_amdgcn_readfirstlane(__shfl(k, (int)__builtin_ctzll(bm), 64));
+          if (todo && k == kk) {
+            int qoff = J.qoff, nq = J.nq;
+            if (kk < 7) {  // Dunavant offsets {0,3,7,13,20,32,45} and sizes {3,4,6,7,12,13,19}, 6 bits each
+              qoff = ANDR_DUN_OFF0 + (int)((0x2D8143470C0ull >> (6 * kk)) & 63);
+              nq = (int)((0x1334C1C6103ull >> (6 * kk)) & 63);
+            }
+            pair(e, qoff, nq);
+            todo = false;
+          }
+        }
+      }
+      __syncthreads();  // s_list and s_cnt are rewritten by the next chunk
+    }
+  }
+  double accd[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; ++i) accd[i] = (double)acc[i];
+  block_sum<NACC>(accd, red);
+  if (threadIdx.x == 0) {
+    const double *S = accd, *tsum = accd + 6;
+    // rotation moments sum_e sum_v T_k(e, v) flag_v (x_v - centre)_d: as integrated (per-vertex
+    // form), or S_k (s - centre)_d (vertex sums: x_v - centre = (x_v - s) + (s - centre), and the
+    // first part only adds third moments that the rotations cancel)
+    double M[18];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) M[3 * k + d] = SUM ? S[k] * (s[d] - J.center[d]) : accd[12 + 3 * k + d];
+    const double cfree[3] = {-tsum[0] + J.extra, -tsum[3] + J.extra, -tsum[5] + J.extra};
+    const int *cn = g.con + 3 * r;
+    double ubar[3][ANDR_NRHS];  // (1/3) sum_v u_k(node v), component c
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < ANDR_NRHS; ++k) ubar[c][k] = 0.0;
+    for (int v = 0; v < 3; ++v) {
+      const double w = (J.moving ? (double)J.moving[cn[v]] : 1.0) / 3.0;
+      const double *p = g.xyz + 3 * cn[v];
+      const double q[3] = {p[0] - J.center[0], p[1] - J.center[1], p[2] - J.center[2]};
+      if (J.strain) {
+        ubar[1][0] += w * q[2];
+        ubar[2][0] += w * q[1];
+        ubar[0][1] += w * q[2];
+        ubar[2][1] += w * q[0];
+        ubar[0][2] += w * q[1];
+        ubar[1][2] += w * q[0];
+        continue;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ubar[c][c] += w;
+      ubar[1][3] -= w * q[2];
+      ubar[2][3] += w * q[1];
+      ubar[0][4] += w * q[2];
+      ubar[2][4] -= w * q[0];
+      ubar[0][5] -= w * q[1];
+      ubar[1][5] += w * q[0];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double *Mc[3] = {M + 3 * sym3(c, 0), M + 3 * sym3(c, 1), M + 3 * sym3(c, 2)};
+      double b[ANDR_NRHS];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) b[k] = S[sym3(c, k)];
+      b[3] = -Mc[1][2] + Mc[2][1];
+      b[4] = Mc[0][2] - Mc[2][0];
+      b[5] = -Mc[0][1] + Mc[1][0];
+#pragma unroll
+      for (int k = 6; k < ANDR_NRHS; ++k) b[k] = 0.0;
+      if (J.strain) {
+        b[0] = Mc[1][2] + Mc[2][1];
+        b[1] = Mc[0][2] + Mc[2][0];
+        b[2] = Mc[0][1] + Mc[1][0];
+        b[3] = b[4] = b[5] = 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < ANDR_NRHS; ++k) J.B[(size_t)(c * ne + r) * ANDR_NRHS + k] = b[k] + cfree[c] * ubar[c][k];
+    }
+    const double osemPi = (double)(atanf(1.0f)) * 4.0 * 8.0;
+    double err = fmax(fabs(tsum[1]), fmax(fabs(tsum[4]), fabs(tsum[2])));
+    for (int c = 0; c < 3; ++c) err = fmax(err, fabs(cfree[c] - J.extra - 0.5 * osemPi));
+    if (J.err_rows) J.err_rows[r] = err;
+  }
+}
+
 int andr_launch_stokes_system(andr_ctx *ctx, const SysJob *d_jobs, const int2 *d_rowmap, int nrows, bool fp32_quad,
-                              bool fp32_store, double alg_bytes, double alg_flops) {
+                              bool fp32_store, double alg_bytes, double alg_flops, bool vertex_sums) {
   if (nrows <= 0) return ANDR_OK;
   const int tok = andr_prof_begin(ctx, fp32_quad ? "stokes_system_fp32" : "stokes_system_fp64", alg_bytes, alg_flops);
   if (!fp32_quad && !fp32_store)
     hipLaunchKernelGGL((k_stokes_system<double, double>), dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
+  else if (fp32_quad && fp32_store && vertex_sums)  // symmetric blocks, diagonal from k_self_blocks
+    hipLaunchKernelGGL(k_stokes_system_fast<true>, dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
   else if (fp32_quad && fp32_store)
-    hipLaunchKernelGGL((k_stokes_system<float, float>), dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
+    hipLaunchKernelGGL(k_stokes_system_fast<false>, dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
   else if (!fp32_quad && fp32_store)
     hipLaunchKernelGGL((k_stokes_system<double, float>), dim3(nrows), dim3(256), 0, ctx->stream, d_jobs, d_rowmap);
   else
@@ -1213,6 +1444,13 @@ __global__ void k_self_blocks(const SysJob *__restrict__ jobs, const int2 *__res
   double Gs[6];
   andr_sing_stk(x1, x2, x3, s, 4, Gs);
   const double sc = -g.mul[e] / J.mu;
+  if (J.lda == 0) {  // symmetric FP32 blocks: their diagonal entries (k_stokes_system_fast skips them)
+    float *A = J.template Aptr<float>();
+    const int KB = (g.ne + 15) / 16;
+    const double scs = -g.mul[e] * (1.0 / J.mu);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) A[andr_sym_off(q, e, e, KB)] = (float)(scs * Gs[q]);
+  }
   const double dfl = J.defl ? J.defl[g.bcid[e]] * g.area[e] : 0.0;
   const double *n = g.nrm + 3 * e;
   double a[3][3];

@@ -16,4 +16,4 @@ struct LearnedStats {
 // classifies every pair of the listed rows (writes SysJob::cls), then assembles with those rules;
 // stats[njobs] (may be null) gets the counts of each job
 int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJob *d_jobs, const int2 *d_rowmap,
-                                      int nrows, int njobs, LearnedStats *stats);
+                                      int nrows, int njobs, bool vertex_sums, LearnedStats *stats);

@@ -821,7 +821,7 @@ extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *params, flo
 }
 
 int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJob *d_jobs, const int2 *d_rowmap,
-                                      int nrows, int njobs, LearnedStats *stats) {
+                                      int nrows, int njobs, bool summed, LearnedStats *stats) {
   if (!ctx->has_model) return andr_fail(ctx, ANDR_ERR_STATE, "no quadrature model");
   char *base = (char *)ctx->d_model;
   const float *fp = (const float *)base;
@@ -860,7 +860,9 @@ int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJo
     r.flops = near * 2.0 * ANDR_MLP_FLOPS_PER_PASS;
     r.bytes = cls_bytes;
   }
-  return andr_launch_stokes_system(ctx, d_jobs, d_rowmap, nrows, true, true, bytes, points * ANDR_FLOPS_PER_STOKES_EVAL);
+  return andr_launch_stokes_system(ctx, d_jobs, d_rowmap, nrows, true, true, bytes,
+                                   points * (summed ? ANDR_FLOPS_PER_STOKES_EVAL_SUM : ANDR_FLOPS_PER_STOKES_EVAL),
+                                   summed);
 }
 
 int andr_upload_rules_learned(andr_ctx *ctx, const double *tab, const float *tabf) {

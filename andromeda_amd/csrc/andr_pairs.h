@@ -133,3 +133,57 @@ __device__ inline void stokes_pair_regular(const double s[3], const double *x1, 
   }
 }
 
+// As stokes_pair_regular, with the stresslet moments summed over the vertices: Ts[k] = sum_v Tv[k][v]
+// (the shape functions sum to one, so the vertex weights drop out of the quadrature loop). Enough for
+// a velocity field that is affine over the element (a rigid motion of all three vertices): then
+// sum_v Tv[k][v] u(x_v) = int T_k u(x), and the rotation terms reduce to Ts and the collocation point
+// (the third moments int T_{c,i} r_j enter the rotations antisymmetrised in (i, j), where
+// T_{c,i} r_j = t r_c r_i r_j cancels at every quadrature point).
+template <class CT>
+__device__ inline void stokes_pair_sum(const double s[3], const double *x1, const double *x2, const double *x3,
+                                       double nx, double ny, double nz, double area, int qoff, int nq, double Gs[6],
+                                       CT Ts[6]) {
+  const CT d0 = (CT)(x3[0] - s[0]), d1 = (CT)(x3[1] - s[1]), d2 = (CT)(x3[2] - s[2]);
+  const CT a0 = (CT)(x1[0] - x3[0]), a1 = (CT)(x1[1] - x3[1]), a2 = (CT)(x1[2] - x3[2]);
+  const CT b0 = (CT)(x2[0] - x3[0]), b1 = (CT)(x2[1] - x3[1]), b2 = (CT)(x2[2] - x3[2]);
+  const CT n0 = (CT)nx, n1 = (CT)ny, n2 = (CT)nz;
+  CT g[6] = {0, 0, 0, 0, 0, 0};
+  CT t[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < nq; ++q) {
+    CT l1, l2, w;
+    if constexpr (sizeof(CT) == 8) {
+      l1 = c_qd[qoff + q][0]; l2 = c_qd[qoff + q][1]; w = c_qd[qoff + q][3];
+    } else {
+      l1 = c_qf[qoff + q][0]; l2 = c_qf[qoff + q][1]; w = c_qf[qoff + q][3];
+    }
+    const CT rx = d0 + l1 * a0 + l2 * b0;
+    const CT ry = d1 + l1 * a1 + l2 * b1;
+    const CT rz = d2 + l1 * a2 + l2 * b2;
+    const CT rxx = rx * rx, rxy = rx * ry, rxz = rx * rz, ryy = ry * ry, ryz = ry * rz, rzz = rz * rz;
+    CT ir;
+    if constexpr (sizeof(CT) == 8) ir = rsqrt(rxx + ryy + rzz);
+    else ir = __frsqrt_rn(rxx + ryy + rzz);
+    const CT ir2 = ir * ir, ir3 = ir2 * ir;
+    const CT wir = w * ir, wir3 = w * ir3;
+    g[0] += wir + rxx * wir3;
+    g[1] += rxy * wir3;
+    g[2] += rxz * wir3;
+    g[3] += wir + ryy * wir3;
+    g[4] += ryz * wir3;
+    g[5] += wir + rzz * wir3;
+    const CT wc = (CT)(-6.0) * wir3 * ir2 * (rx * n0 + ry * n1 + rz * n2);
+    t[0] += wc * rxx;
+    t[1] += wc * rxy;
+    t[2] += wc * rxz;
+    t[3] += wc * ryy;
+    t[4] += wc * ryz;
+    t[5] += wc * rzz;
+  }
+  const CT ca = (CT)area;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    Gs[k] = (double)g[k] * area;
+    Ts[k] = t[k] * ca;
+  }
+}
+

@@ -29,6 +29,7 @@ struct Prepared {
   std::vector<int> node_wall;
   std::vector<double> defl;  // per wall id 0..nwalls (iterative forms)
   int ne = 0, nn = 0;
+  bool moving_alike = true;  // bounded form: the three nodes of every element all move or all rest
 };
 
 // Null-space deflation for the iterative (mixed and learned) forms. On a closed surface the
@@ -148,6 +149,11 @@ int prepare_bodies(andr_ctx *ctx, const andr_mesh *meshes, int b0, int b1, bool 
         p.node_wall[nd] = std::max(p.node_wall[nd], w);
       }
     }
+    for (int e = 0; e < p.ne && p.moving_alike; ++e) {  // moving nodes: those of wall 2 (run_wave)
+      const int *c = m->con + 3 * e;
+      const bool m0 = p.node_wall[c[0] - m->index_base] == 2;
+      p.moving_alike = m0 == (p.node_wall[c[1] - m->index_base] == 2) && m0 == (p.node_wall[c[2] - m->index_base] == 2);
+    }
   }
   return ANDR_OK;
 }
@@ -246,16 +252,20 @@ int run_wave(andr_ctx *ctx, Prepared *const *P, int nb, const andr_resistance_op
   ANDR_HIP(ctx, hipMemcpyAsync(drow.p, rowmap.data(), sizeof(int2) * rowmap.size(), hipMemcpyHostToDevice, ctx->stream));
   std::vector<LearnedStats> ls(nb, LearnedStats{0, 0, 0});
   double alg_bytes = 0.0, alg_flops = 0.0;
+  // fast forms: vertex-summed stresslet moments when every element moves rigidly as a whole
+  bool summed = !plain && opts->problems == ANDR_PROBLEMS_RIGID;
+  for (int i = 0; i < nb; ++i) summed = summed && (exterior || P[i]->moving_alike);
   for (int i = 0; i < nb; ++i) {
     const double ne = P[i]->ne;
     // plain: the full 3F x 3F FP64 matrix; fast forms: the six symmetric FP32 blocks (6 F^2)
     alg_bytes += (plain ? 9.0 : 6.0) * ne * ne * esz + 3.0 * ne * ANDR_NRHS * 8.0;
-    alg_flops += ne * (ne - 1.0) * jobs[i].nq * ANDR_FLOPS_PER_STOKES_EVAL;
+    alg_flops += ne * (ne - 1.0) * jobs[i].nq * (summed ? ANDR_FLOPS_PER_STOKES_EVAL_SUM : ANDR_FLOPS_PER_STOKES_EVAL);
   }
   if (!plain && (rc = andr_launch_self_blocks(ctx, djobs.p, drow.p, (int)rowmap.size()))) return rc;
-  if (learned) rc = andr_launch_stokes_system_learned(ctx, jobs.data(), djobs.p, drow.p, (int)rowmap.size(), nb, ls.data());
+  if (learned)
+    rc = andr_launch_stokes_system_learned(ctx, jobs.data(), djobs.p, drow.p, (int)rowmap.size(), nb, summed, ls.data());
   else
-    rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain, alg_bytes, alg_flops);
+    rc = andr_launch_stokes_system(ctx, djobs.p, drow.p, (int)rowmap.size(), !plain, !plain, alg_bytes, alg_flops, summed);
   if (rc) return rc;
   ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const double ta1 = andr_now();
