@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -52,6 +53,7 @@ struct GSys {  // device view of one system
   StokesOp op;
   int nep;       // op.ne rounded up to 32
   double *ZT;    // 3 x NB x nep: z = M^-1 q_j, [component][rhs][element] (Krylov product operand)
+  float *ZT32;   // the same in FP32 (FP32 products), null: FP64 products
   double *Z64;   // 3 ne x NB: z = M^-1 x in FP64 (residual)
   double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
   int et0, net;  // element tiles (32 elements) of the system in the current map
@@ -95,7 +97,10 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
     const double *mi = O.minv + 9 * (size_t)e;
     const double z[3] = {mi[0] * q0 + mi[1] * q1 + mi[2] * q2, mi[3] * q0 + mi[4] * q1 + mi[5] * q2,
                          mi[6] * q0 + mi[7] * q1 + mi[8] * q2};
-    if (mode == 0) {
+    if (mode == 0 && S.ZT32) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) S.ZT32[((size_t)c * NB + k) * S.nep + e] = (float)z[c];
+    } else if (mode == 0) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = z[c];
     } else {
@@ -106,7 +111,10 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
     dz = O.nrm[3 * e] * z[0] + O.nrm[3 * e + 1] * z[1] + O.nrm[3 * e + 2] * z[2];
   } else if (mode == 0 && e < S.nep) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      if (S.ZT32) S.ZT32[((size_t)c * NB + k) * S.nep + e] = 0.f;
+      else S.ZT[((size_t)c * NB + k) * S.nep + e] = 0.0;
+    }
   }
 #pragma unroll
   for (int ww = 0; ww < MAXW; ++ww) {
@@ -151,10 +159,14 @@ __device__ inline double defl_term(const StokesOp &O, const double *sdef, int r,
   return d != 0.0 ? d * O.area[r] * O.nrm[3 * r + c] * sdef[(w - 1) * NB + k] : 0.0;
 }
 
-// The Krylov product W = A M^-1 Q_j on the FP64 matrix cores: y_c = sum_en S_{c,en} z_en (+ the
-// null-space term), S the six symmetric FP32 blocks (converted to FP64 in registers, exactly),
-// z = M^-1 q_j in FP64, products and sums in FP64 (v_mfma_f64_16x16x4f64), so the Krylov relation
-// holds to FP64 rounding and the true residual follows the GMRES estimate.
+// The Krylov product W = A M^-1 Q_j on the matrix cores: y_c = sum_en S_{c,en} z_en (+ the
+// null-space term), S the six symmetric FP32 blocks, z = M^-1 q_j.
+//  * F32 = false: S converted to FP64 in registers (exactly), z in FP64, products and sums in FP64
+//    (v_mfma_f64_16x16x4f64), so the Krylov relation holds to FP64 rounding;
+//  * F32 = true: z rounded to FP32, products on the FP32 matrix cores (v_mfma_f32_16x16x4f32, twice
+//    the FP64 rate) summed in FP32 over one 16-column block and the block sums accumulated in FP64:
+//    the product carries a relative error of ~1e-7, which GMRES tolerates at relative residuals of
+//    1e-6 and above (the product then no longer bounds the kernel, the matrix stream does).
 //
 // The B operand has 16 columns for 8 right-hand sides: each block product fills both halves.
 // With h = column half, three accumulators per 16-row band take the six blocks:
@@ -164,10 +176,13 @@ __device__ inline double defl_term(const StokesOp &O, const double *sdef, int r,
 // so y_x = P_lo + Q_hi, y_y = R_lo + P_hi, y_z = Q_lo + R_hi: six block products per column step,
 // each matrix block read once. A wave owns 32 element rows (two bands) over a column segment; lane
 // (g = lane / 16, c = lane % 16) feeds row 16 T + c with columns 16 kb + 4 g + t of every block (one
-// float4 per block and band) and column c of B with the same columns. The FP64 variant returns
-// D[g + 4 v][c] in register v of lane (g, c) (rows interleaved, unlike the FP32 16x16x4 form).
+// float4 per block and band) and column c of B with the same columns. The FP64 form returns
+// D[g + 4 v][c] in register v of lane (g, c), the FP32 form D[4 g + v][c].
+template <bool F32>
 __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
                                                    int nwaves) {
+  using ZT_t = typename std::conditional<F32, float, double>::type;  // z operand
+  typedef float f4n __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
@@ -180,43 +195,75 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   const float *ab[2];
 #pragma unroll
   for (int T = 0; T < 2; ++T) ab[T] = O.S + (size_t)((row0 >> 4) + T) * KBe * 1536 + 16 * c + 4 * g;
-  const double *zb[3];
+  const ZT_t *zt;
+  if constexpr (F32) zt = S.ZT32;
+  else zt = S.ZT;
+  const ZT_t *zb[3];
 #pragma unroll
-  for (int en = 0; en < 3; ++en) zb[en] = S.ZT + ((size_t)en * NB + (c & (NB - 1))) * S.nep + 4 * g;
+  for (int en = 0; en < 3; ++en) zb[en] = zt + ((size_t)en * NB + (c & (NB - 1))) * S.nep + 4 * g;
   d4v acc[2][3];
+  f4n accf[2][3];  // F32: the current block's sums
 #pragma unroll
   for (int T = 0; T < 2; ++T)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) acc[T][a] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < 3; ++a) {
+      acc[T][a] = d4v{0.0, 0.0, 0.0, 0.0};
+      accf[T][a] = f4n{0.f, 0.f, 0.f, 0.f};
+    }
   // one column step t of a 16-column block: operands a[T][q][t], z[en][t]
-  auto step = [&](const float (&a)[2][6][4], const double (&z)[3][4], int t) {
-    const double z0 = z[0][t], z1 = z[1][t], z2 = z[2][t];
-    const double b0 = hi ? 0.0 : z0, b1 = hi ? z0 : z1, b3 = hi ? z1 : 0.0;
-    const double b2 = hi ? z2 : z0, b5 = hi ? 0.0 : z2, b4 = hi ? z1 : z2;
+  auto step = [&](const float (&a)[2][6][4], const ZT_t (&z)[3][4], int t) {
+    const ZT_t z0 = z[0][t], z1 = z[1][t], z2 = z[2][t];
+    const ZT_t b0 = hi ? (ZT_t)0 : z0, b1 = hi ? z0 : z1, b3 = hi ? z1 : (ZT_t)0;
+    const ZT_t b2 = hi ? z2 : z0, b5 = hi ? (ZT_t)0 : z2, b4 = hi ? z1 : z2;
     // bands interleaved so that consecutive instructions never share an accumulator
+    if constexpr (F32) {
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][0][t], b0, acc[T][0], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][0][t], b0, accf[T][0], 0, 0, 0);
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][2][t], b2, acc[T][1], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][2][t], b2, accf[T][1], 0, 0, 0);
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][2] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][4][t], b4, acc[T][2], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][4][t], b4, accf[T][2], 0, 0, 0);
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][1][t], b1, acc[T][0], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][1][t], b1, accf[T][0], 0, 0, 0);
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][5][t], b5, acc[T][1], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][5][t], b5, accf[T][1], 0, 0, 0);
 #pragma unroll
-    for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][3][t], b3, acc[T][0], 0, 0, 0);
+      for (int T = 0; T < 2; ++T) accf[T][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T][3][t], b3, accf[T][0], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][0][t], b0, acc[T][0], 0, 0, 0);
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][2][t], b2, acc[T][1], 0, 0, 0);
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][2] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][4][t], b4, acc[T][2], 0, 0, 0);
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][1][t], b1, acc[T][0], 0, 0, 0);
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][1] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][5][t], b5, acc[T][1], 0, 0, 0);
+#pragma unroll
+      for (int T = 0; T < 2; ++T) acc[T][0] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[T][3][t], b3, acc[T][0], 0, 0, 0);
+    }
   };
-  // full column blocks, software-pipelined: the operands of block kb + 1 are requested before the
-  // products of block kb, so that their memory latency overlaps the matrix-core work
+  // F32: a block's FP32 sums into the FP64 totals
+  auto flush = [&]() {
+    if constexpr (F32) {
+#pragma unroll
+      for (int T = 0; T < 2; ++T)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[T][a][v] += (double)accf[T][a][v];
+          accf[T][a] = f4n{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  };
   // global-address-space views of the operands (plain global loads instead of flat ones)
   typedef const float __attribute__((address_space(1))) *gfp;
-  typedef const double __attribute__((address_space(1))) *gdp;
-  typedef float f4n __attribute__((ext_vector_type(4)));
+  typedef const ZT_t __attribute__((address_space(1))) *gzp;
   typedef double d2n __attribute__((ext_vector_type(2)));
   const gfp ga[2] = {(gfp)ab[0], (gfp)ab[1]};
-  const gdp gz[3] = {(gdp)zb[0], (gdp)zb[1], (gdp)zb[2]};
-  auto load = [&](int k, float (&a)[2][6][4], double (&z)[3][4]) {
+  const gzp gz[3] = {(gzp)zb[0], (gzp)zb[1], (gzp)zb[2]};
+  auto load = [&](int k, float (&a)[2][6][4], ZT_t (&z)[3][4]) {
 #pragma unroll
     for (int T = 0; T < 2; ++T)
 #pragma unroll
@@ -230,12 +277,20 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
       }
 #pragma unroll
     for (int en = 0; en < 3; ++en) {
-      const d2n u = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k);
-      const d2n w = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k + 2);
-      z[en][0] = u.x;
-      z[en][1] = u.y;
-      z[en][2] = w.x;
-      z[en][3] = w.y;
+      if constexpr (F32) {
+        const f4n u = *(const f4n __attribute__((address_space(1))) *)(gz[en] + 16 * k);
+        z[en][0] = u.x;
+        z[en][1] = u.y;
+        z[en][2] = u.z;
+        z[en][3] = u.w;
+      } else {
+        const d2n u = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k);
+        const d2n w = *(const d2n __attribute__((address_space(1))) *)(gz[en] + 16 * k + 2);
+        z[en][0] = u.x;
+        z[en][1] = u.y;
+        z[en][2] = w.x;
+        z[en][3] = w.y;
+      }
     }
   };
   // full column blocks kb < kb_end, two register sets in turn: the operands of the next block are
@@ -245,7 +300,7 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   const int kb_end = kend >> 4;
   if (kb < kb_end) {
     float a0[2][6][4], a1[2][6][4];
-    double z0[3][4], z1[3][4];
+    ZT_t z0[3][4], z1[3][4];
     load(kb, a0, z0);
     for (; kb + 1 < kb_end; kb += 2) {
       // the scheduling barriers keep each group of loads ahead of the products it overlaps (left
@@ -254,27 +309,30 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int t = 0; t < 4; ++t) step(a0, z0, t);
+      flush();
       __builtin_amdgcn_sched_barrier(0);
       load(min(kb + 2, kb_end - 1), a0, z0);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int t = 0; t < 4; ++t) step(a1, z1, t);
+      flush();
       __builtin_amdgcn_sched_barrier(0);
     }
     if (kb < kb_end) {  // odd count: a0 holds block kb
 #pragma unroll
       for (int t = 0; t < 4; ++t) step(a0, z0, t);
+      flush();
       ++kb;
     }
   }
   if (16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
     float a[2][6][4];
-    double z[3][4];
+    ZT_t z[3][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const bool in = 16 * kb + 4 * g + t < kend;
 #pragma unroll
-      for (int en = 0; en < 3; ++en) z[en][t] = in ? zb[en][16 * kb + t] : 0.0;
+      for (int en = 0; en < 3; ++en) z[en][t] = in ? zb[en][16 * kb + t] : (ZT_t)0;
 #pragma unroll
       for (int T = 0; T < 2; ++T)
 #pragma unroll
@@ -282,6 +340,7 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) step(a, z, t);
+    flush();
   }
   const bool direct = S.nseg <= 1;
   double *Y = direct ? S.W : S.P + (size_t)wm.seg * S.n * NB;
@@ -291,7 +350,7 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
     for (int v = 0; v < 4; ++v) {
       const double p = acc[T][0][v], q = acc[T][1][v], rr = acc[T][2][v];
       const double ph = __shfl_xor(p, NB, 64), qh = __shfl_xor(q, NB, 64), rh = __shfl_xor(rr, NB, 64);
-      const int r = row0 + 16 * T + g + 4 * v;
+      const int r = row0 + 16 * T + (F32 ? 4 * g + v : g + 4 * v);
       if (hi || r >= ne) continue;
       const double y[3] = {p + qh, rr + ph, q + rh};
 #pragma unroll
@@ -868,6 +927,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   const size_t state_per = (size_t)NB * 8;
   DevBuf<double> R, W, Q, SM, ST8, partW, partT, Z64, SDEF, partS;
   DevBuf<double> ZT;
+  DevBuf<float> ZT32;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
   if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
@@ -875,7 +935,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       partT.alloc((size_t)nt_full * part_stride(m) * NB, ctx) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
-  if ((Z64.alloc(vec_elems, ctx) || ZT.alloc(zt_elems, ctx) ||
+  // Krylov products on the FP32 matrix cores at relative residuals of 1e-6 and above (their ~1e-7
+  // relative error stays below the tolerance); FP64 products for tighter tolerances.
+  // ANDR_GMRES_FP64_PRODUCTS=1 keeps FP64 products at every tolerance.
+  const bool f32prod = tol >= 1e-6 && getenv("ANDR_GMRES_FP64_PRODUCTS") == nullptr;
+  if ((Z64.alloc(vec_elems, ctx) || (f32prod ? ZT32.alloc(zt_elems, ctx) : ZT.alloc(zt_elems, ctx)) ||
                      SDEF.alloc((size_t)nsys * MAXW * NB, ctx) || partS.alloc((size_t)net_full * MAXW * NB, ctx)))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES operator workspace");
   // a map set over a list of systems: residual blocks of 64 element rows, tiles of
@@ -982,7 +1046,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     const int n = in[s].n;
     g.op = in[s].op;
     g.nep = (g.op.ne + 31) / 32 * 32;
-    g.ZT = ZT.p + zo;
+    g.ZT = f32prod ? nullptr : ZT.p + zo;
+    g.ZT32 = f32prod ? ZT32.p + zo : nullptr;
     g.Z64 = Z64.p + vo;
     g.sdef = SDEF.p + (size_t)s * MAXW * NB;
     zo += (size_t)3 * NB * g.nep;
@@ -1076,7 +1141,10 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
     hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
     if (krylov) {
-      hipLaunchKernelGGL(kb_gemv_sym, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
+      if (f32prod)
+        hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
+      else
+        hipLaunchKernelGGL(kb_gemv_sym<false>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
       if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
     } else {
       hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);

@@ -222,7 +222,7 @@ typedef struct {
   int32_t form;      /* ANDR_FORM_* */
   int32_t precision; /* ANDR_PREC_* */
   double mu;         /* viscosity (subdomain diff), default 1 */
-  double eps;        /* solver tolerance; 0 -> 1e-12 (LSQR eps, plain) or 1e-7 (GMRES relative
+  double eps;        /* solver tolerance; 0 -> 1e-12 (LSQR eps, plain) or 1e-6 (GMRES relative
                         residual, mixed and learned) */
   int32_t maxit;     /* 0 -> default */
   double center[3];  /* torque and rotation reference point */

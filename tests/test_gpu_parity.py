@@ -152,6 +152,31 @@ def test_mixed_vs_plain(ctx):
     assert tensor_error(x.FT, p.FT) < 1e-6
 
 
+def test_krylov_product_precisions(ctx, monkeypatch):
+    """At the default relative residual (1e-6) the Krylov products run on the FP32 matrix cores,
+    at 1e-8 on the FP64 ones; ANDR_GMRES_FP64_PRODUCTS=1 forces FP64 products. On a superellipsoid
+    and a fibre of the batched workload both product precisions give tensors within 1e-6 of the
+    plain FP64 path and within 3e-7 of each other, in about the same number of Krylov steps."""
+    allb = [m for m, _ in M.batch_bodies(512, 64)]
+    bodies = [allb[42], allb[534]]
+    plain = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
+    f32 = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    monkeypatch.setenv("ANDR_GMRES_FP64_PRODUCTS", "1")
+    f64 = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    monkeypatch.delenv("ANDR_GMRES_FP64_PRODUCTS")
+    tight = ctx.resistance_batch(bodies, precision=api.PREC_MIXED, eps=1e-8)
+    for p, a, b, t in zip(plain, f32, f64, tight):
+        print(f"F {a.FT.shape}: its f32 {a.nit.max()} f64 {b.nit.max()} (1e-8: {t.nit.max()}); "
+              f"err f32 {tensor_error(a.FT, p.FT):.1e} f64 {tensor_error(b.FT, p.FT):.1e} "
+              f"1e-8 {tensor_error(t.FT, p.FT):.1e}")
+        assert a.status == 0 and b.status == 0 and t.status == 0
+        assert tensor_error(a.FT, p.FT) < 1e-6 and tensor_error(b.FT, p.FT) < 1e-6
+        assert tensor_error(t.FT, p.FT) < 1e-6
+        assert tensor_error(a.FT, b.FT) < 3e-7
+        assert a.nit.max() <= 1.15 * b.nit.max() + 2
+        assert t.nit.max() > b.nit.max()
+
+
 def test_slender_body_long_restarts(ctx):
     """A 496-element fibre of the batched workload whose hardest column needs several hundred
     Krylov steps with a first cycle of 128: the cycles after the first run with a longer restart in
