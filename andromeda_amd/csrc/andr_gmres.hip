@@ -518,38 +518,42 @@ __host__ __device__ constexpr int part_stride(int m) { return 2 * (m + 1) + 3; }
 // per (tile, chunk of ICH basis vectors): partial s_i = Q_i . u and t_i = Q_i . W over the tile's
 // rows, part[(tile * PS + slot) * NB + k]; the first chunk also adds u.u, u.W, W.W. fin: W = 0
 // (closing a cycle: only the reorthogonalisation of the last candidate). Wave w of the block takes
-// the basis vectors i0 + w, i0 + w + 4, ... over all 64 rows of the tile (lane: row within a group
-// of 8, rhs), so that each dot product is reduced across lanes once per tile.
+// the basis vectors i0 + w, i0 + w + 4, ... over all 64 rows of the tile; a lane holds two
+// right-hand sides of one row (16-byte loads, 16 rows per load), so that each dot product is
+// reduced across lanes once per tile.
 __global__ __launch_bounds__(256) void kb_adots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
                                                 int m, double *__restrict__ part, int fin) {
   constexpr int PW = ICH / 4;  // basis vectors per wave
+  constexpr int RT = TILE / 16;  // rows per lane
+  typedef double d2 __attribute__((ext_vector_type(2)));
   const int i0 = blockIdx.y * ICH;
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n, PS = part_stride(m);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int k = lane & 7, r8 = lane >> 3;
-  const double *U = S.Q + (size_t)j * n * NB;
-  double u[TILE / 8], w[TILE / 8];
-  int rows[TILE / 8];
+  const int kp = lane & 3, r16 = lane >> 2;  // right-hand sides 2 kp, 2 kp + 1; row within 16
+  const d2 *U = (const d2 *)(S.Q + (size_t)j * n * NB) + kp;
+  const d2 *Wv = (const d2 *)S.W + kp;
+  d2 u[RT], w[RT];
+  int rows[RT];
 #pragma unroll
-  for (int t = 0; t < TILE / 8; ++t) {
-    const int row = tm.row0 + 8 * t + r8;
+  for (int t = 0; t < RT; ++t) {
+    const int row = tm.row0 + 16 * t + r16;
     rows[t] = row < n ? row : -1;
-    u[t] = row < n ? U[(size_t)row * NB + k] : 0.0;
-    w[t] = (row < n && !fin) ? S.W[(size_t)row * NB + k] : 0.0;
+    u[t] = row < n ? U[(size_t)row * (NB / 2)] : d2{0.0, 0.0};
+    w[t] = (row < n && !fin) ? Wv[(size_t)row * (NB / 2)] : d2{0.0, 0.0};
   }
-  double as[PW], at[PW];
+  d2 as[PW], at[PW];
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
     const int i = i0 + wid + 4 * p;
-    double a = 0.0, b = 0.0;
+    d2 a = {0.0, 0.0}, b = {0.0, 0.0};
     if (i < j) {
-      const double *Qi = S.Q + (size_t)i * n * NB + k;
+      const d2 *Qi = (const d2 *)(S.Q + (size_t)i * n * NB) + kp;
 #pragma unroll
-      for (int t = 0; t < TILE / 8; ++t)
+      for (int t = 0; t < RT; ++t)
         if (rows[t] >= 0) {
-          const double q = Qi[(size_t)rows[t] * NB];
+          const d2 q = Qi[(size_t)rows[t] * (NB / 2)];
           a += q * u[t];
           b += q * w[t];
         }
@@ -557,7 +561,8 @@ __global__ __launch_bounds__(256) void kb_adots(const GSys *__restrict__ sys, co
     as[p] = a;
     at[p] = b;
   }
-  auto rsum = [&](double x) {  // over the 8 row groups of the wave
+  auto rsum = [&](double x) {  // over the 16 row groups of the wave
+    x += __shfl_xor(x, 4, 64);
     x += __shfl_xor(x, 8, 64);
     x += __shfl_xor(x, 16, 64);
     x += __shfl_xor(x, 32, 64);
@@ -565,25 +570,33 @@ __global__ __launch_bounds__(256) void kb_adots(const GSys *__restrict__ sys, co
   };
 #pragma unroll
   for (int p = 0; p < PW; ++p) {
-    const double a = rsum(as[p]), b = rsum(at[p]);
+    const double a0 = rsum(as[p].x), a1 = rsum(as[p].y), b0 = rsum(at[p].x), b1 = rsum(at[p].y);
     const int i = i0 + wid + 4 * p;
-    if (lane < 8 && i < j) {
-      part[((size_t)blockIdx.x * PS + i) * NB + k] = a;
-      part[((size_t)blockIdx.x * PS + (m + 1) + i) * NB + k] = b;
+    if (lane < 4 && i < j) {
+      double *ps = part + ((size_t)blockIdx.x * PS + i) * NB + 2 * kp;
+      double *pt = part + ((size_t)blockIdx.x * PS + (m + 1) + i) * NB + 2 * kp;
+      ps[0] = a0;
+      ps[1] = a1;
+      pt[0] = b0;
+      pt[1] = b1;
     }
   }
   if (blockIdx.y == 0 && wid == 0) {
-    double sc[3] = {0.0, 0.0, 0.0};
+    d2 sc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
-    for (int t = 0; t < TILE / 8; ++t) {
+    for (int t = 0; t < RT; ++t) {
       sc[0] += u[t] * u[t];
       sc[1] += u[t] * w[t];
       sc[2] += w[t] * w[t];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double v = rsum(sc[c]);
-      if (lane < 8) part[((size_t)blockIdx.x * PS + 2 * (m + 1) + c) * NB + k] = v;
+      const double v0 = rsum(sc[c].x), v1 = rsum(sc[c].y);
+      if (lane < 4) {
+        double *pc = part + ((size_t)blockIdx.x * PS + 2 * (m + 1) + c) * NB + 2 * kp;
+        pc[0] = v0;
+        pc[1] = v1;
+      }
     }
   }
 }
@@ -706,25 +719,27 @@ __global__ __launch_bounds__(64) void kb_arnoldi(const GSys *__restrict__ sys, i
 
 // per tile: v_j = (u_j - V s) / rho into Q_j, u_{j+1} = (W - V t - delta v_j) / (rho eta) into Q_{j+1}.
 // The basis vectors are taken in chunks of CCH: the block stages the chunk's step coefficients s, t of
-// its system in LDS (lc[i NB + k] = s_(c0+i), lc[(cn + i) NB + k] = t_(c0+i)); both rows of a thread
-// and four basis vectors per iteration (eight loads) are in flight together.
+// its system in LDS (lc[i NB + k] = s_(c0+i), lc[(cn + i) NB + k] = t_(c0+i)). A thread holds two
+// right-hand sides of one row (16-byte loads); eight basis vectors per iteration are in flight.
 __global__ __launch_bounds__(256) void kb_aupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int j,
                                                   int m) {
   constexpr int CCH = 128;
+  typedef double d2 __attribute__((ext_vector_type(2)));
   __shared__ double lc[2 * CCH * NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n;
-  const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
-  static_assert(TILE == 64, "two rows per thread");
-  const int r0 = tm.row0 + rr, r1 = tm.row0 + rr + 32;
-  const bool v0 = r0 < n, v1 = r1 < n;
-  const int q0 = v0 ? r0 : 0, q1 = v1 ? r1 : q0;  // in-range rows for the loads
-  const double *cw = S.cw + (size_t)k * cw_stride(m);
-  const double irho = cw[2 * m + 2], delta = cw[2 * m + 3], ieta = cw[2 * m + 4];
-  const size_t vs = (size_t)n * NB;
-  const double *Qa = S.Q + (size_t)q0 * NB + k, *Qb = S.Q + (size_t)q1 * NB + k;
-  double sa0 = 0.0, sa1 = 0.0, ta0 = 0.0, ta1 = 0.0, sb0 = 0.0, sb1 = 0.0, tb0 = 0.0, tb1 = 0.0;
+  const int kp = threadIdx.x & 3, rr = threadIdx.x >> 2;
+  static_assert(TILE == 64, "one row per thread");
+  const int r0 = tm.row0 + rr;
+  const bool v0 = r0 < n;
+  const int q0 = v0 ? r0 : tm.row0;  // an in-range row for the loads
+  const double *cwa = S.cw + (size_t)(2 * kp) * cw_stride(m), *cwb = cwa + cw_stride(m);
+  const d2 irho = {cwa[2 * m + 2], cwb[2 * m + 2]}, delta = {cwa[2 * m + 3], cwb[2 * m + 3]},
+           ieta = {cwa[2 * m + 4], cwb[2 * m + 4]};
+  const size_t vs = (size_t)n * (NB / 2);  // vector stride in d2
+  const d2 *Qa = (const d2 *)S.Q + (size_t)q0 * (NB / 2) + kp;
+  d2 sa0 = {0.0, 0.0}, sa1 = {0.0, 0.0}, ta0 = {0.0, 0.0}, ta1 = {0.0, 0.0};
   for (int c0 = 0; c0 < j; c0 += CCH) {  // j is uniform over the block: every thread meets the barriers
     const int cn = min(CCH, j - c0);
     __syncthreads();
@@ -733,50 +748,33 @@ __global__ __launch_bounds__(256) void kb_aupdate(const GSys *__restrict__ sys, 
       lc[q] = S.cw[(size_t)kk * cw_stride(m) + (ii < cn ? c0 + ii : (m + 1) + c0 + (ii - cn))];
     }
     __syncthreads();
-    const double *cs = lc + k, *ct = lc + (size_t)cn * NB + k;
-    const double *Pa = Qa + (size_t)c0 * vs, *Pb = Qb + (size_t)c0 * vs;
+    const d2 *cs = (const d2 *)lc + kp, *ct = (const d2 *)(lc + (size_t)cn * NB) + kp;
+    const d2 *Pa = Qa + (size_t)c0 * vs;
     int i = 0;
-    for (; i + 4 <= cn; i += 4) {
-      double qa[4], qb[4];
+    for (; i + 8 <= cn; i += 8) {
+      d2 qa[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        qa[u] = Pa[(size_t)(i + u) * vs];
-        qb[u] = Pb[(size_t)(i + u) * vs];
-      }
+      for (int u = 0; u < 8; ++u) qa[u] = Pa[(size_t)(i + u) * vs];
 #pragma unroll
-      for (int u = 0; u < 4; u += 2) {
-        const double s0 = cs[(i + u) * NB], s1 = cs[(i + u + 1) * NB];
-        const double t0 = ct[(i + u) * NB], t1 = ct[(i + u + 1) * NB];
-        sa0 += qa[u] * s0;
-        sa1 += qa[u + 1] * s1;
-        ta0 += qa[u] * t0;
-        ta1 += qa[u + 1] * t1;
-        sb0 += qb[u] * s0;
-        sb1 += qb[u + 1] * s1;
-        tb0 += qb[u] * t0;
-        tb1 += qb[u + 1] * t1;
+      for (int u = 0; u < 8; u += 2) {
+        sa0 += qa[u] * cs[(i + u) * (NB / 2)];
+        sa1 += qa[u + 1] * cs[(i + u + 1) * (NB / 2)];
+        ta0 += qa[u] * ct[(i + u) * (NB / 2)];
+        ta1 += qa[u + 1] * ct[(i + u + 1) * (NB / 2)];
       }
     }
     for (; i < cn; ++i) {
-      const double qa = Pa[(size_t)i * vs], qb = Pb[(size_t)i * vs];
-      sa0 += qa * cs[i * NB];
-      ta0 += qa * ct[i * NB];
-      sb0 += qb * cs[i * NB];
-      tb0 += qb * ct[i * NB];
+      const d2 qa = Pa[(size_t)i * vs];
+      sa0 += qa * cs[i * (NB / 2)];
+      ta0 += qa * ct[i * (NB / 2)];
     }
   }
-  double *U = S.Q + (size_t)j * vs, *Un = S.Q + (size_t)(j + 1) * vs;
+  d2 *U = (d2 *)(S.Q + (size_t)j * n * NB), *Un = (d2 *)(S.Q + (size_t)(j + 1) * n * NB);
   if (v0) {
-    const size_t o = (size_t)r0 * NB + k;
-    const double v = (U[o] - (sa0 + sa1)) * irho;
+    const size_t o = (size_t)r0 * (NB / 2) + kp;
+    const d2 v = (U[o] - (sa0 + sa1)) * irho;
     U[o] = v;
-    Un[o] = (S.W[o] - (ta0 + ta1) - delta * v) * irho * ieta;
-  }
-  if (v1) {
-    const size_t o = (size_t)r1 * NB + k;
-    const double v = (U[o] - (sb0 + sb1)) * irho;
-    U[o] = v;
-    Un[o] = (S.W[o] - (tb0 + tb1) - delta * v) * irho * ieta;
+    Un[o] = (((const d2 *)S.W)[o] - (ta0 + ta1) - delta * v) * irho * ieta;
   }
 }
 

@@ -176,6 +176,9 @@ def main():
     ap.add_argument("--plain-only", action="store_true",
                     help="only the plain FP64 path on the sample, one body at a time with per-kernel timing "
                          "(the workload of the plain path's kernel traces and counters)")
+    ap.add_argument("--sphere20480", action="store_true",
+                    help="only the single-system configuration: the plain FP64 path (assembly and LSQR of a "
+                         "61440^2 system) on the 20480-triangle unit sphere, with its kernel roofline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches): the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -218,6 +221,32 @@ def main():
         print(json.dumps({"domain_values": {"points": args.domain_only, "triangles": m.nelem,
                                             "points_per_s": round(args.domain_only / el, 1),
                                             "roofline": roofline, "kernels": kernels}}), flush=True)
+        return
+
+    if args.sphere20480:
+        from andromeda_amd import mesh as M
+        m = M.icosphere(5)
+        ctx = api.Context(0)
+        for w in range(args.warmup):
+            ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            r = ctx.resistance(m, precision=api.PREC_FP64, eps=1e-12)
+        el = (time.perf_counter() - t0) / args.steps
+        kernels, roofline = kernel_table(ctx.profile_read())
+        ctx.profile(False)
+        K, Om = np.diag(r.FT[:3, :3]), np.diag(r.FT[3:, 3:])
+        drag = float(np.abs(K / (6 * np.pi) - 1).max())
+        log(f"[bench] 20480-triangle sphere, plain FP64: {el:.3f} s per tensor (assembly {r.t_assemble:.3f} s, "
+            f"LSQR {r.t_solve:.3f} s, its {r.nit.tolist()}); drag vs 6 pi: {drag:.2e}; {kernels}")
+        print(json.dumps({"sphere_20480_plain_fp64": {
+            "triangles": m.nelem, "unknowns": 3 * m.nelem, "system_bytes": 8 * (3 * m.nelem) ** 2,
+            "tensors_per_s": round(1.0 / el, 4), "s_per_tensor": round(el, 3),
+            "assembly_s": round(r.t_assemble, 3), "lsqr_s": round(r.t_solve, 3), "lsqr_its": r.nit.tolist(),
+            "status": int(r.status), "drag_rel_err_vs_6pi": float(f"{drag:.3e}"),
+            "torque_rel_err_vs_8pi": float(f"{np.abs(Om / (8 * np.pi) - 1).max():.3e}"),
+            "roofline": roofline, "kernels": kernels}}), flush=True)
         return
 
     n_fibre = args.bodies // 9
@@ -280,6 +309,23 @@ def main():
     value = args.steps * len(bodies) / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] fast form ({prec_name}): {value:.2f} tensors/s, {ms_per_step:.0f} ms/step")
+    # ablation on the same batch, one step each after a warm call: the fixed-order FP32 rule instead
+    # of the learned rule order (mixed form), and the learned form with FP64 Krylov products
+    ablation = {"learned_step_ms": round(ms_per_step, 1)}
+    if prec == api.PREC_LEARNED:
+        ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
+        ta = time.perf_counter()
+        ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
+        ablation["mixed_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
+    os.environ["ANDR_GMRES_FP64_PRODUCTS"] = "1"
+    try:
+        ctx.resistance_batch_resident(resident, precision=prec)
+        ta = time.perf_counter()
+        ctx.resistance_batch_resident(resident, precision=prec)
+        ablation[f"{prec_name}_fp64_krylov_products_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
+    finally:
+        os.environ.pop("ANDR_GMRES_FP64_PRODUCTS", None)
+    log(f"[bench] ablation: {ablation}")
     # the same step from host meshes (upload over PCIe inside the call), reported beside value
     th = time.perf_counter()
     ctx.resistance_batch(bodies, precision=prec)
@@ -350,7 +396,8 @@ def main():
         "dtype": "f32+f64",
         "precision": {"quadrature": "f32" if prec_name == "mixed" else "f32, learned rule order",
                       "singular_integrals": "f64", "matrix_storage": "f32",
-                      "krylov_product_and_solve": "f64", "refinement_residual": "f64"},
+                      "krylov_product": "f32 matrix cores, f64 accumulation of 16-column blocks",
+                      "krylov_basis_and_solve": "f64", "refinement_residual": "f64"},
         "data": "synthetic (seeded superellipsoids and fibres)",
         "config": {"workload": f"{len(bodies)} bodies ({n_super} superellipsoids + {n_fibre} fibres), "
                                f"F log-uniform [500, 8000], 6 rigid-body RHS, exterior Stokes",
@@ -361,6 +408,11 @@ def main():
                        "fast_tensors_per_s_same_sample": round(len(sample) / t_fast, 3),
                        "speedup_same_sample": round(t_plain / t_fast, 2),
                        "roofline": plain_roofline, "kernels": plain_kernels},
+        "ablation": dict(ablation, plain_fp64_lsqr_tensors_per_s_on_sample=round(plain_rate, 3),
+                         fast_tensors_per_s_on_sample=round(len(sample) / t_fast, 3),
+                         note="steps of the whole batch: learned rule order (value), fixed-order FP32 rule "
+                              "(mixed), FP64 instead of FP32 Krylov products; the plain FP64 LSQR path and the "
+                              "fast form on the stratified sample"),
         "host_meshes": {"tensors_per_s": round(host_rate, 3),
                         "note": "one step from host meshes, mesh upload over PCIe inside the call"},
         "accuracy": {"max_tensor_err_vs_fp64": float(f"{max(errs):.3e}"), "bodies": len(sample),
