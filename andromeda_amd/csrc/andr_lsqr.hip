@@ -106,27 +106,41 @@ __global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A,
                                                    double *__restrict__ out) {
   typedef const double __attribute__((address_space(1))) *gdp;
   typedef double d2n __attribute__((ext_vector_type(2)));
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // VEC: the tiles are read in whole 128-byte row segments (lane (rr = lane / 8, p = lane % 8) takes
+  // columns 2 p, 2 p + 1 of row rr of an 8-row half tile: a load covers 8 full lines, where the
+  // matrix-core lane pattern covers 16 half lines, measured 6.9 against 4.6-5.3 TB/s on a 30 GB
+  // matrix) and redistributed to the matrix-core layout through LDS (16 x 16 tile per T, rows padded
+  // to 18 doubles)
+  constexpr int LR = 18;
+  __shared__ __attribute__((aligned(16))) double lt[VEC ? 4 : 1][VEC ? GN_T : 1][16 * LR];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wid;
   const int row0 = wave * 16 * GN_T;
-  if (row0 >= m) return;
+  if (row0 >= m) return;  // uniform over the wave; the LDS tiles are per wave
   const int g = lane >> 4, c = lane & 15;
+  const int rr = lane >> 3, p = lane & 7;
   const bool hi = c >= NB;
   const int k0 = blockIdx.y * seglen, kend = min(n, k0 + seglen);
-  gdp arow[GN_T];
+  gdp arow[GN_T];   // matrix-core pattern: row 16 T + c, columns 4 g ..
+  gdp lrow[GN_T][2];  // row-segment pattern: rows 16 T + rr and 16 T + 8 + rr, columns 2 p ..
 #pragma unroll
-  for (int T = 0; T < GN_T; ++T) arow[T] = (gdp)(A + (size_t)min(row0 + 16 * T + c, m - 1) * lda + 4 * g);
+  for (int T = 0; T < GN_T; ++T) {
+    arow[T] = (gdp)(A + (size_t)min(row0 + 16 * T + c, m - 1) * lda + 4 * g);
+    lrow[T][0] = (gdp)(A + (size_t)min(row0 + 16 * T + rr, m - 1) * lda + 2 * p);
+    lrow[T][1] = (gdp)(A + (size_t)min(row0 + 16 * T + 8 + rr, m - 1) * lda + 2 * p);
+  }
   const gdp xb = (gdp)(X + (size_t)(4 * g) * NB + (c & (NB - 1)));
   d4v acc[GN_T];
 #pragma unroll
   for (int T = 0; T < GN_T; ++T) acc[T] = d4v{0.0, 0.0, 0.0, 0.0};
+  // raw operands of one 16-column block: VEC: two row segments per T; otherwise the matrix-core
+  // values themselves
   auto load = [&](int kb, double (&a)[GN_T][4], double (&x)[4]) {
 #pragma unroll
     for (int T = 0; T < GN_T; ++T) {
       if (VEC) {  // the matrix is streamed once: non-temporal, so that X keeps its place in L2
-        const d2n u = __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(arow[T] + 16 * kb));
-        const d2n w =
-            __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(arow[T] + 16 * kb + 2));
+        const d2n u = __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(lrow[T][0] + 16 * kb));
+        const d2n w = __builtin_nontemporal_load((const d2n __attribute__((address_space(1))) *)(lrow[T][1] + 16 * kb));
         a[T][0] = u.x;
         a[T][1] = u.y;
         a[T][2] = w.x;
@@ -138,6 +152,29 @@ __global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A,
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) x[t] = xb[(size_t)(16 * kb + t) * NB];
+  };
+  // VEC: row segments -> LDS -> matrix-core values (in place)
+  auto stage = [&](double (&a)[GN_T][4]) {
+    if (VEC) {
+#pragma unroll
+      for (int T = 0; T < GN_T; ++T) {
+        *(d2n *)&lt[wid][T][rr * LR + 2 * p] = d2n{a[T][0], a[T][1]};
+        *(d2n *)&lt[wid][T][(8 + rr) * LR + 2 * p] = d2n{a[T][2], a[T][3]};
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int T = 0; T < GN_T; ++T) {
+        const d2n u = *(const d2n *)&lt[wid][T][c * LR + 4 * g];
+        const d2n w = *(const d2n *)&lt[wid][T][c * LR + 4 * g + 2];
+        a[T][0] = u.x;
+        a[T][1] = u.y;
+        a[T][2] = w.x;
+        a[T][3] = w.y;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
   };
   auto step = [&](const double (&a)[GN_T][4], const double (&x)[4]) {
 #pragma unroll
@@ -157,14 +194,17 @@ __global__ __launch_bounds__(256) void k_gemv_n_mf(const double *__restrict__ A,
     for (; kb + 1 < kb_end; kb += 2) {
       load(kb + 1, a1, x1);
       __builtin_amdgcn_sched_barrier(0);
+      stage(a0);
       step(a0, x0);
       __builtin_amdgcn_sched_barrier(0);
       load(min(kb + 2, kb_end - 1), a0, x0);
       __builtin_amdgcn_sched_barrier(0);
+      stage(a1);
       step(a1, x1);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (kb < kb_end) {  // odd count: a0 holds block kb
+      stage(a0);
       step(a0, x0);
       ++kb;
     }
