@@ -308,6 +308,47 @@ __global__ __launch_bounds__(256) void k_gemv_t(const ST *__restrict__ A, int64_
   }
 }
 
+// The same for FP64 A with even lda on a 16-byte boundary: two columns per thread, so that a load
+// covers 1 KB of a row (64 lanes x 16 B) instead of 512 B.
+__global__ __launch_bounds__(256) void k_gemv_t2(const double *__restrict__ A, int64_t lda, int m, int n,
+                                                 const double *__restrict__ Y, int rows_per_split,
+                                                 double *__restrict__ P) {
+  typedef double d2n __attribute__((ext_vector_type(2)));
+  typedef const d2n __attribute__((address_space(1))) *gd2p;
+  const int j = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int s = blockIdx.y;
+  const int i0 = s * rows_per_split;
+  const int i1 = min(m, i0 + rows_per_split);
+  if (j >= n) return;
+  d2n acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = d2n{0.0, 0.0};
+  // columns j, j + 1 (the second may be the row padding when n is odd: read, never stored)
+  const gd2p col = (gd2p)(A + j);
+  int i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    d2n a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = __builtin_nontemporal_load(col + (size_t)(i + u) * (lda / 2));
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += a[u] * Y[(size_t)(i + u) * NB + k];
+  }
+  for (; i < i1; ++i) {
+    const d2n a = __builtin_nontemporal_load(col + (size_t)i * (lda / 2));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += a * Y[(size_t)i * NB + k];
+  }
+  double4 *p = reinterpret_cast<double4 *>(P + ((size_t)s * n + j) * NB);
+  p[0] = make_double4(acc[0].x, acc[1].x, acc[2].x, acc[3].x);
+  p[1] = make_double4(acc[4].x, acc[5].x, acc[6].x, acc[7].x);
+  if (j + 1 < n) {
+    p[2] = make_double4(acc[0].y, acc[1].y, acc[2].y, acc[3].y);
+    p[3] = make_double4(acc[4].y, acc[5].y, acc[6].y, acc[7].y);
+  }
+}
+
 // V = sx .* sum_s P[s] + sy .* V; partV[block][k] = sum V^2, partW[block][k] = sum W^2 (old W).
 __global__ __launch_bounds__(256) void k_combine_t(int n, int nsplit, const double *__restrict__ P,
                                                    double *__restrict__ V, const double *__restrict__ W,
@@ -746,6 +787,9 @@ struct LsqrRun {
     dim3 g((n + 255) / 256, ns);
     if (fp32)
       hipLaunchKernelGGL(k_gemv_t<float>, g, dim3(256), 0, st, (const float *)A, lda, m, n, Y, rps, P.p);
+    else if (vecN && (n % 2 == 0 || n < lda))  // column pairs: an odd last column reads the row padding
+      hipLaunchKernelGGL(k_gemv_t2, dim3((n + 511) / 512, ns), dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps,
+                         P.p);
     else
       hipLaunchKernelGGL(k_gemv_t<double>, g, dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps, P.p);
     andr_prof_end(ctx, t);
