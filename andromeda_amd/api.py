@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH):
                                                 C.c_void_p, C.c_double, C.c_int32, C.POINTER(_LsqrStats)]
     lib.andr_autotune.argtypes = [C.c_void_p, C.c_int32]
     lib.andr_autotune_read.argtypes = [C.c_void_p, C.c_int32, _I, _I, _I, _D]
+    lib.andr_autotune_read_krylov.argtypes = [C.c_void_p, C.c_int32, _I, _I, _D]
     _lib = lib
     return lib
 
@@ -145,7 +146,7 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_assemble_stokes", "lsqr_precon_", "lsqr_solve_", "andr_lsqr_forget", "andr_lsqr_dense",
             "andr_resistance", "andr_resistance_batch", "andr_quad_training_pairs", "andr_mlp_train",
             "andr_mlp_infer", "andr_mlp_loss_grad", "andr_set_quadrature_model", "andr_profile",
-            "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read",
+            "andr_profile_read", "andr_quad_far_calibration", "andr_autotune", "andr_autotune_read", "andr_autotune_read_krylov",
             "andr_bodies_upload", "andr_bodies_count", "andr_resistance_batch_resident", "andr_bodies_free",
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
@@ -615,6 +616,17 @@ class Context:
         lr, lc, rpw, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         while self.lib.andr_autotune_read(self.h, i, C.byref(lr), C.byref(lc), C.byref(rpw), C.byref(ms)) == 1:
             out.append({"log2_rows": lr.value, "log2_cols": lc.value, "rows_per_wave": rpw.value, "ms": ms.value})
+            i += 1
+        return out
+
+    def autotune_table_krylov(self) -> list:
+        """Autotuner choices of the fast forms' Krylov product: the waves its column segments aim at,
+        per bucket of live 32-row waves."""
+        out = []
+        i = 0
+        lw, tw, ms = C.c_int32(), C.c_int32(), C.c_double()
+        while self.lib.andr_autotune_read_krylov(self.h, i, C.byref(lw), C.byref(tw), C.byref(ms)) == 1:
+            out.append({"log2_row_waves": lw.value, "target_waves": tw.value, "ms": ms.value})
             i += 1
         return out
 

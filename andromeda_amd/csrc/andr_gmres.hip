@@ -944,7 +944,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // TILE rows (vector kernels) and matrix-core waves (64 rows x a column segment). When the list
   // has few rows (the last, slowest systems of a batch) the columns are split into segments so
   // that the product still fills the chip; the segments are summed by kb_seg_sum.
-  constexpr int MF_TARGET_WAVES = 2048, MF_MAX_SEG = 8;
+  constexpr int MF_TARGET_WAVES = 2048, MF_MAX_SEG = 16;
   // structured operator: w = residual blocks of 64 element rows (4 partial entries each), mw =
   // matrix-core waves of 32 element rows, et = element tiles of 32 (z = M^-1 v and wall sums)
   struct MapSet {
@@ -966,7 +966,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   std::vector<GSys> hs(nsys);
   DevBuf<double> Pseg;
   size_t pseg_cap = 0;
-  auto build = [&](const std::vector<int> &list, MapSet &ms) -> int {
+  auto build = [&](const std::vector<int> &list, MapSet &ms, int target_waves = MF_TARGET_WAVES) -> int {
     std::vector<WaveMap> wm;
     std::vector<TileMap> tm, em;
     std::vector<MWave> mw;
@@ -981,7 +981,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ms.max_seg = 1;
     int row_waves = 0;
     for (int s : list) row_waves += (in[s].op.ne + 31) / 32;
-    const int want = row_waves > 0 ? std::min(MF_MAX_SEG, (MF_TARGET_WAVES + row_waves - 1) / row_waves) : 1;
+    const int want = row_waves > 0 ? std::min(MF_MAX_SEG, (target_waves + row_waves - 1) / row_waves) : 1;
     for (int s : list) {
       const int n = in[s].n, ne = in[s].op.ne;
       w0[s] = 4 * (int)wm.size();
@@ -1124,7 +1124,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
 
   const int gs = (nsys * NB + 63) / 64;
   // which_x 0: residual R = B - A X (R holds B); 1: Krylov product W = A Q_j
-  auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales) {
+  auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales, bool prof = true) {
     if (ms.n_w == 0) return;
     double n2 = 0.0, n1 = 0.0;
     for (int s = 0; s < nsys; ++s)
@@ -1134,8 +1134,9 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         n1 += in[s].n;
       }
     const bool krylov = which_x == 1;
-    const int tok = andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
-                                    2.0 * 1.5 * n2 * nrhs);
+    const int tok = prof ? andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
+                                           2.0 * 1.5 * n2 * nrhs)
+                         : -1;
     hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
     hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
     if (krylov) {
@@ -1148,6 +1149,48 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
     }
     andr_prof_end(ctx, tok);
+  };
+  // Autotuner of the product's column segments: when few rows are live (the slowest bodies of a
+  // batch) the columns are split so that the product still fills the chip; the split aims at a
+  // number of waves. The first time a size bucket (ceil log2 of the live 32-row waves) occurs, the
+  // candidates are timed on the live systems' current Krylov vector and the fastest is kept in the
+  // context. The split changes the summation order of the product (segments are summed apart).
+  auto krylov_target = [&](const std::vector<int> &list, int jq) -> int {
+    int row_waves = 0;
+    for (int s : list) row_waves += (in[s].op.ne + 31) / 32;
+    if (!ctx->autotune || row_waves <= 0 || row_waves >= 4096) return MF_TARGET_WAVES;
+    int key = 0;
+    while ((1 << key) < row_waves) ++key;
+    auto it = ctx->tune_krylov.find(key);
+    if (it != ctx->tune_krylov.end()) return it->second.first;
+    hipEvent_t e0, e1;
+    ANDR_HIP(ctx, hipEventCreate(&e0));
+    ANDR_HIP(ctx, hipEventCreate(&e1));
+    int best = MF_TARGET_WAVES;
+    float best_ms = 1e30f;
+    for (int target : {1024, 2048, 4096, 8192}) {
+      if (build(list, live, target) || use_tiles(live)) break;
+      gemv(live, 1, jq, d_arn.p, false);  // warm-up
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        (void)hipEventRecord(e0, st);
+        gemv(live, 1, jq, d_arn.p, false);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms = std::min(ms, t);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = target;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ANDR_HIP(ctx, hipGetLastError());
+    ctx->tune_krylov[key] = {best, best_ms};
+    return best;
   };
   std::vector<int> it_count((size_t)nsys * NB, 0);
   std::vector<double> hsmall(state_per * nsys);
@@ -1239,7 +1282,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         int nlive = 0;
         for (int s = 0; s < nsys; ++s) nlive += live.ntiles[s] ? 1 : 0;
         if ((int)still.size() < nlive) {
-          if ((rc = build(still, live))) return rc;
+          const int target = krylov_target(still, j + 1);
+          if ((rc = build(still, live, target))) return rc;
           if ((rc = use_tiles(live))) return rc;
         }
       }

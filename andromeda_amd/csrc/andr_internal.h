@@ -47,6 +47,9 @@ struct andr_ctx {
   bool autotune = true;
   int gemv_n_tiles = 4;  // with the autotuner off
   std::map<std::pair<int, int>, std::pair<int, float>> tune_gemv_n;
+  // autotuner of the fast form's Krylov product: ceil log2 of the live 32-row waves -> (waves the
+  // column segments aim for, best time in ms); tuned when few rows are live (the segments matter)
+  std::map<int, std::pair<int, float>> tune_krylov;
 };
 
 // status helpers ------------------------------------------------------------------------------

@@ -110,3 +110,14 @@ extern "C" int andr_autotune_read(andr_ctx *ctx, int32_t idx, int32_t *log2_rows
   if (ms) *ms = it->second.second;
   return 1;
 }
+
+extern "C" int andr_autotune_read_krylov(andr_ctx *ctx, int32_t idx, int32_t *log2_row_waves, int32_t *target_waves,
+                                         double *ms) {
+  if (!ctx || idx < 0 || idx >= (int)ctx->tune_krylov.size()) return 0;
+  auto it = ctx->tune_krylov.begin();
+  std::advance(it, idx);
+  if (log2_row_waves) *log2_row_waves = it->first;
+  if (target_waves) *target_waves = it->second.first;
+  if (ms) *ms = it->second.second;
+  return 1;
+}

@@ -170,6 +170,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "learned", "auto"], default="auto")
     ap.add_argument("--model", default=os.path.join(ROOT, "andromeda_amd", "quad_model.npz"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the ablation steps (profiled runs)")
     ap.add_argument("--domain-only", type=int, default=0, metavar="NPTS",
                     help="only the domain evaluation: velocity and pressure at NPTS points inside a "
                          "20480-triangle sphere (the kernel's trace and roofline)")
@@ -312,17 +313,21 @@ def main():
     # ablation on the same batch, one step each after a warm call: the fixed-order FP32 rule instead
     # of the learned rule order (mixed form), and the learned form with FP64 Krylov products
     ablation = {"learned_step_ms": round(ms_per_step, 1)}
-    if prec == api.PREC_LEARNED:
+    if prec == api.PREC_LEARNED and not args.no_ablation:
         ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
         ta = time.perf_counter()
         ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
         ablation["mixed_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
     os.environ["ANDR_GMRES_FP64_PRODUCTS"] = "1"
     try:
+        if args.no_ablation:
+            raise StopIteration
         ctx.resistance_batch_resident(resident, precision=prec)
         ta = time.perf_counter()
         ctx.resistance_batch_resident(resident, precision=prec)
         ablation[f"{prec_name}_fp64_krylov_products_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
+    except StopIteration:
+        pass
     finally:
         os.environ.pop("ANDR_GMRES_FP64_PRODUCTS", None)
     log(f"[bench] ablation: {ablation}")
@@ -422,6 +427,8 @@ def main():
                      "within_1e-6": bool(accuracy_ok)},
         "roofline": roofline,
         "kernels": kernels,
+        "autotuner": {"krylov_product_segments": ctx.autotune_table_krylov(),
+                      "plain_gemv_n_rows_per_wave": ctx.autotune_table()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

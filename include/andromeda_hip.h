@@ -379,6 +379,14 @@ int andr_profile_read(andr_ctx *ctx, int32_t idx, char *name, int32_t namelen, i
 int andr_autotune(andr_ctx *ctx, int32_t mode);
 int andr_autotune_read(andr_ctx *ctx, int32_t idx, int32_t *log2_rows, int32_t *log2_cols, int32_t *rows_per_wave,
                        double *ms);
+/* The fast forms' Krylov product splits the columns of the live systems into segments when few
+ * rows are live (the slowest bodies of a batch), aiming at a number of waves (1024, 2048, 4096 or
+ * 8192). With the autotuner on (mode 1), the first time a bucket of live rows (ceil log2 of the live
+ * 32-row waves, below 4096 waves) occurs the candidates are timed on the live systems and the
+ * fastest is kept in the context; otherwise 2048. andr_autotune_read_krylov returns 1 and fills
+ * entry idx (0..) of that table, 0 past the end. */
+int andr_autotune_read_krylov(andr_ctx *ctx, int32_t idx, int32_t *log2_row_waves, int32_t *target_waves,
+                              double *ms);
 
 #ifdef __cplusplus
 }

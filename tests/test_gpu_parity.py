@@ -177,6 +177,28 @@ def test_krylov_product_precisions(ctx, monkeypatch):
         assert t.nit.max() > b.nit.max()
 
 
+def test_krylov_product_autotuner(ctx):
+    """A slow fibre next to a fast body: once the fast body is done the fibre alone is live, its
+    columns are split into segments, and the autotuner times the candidate splits for that bucket
+    and keeps the fastest. With the tuner off (fixed split) the tensors agree to rounding."""
+    allb = [m for m, _ in M.batch_bodies(512, 64)]
+    bodies = [allb[523], allb[29]]
+    ctx.autotune(True)
+    tuned = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    table = ctx.autotune_table_krylov()
+    print("krylov autotuner:", table)
+    assert table and all(e["target_waves"] in (1024, 2048, 4096, 8192) and e["ms"] > 0 for e in table)
+    ctx.autotune(False)
+    try:
+        fixed = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+    finally:
+        ctx.autotune(True)
+    for a, b in zip(tuned, fixed):
+        assert a.status == 0 and b.status == 0
+        assert tensor_error(a.FT, b.FT) < 1e-9
+        assert abs(int(a.nit.max()) - int(b.nit.max())) <= 2
+
+
 def test_slender_body_long_restarts(ctx):
     """A 496-element fibre of the batched workload whose hardest column needs several hundred
     Krylov steps with a first cycle of 128: the cycles after the first run with a longer restart in
