@@ -318,18 +318,15 @@ def main():
         ta = time.perf_counter()
         ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
         ablation["mixed_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
-    os.environ["ANDR_GMRES_FP64_PRODUCTS"] = "1"
-    try:
-        if args.no_ablation:
-            raise StopIteration
-        ctx.resistance_batch_resident(resident, precision=prec)
-        ta = time.perf_counter()
-        ctx.resistance_batch_resident(resident, precision=prec)
-        ablation[f"{prec_name}_fp64_krylov_products_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
-    except StopIteration:
-        pass
-    finally:
-        os.environ.pop("ANDR_GMRES_FP64_PRODUCTS", None)
+    if not args.no_ablation:
+        os.environ["ANDR_GMRES_FP64_PRODUCTS"] = "1"
+        try:
+            ctx.resistance_batch_resident(resident, precision=prec)
+            ta = time.perf_counter()
+            ctx.resistance_batch_resident(resident, precision=prec)
+            ablation[f"{prec_name}_fp64_krylov_products_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
+        finally:
+            os.environ.pop("ANDR_GMRES_FP64_PRODUCTS", None)
     log(f"[bench] ablation: {ablation}")
     # the same step from host meshes (upload over PCIe inside the call), reported beside value
     th = time.perf_counter()
