@@ -102,6 +102,21 @@ def test_tensor_error_measure():
     assert abs(tensor_error(X, R) - 1e-6) < 1e-12
 
 
+def test_entry_error_measure():
+    """Entry-wise relative error over the entries above 1e-3 of their block's norm: small entries
+    (below the floor) are left out, the coupling blocks are floored on sqrt(|K| |Omega|)."""
+    from andromeda_amd.metrics import entry_error
+    R = np.diag([6.0, 6.0, 6.0, 8.0, 8.0, 8.0])
+    R[0, 1] = R[1, 0] = 1e-4   # below 1e-3 |K|: not judged
+    R[0, 4] = 0.5              # coupling entry above 1e-3 sqrt(|K| |Omega|)
+    X = R.copy()
+    X[0, 1] *= 2.0
+    assert entry_error(X, R) == 0.0
+    X[0, 4] = 0.5 * (1 + 3e-6)
+    X[3, 3] = 8.0 * (1 - 1e-6)
+    assert abs(entry_error(X, R) - 3e-6) < 1e-15
+
+
 def test_mesh_validate_rejects_bad_inputs():
     """andr_mesh_validate (host only): every check an entry point makes before it reads a mesh --
     wall ids outside 1..nwalls, connectivity out of range, bad rule indices -- is refused with
