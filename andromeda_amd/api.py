@@ -696,6 +696,11 @@ def lsqr_precon(m: int, n: int, ia, ja, a):
     return q
 
 
+def lsqr_forget():
+    """Releases the seam's cached device copy of the last matrix (andr_lsqr_forget)."""
+    load_library().andr_lsqr_forget()
+
+
 def lsqr_solve(m: int, n: int, ia, ja, a, q, b, x, maxit: int = 0, eps: float = 1e-15):
     lib = load_library()
     mm, nn_, nnz = C.c_int32(m), C.c_int32(n), C.c_int64(a.size)

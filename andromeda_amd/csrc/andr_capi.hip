@@ -305,7 +305,22 @@ struct LsqrCache {
   int64_t nnz = 0;
   int m = 0, n = 0;
   uint64_t sum = 0;
-  DevBuf<double> D, q;  // D: dense m x n, leading dimension dense_ld(n)
+  bool sparse = false;
+  DevBuf<double> D;  // dense m x n, leading dimension dense_ld(n)
+  // sparse: A and A^T in CSR with 0-based row pointers and columns
+  DevBuf<int64_t> cia, tia;
+  DevBuf<int32_t> cja, tja;
+  DevBuf<double> cva, tva;
+  void release() {
+    D.release();
+    cia.release();
+    tia.release();
+    cja.release();
+    tja.release();
+    cva.release();
+    tva.release();
+    key_a = nullptr;
+  }
 };
 std::mutex g_mu;
 andr_ctx *g_ctx = nullptr;
@@ -360,34 +375,84 @@ void cache_key(LsqrCache &c, int m, int n, int64_t nnz, const int64_t *ia, const
   c.sum = csr_checksum(m, nnz, ia, ja, a);
 }
 bool cache_hit(const LsqrCache &c, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a) {
-  return c.key_a == a && c.key_ia == ia && c.key_ja == ja && c.nnz == nnz && c.m == m && c.n == n && c.D.p &&
-         c.sum == csr_checksum(m, nnz, ia, ja, a);
+  return c.key_a == a && c.key_ia == ia && c.key_ja == ja && c.nnz == nnz && c.m == m && c.n == n &&
+         (c.sparse ? c.cva.p != nullptr : c.D.p != nullptr) && c.sum == csr_checksum(m, nnz, ia, ja, a);
 }
+// A CSR whose entries fill less than a third of the m x n matrix stays sparse on the device (A and its
+// transpose, 12 B per entry each); a denser one (the reference's systems are dense matrices stored
+// in CSR) is expanded to a dense copy for the matrix-core products.
+bool sparse_csr(int m, int n, int64_t nnz) { return 3.0 * (double)nnz < (double)m * (double)n; }
 // leading dimension of the dense device copy: rows start on 128-byte lines
 int64_t dense_ld(int n) { return ((int64_t)n + 15) / 16 * 16; }
 
-int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
-               DevBuf<double> &D) {
-  // the device scatter trusts these indices: check the 1-based CSR on the host first
+// checks the 1-based CSR on the host (the device kernels trust its indices); colsq (n entries, may be
+// null) receives the sum of the squared stored entries of every column in row order, duplicates
+// squared apart (lsqr_tmatvec3, src/lib_lsqr.f:267-299)
+int check_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
+              double *colsq) {
   if (m <= 0 || n <= 0 || nnz < 0 || ia[0] != 1 || ia[m] != nnz + 1)
     return andr_fail(ctx, ANDR_ERR_ARG, "CSR row pointers inconsistent (m=%d nnz=%lld)", m, (long long)nnz);
   for (int i = 0; i < m; ++i)
     if (ia[i + 1] < ia[i]) return andr_fail(ctx, ANDR_ERR_ARG, "CSR row pointer decreases at row %d", i + 1);
-  for (int64_t p = 0; p < nnz; ++p)
+  if (colsq)
+    for (int j = 0; j < n; ++j) colsq[j] = 0.0;
+  for (int64_t p = 0; p < nnz; ++p) {
     if (ja[p] < 1 || ja[p] > n) return andr_fail(ctx, ANDR_ERR_ARG, "CSR column %d out of range", ja[p]);
+    if (colsq) colsq[ja[p] - 1] += a[p] * a[p];
+  }
+  return ANDR_OK;
+}
+
+// the device copy of a checked CSR: dense, or sparse with its transpose (sparse_csr)
+int upload_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja, const double *a,
+               LsqrCache &c) {
+  c.release();
+  c.sparse = sparse_csr(m, n, nnz);
+  hipStream_t st = ctx->stream;
+  if (c.sparse) {
+    std::vector<int64_t> hia(m + 1), tia(n + 1, 0);
+    std::vector<int32_t> hja(nnz), tja(nnz);
+    std::vector<double> tva(nnz);
+    for (int i = 0; i <= m; ++i) hia[i] = ia[i] - 1;
+    for (int64_t p = 0; p < nnz; ++p) {
+      hja[p] = ja[p] - 1;
+      ++tia[ja[p]];
+    }
+    for (int j = 0; j < n; ++j) tia[j + 1] += tia[j];
+    std::vector<int64_t> fill(tia.begin(), tia.end() - 1);
+    for (int i = 0; i < m; ++i)  // row order within every column: the products' sums are fixed
+      for (int64_t p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+        const int64_t o = fill[ja[p] - 1]++;
+        tja[o] = i;
+        tva[o] = a[p];
+      }
+    if (c.cia.alloc(m + 1) || c.cja.alloc(std::max<int64_t>(nnz, 1)) || c.cva.alloc(std::max<int64_t>(nnz, 1)) ||
+        c.tia.alloc(n + 1) || c.tja.alloc(std::max<int64_t>(nnz, 1)) || c.tva.alloc(std::max<int64_t>(nnz, 1)))
+      return andr_fail(ctx, ANDR_ERR_ALLOC, "sparse LSQR matrix %d x %d (%lld entries)", m, n, (long long)nnz);
+    ANDR_HIP(ctx, hipMemcpyAsync(c.cia.p, hia.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, st));
+    ANDR_HIP(ctx, hipMemcpyAsync(c.tia.p, tia.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (nnz > 0) {
+      ANDR_HIP(ctx, hipMemcpyAsync(c.cja.p, hja.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+      ANDR_HIP(ctx, hipMemcpyAsync(c.cva.p, a, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+      ANDR_HIP(ctx, hipMemcpyAsync(c.tja.p, tja.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+      ANDR_HIP(ctx, hipMemcpyAsync(c.tva.p, tva.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    }
+    ANDR_HIP(ctx, hipStreamSynchronize(st));  // the host vectors go out of scope
+    return ANDR_OK;
+  }
   DevBuf<int64_t> dia;
   DevBuf<int32_t> dja;
   DevBuf<double> da;
   const int64_t ld = dense_ld(n);
-  if (D.alloc((size_t)m * ld) || dia.alloc(m + 1) || dja.alloc(nnz) || da.alloc(nnz))
+  if (c.D.alloc((size_t)m * ld) || dia.alloc(m + 1) || dja.alloc(nnz) || da.alloc(nnz))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "LSQR matrix %d x %d", m, n);
-  ANDR_HIP(ctx, hipMemsetAsync(D.p, 0, sizeof(double) * (size_t)m * ld, ctx->stream));
-  ANDR_HIP(ctx, hipMemcpyAsync(dia.p, ia, sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, ctx->stream));
-  ANDR_HIP(ctx, hipMemcpyAsync(dja.p, ja, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
-  ANDR_HIP(ctx, hipMemcpyAsync(da.p, a, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, ctx->stream, m, ld, dia.p, dja.p, da.p, D.p);
+  ANDR_HIP(ctx, hipMemsetAsync(c.D.p, 0, sizeof(double) * (size_t)m * ld, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dia.p, ia, sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(dja.p, ja, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+  ANDR_HIP(ctx, hipMemcpyAsync(da.p, a, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, st, m, ld, dia.p, dja.p, da.p, c.D.p);
   ANDR_HIP(ctx, hipGetLastError());
-  ANDR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ANDR_HIP(ctx, hipStreamSynchronize(st));
   return ANDR_OK;
 }
 }  // namespace
@@ -403,9 +468,7 @@ andr_ctx *andr_default_ctx() {
 
 extern "C" void andr_lsqr_forget(void) {
   std::lock_guard<std::mutex> lk(g_mu);
-  g_cache.D.release();
-  g_cache.q.release();
-  g_cache.key_a = nullptr;
+  g_cache.release();
   g_precon_failed = false;
 }
 
@@ -426,14 +489,15 @@ extern "C" void lsqr_precon_(const int32_t *m, const int32_t *n, const int64_t *
   andr_ctx *ctx = andr_default_ctx();
   if (!ctx) return fail("no device");
   (void)hipSetDevice(ctx->device);
-  if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D) || c.q.alloc(*n) ||
-      andr_colscale(ctx, c.D.p, false, dense_ld(*n), *m, *n, c.q.p))
-    return fail(ctx->err.c_str());
-  if (hipMemcpy(q, c.q.p, sizeof(double) * (*n), hipMemcpyDeviceToHost) != hipSuccess)
-    return fail("copy of the column scales");
-  // A <- A diag(q) on the host array too (lsqr_matvec4, src/lib_lsqr.f:301-329)
+  // q = 1 / sqrt(column sums of squares) as lsqr_tmatvec3 forms it (host, one pass with the checks),
+  // then A <- A diag(q) in place (lsqr_matvec4, src/lib_lsqr.f:301-329) and the device copy of the
+  // scaled matrix, which the next lsqr_solve_ finds in the cache
+  std::vector<double> colsq(*n > 0 ? *n : 1);
+  if (check_csr(ctx, *m, *n, *nnz, ia, ja, a, colsq.data())) return fail(ctx->err.c_str());
+  for (int j = 0; j < *n; ++j) q[j] = 1.0 / std::sqrt(colsq[j]);
   for (int i = 0; i < *m; ++i)
     for (int64_t p = ia[i] - 1; p < ia[i + 1] - 1; ++p) a[p] = a[p] * q[ja[p] - 1];
+  if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c)) return fail(ctx->err.c_str());
   cache_key(c, *m, *n, *nnz, ia, ja, a);
   g_precon_failed = false;
 }
@@ -451,8 +515,8 @@ extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *n
   (void)hipSetDevice(ctx->device);
   LsqrCache &c = g_cache;
   if (!cache_hit(c, *m, *n, *nnz, ia, ja, a)) {
-    if (upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c.D)) {
-      c.key_a = nullptr;
+    if (check_csr(ctx, *m, *n, *nnz, ia, ja, a, nullptr) || upload_csr(ctx, *m, *n, *nnz, ia, ja, a, c)) {
+      c.release();
       *ierr = -2;
       return;
     }
@@ -472,7 +536,9 @@ extern "C" void lsqr_solve_(const int32_t *m, const int32_t *n, const int64_t *n
   (void)hipMemcpyAsync(tmp.p, x, sizeof(double) * N, hipMemcpyHostToDevice, st);
   hipLaunchKernelGGL(k_pack_col, dim3((N * ANDR_NB + 255) / 256), dim3(256), 0, st, N, tmp.p, dX.p);
   int it = 0, is = 0;
-  int rc = andr_lsqr_device(ctx, c.D.p, false, dense_ld(N), M, N, dq.p, dB.p, dX.p, 1, *maxit, *eps, &it, &is);
+  int rc = c.sparse ? andr_lsqr_device_csr(ctx, M, N, *nnz, c.cia.p, c.cja.p, c.cva.p, c.tia.p, c.tja.p, c.tva.p, dq.p,
+                                            dB.p, dX.p, 1, *maxit, *eps, &it, &is)
+                    : andr_lsqr_device(ctx, c.D.p, false, dense_ld(N), M, N, dq.p, dB.p, dX.p, 1, *maxit, *eps, &it, &is);
   if (rc) {
     fprintf(stderr, "andromeda_hip: lsqr_solve_: %s\n", ctx->err.c_str());
     *ierr = rc;

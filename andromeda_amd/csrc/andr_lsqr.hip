@@ -612,6 +612,45 @@ __global__ void k_sumsq_parts(int n, const double *__restrict__ Y, double *__res
   }
 }
 
+// ---- sparse products (CSR) -----------------------------------------------------------------------
+// Row-parallel CSR product over NB interleaved columns, 0-based row pointers and columns: thread
+// (row r, column k). UPDATE: Y = sx .* (A X) + sy .* Y and part[block][k] = sum of the new Y^2 over
+// the block's 32 rows (the layout of k_combine_n); otherwise the raw product goes to Y (A^T U into
+// the split partials of k_combine_t, one split). Each row sums its entries in stored order.
+template <bool UPDATE>
+__global__ __launch_bounds__(256) void k_csr_mv(int rows, const int64_t *__restrict__ ia, const int32_t *__restrict__ ja,
+                                                const double *__restrict__ va, const double *__restrict__ X,
+                                                double *__restrict__ Y, const double *__restrict__ sx,
+                                                const double *__restrict__ sy, double *__restrict__ part) {
+  __shared__ double red[4][NB];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t >> 3, k = t & 7;
+  double y = 0.0;
+  if (r < rows) {
+    double acc = 0.0;
+    for (int64_t p = ia[r]; p < ia[r + 1]; ++p) acc += va[p] * X[(size_t)ja[p] * NB + k];
+    double *py = Y + (size_t)r * NB + k;
+    if (UPDATE) {
+      y = sx[k] * acc + sy[k] * (*py);
+      *py = y;
+    } else {
+      *py = acc;
+    }
+  }
+  if (UPDATE) {
+    double y2 = y * y;
+    y2 += __shfl_xor(y2, 8, 64);
+    y2 += __shfl_xor(y2, 16, 64);
+    y2 += __shfl_xor(y2, 32, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane < NB) red[wid][lane] = y2;
+    __syncthreads();
+    if (threadIdx.x < NB)
+      part[(size_t)blockIdx.x * NB + threadIdx.x] =
+          (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  }
+}
+
 // ---- column scaling (lsqr_precon) ------------------------------------------------------------
 template <class ST>
 __global__ __launch_bounds__(256) void k_colsq(const ST *__restrict__ A, int64_t lda, int m, int n, int rows_per_split,
@@ -693,6 +732,11 @@ struct LsqrRun {
   int tilesN = 4;                           // A X: 16-row tiles per wave (autotuned)
   bool vecN = false;
   bool quiet = false;  // no profiling records (the no-allocation path: event creation is skipped)
+  // sparse mode: A and A^T in CSR (0-based, device); A is then unused
+  const int64_t *cia = nullptr, *tia = nullptr;
+  const int32_t *cja = nullptr, *tja = nullptr;
+  const double *cva = nullptr, *tva = nullptr;
+  int64_t nnz = 0;
   double *pn = nullptr;  // A X segment partials when they come from a caller's workspace
   DevBuf<double> U, V, W, Xl, X0, Xs, P, PN, partU, partV, partW, partB, sc;
   DevBuf<LsqrState> S;
@@ -772,7 +816,9 @@ struct LsqrRun {
 
   void gemv_n(const double *Xin, double *Y, const double *sx, const double *sy, double *part) {
     const int t = tok("lsqr_gemv_n", abytes + vbytes + (double)m * NB * 8.0);
-    if (fp32) {
+    if (cia) {
+      hipLaunchKernelGGL(k_csr_mv<true>, dim3(blocksM), dim3(256), 0, st, m, cia, cja, cva, Xin, Y, sx, sy, part);
+    } else if (fp32) {
       hipLaunchKernelGGL(k_gemv_n<float>, dim3(blocksN), dim3(256), 0, st, (const float *)A, lda, m, n, Xin, Y, sx, sy,
                          part);
     } else {
@@ -785,7 +831,10 @@ struct LsqrRun {
   void gemv_t(const double *Y) {
     const int t = tok("lsqr_gemv_t", abytes + vbytes + (double)ns * n * NB * 8.0);
     dim3 g((n + 255) / 256, ns);
-    if (fp32)
+    if (cia)
+      hipLaunchKernelGGL(k_csr_mv<false>, dim3(blocksC), dim3(256), 0, st, n, tia, tja, tva, Y, P.p,
+                         (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
+    else if (fp32)
       hipLaunchKernelGGL(k_gemv_t<float>, g, dim3(256), 0, st, (const float *)A, lda, m, n, Y, rps, P.p);
     else if (vecN && (n % 2 == 0 || n < lda))  // column pairs: an odd last column reads the row padding
       hipLaunchKernelGGL(k_gemv_t2, dim3((n + 511) / 512, ns), dim3(256), 0, st, (const double *)A, lda, m, n, Y, rps,
@@ -810,7 +859,10 @@ struct LsqrRun {
     nrhs = nrhs_;
     if (nrhs < 1 || nrhs > NB) return andr_fail(ctx, ANDR_ERR_ARG, "nrhs must be 1..%d", NB);
     if (m < 1 || n < 1) return andr_fail(ctx, ANDR_ERR_ARG, "empty system");
-    if (fp32) {  // VALU product: one wave per 8 rows
+    if (cia) {  // CSR products: one thread per (row, column); partials per block of 32 rows
+      nsegN = 1;
+      partsN = (m * NB + 255) / 256;
+    } else if (fp32) {  // VALU product: one wave per 8 rows
       wavesN = (m + 7) / 8;
       blocksN = (wavesN + 3) / 4;
       partsN = wavesN;
@@ -825,7 +877,7 @@ struct LsqrRun {
       partsN = std::max((m + 31) / 32, (m * NB + 255) / 256);
       vecN = lda % 2 == 0 && ((uintptr_t)A % 16) == 0;
     }
-    ns = splits_for(m, n);
+    ns = cia ? 1 : splits_for(m, n);
     rps = (m + ns - 1) / ns;
     blocksC = (n * NB + 255) / 256;
     blocksM = (m * NB + 255) / 256;
@@ -863,7 +915,7 @@ struct LsqrRun {
     ANDR_HIP(ctx, hipMemsetAsync(V.p, 0, sizeof(double) * n * NB, st));
     ANDR_HIP(ctx, hipMemsetAsync(Xl.p, 0, sizeof(double) * n * NB, st));
     ANDR_HIP(ctx, hipMemsetAsync(W.p, 0, sizeof(double) * n * NB, st));
-    if (!fp32) {
+    if (!fp32 && !cia) {
       tilesN = ctx->autotune ? 4 : ctx->gemv_n_tiles;
       if (nsegN > 1 && ctx->autotune && (double)m * n * 8.0 >= 2.56e8) {
         const int rc = tune_tiles();
@@ -871,7 +923,7 @@ struct LsqrRun {
       }
       partsN = nsegN == 1 ? (m + 16 * tilesN - 1) / (16 * tilesN) : blocksM;  // one per wave, or per block
     }
-    abytes = (double)m * n * (fp32 ? 4.0 : 8.0);
+    abytes = cia ? (double)nnz * 12.0 + (double)(m + 1) * 8.0 : (double)m * n * (fp32 ? 4.0 : 8.0);
     vbytes = ((double)m + n) * NB * 8.0;
     aflops = 2.0 * m * n * nrhs;
     // x_s = x0 / q ; U = b - A x_s ; ||U||, ||b||
@@ -941,6 +993,26 @@ int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m
                      const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop) {
   LsqrRun R;
   int rc = R.begin(ctx, ctx->stream, A, fp32, lda, m, n, q, B, X, nrhs, maxit, eps);
+  if (rc) return rc;
+  while (!R.done) {
+    R.advance(LSQR_POLL);
+    if ((rc = R.poll_state())) return rc;
+  }
+  return R.finish(nit, istop);
+}
+
+int andr_lsqr_device_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja,
+                         const double *a, const int64_t *tia, const int32_t *tja, const double *ta, const double *q,
+                         const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop) {
+  LsqrRun R;
+  R.cia = ia;
+  R.cja = ja;
+  R.cva = a;
+  R.tia = tia;
+  R.tja = tja;
+  R.tva = ta;
+  R.nnz = nnz;
+  int rc = R.begin(ctx, ctx->stream, nullptr, false, 0, m, n, q, B, X, nrhs, maxit, eps);
   if (rc) return rc;
   while (!R.done) {
     R.advance(LSQR_POLL);

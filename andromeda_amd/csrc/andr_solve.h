@@ -30,6 +30,11 @@ int andr_colscale(andr_ctx *ctx, void *A, bool fp32, int64_t lda, int m, int n, 
 // (0 -> 10(m+n+50)); nit/istop (host, nrhs entries) receive the results.
 int andr_lsqr_device(andr_ctx *ctx, const void *A, bool fp32, int64_t lda, int m, int n, const double *q,
                      const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop);
+// The same on a sparse FP64 system: A (m x n) and its transpose in CSR with 0-based row pointers and
+// columns, all on the device; each product sums a row's entries in stored order.
+int andr_lsqr_device_csr(andr_ctx *ctx, int m, int n, int64_t nnz, const int64_t *ia, const int32_t *ja,
+                         const double *a, const int64_t *tia, const int32_t *tja, const double *ta, const double *q,
+                         const double *B, double *X, int nrhs, int maxit, double eps, int *nit, int *istop);
 // Several independent LSQR solves (as andr_lsqr_device each), advanced together on up to
 // LSQR_STREAMS streams so that the small ones overlap on the device. Waits for the context's stream
 // first; returns when every solve has finished.

@@ -96,7 +96,11 @@ int andr_assemble_stokes_rows(andr_ctx *ctx, const andr_mesh *mesh, int32_t row_
  * lsqr_precon_: q = 1/||column||_2, a <- a diag(q) in place (src/lib_lsqr.f:131-167).
  * lsqr_solve_: hot start from x, maxit == 0 -> 10(m+n+50) written back, ierr = istop if > 3
  * (src/lib_lsqr.f:33-127). Runs on the device of the process default context (ANDR_DEVICE env,
- * default 0). Duplicate (i, j) entries of the CSR add up, as in the reference's products. The
+ * default 0). Duplicate (i, j) entries of the CSR add up, as in the reference's products, and q
+ * sums their squares apart, as the reference's column norms do (computed on the host in row
+ * order). A CSR with entries in under a third of the m x n positions stays sparse on the device
+ * (A and its transpose in CSR, row-parallel products); a denser one (the reference's own systems
+ * are dense matrices stored in CSR) becomes a dense copy for the matrix-core products. The
  * device copy of the matrix is reused by later lsqr_solve_ calls while the array pointers, the
  * sizes and a checksum of every byte of ia, ja and a are unchanged (else the matrix is uploaded
  * again); andr_lsqr_forget() drops it. The Fortran interface has no status argument for
