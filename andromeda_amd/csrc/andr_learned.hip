@@ -331,10 +331,8 @@ void mlp_backward(hipStream_t st, int M, const float *X, const int *lab, const f
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline short f2bf(float x) {  // round to nearest even
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+__device__ inline short f2bf(float x) {  // round to nearest even (v_cvt_pk_bf16_f32)
+  return __builtin_bit_cast(short, (__bf16)x);
 }
 
 constexpr int CLS_THREADS = 256, CLS_WAVES = CLS_THREADS / 64, CLS_Q = 128;
@@ -420,7 +418,7 @@ __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float
   float e[NOUT], sum = 0.f;
 #pragma unroll
   for (int o = 0; o < NOUT; ++o) {
-    e[o] = expf(zz[o] - mx);
+    e[o] = __expf(zz[o] - mx);  // the quantile decision needs no last-ulp exponentials
     sum += e[o];
   }
   const float target = min_conf * sum;
@@ -444,12 +442,12 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
                                                           const int2 *__restrict__ rowmap,
                                                           const uint16_t *__restrict__ wbf, const float *__restrict__ wf,
                                                           const int *__restrict__ far_table,
-                                                          const int *__restrict__ npts_of_cls, double near_ratio,
+                                                          double near_ratio,
                                                           float min_conf, unsigned long long *__restrict__ stats) {
   __shared__ float sb[3 * NH + NOUT];                 // biases b1, b2, b3, b4
   __shared__ int sfar[ANDR_FAR_BINS];
   __shared__ int qe[CLS_WAVES][CLS_Q];                // queued near pairs: element
-  __shared__ float qf[CLS_WAVES][CLS_Q][3];           // and its three geometric features
+  __shared__ float qf[CLS_WAVES][64][3];              // three geometric features of a batch
   __shared__ int qres[CLS_WAVES][64][2];              // class per kernel of a processed batch
   __shared__ unsigned long long red[3];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -486,7 +484,29 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   unsigned long long pts = 0, nl = 0, nf = 0;
   int qn = 0;  // queue length (wave-uniform)
   // classify the first `cnt` queued pairs (all lanes take part: the matrix cores need a full wave)
+  // rule points of a class: Dunavant sizes {3,4,6,7,12,13,19} (6 bits each), the fixed rule's J.nq
+  auto npts = [&](int cl) { return cl == NCLS_DUN ? J.nq : (int)((0x1334C1C6103ull >> (6 * cl)) & 63); };
   auto run_batch = [&](int cnt) {
+    // the features of the batch's pairs, one per lane (the FP64 pair geometry with every lane busy);
+    // unused slots get harmless features
+    {
+      float f0 = 0.f, f1 = 0.f, f2 = 1.f;
+      if (lane < cnt) {
+        const int e = qe[wid][lane];
+        const double *x1 = g_.xyz + 3 * g_.con[3 * e], *x2 = g_.xyz + 3 * g_.con[3 * e + 1],
+                     *x3 = g_.xyz + 3 * g_.con[3 * e + 2];
+        const double n[3] = {g_.nrm[3 * e], g_.nrm[3 * e + 1], g_.nrm[3 * e + 2]};
+        float f[NIN];
+        features(pair_geo(s, ns, x1, x2, x3, n, g_.area[e]), 0, f);
+        f0 = f[0];
+        f1 = f[1];
+        f2 = f[2];
+      }
+      qf[wid][lane][0] = f0;
+      qf[wid][lane][1] = f1;
+      qf[wid][lane][2] = f2;
+      __builtin_amdgcn_wave_barrier();
+    }
     for (int blk = 0; blk < 4; ++blk) {
       if (16 * blk >= cnt) break;  // uniform
       const float *feat = &qf[wid][16 * blk][0];
@@ -510,84 +530,55 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
         cl = (uint8_t)max(a, b);
         ++nl;
       }
-      pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
+      pts += npts(cl);
       crow[qe[wid][lane]] = cl;
     }
     __builtin_amdgcn_wave_barrier();
   };
-  for (int base = wid * 64; base < ne; base += CLS_THREADS) {
-    const int e = base + lane;
-    bool near = false;
-    float f3[3] = {0.f, 0.f, 0.f};
-    if (e < ne) {
-      // far test on squared distances from the stored centroid and longest edge (FP32): dh >=
-      // near_ratio is d^2 >= near_ratio^2 L^2, and the half-octave bin floor(2 log2(dh / near_ratio))
-      // is the binary exponent of d^2 / (near_ratio^2 L^2)
-      const float4 ce = g_.cenl[e];
-      const float dx = sr.x - ce.x, dy = sr.y - ce.y, dz = sr.z - ce.z;
-      const float q = (dx * dx + dy * dy + dz * dz) * ce.w * inv_nr2;
-      if (e == r) {
-        crow[e] = NCLS_DUN;
-      } else if (q >= 1.f) {
-        const int cl = sfar[min(ilogbf(q), ANDR_FAR_BINS - 1)];
-        crow[e] = (uint8_t)cl;
-        pts += cl == NCLS_DUN ? J.nq : npts_of_cls[cl];
-      } else {
-        const double *x1 = g_.xyz + 3 * g_.con[3 * e], *x2 = g_.xyz + 3 * g_.con[3 * e + 1],
-                     *x3 = g_.xyz + 3 * g_.con[3 * e + 2];
-        const double n[3] = {g_.nrm[3 * e], g_.nrm[3 * e + 1], g_.nrm[3 * e + 2]};
-        const PairGeo pg = pair_geo(s, ns, x1, x2, x3, n, g_.area[e]);
-        {
-          near = true;
-          float f[NIN];
-          features(pg, 0, f);
-          f3[0] = f[0];
-          f3[1] = f[1];
-          f3[2] = f[2];
+  // scan the row (the wave takes every fourth group of 64 elements); near pairs are queued and
+  // classified 64 at a time, the rest of the queue once the scan is done (one batch call site)
+  int base = wid * 64;
+  for (;;) {
+    if (base < ne) {
+      const int e = base + lane;
+      bool near = false;
+      if (e < ne) {
+        // far test on squared distances from the stored centroid and longest edge (FP32): dh >=
+        // near_ratio is d^2 >= near_ratio^2 L^2, and the half-octave bin floor(2 log2(dh / near_ratio))
+        // is the binary exponent of d^2 / (near_ratio^2 L^2)
+        const float4 ce = g_.cenl[e];
+        const float dx = sr.x - ce.x, dy = sr.y - ce.y, dz = sr.z - ce.z;
+        const float q = (dx * dx + dy * dy + dz * dz) * ce.w * inv_nr2;
+        if (e == r) {
+          crow[e] = NCLS_DUN;
+        } else if (q >= 1.f) {
+          const int cl = sfar[min(ilogbf(q), ANDR_FAR_BINS - 1)];
+          crow[e] = (uint8_t)cl;
+          pts += npts(cl);
+        } else {
+          near = true;  // queued: its features are formed in the batch, all lanes busy
         }
       }
+      const unsigned long long m = __ballot(near);
+      const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
+      if (near) qe[wid][pos] = e;
+      qn += __popcll(m);
+      base += CLS_THREADS;
+      __builtin_amdgcn_wave_barrier();
     }
-    const unsigned long long m = __ballot(near);
-    const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
-    if (near) {
-      qe[wid][pos] = e;
-      qf[wid][pos][0] = f3[0];
-      qf[wid][pos][1] = f3[1];
-      qf[wid][pos][2] = f3[2];
-    }
-    qn += __popcll(m);
-    __builtin_amdgcn_wave_barrier();
-    if (qn >= 64) {
-      run_batch(64);
-      qn -= 64;
+    const bool scanned = base >= ne;  // uniform over the wave
+    if (qn >= 64 || (scanned && qn > 0)) {
+      const int cnt = min(qn, 64);
+      run_batch(cnt);
+      qn -= cnt;
       // move the rest (< 64) to the front
-      int te = 0;
-      float t0 = 0.f, t1 = 0.f, t2 = 0.f;
-      if (lane < qn) {
-        te = qe[wid][64 + lane];
-        t0 = qf[wid][64 + lane][0];
-        t1 = qf[wid][64 + lane][1];
-        t2 = qf[wid][64 + lane][2];
-      }
+      const int te = lane < qn ? qe[wid][64 + lane] : 0;
       __builtin_amdgcn_wave_barrier();
-      if (lane < qn) {
-        qe[wid][lane] = te;
-        qf[wid][lane][0] = t0;
-        qf[wid][lane][1] = t1;
-        qf[wid][lane][2] = t2;
-      }
+      if (lane < qn) qe[wid][lane] = te;
       __builtin_amdgcn_wave_barrier();
+    } else if (scanned) {
+      break;
     }
-  }
-  if (qn > 0) {
-    // unused queue slots of the last batch get harmless features
-    if (lane >= qn) {
-      qf[wid][lane][0] = 0.f;
-      qf[wid][lane][1] = 0.f;
-      qf[wid][lane][2] = 1.f;
-    }
-    __builtin_amdgcn_wave_barrier();
-    run_batch(qn);
   }
   atomicAdd(&red[0], pts);
   atomicAdd(&red[1], nl);
@@ -827,12 +818,11 @@ int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJo
   const float *fp = (const float *)base;
   const uint16_t *bp = (const uint16_t *)(base + NPARAM * sizeof(float));
   const int *far = (const int *)(base + NPARAM * (sizeof(float) + sizeof(uint16_t)));
-  const int *npts = far + ANDR_FAR_BINS;
   DevBuf<unsigned long long> st;
   if (st.alloc(3 * (size_t)njobs)) return andr_fail(ctx, ANDR_ERR_ALLOC, "stats");
   ANDR_HIP(ctx, hipMemsetAsync(st.p, 0, 3 * sizeof(unsigned long long) * njobs, ctx->stream));
   const int tok = andr_prof_begin(ctx, "classify_pairs", 0.0, 0.0);
-  hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(CLS_THREADS), 0, ctx->stream, d_jobs, d_rowmap, bp, fp, far, npts,
+  hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(CLS_THREADS), 0, ctx->stream, d_jobs, d_rowmap, bp, fp, far,
                      ctx->model_near, ctx->model_conf, st.p);
   andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
