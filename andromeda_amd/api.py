@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
                                               _D]
     lib.andr_laplace_solve.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), C.c_int32, _I,
                                        C.POINTER(_LsqrOpts), _D, _D, _D, C.POINTER(_LapReport)]
+    lib.andr_laplace_solve_mixed.argtypes = lib.andr_laplace_solve.argtypes
     lib.andr_stokes_solve.argtypes = [C.c_void_p, C.POINTER(_Mesh), C.POINTER(_Subdomain), _I, C.POINTER(_LsqrOpts),
                                       _D, _D, _D, C.POINTER(_LapReport)]
     lib.andr_subdomain_node_walls.argtypes = [C.POINTER(_Mesh), C.c_int32, C.POINTER(_Subdomain), _I, C.c_char_p,
@@ -151,7 +152,8 @@ EXPORTED = ["andr_create", "andr_destroy", "andr_last_error", "andr_version", "a
             "andr_stokes_domain_values", "andr_stokes_surface_pressure", "andr_integrals_write_stokes",
             "andr_integrals_read_stokes", "andr_integrals_write_laplace", "andr_integrals_read_laplace",
             "andr_bicgstab_dense", "andr_mesh_validate", "andr_bicgstab_dense_noalloc", "andr_assemble_laplace_rows",
-            "andr_assemble_stokes_rows", "andr_laplace_solve", "andr_subdomain_node_walls", "andr_stokes_solve"]
+            "andr_assemble_stokes_rows", "andr_laplace_solve", "andr_laplace_solve_mixed", "andr_subdomain_node_walls",
+            "andr_stokes_solve"]
 BC_FUNCTION, BC_FLUX, BC_CONTACT = 1, 2, 3
 FAR_BINS = 16
 
@@ -428,10 +430,13 @@ class Context:
         return T, G, prT, prG, err
 
     def laplace_solve(self, mesh: Mesh, subdomains, wall_bc, u, q, eps=1e-15, maxit=500, itri: int = 5,
-                      iquad: int = 4):
+                      iquad: int = 4, precision: int = PREC_FP64):
         """The Laplace boundary-value problem (andr_laplace_solve) for one or more equations:
         wall_bc (neqn, nwalls) of BC_* kinds, u (neqn, nnodes) and q (neqn, nelem) the known and
-        initial values; eps and maxit scalars or per equation. Returns a LaplaceResult per equation."""
+        initial values; eps and maxit scalars or per equation. precision PREC_MIXED: the
+        mixed-precision form (andr_laplace_solve_mixed). Returns a LaplaceResult per equation."""
+        if precision not in (PREC_FP64, PREC_MIXED):
+            raise ValueError("laplace_solve takes PREC_FP64 or PREC_MIXED")
         mv = _MeshView(mesh, None, itri, iquad)
         nn, ne, nw = mv.m.nnodes, mv.m.nelem, mv.s.nwalls
         bc = np.ascontiguousarray(np.atleast_2d(wall_bc), dtype=np.int32)
@@ -446,8 +451,9 @@ class Context:
         flux = np.zeros((neqn, nw))
         reps = (_LapReport * neqn)()
         sv = _SdView(subdomains, nn)
-        self._check(self.lib.andr_laplace_solve(self.h, C.byref(mv.s), sv.n, sv.arr, neqn, _pi(bc), opts, _p(U), _p(Q),
-                                                _p(flux), reps), "andr_laplace_solve")
+        fn = self.lib.andr_laplace_solve_mixed if precision == PREC_MIXED else self.lib.andr_laplace_solve
+        self._check(fn(self.h, C.byref(mv.s), sv.n, sv.arr, neqn, _pi(bc), opts, _p(U), _p(Q), _p(flux), reps),
+                    "andr_laplace_solve")
         return [LaplaceResult(U[k].copy(), Q[k].copy(), flux[k].copy(), r.neq, r.nx, r.nb, r.nit, r.istop,
                               (r.err[0], r.err[1]), r.t_assemble, r.t_solve) for k, r in enumerate(reps)]
 

@@ -239,9 +239,10 @@ def load_case(inp_path: str):
 
 
 def solve_laplace_case(ctx: "api.Context", mesh: Mesh, prob: BicProblem, itri: int = 5, iquad: int = 4,
-                       eps=None, maxit=None):
-    """Every equation of the problem on the device (andr_laplace_solve); eps and maxit override the
-    SOLVER settings. Returns {equation: api.LaplaceResult}."""
+                       eps=None, maxit=None, precision: int = 0):
+    """Every equation of the problem on the device (andr_laplace_solve; precision 1 = api.PREC_MIXED:
+    andr_laplace_solve_mixed); eps and maxit override the SOLVER settings. Returns
+    {equation: api.LaplaceResult}."""
     bcs, us, qs = [], [], []
     for e in prob.equations:
         bc, u, q = conditions(mesh, prob, e)
@@ -251,7 +252,7 @@ def solve_laplace_case(ctx: "api.Context", mesh: Mesh, prob: BicProblem, itri: i
     ep = [prob.solver[e].eps if eps is None else eps for e in prob.equations]
     mi = [prob.solver[e].maxit if maxit is None else maxit for e in prob.equations]
     res = ctx.laplace_solve(mesh, prob.subdomains, np.array(bcs), np.array(us), np.array(qs), eps=ep, maxit=mi,
-                            itri=itri, iquad=iquad)
+                            itri=itri, iquad=iquad, precision=precision)
     return dict(zip(prob.equations, res))
 
 

@@ -464,7 +464,9 @@ __device__ __noinline__ void quad_stokes_pair(const double *x1, const double *x2
 // Laplace rows: sdIntRowU / sdIntRowQ (src/lib_sdBem.f90:438-733), regular integrals by
 // Triangle_3DLapInt (:859-905) with BemKernel (:911-936), singular by the analytic formula.
 // ------------------------------------------------------------------------------------------------
-template <bool ROW_IN_LDS>
+// F32Q: the regular integrals in FP32 (relative geometry formed in FP64, quadrature and sums in FP32,
+// scaled by the area in FP64), the singular ones in FP64 (the mixed-precision Laplace form)
+template <bool ROW_IN_LDS, bool F32Q>
 __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int nq, const int *__restrict__ celems,
                                                       const int *__restrict__ coff, int ncolors,
                                                       double *__restrict__ H, double *__restrict__ G,
@@ -510,7 +512,34 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
       if (!qrow) isrc = (v0 == r) ? 1 : (v1 == r) ? 2 : (v2 == r) ? 3 : 0;
       else if (e == erow) isrc = 4;
       double gv, h1 = 0.0, h2 = 0.0, h3 = 0.0;
-      if (isrc == 0) {
+      if (isrc == 0 && F32Q) {
+        const double m = g.mul[e];
+        const float nx = (float)(m * g.nrm[3 * e]), ny = (float)(m * g.nrm[3 * e + 1]), nz = (float)(m * g.nrm[3 * e + 2]);
+        // r = s - f = d - l1 a - l2 b with d = s - x3, a = x1 - x3, b = x2 - x3 (l3 = 1 - l1 - l2)
+        const float d0 = (float)(s[0] - x3[0]), d1 = (float)(s[1] - x3[1]), d2 = (float)(s[2] - x3[2]);
+        const float a0 = (float)(x1[0] - x3[0]), a1 = (float)(x1[1] - x3[1]), a2 = (float)(x1[2] - x3[2]);
+        const float b0 = (float)(x2[0] - x3[0]), b1 = (float)(x2[1] - x3[1]), b2 = (float)(x2[2] - x3[2]);
+        const float k4 = (float)k4pi;
+        float gs = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f;
+        for (int q = 0; q < nq; ++q) {
+          const float l1 = c_qf[qoff + q][0], l2 = c_qf[qoff + q][1], l3 = c_qf[qoff + q][2], w = c_qf[qoff + q][3];
+          const float rx = d0 - l1 * a0 - l2 * b0;
+          const float ry = d1 - l1 * a1 - l2 * b1;
+          const float rz = d2 - l1 * a2 - l2 * b2;
+          const float ira = __frsqrt_rn(rx * rx + ry * ry + rz * rz);
+          const float fg = k4 * ira;
+          const float wh = w * (rx * nx + ry * ny + rz * nz) * ira * ira * fg;
+          gs += w * fg;
+          f1 += wh * l1;
+          f2 += wh * l2;
+          f3 += wh * l3;
+        }
+        const double A = g.area[e];
+        gv = (double)gs * A;
+        h1 = (double)f1 * A;
+        h2 = (double)f2 * A;
+        h3 = (double)f3 * A;
+      } else if (isrc == 0) {
         const double m = g.mul[e];
         const double nx = m * g.nrm[3 * e], ny = m * g.nrm[3 * e + 1], nz = m * g.nrm[3 * e + 2];
         // r = s - f with f = l1 x1 + l2 x2 + l3 x3
@@ -579,23 +608,21 @@ __global__ __launch_bounds__(256) void k_laplace_rows(BodyGeom g, int qoff, int 
   if (threadIdx.x == 0) err_rows[ro] = err;
 }
 
-int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows, int r0, int nrows) {
+int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows, int r0, int nrows,
+                             bool fp32_quad) {
   QuadRule qr = andr_rule(b.itri);
   const int nsp = nrows < 0 ? b.g.nn + b.g.ne - r0 : nrows;  // rows r0 .. r0 + nsp - 1
   const int ncol = (int)b.color_off.size() - 1;
   const size_t lds = sizeof(double) * b.g.nn;
   const double evals = (double)nsp * b.g.ne * qr.npts;
-  const int tok = andr_prof_begin(ctx, "laplace_rows", 8.0 * nsp * ((double)b.g.nn + b.g.ne),
-                                  evals * ANDR_FLOPS_PER_LAPLACE_EVAL);
-  if (lds <= 128 * 1024) {
-    ANDR_HIP(ctx, hipFuncSetAttribute((const void *)k_laplace_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    hipLaunchKernelGGL(k_laplace_rows<true>, dim3(nsp), dim3(256), lds, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows, r0);
-  } else {
-    hipLaunchKernelGGL(k_laplace_rows<false>, dim3(nsp), dim3(256), 0, ctx->stream, b.g, qr.offset, qr.npts,
-                       b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows, r0);
-  }
+  const int tok = andr_prof_begin(ctx, fp32_quad ? "laplace_rows_fp32" : "laplace_rows",
+                                  8.0 * nsp * ((double)b.g.nn + b.g.ne), evals * ANDR_FLOPS_PER_LAPLACE_EVAL);
+  const bool in_lds = lds <= 128 * 1024;
+  auto kern = in_lds ? (fp32_quad ? k_laplace_rows<true, true> : k_laplace_rows<true, false>)
+                     : (fp32_quad ? k_laplace_rows<false, true> : k_laplace_rows<false, false>);
+  if (in_lds) ANDR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(nsp), dim3(256), in_lds ? lds : 0, ctx->stream, b.g, qr.offset, qr.npts,
+                     b.color_elems.p, b.d_color_off.p, ncol, H, G, err_rows, r0);
   andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
   return ANDR_OK;

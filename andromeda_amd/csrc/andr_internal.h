@@ -149,8 +149,9 @@ int andr_body_upload(andr_ctx *ctx, const andr_mesh *m, DevBody &b, bool with_co
 // kernels exposed between translation units ----------------------------------------------------
 // rows r0 .. r0 + nrows - 1 of the nsp source points (node rows first; nrows < 0: to the end); the
 // outputs hold those rows only (row k of H is source point r0 + k)
+// fp32_quad: regular integrals in FP32 (the mixed-precision form), singular ones in FP64
 int andr_launch_laplace_rows(andr_ctx *ctx, DevBody &b, double *H, double *G, double *err_rows, int r0 = 0,
-                             int nrows = -1);
+                             int nrows = -1, bool fp32_quad = false);
 int andr_launch_stokes_rows(andr_ctx *ctx, DevBody &b, double extra, double *T, double *G, double *prT,
                             double *prG, double *err_rows, int r0 = 0, int nrows = -1);
 // transposes an m x n row-major device matrix into a column-major host buffer

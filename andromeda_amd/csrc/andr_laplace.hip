@@ -96,29 +96,30 @@ int build_lists(const andr_mesh *m, int nsd, const andr_subdomain *sds, std::vec
 
 // One row block per system row of a subdomain: sys[row][col] = H[r][j] or (normmul / diff) G[r][j]
 // for the unknowns' columns, rhs[row] = -(sum of the known columns times the known values).
+template <class ST>
 __global__ __launch_bounds__(256) void k_lap_system(int nn_sd, int ne_sd, const double *__restrict__ H,
                                                     const double *__restrict__ G, const int *__restrict__ rows_r,
                                                     int row0, const int *__restrict__ ncol,
                                                     const uint8_t *__restrict__ nknown, const int *__restrict__ qcol,
                                                     const uint8_t *__restrict__ qknown,
                                                     const double *__restrict__ gmul, const double *__restrict__ b,
-                                                    double *__restrict__ sys, int64_t lda, double *__restrict__ B) {
+                                                    ST *__restrict__ sys, int64_t lda, double *__restrict__ B) {
   __shared__ double red[4];
   const int row = row0 + blockIdx.x;
   const int r = rows_r[blockIdx.x];
   const double *hr = H + (size_t)r * nn_sd;
   const double *gr = G + (size_t)r * ne_sd;
-  double *srow = sys + (size_t)row * lda;
+  ST *srow = sys + (size_t)row * lda;
   double acc = 0.0;
   for (int j = threadIdx.x; j < nn_sd; j += blockDim.x) {
     const double v = hr[j];
     if (nknown[j]) acc -= v * b[ncol[j]];
-    else srow[ncol[j]] = v;
+    else srow[ncol[j]] = (ST)v;
   }
   for (int j = threadIdx.x; j < ne_sd; j += blockDim.x) {
     const double v = gmul[j] * gr[j];
     if (qknown[j]) acc -= v * b[qcol[j]];
-    else srow[qcol[j]] = v;
+    else srow[qcol[j]] = (ST)v;
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -126,9 +127,10 @@ __global__ __launch_bounds__(256) void k_lap_system(int nn_sd, int ne_sd, const 
   if (threadIdx.x == 0) B[(size_t)row * ANDR_NB] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-__global__ void k_lap_fill(size_t n, double *__restrict__ p) {
+template <class T>
+__global__ void k_lap_fill(size_t n, T *__restrict__ p) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = 0.0;
+  if (t < n) p[t] = (T)0;
 }
 
 // the solution back to the boundary values: u[i] = x[src] where src >= 0 (column 0 of the packed X)
@@ -235,9 +237,12 @@ extern "C" int andr_subdomain_node_walls(const andr_mesh *mesh, int32_t nsd, con
   return ANDR_OK;
 }
 
-extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds,
-                                  int32_t neqn, const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u,
-                                  double *q, double *wall_flux, andr_laplace_report *reports) {
+namespace {
+// mixed: the mixed-precision form (FP32 regular integrals, FP32 system storage with FP64 products
+// and LSQR vectors); otherwise the plain FP64 path
+int laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds, int32_t neqn,
+                  const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q, double *wall_flux,
+                  andr_laplace_report *reports, bool mixed) {
   if (!ctx) return ANDR_ERR_ARG;
   if (neqn <= 0 || !wall_bc || !u || !q) return andr_fail(ctx, ANDR_ERR_ARG, "andr_laplace_solve arguments");
   int rc = andr_check_mesh(ctx, mesh);
@@ -273,7 +278,7 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
     const int nsp = d.nn + d.ne;
     if (d.H.alloc((size_t)nsp * d.nn, ctx) || d.G.alloc((size_t)nsp * d.ne, ctx) || d.E.alloc(nsp, ctx))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "integral matrices of subdomain %d", s);
-    if ((rc = andr_launch_laplace_rows(ctx, body, d.H.p, d.G.p, d.E.p))) return rc;
+    if ((rc = andr_launch_laplace_rows(ctx, body, d.H.p, d.G.p, d.E.p, 0, -1, mixed))) return rc;
     std::vector<double> he(nsp);
     ANDR_HIP(ctx, hipMemcpyAsync(he.data(), d.E.p, sizeof(double) * nsp, hipMemcpyDeviceToHost, st));
     ANDR_HIP(ctx, hipStreamSynchronize(st));  // the body's buffers go back to the cache after this
@@ -347,13 +352,20 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
         }
       }
     }
-    const int64_t lda = ((int64_t)nx + 15) / 16 * 16;  // rows start on 128-byte lines
+    // rows start on 128-byte lines (16 FP64 or 32 FP32 entries)
+    const int64_t lda = mixed ? ((int64_t)nx + 31) / 32 * 32 : ((int64_t)nx + 15) / 16 * 16;
     DevBuf<double> sys, B, X, db, qs, du, dq;
-    if (sys.alloc((size_t)neq * lda, ctx) || B.alloc((size_t)neq * ANDR_NB, ctx) || X.alloc((size_t)nx * ANDR_NB, ctx) ||
-        qs.alloc(nx, ctx))
+    DevBuf<float> sys32;
+    if ((mixed ? sys32.alloc((size_t)neq * lda, ctx) : sys.alloc((size_t)neq * lda, ctx)) ||
+        B.alloc((size_t)neq * ANDR_NB, ctx) || X.alloc((size_t)nx * ANDR_NB, ctx) || qs.alloc(nx, ctx))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "Laplace system %d x %d", neq, nx);
-    hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st, (size_t)neq * lda, sys.p);
-    hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * ANDR_NB + 255) / 256), dim3(256), 0, st,
+    if (mixed)
+      hipLaunchKernelGGL(k_lap_fill<float>, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st, (size_t)neq * lda,
+                         sys32.p);
+    else
+      hipLaunchKernelGGL(k_lap_fill<double>, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st,
+                         (size_t)neq * lda, sys.p);
+    hipLaunchKernelGGL(k_lap_fill<double>, dim3(((size_t)neq * ANDR_NB + 255) / 256), dim3(256), 0, st,
                        (size_t)neq * ANDR_NB, B.p);
     {
       std::vector<double> xp((size_t)nx * ANDR_NB, 0.0);
@@ -397,8 +409,12 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
         if ((rc = upload(ctx, drr, rr)) || (rc = upload(ctx, dnc, nc)) || (rc = upload(ctx, dqc, qc)) ||
             (rc = upload(ctx, dnk, nk)) || (rc = upload(ctx, dqk, qk)) || (rc = upload(ctx, dgm, gm)))
           return rc;
-        hipLaunchKernelGGL(k_lap_system, dim3((int)rr.size()), dim3(256), 0, st, D[s].nn, D[s].ne, D[s].H.p, D[s].G.p,
-                           drr.p, row, dnc.p, dnk.p, dqc.p, dqk.p, dgm.p, db.p, sys.p, lda, B.p);
+        if (mixed)
+          hipLaunchKernelGGL(k_lap_system<float>, dim3((int)rr.size()), dim3(256), 0, st, D[s].nn, D[s].ne, D[s].H.p,
+                             D[s].G.p, drr.p, row, dnc.p, dnk.p, dqc.p, dqk.p, dgm.p, db.p, sys32.p, lda, B.p);
+        else
+          hipLaunchKernelGGL(k_lap_system<double>, dim3((int)rr.size()), dim3(256), 0, st, D[s].nn, D[s].ne, D[s].H.p,
+                             D[s].G.p, drr.p, row, dnc.p, dnk.p, dqc.p, dqk.p, dgm.p, db.p, sys.p, lda, B.p);
         ANDR_HIP(ctx, hipGetLastError());
         ANDR_HIP(ctx, hipStreamSynchronize(st));  // the maps go back to the cache after this
         row += (int)rr.size();
@@ -408,9 +424,10 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
     // lsqr_precon + lsqr_solve (column scaling in place, hot start, the reference's stopping rules)
     const int maxit = opts ? opts[en].maxit : 0;
     const double eps = opts && opts[en].eps > 0.0 ? opts[en].eps : 1e-15;
-    if ((rc = andr_colscale(ctx, sys.p, false, lda, neq, nx, qs.p))) return rc;
+    void *A = mixed ? (void *)sys32.p : (void *)sys.p;
+    if ((rc = andr_colscale(ctx, A, mixed, lda, neq, nx, qs.p))) return rc;
     int nit = 0, istop = 0;
-    if ((rc = andr_lsqr_device(ctx, sys.p, false, lda, neq, nx, qs.p, B.p, X.p, 1, maxit, eps, &nit, &istop)))
+    if ((rc = andr_lsqr_device(ctx, A, mixed, lda, neq, nx, qs.p, B.p, X.p, 1, maxit, eps, &nit, &istop)))
       return rc;
     // the solution back to u and q, then the wall fluxes
     DevBuf<int> dsu, dsq;
@@ -444,6 +461,20 @@ extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t 
     }
   }
   return ANDR_OK;
+}
+}  // namespace
+
+extern "C" int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds,
+                                  int32_t neqn, const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u,
+                                  double *q, double *wall_flux, andr_laplace_report *reports) {
+  return laplace_solve(ctx, mesh, nsd, sds, neqn, wall_bc, opts, u, q, wall_flux, reports, false);
+}
+
+extern "C" int andr_laplace_solve_mixed(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd,
+                                        const andr_subdomain *sds, int32_t neqn, const int32_t *wall_bc,
+                                        const andr_lsqr_opts *opts, double *u, double *q, double *wall_flux,
+                                        andr_laplace_report *reports) {
+  return laplace_solve(ctx, mesh, nsd, sds, neqn, wall_bc, opts, u, q, wall_flux, reports, true);
 }
 
 // ---- the general Stokes problem (one subdomain) ------------------------------------------------------
@@ -598,8 +629,8 @@ extern "C" int andr_stokes_solve(andr_ctx *ctx, const andr_mesh *mesh, const and
       (rc = upload(ctx, dqk, qknown)) || (rc = upload(ctx, dsu, src_u)) || (rc = upload(ctx, dsq, src_q)))
     return rc;
   ANDR_HIP(ctx, hipMemcpyAsync(X.p, xp.data(), sizeof(double) * xp.size(), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st, (size_t)neq * lda, sys.p);
-  hipLaunchKernelGGL(k_lap_fill, dim3(((size_t)neq * ANDR_NB + 255) / 256), dim3(256), 0, st, (size_t)neq * ANDR_NB,
+  hipLaunchKernelGGL(k_lap_fill<double>, dim3(((size_t)neq * lda + 255) / 256), dim3(256), 0, st, (size_t)neq * lda, sys.p);
+  hipLaunchKernelGGL(k_lap_fill<double>, dim3(((size_t)neq * ANDR_NB + 255) / 256), dim3(256), 0, st, (size_t)neq * ANDR_NB,
                      B.p);
   hipLaunchKernelGGL(k_stk_system, dim3(neq), dim3(256), 0, st, nns, nes, nsp, dT.p, dG.p, drows.p, dnc.p, dnk.p,
                      dqc.p, dqk.p, dgm.p, db.p, sys.p, lda, B.p);

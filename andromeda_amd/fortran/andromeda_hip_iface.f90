@@ -221,6 +221,22 @@ module andromeda_hip
       type(andr_laplace_report), intent(out) :: reports(*)
     end function
 
+    ! the mixed-precision form of andr_laplace_solve (FP32 regular integrals and system storage)
+    integer(c_int) function andr_laplace_solve_mixed(ctx, mesh, nsd, sds, neqn, wall_bc, opts, u, q, wall_flux, &
+        reports) bind(C, name="andr_laplace_solve_mixed")
+      import :: c_int, c_ptr, c_int32_t, c_double, andr_mesh, andr_subdomain, andr_lsqr_opts, andr_laplace_report
+      type(c_ptr), value :: ctx
+      type(andr_mesh), intent(in) :: mesh
+      integer(c_int32_t), value :: nsd
+      type(andr_subdomain), intent(in) :: sds(*)
+      integer(c_int32_t), value :: neqn
+      integer(c_int32_t), intent(in) :: wall_bc(*)
+      type(andr_lsqr_opts), intent(in) :: opts(*)
+      real(c_double), intent(inout) :: u(*), q(*)
+      real(c_double), intent(out) :: wall_flux(*)
+      type(andr_laplace_report), intent(out) :: reports(*)
+    end function
+
     integer(c_int) function andr_set_quadrature_model(ctx, params, confidence, near_ratio, far_table) &
         bind(C, name="andr_set_quadrature_model")
       import :: c_int, c_ptr, c_float, c_double, c_int32_t

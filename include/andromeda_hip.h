@@ -191,6 +191,14 @@ typedef struct {
 int andr_laplace_solve(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds, int32_t neqn,
                        const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q, double *wall_flux,
                        andr_laplace_report *reports);
+/* The mixed-precision form of andr_laplace_solve (no counterpart in the reference; same arguments and
+ * results): the regular integrals in FP32 (singular ones, the free terms and the right-hand side in
+ * FP64), the system stored in FP32, the LSQR products accumulated in FP64 and every LSQR vector and
+ * the solution update in FP64. On the C1 sphere (1280 triangles, u = x) the fluxes stay within
+ * 1e-6 (relative L2) of andr_laplace_solve's. */
+int andr_laplace_solve_mixed(andr_ctx *ctx, const andr_mesh *mesh, int32_t nsd, const andr_subdomain *sds,
+                             int32_t neqn, const int32_t *wall_bc, const andr_lsqr_opts *opts, double *u, double *q,
+                             double *wall_flux, andr_laplace_report *reports);
 
 /* The general Stokes problem of the reference's Stokes driver on one subdomain (its Stokes path has
  * one, src/lib_StokesFull.f90:183): the 12 integral matrices (sdFormIntegralMatricesStokes, bounded
