@@ -178,33 +178,106 @@ __global__ __launch_bounds__(256) void k_pair_labels(BodyGeom g, int qoff_ref, i
   }
 }
 
-// ---- generic small FP32 GEMM: C = alpha op(A) op(B) + beta C (row-major) -------------------------
-__global__ void k_sgemm(int M, int N, int K, const float *__restrict__ A, int lda, int ta, const float *__restrict__ B,
-                        int ldb, int tb, float *__restrict__ C, int ldc, float alpha, float beta) {
-  __shared__ float As[16][17], Bs[16][17];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
-  float acc = 0.f;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const int ka = k0 + tx, kb = k0 + ty;
-    As[ty][tx] = (row < M && ka < K) ? (ta ? A[(size_t)ka * lda + row] : A[(size_t)row * lda + ka]) : 0.f;
-    Bs[ty][tx] = (kb < K && col < N) ? (tb ? B[(size_t)col * ldb + kb] : B[(size_t)kb * ldb + col]) : 0.f;
-    __syncthreads();
+// ---- small FP32 GEMMs on the matrix cores (v_mfma_f32_16x16x4f32), row-major -------------------
+// Lane layout of one 16x16x4 step: lane 16 g + i supplies A[i][g] and B[g][i] and, after the K
+// loop, holds C[4 g + v][i], v = 0..3. Inside a 16-deep K block, step t gives lane group g the index
+// k0 + 4 g + t on both operands (one permutation of K, so every k is summed once); the four steps
+// go to four accumulators, added at the end. Out-of-range rows, columns and k read as 0.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Z = op(A) op(B) (+ bias[col]); H = relu(Z) if H; Z = 0 where mask <= 0 if mask (ReLU backward,
+// mask = the forward pre-activation of the same shape). One wave per 16x16 tile of Z.
+__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, const float *__restrict__ A, int lda, int ta,
+                                               const float *__restrict__ B, int ldb, int tb, float *__restrict__ Z,
+                                               int ldz, const float *__restrict__ bias, float *__restrict__ H,
+                                               const float *__restrict__ mask) {
+  const int lane = threadIdx.x & 63;
+  const int tiles_n = (N + 15) >> 4;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= tiles_n * ((M + 15) >> 4)) return;
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int g = lane >> 4, i = lane & 15;
+  const int row = 16 * tm + i, col = 16 * tn + i;
+  const bool rok = row < M, cok = col < N;
+  f32x4 acc[4];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
-    __syncthreads();
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = k0 + 4 * g + t;
+      const bool kok = k < K;
+      const float a = (rok && kok) ? (ta ? A[(size_t)k * lda + row] : A[(size_t)row * lda + k]) : 0.f;
+      const float b = (cok && kok) ? (tb ? B[(size_t)col * ldb + k] : B[(size_t)k * ldb + col]) : 0.f;
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+    }
   }
-  if (row < M && col < N) C[(size_t)row * ldc + col] = alpha * acc + (beta != 0.f ? beta * C[(size_t)row * ldc + col] : 0.f);
+  if (!cok) return;
+  const float bc = bias ? bias[col] : 0.f;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 16 * tm + 4 * g + v;
+    if (r >= M) break;
+    const size_t o = (size_t)r * ldz + col;
+    float z = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]) + bc;
+    if (mask && !(mask[o] > 0.f)) z = 0.f;
+    Z[o] = z;
+    if (H) H[o] = fmaxf(z, 0.f);
+  }
 }
 
-// Z += b (broadcast over rows); H = relu(Z) (H may alias nothing)
-__global__ void k_bias_relu(int M, int N, float *__restrict__ Z, const float *__restrict__ b, float *__restrict__ H,
-                            int relu) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M * N) return;
-  const float z = Z[t] + b[t % N];
-  Z[t] = z;
-  if (H) H[t] = relu ? fmaxf(z, 0.f) : z;
+// Weight and bias gradients of one layer: G = A^T D (A: M x NI, D: M x NO, row-major with leading
+// dimensions NI and NO; G: NI x NO) and gb = the column sums of D. One 1024-thread block per 16x16
+// tile of G: its 16 waves take the samples 16 at a time in turn and their partial tiles are added
+// in a fixed order through LDS. The blocks of the first row of tiles (tm = 0) also write gb.
+constexpr int WG_WAVES = 16;
+__global__ __launch_bounds__(1024) void k_wgrad(int M, int NI, int NO, const float *__restrict__ A,
+                                                const float *__restrict__ D, float *__restrict__ G,
+                                                float *__restrict__ gb) {
+  __shared__ float part[WG_WAVES][5][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tiles_n = (NO + 15) >> 4;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  const int g = lane >> 4, i = lane & 15;
+  const int row = 16 * tm + i, col = 16 * tn + i;
+  const bool rok = row < NI, cok = col < NO;
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float cs = 0.f;  // this lane's share of the column sum of D
+  for (int m0 = 16 * w; m0 < M; m0 += 16 * WG_WAVES) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int m = m0 + 4 * g + t;
+      const bool mok = m < M;
+      const float a = (rok && mok) ? A[(size_t)m * NI + row] : 0.f;
+      const float d = (cok && mok) ? D[(size_t)m * NO + col] : 0.f;
+      cs += d;
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, d, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) part[w][v][lane] = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+  part[w][4][lane] = cs;
+  __syncthreads();
+  if (w != 0) return;
+  float s[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    s[v] = 0.f;
+    for (int u = 0; u < WG_WAVES; ++u) s[v] += part[u][v][lane];
+  }
+  // the column sum: the four lane groups hold sums over disjoint samples (all lanes take part)
+  float c = s[4];
+  c += __shfl_xor(c, 16, 64);
+  c += __shfl_xor(c, 32, 64);
+  if (!cok) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 16 * tm + 4 * g + v;
+    if (r < NI) G[(size_t)r * NO + col] = s[v];
+  }
+  if (gb && tm == 0 && g == 0) gb[col] = c;
 }
 
 // softmax cross-entropy over NOUT logits: P, per-sample loss, dZ = (P - onehot) * scale
@@ -226,19 +299,6 @@ __global__ void k_softmax_xent(int M, const float *__restrict__ Z, const int *__
     if (dZ) dZ[(size_t)i * NOUT + k] = (p - (lab && lab[i] == k ? 1.f : 0.f)) * scale;
   }
   if (loss && lab) loss[i] = -(z[lab[i]] - mx - logf(s));
-}
-
-__global__ void k_relu_bwd(int n, float *__restrict__ dH, const float *__restrict__ Z) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n && !(Z[t] > 0.f)) dH[t] = 0.f;
-}
-
-__global__ void k_colsum(int M, int N, const float *__restrict__ X, float *__restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  float s = 0.f;
-  for (int i = 0; i < M; ++i) s += X[(size_t)i * N + j];
-  out[j] = s;
 }
 
 __global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
@@ -277,47 +337,40 @@ struct MlpWork {
 };
 
 void sgemm(hipStream_t st, int M, int N, int K, const float *A, int lda, int ta, const float *B, int ldb, int tb,
-           float *C, int ldc, float alpha = 1.f, float beta = 0.f) {
-  dim3 grid((N + 15) / 16, (M + 15) / 16);
-  hipLaunchKernelGGL(k_sgemm, grid, dim3(16, 16), 0, st, M, N, K, A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
+           float *Z, int ldz, const float *bias = nullptr, float *H = nullptr, const float *mask = nullptr) {
+  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+  hipLaunchKernelGGL(k_sgemm, dim3((tiles + 3) / 4), dim3(256), 0, st, M, N, K, A, lda, ta, B, ldb, tb, Z, ldz, bias, H,
+                     mask);
 }
 
-// forward pass of M samples (X: M x NIN) with parameters p
+void wgrad(hipStream_t st, int M, int NI, int NO, const float *A, const float *D, float *G, float *gb) {
+  const int tiles = ((NI + 15) / 16) * ((NO + 15) / 16);
+  hipLaunchKernelGGL(k_wgrad, dim3(tiles), dim3(64 * WG_WAVES), 0, st, M, NI, NO, A, D, G, gb);
+}
+
+// forward pass of M samples (X: M x NIN) with parameters p: Z_l = H_(l-1) W_l + b_l, H_l = relu(Z_l)
 void mlp_forward(hipStream_t st, int M, const float *X, const float *p, MlpWork &w) {
-  const int g1 = (M * NH + 255) / 256, g4 = (M * NOUT + 255) / 256;
-  sgemm(st, M, NH, NIN, X, NIN, 0, p + P_W1, NH, 0, w.Z1.p, NH);
-  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z1.p, p + P_B1, w.H1.p, 1);
-  sgemm(st, M, NH, NH, w.H1.p, NH, 0, p + P_W2, NH, 0, w.Z2.p, NH);
-  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z2.p, p + P_B2, w.H2.p, 1);
-  sgemm(st, M, NH, NH, w.H2.p, NH, 0, p + P_W3, NH, 0, w.Z3.p, NH);
-  hipLaunchKernelGGL(k_bias_relu, dim3(g1), dim3(256), 0, st, M, NH, w.Z3.p, p + P_B3, w.H3.p, 1);
-  sgemm(st, M, NOUT, NH, w.H3.p, NH, 0, p + P_W4, NOUT, 0, w.Z4.p, NOUT);
-  hipLaunchKernelGGL(k_bias_relu, dim3(g4), dim3(256), 0, st, M, NOUT, w.Z4.p, p + P_B4, (float *)nullptr, 0);
+  sgemm(st, M, NH, NIN, X, NIN, 0, p + P_W1, NH, 0, w.Z1.p, NH, p + P_B1, w.H1.p);
+  sgemm(st, M, NH, NH, w.H1.p, NH, 0, p + P_W2, NH, 0, w.Z2.p, NH, p + P_B2, w.H2.p);
+  sgemm(st, M, NH, NH, w.H2.p, NH, 0, p + P_W3, NH, 0, w.Z3.p, NH, p + P_B3, w.H3.p);
+  sgemm(st, M, NOUT, NH, w.H3.p, NH, 0, p + P_W4, NOUT, 0, w.Z4.p, NOUT, p + P_B4);
 }
 
 // backward of the mean cross-entropy: grad (NPARAM) from a forward pass already in w
 void mlp_backward(hipStream_t st, int M, const float *X, const int *lab, const float *p, MlpWork &w, float *grad) {
-  const int gm = (M + 255) / 256, g1 = (M * NH + 255) / 256;
-  hipLaunchKernelGGL(k_softmax_xent, dim3(gm), dim3(256), 0, st, M, w.Z4.p, lab, w.P.p, w.loss.p, w.dZ4.p,
-                     1.f / (float)M);
-  // layer 4
-  sgemm(st, NH, NOUT, M, w.H3.p, NH, 1, w.dZ4.p, NOUT, 0, grad + P_W4, NOUT);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NOUT, w.dZ4.p, grad + P_B4);
-  sgemm(st, M, NH, NOUT, w.dZ4.p, NOUT, 0, p + P_W4, NOUT, 1, w.dH.p, NH);
-  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dH.p, w.Z3.p);
+  hipLaunchKernelGGL(k_softmax_xent, dim3((M + 255) / 256), dim3(256), 0, st, M, w.Z4.p, lab, w.P.p, w.loss.p,
+                     w.dZ4.p, 1.f / (float)M);
+  // layer 4, then the gradient through W4 and the ReLU of layer 3
+  wgrad(st, M, NH, NOUT, w.H3.p, w.dZ4.p, grad + P_W4, grad + P_B4);
+  sgemm(st, M, NH, NOUT, w.dZ4.p, NOUT, 0, p + P_W4, NOUT, 1, w.dH.p, NH, nullptr, nullptr, w.Z3.p);
   // layer 3
-  sgemm(st, NH, NH, M, w.H2.p, NH, 1, w.dH.p, NH, 0, grad + P_W3, NH);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dH.p, grad + P_B3);
-  sgemm(st, M, NH, NH, w.dH.p, NH, 0, p + P_W3, NH, 1, w.dZb.p, NH);
-  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dZb.p, w.Z2.p);
+  wgrad(st, M, NH, NH, w.H2.p, w.dH.p, grad + P_W3, grad + P_B3);
+  sgemm(st, M, NH, NH, w.dH.p, NH, 0, p + P_W3, NH, 1, w.dZb.p, NH, nullptr, nullptr, w.Z2.p);
   // layer 2
-  sgemm(st, NH, NH, M, w.H1.p, NH, 1, w.dZb.p, NH, 0, grad + P_W2, NH);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dZb.p, grad + P_B2);
-  sgemm(st, M, NH, NH, w.dZb.p, NH, 0, p + P_W2, NH, 1, w.dH.p, NH);
-  hipLaunchKernelGGL(k_relu_bwd, dim3(g1), dim3(256), 0, st, M * NH, w.dH.p, w.Z1.p);
+  wgrad(st, M, NH, NH, w.H1.p, w.dZb.p, grad + P_W2, grad + P_B2);
+  sgemm(st, M, NH, NH, w.dZb.p, NH, 0, p + P_W2, NH, 1, w.dH.p, NH, nullptr, nullptr, w.Z1.p);
   // layer 1
-  sgemm(st, NIN, NH, M, X, NIN, 1, w.dH.p, NH, 0, grad + P_W1, NH);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, M, NH, w.dH.p, grad + P_B1);
+  wgrad(st, M, NIN, NH, X, w.dH.p, grad + P_W1, grad + P_B1);
 }
 
 // ---- inference inside the assembly: BF16 matrix cores, FP32 accumulation --------------------------
@@ -329,7 +382,6 @@ void mlp_backward(hipStream_t st, int M, const float *X, const int *lab, const f
 // registers, collects the near pairs of its row in an LDS queue and evaluates them 64 at a time
 // (4 column blocks of 16 pairs), for the Stokes-G and Stokes-H kernel ids.
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline short f2bf(float x) {  // round to nearest even (v_cvt_pk_bf16_f32)
   return __builtin_bit_cast(short, (__bf16)x);
