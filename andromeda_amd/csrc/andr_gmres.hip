@@ -192,6 +192,13 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   const int ne = O.ne, row0 = wm.row0, kend = min(wm.k1, ne), KBe = (ne + 15) / 16;
   const int g = lane >> 4, c = lane & 15;
   const bool hi = c >= NB;
+  // A system whose every column has converged in this cycle (its step count fixed by the Arnoldi
+  // step) needs no more products until the host drops it from the map every few steps: its waves
+  // skip the matrix and write the null-space term alone (the later steps of such a column are
+  // never used, and stay finite).
+  bool need = false;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) need |= S.small[8 * k + 4] > 0.0 && S.small[8 * k + 5] < 0.0;
   const float *ab[2];
 #pragma unroll
   for (int T = 0; T < 2; ++T) ab[T] = O.S + (size_t)((row0 >> 4) + T) * KBe * 1536 + 16 * c + 4 * g;
@@ -297,7 +304,7 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   // requested before the products of the current one. The prefetch is unconditional (the last
   // one re-reads the final block) so that no branch sits between a load and its use.
   int kb = wm.k0 >> 4;
-  const int kb_end = kend >> 4;
+  const int kb_end = need ? kend >> 4 : kb;
   if (kb < kb_end) {
     float a0[2][6][4], a1[2][6][4];
     ZT_t z0[3][4], z1[3][4];
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
       ++kb;
     }
   }
-  if (16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
+  if (need && 16 * kb < kend) {  // last, partial column block: masked (padding columns are not written)
     float a[2][6][4];
     ZT_t z[3][4];
 #pragma unroll
@@ -1200,6 +1207,25 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     return ANDR_OK;
   };
   auto sm_of = [&](int s, int k) { return hsmall.data() + state_per * s + k * 8; };
+  // The Krylov product skips a system once every column has converged in the cycle (kb_gemv_sym);
+  // the host learns the step afterwards from the columns' step counts, and takes the skipped
+  // matrix reads out of the profiled totals (a system once, when it leaves the live map).
+  std::vector<char> skip_counted(nsys, 0);
+  auto account_skips = [&](int s, int jlast) {
+    if (skip_counted[s] || !andr_prof(ctx).on) return;
+    int fz = -1;
+    for (int k = 0; k < NB; ++k) {
+      const double *sm = sm_of(s, k);
+      if (sm[4] <= 0.0) continue;
+      if (sm[5] < 0.0) return;  // a live column: no launch was skipped
+      fz = std::max(fz, (int)sm[5]);
+    }
+    skip_counted[s] = 1;
+    if (fz < 0 || jlast <= fz) return;
+    const double ne = in[s].op.ne, nskip = jlast - fz;
+    andr_prof_adjust(ctx, "kb_gemv_sym", -nskip * (6.0 * ne * ne * 4.0 + in[s].n * NB * 8.0),
+                     -nskip * 2.0 * 1.5 * 6.0 * ne * ne * nrhs);
+  };
   std::vector<int> cyc_list(nsys);
   for (int s = 0; s < nsys; ++s) cyc_list[s] = s;
   int rc = build(cyc_list, cyc);
@@ -1246,6 +1272,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     layout(cyc_list, mc);
     first_cycle = false;
     if ((rc = use_tiles(live))) return rc;
+    std::fill(skip_counted.begin(), skip_counted.end(), 0);
     hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
     hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
     const int steps_max = std::min(mc, maxit - total);
@@ -1277,6 +1304,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
           for (int k = 0; k < NB; ++k)
             if (sm_of(s, k)[4] > 0.0 && sm_of(s, k)[5] < 0.0) l = true;
           if (l) still.push_back(s);
+          else account_skips(s, j);
         }
         if (still.empty()) break;
         int nlive = 0;
@@ -1302,6 +1330,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, mc);
     ANDR_HIP(ctx, hipGetLastError());
     if ((rc = read_small())) return rc;
+    for (int s : cyc_list)
+      if (live.ntiles[s]) account_skips(s, steps - 1);
     for (int s : cyc_list)
       for (int k = 0; k < NB; ++k) {
         const double *sm = sm_of(s, k);

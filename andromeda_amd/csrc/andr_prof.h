@@ -28,3 +28,6 @@ ProfState &andr_prof(andr_ctx *ctx);
 int andr_prof_begin(andr_ctx *ctx, const char *name, double bytes, double flops);
 void andr_prof_end(andr_ctx *ctx, int token);
 void andr_prof_flush(andr_ctx *ctx);
+// corrects the totals of a kernel name by dbytes and dflops (work the launches skipped on the
+// device, known to the host only afterwards); no effect when profiling is off
+void andr_prof_adjust(andr_ctx *ctx, const char *name, double dbytes, double dflops);

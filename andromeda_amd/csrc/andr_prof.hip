@@ -62,6 +62,14 @@ void andr_prof_flush(andr_ctx *ctx) {
   P.pending.clear();
 }
 
+void andr_prof_adjust(andr_ctx *ctx, const char *name, double dbytes, double dflops) {
+  ProfState &P = andr_prof(ctx);
+  if (!P.on) return;
+  ProfAcc &A = P.acc[name];
+  A.bytes += dbytes;
+  A.flops += dflops;
+}
+
 extern "C" int andr_profile(andr_ctx *ctx, int32_t enable) {
   if (!ctx) return ANDR_ERR_ARG;
   andr_prof_flush(ctx);
