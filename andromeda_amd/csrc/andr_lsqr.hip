@@ -696,7 +696,7 @@ int andr_colscale(andr_ctx *ctx, void *A, bool fp32, int64_t lda, int m, int n, 
   const int ns = splits_for(m, n);
   const int rps = (m + ns - 1) / ns;
   DevBuf<double> P;
-  if (P.alloc((size_t)ns * n)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scaling scratch");
+  if (P.alloc((size_t)ns * n, ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "column scaling scratch");
   dim3 g1((n + 255) / 256, ns);
   if (fp32)
     hipLaunchKernelGGL(k_colsq<float>, g1, dim3(256), 0, ctx->stream, (const float *)A, lda, m, n, rps, P.p);
