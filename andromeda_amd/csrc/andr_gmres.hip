@@ -1275,6 +1275,14 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     std::fill(skip_counted.begin(), skip_counted.end(), 0);
     hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
     hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
+    {  // a cycle that starts with few rows (a small batch) takes the tuned column split at once
+      const int target = krylov_target(cyc_list, 0);
+      if (target < 0) return target;  // a failed timing run
+      if (target != MF_TARGET_WAVES) {
+        if ((rc = build(cyc_list, live, target))) return rc;
+        if ((rc = use_tiles(live))) return rc;
+      }
+    }
     const int steps_max = std::min(mc, maxit - total);
     int steps = 0;
     for (int j = 0; j < steps_max; ++j) {
@@ -1311,6 +1319,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         for (int s = 0; s < nsys; ++s) nlive += live.ntiles[s] ? 1 : 0;
         if ((int)still.size() < nlive) {
           const int target = krylov_target(still, j + 1);
+          if (target < 0) return target;
           if ((rc = build(still, live, target))) return rc;
           if ((rc = use_tiles(live))) return rc;
         }
