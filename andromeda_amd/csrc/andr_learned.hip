@@ -397,14 +397,22 @@ struct MlpTiles {     // A operands (W^T tiles) of one lane
 };
 
 // lane (g, i): A[m = 16 mt + i][k = 16 kt + 4 g + t] = W[in = 16 kt + 4 g + t][out = 16 mt + i]
-__device__ inline bf16x4 load_tile(const uint16_t *w, int nin, int nout, int mt, int kt, int g, int i) {
-  bf16x4 r;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int in = 16 * kt + 4 * g + t, out = 16 * mt + i;
-    r[t] = (in < nin && out < nout) ? (short)w[in * nout + out] : (short)0;
+// (0 outside the layer). The 40 tiles are packed on the host when the model is set, lane by lane in
+// register order: tile index CLS_T_* + ..., then lane, then t.
+constexpr int CLS_T_W1 = 0, CLS_T_W2 = 4, CLS_T_W3 = 20, CLS_T_W4 = 36, CLS_NTILES = 40;
+// the device model buffer: FP32 parameters, BF16 parameters, far table, rule sizes, then the packed
+// tiles (16-byte aligned)
+constexpr size_t model_tiles_off() {
+  return (NPARAM * (sizeof(uint16_t) + sizeof(float)) + ANDR_FAR_BINS * sizeof(int) + 8 * sizeof(int) + 15) / 16 * 16;
+}
+static void pack_tile(const uint16_t *w, int nin, int nout, int mt, int kt, uint16_t *out) {
+  for (int lane = 0; lane < 64; ++lane) {
+    const int g = lane >> 4, i = lane & 15;
+    for (int t = 0; t < 4; ++t) {
+      const int in = 16 * kt + 4 * g + t, o = 16 * mt + i;
+      out[lane * 4 + t] = (in < nin && o < nout) ? w[in * nout + o] : (uint16_t)0;
+    }
   }
-  return r;
 }
 
 // bias + ReLU + BF16 of an accumulator tile -> B operand of the next layer
@@ -492,7 +500,7 @@ __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float
 // one block per (job, row): rule class of every pair of the row
 __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restrict__ jobs,
                                                           const int2 *__restrict__ rowmap,
-                                                          const uint16_t *__restrict__ wbf, const float *__restrict__ wf,
+                                                          const bf16x4 *__restrict__ tiles, const float *__restrict__ wf,
                                                           const int *__restrict__ far_table,
                                                           double near_ratio,
                                                           float min_conf, unsigned long long *__restrict__ stats) {
@@ -512,15 +520,15 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   if (threadIdx.x < NOUT) sb[3 * NH + threadIdx.x] = wf[P_B4 + threadIdx.x];
   if (threadIdx.x < ANDR_FAR_BINS) sfar[threadIdx.x] = far_table[threadIdx.x];
   if (threadIdx.x < 3) red[threadIdx.x] = 0ull;
-  MlpTiles T;
+  MlpTiles T;  // the lane's operand tiles, packed in this order when the model was set (one 8-byte load each)
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    T.w1[mt] = load_tile(wbf + P_W1, NIN, NH, mt, 0, g, c);
-    T.w4[mt] = load_tile(wbf + P_W4, NH, NOUT, 0, mt, g, c);
+    T.w1[mt] = tiles[(CLS_T_W1 + mt) * 64 + lane];
+    T.w4[mt] = tiles[(CLS_T_W4 + mt) * 64 + lane];
 #pragma unroll
     for (int kt = 0; kt < 4; ++kt) {
-      T.w2[mt][kt] = load_tile(wbf + P_W2, NH, NH, mt, kt, g, c);
-      T.w3[mt][kt] = load_tile(wbf + P_W3, NH, NH, mt, kt, g, c);
+      T.w2[mt][kt] = tiles[(CLS_T_W2 + 4 * mt + kt) * 64 + lane];
+      T.w3[mt][kt] = tiles[(CLS_T_W3 + 4 * mt + kt) * 64 + lane];
     }
   }
   __syncthreads();
@@ -839,7 +847,18 @@ extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *params, flo
     u += 0x7fffu + ((u >> 16) & 1u);
     wb[i] = (uint16_t)(u >> 16);
   }
-  const size_t bytes = NPARAM * (sizeof(uint16_t) + sizeof(float)) + ANDR_FAR_BINS * sizeof(int) + 8 * sizeof(int);
+  // packed operand tiles of the classifier after the tables (16-byte aligned)
+  const size_t tiles_off = model_tiles_off();
+  const size_t bytes = tiles_off + (size_t)CLS_NTILES * 64 * 4 * sizeof(uint16_t);
+  std::vector<uint16_t> packed((size_t)CLS_NTILES * 64 * 4);
+  for (int mt = 0; mt < 4; ++mt) {
+    pack_tile(wb.data() + P_W1, NIN, NH, mt, 0, packed.data() + (size_t)(CLS_T_W1 + mt) * 256);
+    pack_tile(wb.data() + P_W4, NH, NOUT, 0, mt, packed.data() + (size_t)(CLS_T_W4 + mt) * 256);
+    for (int kt = 0; kt < 4; ++kt) {
+      pack_tile(wb.data() + P_W2, NH, NH, mt, kt, packed.data() + (size_t)(CLS_T_W2 + 4 * mt + kt) * 256);
+      pack_tile(wb.data() + P_W3, NH, NH, mt, kt, packed.data() + (size_t)(CLS_T_W3 + 4 * mt + kt) * 256);
+    }
+  }
   // the previous model is gone from here on: a learned call made after a failure below is refused
   ctx->has_model = false;
   if (ctx->d_model) (void)hipFree(ctx->d_model);
@@ -857,6 +876,7 @@ extern "C" int andr_set_quadrature_model(andr_ctx *ctx, const float *params, flo
   ANDR_HIP(ctx, hipMemcpy(bp, wb.data(), sizeof(uint16_t) * NPARAM, hipMemcpyHostToDevice));
   ANDR_HIP(ctx, hipMemcpy(far, far_table, sizeof(int) * ANDR_FAR_BINS, hipMemcpyHostToDevice));
   ANDR_HIP(ctx, hipMemcpy(npts, hn, sizeof(hn), hipMemcpyHostToDevice));
+  ANDR_HIP(ctx, hipMemcpy(base + tiles_off, packed.data(), sizeof(uint16_t) * packed.size(), hipMemcpyHostToDevice));
   ctx->has_model = true;
   ctx->model_conf = confidence;
   ctx->model_near = near_ratio;
@@ -868,13 +888,14 @@ int andr_launch_stokes_system_learned(andr_ctx *ctx, SysJob *h_jobs, const SysJo
   if (!ctx->has_model) return andr_fail(ctx, ANDR_ERR_STATE, "no quadrature model");
   char *base = (char *)ctx->d_model;
   const float *fp = (const float *)base;
-  const uint16_t *bp = (const uint16_t *)(base + NPARAM * sizeof(float));
   const int *far = (const int *)(base + NPARAM * (sizeof(float) + sizeof(uint16_t)));
+  const size_t tiles_off = model_tiles_off();
+  const bf16x4 *tiles = (const bf16x4 *)(base + tiles_off);
   DevBuf<unsigned long long> st;
   if (st.alloc(3 * (size_t)njobs)) return andr_fail(ctx, ANDR_ERR_ALLOC, "stats");
   ANDR_HIP(ctx, hipMemsetAsync(st.p, 0, 3 * sizeof(unsigned long long) * njobs, ctx->stream));
   const int tok = andr_prof_begin(ctx, "classify_pairs", 0.0, 0.0);
-  hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(CLS_THREADS), 0, ctx->stream, d_jobs, d_rowmap, bp, fp, far,
+  hipLaunchKernelGGL(k_classify, dim3(nrows), dim3(CLS_THREADS), 0, ctx->stream, d_jobs, d_rowmap, tiles, fp, far,
                      ctx->model_near, ctx->model_conf, st.p);
   andr_prof_end(ctx, tok);
   ANDR_HIP(ctx, hipGetLastError());
