@@ -810,7 +810,7 @@ __global__ __launch_bounds__(64) void kb_solve_y(const GSys *__restrict__ sys, i
   }
 }
 
-// per tile: X += sum_j Q_j y_j
+// per tile: X += sum_j Q_j y_j over the steps each column used (y_j = 0 beyond them, as in kb_solve_y)
 __global__ __launch_bounds__(256) void kb_xupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int steps,
                                                   int m) {
   const TileMap tm = tmap[blockIdx.x];
@@ -818,12 +818,15 @@ __global__ __launch_bounds__(256) void kb_xupdate(const GSys *__restrict__ sys, 
   const int n = S.n;
   const int k = threadIdx.x & 7, rr = threadIdx.x >> 3;
   const double *y = S.cw + (size_t)k * cw_stride(m);
+  const double *sm = S.small + k * 8;
+  int kk = sm[5] > 0.0 ? min((int)sm[5], steps) : steps;
+  if (sm[4] <= 0.0) kk = 0;
 #pragma unroll
   for (int t = 0; t < TILE / 32; ++t) {
     const int row = tm.row0 + rr + 32 * t;
     if (row >= n) continue;
     double x = S.X[(size_t)row * NB + k];
-    for (int i = 0; i < steps; ++i) x += S.Q[((size_t)i * n + row) * NB + k] * y[i];
+    for (int i = 0; i < kk; ++i) x += S.Q[((size_t)i * n + row) * NB + k] * y[i];
     S.X[(size_t)row * NB + k] = x;
   }
 }
