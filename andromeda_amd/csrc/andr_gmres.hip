@@ -52,9 +52,8 @@ struct GSys {  // device view of one system
   // structured Stokes operator (fp32 path)
   StokesOp op;
   int nep;       // op.ne rounded up to 32
-  double *ZT;    // 3 x NB x nep: z = M^-1 q_j, [component][rhs][element] (Krylov product operand)
+  double *ZT;    // 3 x NB x nep: z = M^-1 q_j (FP64 products) or M^-1 x (residual), [component][rhs][element]
   float *ZT32;   // the same in FP32 (FP32 products), null: FP64 products
-  double *Z64;   // 3 ne x NB: z = M^-1 x in FP64 (residual)
   double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
   int et0, net;  // element tiles (32 elements) of the system in the current map
   int incyc;     // 1 while the system takes part in the current restart cycle
@@ -76,7 +75,7 @@ constexpr int MAXW = ANDR_OP_MAXW;
 
 // ---- structured Stokes operator --------------------------------------------------------------
 // per tile of 32 elements (256 threads = 32 elements x 8 right-hand sides): z_e = M^-1_e v_e for the
-// three components of element e, v = Q_j (mode 0, z to ZT) or v = X (mode 1, z to Z64 in
+// three components of element e, v = Q_j (mode 0, z to ZT or ZT32) or v = X (mode 1, z to ZT in
 // FP64); partial sums of n_e . z_e per wall (1..MAXW) and rhs for the null-space term.
 __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, const TileMap *__restrict__ emap, int mode,
                                                int jq, double *__restrict__ part_s) {
@@ -103,16 +102,16 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
     } else if (mode == 0) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = z[c];
-    } else {
+    } else {  // the residual's operand, FP64 in the product's layout
 #pragma unroll
-      for (int c = 0; c < 3; ++c) S.Z64[(size_t)(c * ne + e) * NB + k] = z[c];
+      for (int c = 0; c < 3; ++c) S.ZT[((size_t)c * NB + k) * S.nep + e] = z[c];
     }
     w = O.bcid[e];
     dz = O.nrm[3 * e] * z[0] + O.nrm[3 * e + 1] * z[1] + O.nrm[3 * e + 2] * z[2];
-  } else if (mode == 0 && e < S.nep) {
+  } else if (e < S.nep) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (S.ZT32) S.ZT32[((size_t)c * NB + k) * S.nep + e] = 0.f;
+      if (mode == 0 && S.ZT32) S.ZT32[((size_t)c * NB + k) * S.nep + e] = 0.f;
       else S.ZT[((size_t)c * NB + k) * S.nep + e] = 0.0;
     }
   }
@@ -180,7 +179,7 @@ __device__ inline double defl_term(const StokesOp &O, const double *sdef, int r,
 // D[g + 4 v][c] in register v of lane (g, c), the FP32 form D[4 g + v][c].
 template <bool F32>
 __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ sys, const MWave *__restrict__ wmap,
-                                                   int nwaves) {
+                                                   int nwaves, int all) {
   using ZT_t = typename std::conditional<F32, float, double>::type;  // z operand
   typedef float f4n __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63;
@@ -196,7 +195,8 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
   // step) needs no more products until the host drops it from the map every few steps: its waves
   // skip the matrix and write the null-space term alone (the later steps of such a column are
   // never used, and stay finite).
-  bool need = false;
+  // (all: every system's product, as the residual of the restart test needs)
+  bool need = all != 0;
 #pragma unroll
   for (int k = 0; k < NB; ++k) need |= S.small[8 * k + 4] > 0.0 && S.small[8 * k + 5] < 0.0;
   const float *ab[2];
@@ -366,82 +366,31 @@ __global__ __launch_bounds__(256, 2) void kb_gemv_sym(const GSys *__restrict__ s
     }
 }
 
-// R = B - A M^-1 X for the structured operator in FP64 (the residual of the refinement loop), from
-// z = M^-1 X already in Z64. Block of 4 waves per 64 element rows of one system (one 16-row band per
-// wave); the 16 x 3 x 8 values of z for the current column block are staged in LDS and shared.
-// Lane (g, c) covers row c of its band and columns 4 g..4 g+3; the column groups are summed across
-// lanes at the end. part[4 block + wave][k] = sum over the band's rows of R^2 (rows of all three
-// components).
-__global__ __launch_bounds__(256) void kb_resid_sym(const GSys *__restrict__ sys, const WaveMap *__restrict__ bmap,
+// R = B - A M^-1 X, the residual of the refinement loop: W = A M^-1 X (+ the null-space term) comes
+// from the product on the FP64 matrix cores (kb_gemv_sym<false> on z = M^-1 X in FP64, every
+// system); R holds B. Block of 4 waves per 64 element rows of one system (one 16-row band per wave);
+// lane (g, c) takes row c of the band, component g (g < 3), all right-hand sides.
+// part[4 block + wave][k] = sum over the band's rows of R^2 (rows of all three components).
+__global__ __launch_bounds__(256) void kb_resid_fin(const GSys *__restrict__ sys, const WaveMap *__restrict__ bmap,
                                                     double *__restrict__ part) {
-  __shared__ double zc[3][16][NB];
   const WaveMap bm = bmap[blockIdx.x];
   const GSys &S = sys[bm.sys];
-  const StokesOp &O = S.op;
-  const int ne = O.ne, KBe = (ne + 15) / 16;
+  const int ne = S.op.ne;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
   const int band = (bm.row0 >> 4) + wid;
-  const float *ab = O.S + (size_t)band * KBe * 1536 + 16 * c + 4 * g;
-  double acc[3][NB];
-#pragma unroll
-  for (int co = 0; co < 3; ++co)
-#pragma unroll
-    for (int k = 0; k < NB; ++k) acc[co][k] = 0.0;
-  const bool band_in = 16 * band < ne;  // all waves take part in the staging
-  for (int kb = 0; kb < KBe; ++kb) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < 3 * 16 * NB; i += blockDim.x) {
-      const int en = i / (16 * NB), rem = i % (16 * NB), col = rem / NB, k = rem % NB;
-      const int e = 16 * kb + col;
-      zc[en][col][k] = e < ne ? S.Z64[(size_t)(en * ne + e) * NB + k] : 0.0;
-    }
-    __syncthreads();
-    if (!band_in) continue;
-    double a[6][4];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const float4 v = *reinterpret_cast<const float4 *>(ab + (size_t)kb * 1536 + q * 256);
-      const int c0 = 16 * kb + 4 * g;
-      a[q][0] = c0 < ne ? (double)v.x : 0.0;
-      a[q][1] = c0 + 1 < ne ? (double)v.y : 0.0;
-      a[q][2] = c0 + 2 < ne ? (double)v.z : 0.0;
-      a[q][3] = c0 + 3 < ne ? (double)v.w : 0.0;
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int col = 4 * g + t;
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        const double z0 = zc[0][col][k], z1 = zc[1][col][k], z2 = zc[2][col][k];
-        acc[0][k] += a[0][t] * z0 + a[1][t] * z1 + a[2][t] * z2;
-        acc[1][k] += a[1][t] * z0 + a[3][t] * z1 + a[4][t] * z2;
-        acc[2][k] += a[2][t] * z0 + a[4][t] * z1 + a[5][t] * z2;
-      }
-    }
-  }
-#pragma unroll
-  for (int co = 0; co < 3; ++co)
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      acc[co][k] += __shfl_xor(acc[co][k], 16, 64);
-      acc[co][k] += __shfl_xor(acc[co][k], 32, 64);
-    }
+  const int r = 16 * band + c;
   double r2[NB];
 #pragma unroll
   for (int k = 0; k < NB; ++k) r2[k] = 0.0;
-  const int r = 16 * band + c;
-  if (band_in && g == 0 && r < ne) {
+  if (g < 3 && r < ne) {
+    const size_t o = (size_t)(g * ne + r) * NB;
 #pragma unroll
-    for (int co = 0; co < 3; ++co)
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        const double y = acc[co][k] + defl_term(O, S.sdef, r, co, k);
-        double *pr = S.R + (size_t)(co * ne + r) * NB + k;
-        const double v = *pr - y;  // R holds B
-        *pr = v;
-        r2[k] += v * v;
-      }
+    for (int k = 0; k < NB; ++k) {
+      const double v = S.R[o + k] - S.W[o + k];  // R holds B
+      S.R[o + k] = v;
+      r2[k] = v * v;
+    }
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k)
@@ -455,7 +404,7 @@ __global__ __launch_bounds__(256) void kb_resid_sym(const GSys *__restrict__ sys
 }
 
 // The residual of the first cycle, x = 0: R = B (already copied) and the same partial sums of R^2
-// as kb_resid_sym, over the same bands, without the product (A M^-1 0 contributes exact zeros).
+// as kb_resid_fin, over the same bands, without the product (A M^-1 0 contributes exact zeros).
 __global__ __launch_bounds__(256) void kb_resid_zero(const GSys *__restrict__ sys, const WaveMap *__restrict__ bmap,
                                                      double *__restrict__ part) {
   const WaveMap bm = bmap[blockIdx.x];
@@ -933,7 +882,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   };
   const size_t small_per = small_of(m);
   const size_t state_per = (size_t)NB * 8;
-  DevBuf<double> R, W, Q, SM, ST8, partW, partT, Z64, SDEF, partS;
+  DevBuf<double> R, W, Q, SM, ST8, partW, partT, SDEF, partS;
   DevBuf<double> ZT;
   DevBuf<float> ZT32;
   DevBuf<GSys> dsys;
@@ -947,7 +896,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // relative error stays below the tolerance); FP64 products for tighter tolerances.
   // ANDR_GMRES_FP64_PRODUCTS=1 keeps FP64 products at every tolerance.
   const bool f32prod = tol >= 1e-6 && getenv("ANDR_GMRES_FP64_PRODUCTS") == nullptr;
-  if ((Z64.alloc(vec_elems, ctx) || (f32prod ? ZT32.alloc(zt_elems, ctx) : ZT.alloc(zt_elems, ctx)) ||
+  if ((ZT.alloc(zt_elems, ctx) || (f32prod && ZT32.alloc(zt_elems, ctx)) ||
                      SDEF.alloc((size_t)nsys * MAXW * NB, ctx) || partS.alloc((size_t)net_full * MAXW * NB, ctx)))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES operator workspace");
   // a map set over a list of systems: residual blocks of 64 element rows, tiles of
@@ -1054,9 +1003,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     const int n = in[s].n;
     g.op = in[s].op;
     g.nep = (g.op.ne + 31) / 32 * 32;
-    g.ZT = f32prod ? nullptr : ZT.p + zo;
+    g.ZT = ZT.p + zo;
     g.ZT32 = f32prod ? ZT32.p + zo : nullptr;
-    g.Z64 = Z64.p + vo;
     g.sdef = SDEF.p + (size_t)s * MAXW * NB;
     zo += (size_t)3 * NB * g.nep;
     g.A = in[s].A;
@@ -1118,22 +1066,19 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemsetAsync(SM.p, 0, sizeof(double) * small_per * nsys, st));
   ANDR_HIP(ctx, hipMemsetAsync(ST8.p, 0, sizeof(double) * state_per * nsys, st));
   for (int s = 0; s < nsys; ++s) ANDR_HIP(ctx, hipMemsetAsync(in[s].X, 0, sizeof(double) * in[s].n * NB, st));
-  // scale factors: residual product (sx = -1, sy = 1) and Arnoldi product (sx = 1, sy = 0)
-  std::vector<double> h_res(2 * NB * (size_t)nsys), h_arn(2 * NB * (size_t)nsys);
+  // scale factors of the segment sums (sx = 1, sy = 0): W = A z for the Krylov and residual products
+  std::vector<double> h_arn(2 * NB * (size_t)nsys);
   for (int s = 0; s < nsys; ++s)
     for (int k = 0; k < NB; ++k) {
-      h_res[s * 2 * NB + k] = -1.0;
-      h_res[s * 2 * NB + NB + k] = 1.0;
       h_arn[s * 2 * NB + k] = 1.0;
       h_arn[s * 2 * NB + NB + k] = 0.0;
     }
-  DevBuf<double> d_res, d_arn;
-  if (d_res.alloc(h_res.size(), ctx) || d_arn.alloc(h_arn.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES scales");
-  ANDR_HIP(ctx, hipMemcpyAsync(d_res.p, h_res.data(), sizeof(double) * h_res.size(), hipMemcpyHostToDevice, st));
+  DevBuf<double> d_arn;
+  if (d_arn.alloc(h_arn.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES scales");
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
   const int gs = (nsys * NB + 63) / 64;
-  // which_x 0: residual R = B - A X (R holds B); 1: Krylov product W = A Q_j
+  // which_x 0: residual R = B - A M^-1 X (R holds B); 1: Krylov product W = A M^-1 Q_j
   auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales, bool prof = true) {
     if (ms.n_w == 0) return;
     double n2 = 0.0, n1 = 0.0;
@@ -1144,20 +1089,19 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         n1 += in[s].n;
       }
     const bool krylov = which_x == 1;
-    const int tok = prof ? andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "kb_resid_sym", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
+    // the residual (FP64 products of every system, then R = B - W) or the Krylov product
+    const int tok = prof ? andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "gmres_residual", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
                                            2.0 * 1.5 * n2 * nrhs)
                          : -1;
     hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
     hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
-    if (krylov) {
-      if (f32prod)
-        hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
-      else
-        hipLaunchKernelGGL(kb_gemv_sym<false>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw);
-      if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
-    } else {
-      hipLaunchKernelGGL(kb_resid_sym, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
-    }
+    if (krylov && f32prod)
+      hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw, 0);
+    else
+      hipLaunchKernelGGL(kb_gemv_sym<false>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw,
+                         krylov ? 0 : 1);
+    if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
+    if (!krylov) hipLaunchKernelGGL(kb_resid_fin, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
     andr_prof_end(ctx, tok);
   };
   // Autotuner of the product's column segments: when few rows are live (the slowest bodies of a
@@ -1243,7 +1187,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     if (first) {
       if (cyc.n_w) hipLaunchKernelGGL(kb_resid_zero, dim3(cyc.n_w), dim3(256), 0, st, dsys.p, cyc.w.p, partW.p);
     } else {
-      gemv(cyc, 0, 0, d_res.p);
+      gemv(cyc, 0, 0, d_arn.p);
     }
     ANDR_HIP(ctx, hipMemsetAsync(ddone.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(kb_restart, dim3(gs), dim3(64), 0, st, dsys.p, nsys, partW.p, cyc.w0.p, cyc.nw.p, tol,
