@@ -28,7 +28,7 @@ namespace {
 constexpr int NB = ANDR_NB;
 static_assert(NB == 8, "the matrix-core product packs two 8-column halves into the 16 B columns");
 constexpr int TILE = 64;   // rows per vector tile (256 threads: 32 rows x 8 right-hand sides per pass)
-constexpr int ICH = 16;   // basis vectors per block of the projection kernel
+constexpr int ICH = 8;   // basis vectors per block of the projection kernel
 
 struct GSys {  // device view of one system
   const void *A;
