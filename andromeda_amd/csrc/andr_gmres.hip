@@ -1078,6 +1078,9 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
   const int gs = (nsys * NB + 63) / 64;
+  // ANDR_GMRES_NO_SKIP=1: converged systems keep their products until the next map rebuild (the
+  // steps after convergence are never used, so the results are the same bit for bit)
+  const int noskip = getenv("ANDR_GMRES_NO_SKIP") ? 1 : 0;
   // which_x 0: residual R = B - A M^-1 X (R holds B); 1: Krylov product W = A M^-1 Q_j
   auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales, bool prof = true) {
     if (ms.n_w == 0) return;
@@ -1096,10 +1099,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
     hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
     if (krylov && f32prod)
-      hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw, 0);
+      hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw,
+                         noskip);
     else
       hipLaunchKernelGGL(kb_gemv_sym<false>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw,
-                         krylov ? 0 : 1);
+                         krylov ? noskip : 1);
     if (ms.max_seg > 1) hipLaunchKernelGGL(kb_seg_sum, dim3(ms.n_t), dim3(256), 0, st, dsys.p, ms.t.p, scales);
     if (!krylov) hipLaunchKernelGGL(kb_resid_fin, dim3(ms.n_w), dim3(256), 0, st, dsys.p, ms.w.p, partW.p);
     andr_prof_end(ctx, tok);
@@ -1159,7 +1163,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // matrix reads out of the profiled totals (a system once, when it leaves the live map).
   std::vector<char> skip_counted(nsys, 0);
   auto account_skips = [&](int s, int jlast) {
-    if (skip_counted[s] || !andr_prof(ctx).on) return;
+    if (noskip || skip_counted[s] || !andr_prof(ctx).on) return;
     int fz = -1;
     for (int k = 0; k < NB; ++k) {
       const double *sm = sm_of(s, k);

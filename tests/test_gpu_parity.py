@@ -177,6 +177,26 @@ def test_krylov_product_precisions(ctx, monkeypatch):
         assert t.nit.max() > b.nit.max()
 
 
+def test_converged_system_skip_is_exact(ctx, monkeypatch):
+    """The Krylov product skips a system once every column has converged in the cycle (until the
+    host's next map rebuild). The steps after a column's convergence are never used, so tensors and
+    step counts with and without the skip are the same bit for bit (autotuner off: fixed splits)."""
+    allb = [m for m, _ in M.batch_bodies(512, 64)]
+    bodies = [allb[i] for i in (29, 111, 222, 370, 519, 534)]  # superellipsoids and fibres, 500..5120
+    ctx.autotune(False)
+    try:
+        a = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+        monkeypatch.setenv("ANDR_GMRES_NO_SKIP", "1")
+        b = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
+        monkeypatch.delenv("ANDR_GMRES_NO_SKIP")
+    finally:
+        ctx.autotune(True)
+    for x, y in zip(a, b):
+        assert x.status == 0 and y.status == 0
+        assert np.array_equal(x.nit, y.nit)
+        assert np.array_equal(x.FT, y.FT)
+
+
 def test_krylov_product_autotuner(ctx):
     """A slow fibre next to a fast body: once the fast body is done the fibre alone is live, its
     columns are split into segments, and the autotuner times the candidate splits for that bucket
