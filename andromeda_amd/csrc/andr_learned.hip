@@ -389,16 +389,9 @@ __device__ inline short f2bf(float x) {  // round to nearest even (v_cvt_pk_bf16
 
 constexpr int CLS_THREADS = 256, CLS_WAVES = CLS_THREADS / 64, CLS_Q = 128;
 
-struct MlpTiles {     // A operands (W^T tiles) of one lane
-  bf16x4 w1[4];       // layer 1: 4 row blocks x 1 K block (6 inputs padded to 16)
-  bf16x4 w2[4][4];    // layers 2, 3: 4 x 4
-  bf16x4 w3[4][4];
-  bf16x4 w4[4];       // layer 4: 1 row block (8 classes padded to 16) x 4 K blocks
-};
-
 // lane (g, i): A[m = 16 mt + i][k = 16 kt + 4 g + t] = W[in = 16 kt + 4 g + t][out = 16 mt + i]
 // (0 outside the layer). The 40 tiles are packed on the host when the model is set, lane by lane in
-// register order: tile index CLS_T_* + ..., then lane, then t.
+// operand order: tile index CLS_T_* + ..., then lane, then t (the A operand of one lane).
 constexpr int CLS_T_W1 = 0, CLS_T_W2 = 4, CLS_T_W3 = 20, CLS_T_W4 = 36, CLS_NTILES = 40;
 // the device model buffer: FP32 parameters, BF16 parameters, far table, rule sizes, then the packed
 // tiles (16-byte aligned)
@@ -425,8 +418,9 @@ __device__ inline bf16x4 act_tile(f32x4 c, const float *bias, int mt, int g) {
 
 // logits of 16 pairs (column block) for one kernel id; returns in lanes g == 0 the class and the
 // rule class of pair c (quantile decision, see below)
-__device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float *feat /* [16][3] */, int kernel,
-                                 int g, int c, float min_conf, int *cls, float *conf) {
+__device__ inline void mlp_block(const bf16x4 *lt /* this lane's entry of tile 0, in LDS */, const float *sb,
+                                 const float *feat /* [16][3] */, int kernel, int g, int c, float min_conf, int *cls,
+                                 float *conf) {
   bf16x4 x;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -440,26 +434,26 @@ __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w1[mt], x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lt[(CLS_T_W1 + mt) * 64], x, acc, 0, 0, 0);
     h1[mt] = act_tile(acc, sb + 0, mt, g);
   }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w2[mt][kt], h1[kt], acc, 0, 0, 0);
+    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lt[(CLS_T_W2 + 4 * mt + kt) * 64], h1[kt], acc, 0, 0, 0);
     h2[mt] = act_tile(acc, sb + NH, mt, g);
   }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w3[mt][kt], h2[kt], acc, 0, 0, 0);
+    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lt[(CLS_T_W3 + 4 * mt + kt) * 64], h2[kt], acc, 0, 0, 0);
     h1[mt] = act_tile(acc, sb + 2 * NH, mt, g);
   }
   f32x4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kt = 0; kt < 4; ++kt) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T.w4[kt], h1[kt], z, 0, 0, 0);
+  for (int kt = 0; kt < 4; ++kt) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lt[(CLS_T_W4 + kt) * 64], h1[kt], z, 0, 0, 0);
   // classes 4g + v; lanes g = 0 hold classes 0..3 of pair c, lanes g = 1 classes 4..7
   float zz[8];
 #pragma unroll
@@ -498,7 +492,7 @@ __device__ inline void mlp_block(const MlpTiles &T, const float *sb, const float
 }
 
 // one block per (job, row): rule class of every pair of the row
-__global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restrict__ jobs,
+__global__ __launch_bounds__(CLS_THREADS, 3) void k_classify(const SysJob *__restrict__ jobs,
                                                           const int2 *__restrict__ rowmap,
                                                           const bf16x4 *__restrict__ tiles, const float *__restrict__ wf,
                                                           const int *__restrict__ far_table,
@@ -510,6 +504,7 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   __shared__ float qf[CLS_WAVES][64][3];              // three geometric features of a batch
   __shared__ int qres[CLS_WAVES][64][2];              // class per kernel of a processed batch
   __shared__ unsigned long long red[3];
+  __shared__ bf16x4 s_tiles[CLS_NTILES * 64];  // the network's operand tiles, shared by the block's waves
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
   for (int i = threadIdx.x; i < NH; i += blockDim.x) {
@@ -520,17 +515,10 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
   if (threadIdx.x < NOUT) sb[3 * NH + threadIdx.x] = wf[P_B4 + threadIdx.x];
   if (threadIdx.x < ANDR_FAR_BINS) sfar[threadIdx.x] = far_table[threadIdx.x];
   if (threadIdx.x < 3) red[threadIdx.x] = 0ull;
-  MlpTiles T;  // the lane's operand tiles, packed in this order when the model was set (one 8-byte load each)
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    T.w1[mt] = tiles[(CLS_T_W1 + mt) * 64 + lane];
-    T.w4[mt] = tiles[(CLS_T_W4 + mt) * 64 + lane];
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
-      T.w2[mt][kt] = tiles[(CLS_T_W2 + 4 * mt + kt) * 64 + lane];
-      T.w3[mt][kt] = tiles[(CLS_T_W3 + 4 * mt + kt) * 64 + lane];
-    }
-  }
+  // the operand tiles (packed lane by lane when the model was set) into LDS: the four waves use the
+  // same tiles, and registers stay free for occupancy
+  for (int i = threadIdx.x; i < CLS_NTILES * 64; i += blockDim.x) s_tiles[i] = tiles[i];
+  const bf16x4 *lt = s_tiles + lane;
   __syncthreads();
   const int2 jr = rowmap[blockIdx.x];
   const SysJob &J = jobs[jr.x];
@@ -572,8 +560,8 @@ __global__ __launch_bounds__(CLS_THREADS) void k_classify(const SysJob *__restri
       const float *feat = &qf[wid][16 * blk][0];
       int a, b;
       float ca, cb;  // whether the quantile was reached below the last class (unused here)
-      mlp_block(T, sb, feat, 1, g, c, min_conf, &a, &ca);
-      mlp_block(T, sb, feat, 2, g, c, min_conf, &b, &cb);
+      mlp_block(lt, sb, feat, 1, g, c, min_conf, &a, &ca);
+      mlp_block(lt, sb, feat, 2, g, c, min_conf, &b, &cb);
       if (g == 0) {
         qres[wid][16 * blk + c][0] = a;
         qres[wid][16 * blk + c][1] = b;
