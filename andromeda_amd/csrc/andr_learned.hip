@@ -524,12 +524,10 @@ __global__ __launch_bounds__(CLS_THREADS, 3) void k_classify(const SysJob *__res
   const SysJob &J = jobs[jr.x];
   const BodyGeom &g_ = J.g;
   const int r = jr.y, ne = g_.ne;
-  const double s[3] = {g_.cen[3 * r], g_.cen[3 * r + 1], g_.cen[3 * r + 2]};
-  const double ns[3] = {g_.nrm[3 * r], g_.nrm[3 * r + 1], g_.nrm[3 * r + 2]};
   uint8_t *crow = J.cls + (size_t)r * ne;
   const float inv_nr2 = (float)(1.0 / (near_ratio * near_ratio));
   const float4 sr = g_.cenl[r];
-  unsigned long long pts = 0, nl = 0, nf = 0;
+  unsigned pts = 0, nl = 0, nf = 0;  // per lane and row: at most 8000 x 126 points
   int qn = 0;  // queue length (wave-uniform)
   // classify the first `cnt` queued pairs (all lanes take part: the matrix cores need a full wave)
   // rule points of a class: Dunavant sizes {3,4,6,7,12,13,19} (6 bits each), the fixed rule's J.nq
@@ -545,6 +543,9 @@ __global__ __launch_bounds__(CLS_THREADS, 3) void k_classify(const SysJob *__res
                      *x3 = g_.xyz + 3 * g_.con[3 * e + 2];
         const double n[3] = {g_.nrm[3 * e], g_.nrm[3 * e + 1], g_.nrm[3 * e + 2]};
         float f[NIN];
+        // the row's centroid and normal, read here (not held in registers through the scan)
+        const double s[3] = {g_.cen[3 * r], g_.cen[3 * r + 1], g_.cen[3 * r + 2]};
+        const double ns[3] = {g_.nrm[3 * r], g_.nrm[3 * r + 1], g_.nrm[3 * r + 2]};
         features(pair_geo(s, ns, x1, x2, x3, n, g_.area[e]), 0, f);
         f0 = f[0];
         f1 = f[1];
@@ -628,9 +629,9 @@ __global__ __launch_bounds__(CLS_THREADS, 3) void k_classify(const SysJob *__res
       break;
     }
   }
-  atomicAdd(&red[0], pts);
-  atomicAdd(&red[1], nl);
-  atomicAdd(&red[2], nf);
+  atomicAdd(&red[0], (unsigned long long)pts);
+  atomicAdd(&red[1], (unsigned long long)nl);
+  atomicAdd(&red[2], (unsigned long long)nf);
   __syncthreads();
   if (threadIdx.x < 3) atomicAdd(stats + 3 * (size_t)jr.x + threadIdx.x, red[threadIdx.x]);
 }
