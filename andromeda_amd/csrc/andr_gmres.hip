@@ -29,6 +29,7 @@ constexpr int NB = ANDR_NB;
 static_assert(NB == 8, "the matrix-core product packs two 8-column halves into the 16 B columns");
 constexpr int TILE = 64;   // rows per vector tile (256 threads: 32 rows x 8 right-hand sides per pass)
 constexpr int ICH = 8;   // basis vectors per block of the projection kernel
+constexpr int BT = 256;  // rows per tile of the block Gram-Schmidt sweeps
 
 struct GSys {  // device view of one system
   const void *A;
@@ -57,6 +58,14 @@ struct GSys {  // device view of one system
   double *sdef;  // ANDR_OP_MAXW x NB: sum over wall w of n_e . z_e
   int et0, net;  // element tiles (32 elements) of the system in the current map
   int incyc;     // 1 while the system takes part in the current restart cycle
+  // block GMRES (one Krylov space for all right-hand sides of the system; see kb_bsmall)
+  int t40, nt4;  // tiles of BT rows of the system in the current map
+  double *Rp;    // packed upper triangle of the rotated block Hessenberg: column q at q (q + 1) / 2
+  double *hv;    // Householder reflector of column q: tau, v_1 .. v_L at 8 q
+  double *gB;    // rotated right-hand sides: L (m + 1) rows x NB
+  double *C1;    // first-sweep coefficients: 8 x 8 per basis block ([l][k]); the solution blocks Y at the end
+  double *C2;    // second-sweep coefficients, then (block j + 1) the Gram matrix of the swept vector
+  double *Ri;    // 8 x 8: inverse of the new subdiagonal block (normalises the next basis block)
 };
 
 struct WaveMap {
@@ -851,6 +860,380 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
   }
 }
 
+// ---- Block GMRES ------------------------------------------------------------------------------
+// One Krylov space for the L right-hand sides of a system: round j multiplies the basis block Q_j
+// (n x L, orthonormal columns; a column that takes no part is zero) and orthogonalises W = A M^-1 Q_j
+// against Q_0 .. Q_j by block classical Gram-Schmidt applied twice:
+//   C1 = Q^T W, W1 = W - Q C1;  C2 = Q^T W1, G1 = W1^T W1,
+//   W2 = W1 - Q C2 = Q_(j+1) R_(j+1) with R_(j+1) the Cholesky factor of W2^T W2 = G1 - C2^T C2,
+// so that A M^-1 Q_j = Q C + Q_(j+1) R_(j+1) (C = C1 + C2): column block j of the block Hessenberg
+// matrix. Its least-squares problem min |E - H Y| (E = the Cholesky factor of the initial residual
+// block in the first L rows) is reduced by one Householder reflector per column acting on L + 1
+// rows (the Hessenberg matrix has L subdiagonals); the rows L (j + 1) .. L (j + 2) - 1 of the
+// rotated right-hand sides are the residual of each column. A column whose product has no new
+// direction left (pivot below 1e-20 of its squared norm) ends the cycle: the restart test then takes
+// the true residual. Dead columns (converged, or beyond nrhs) stay exact zeros throughout.
+//
+// Dot sweep: per (tile of BT rows, 8 blocks) partials of C_b = Q_b^T V, b < nb, and with gram V^T V
+// as block nb. Wave w takes blocks 8 y + 2 w and 8 y + 2 w + 1 as the two 8-row halves of the FP64
+// matrix-core A operand (lane (g, i): A[i][g] = Q_(b(i))[row + g][i & 7], B[g][i] = V[row + g][i & 7]);
+// register v of lane (g, i < 8) ends as C_b[l][i] with b, l from row g + 4 v of the product.
+__global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap,
+                                                int which, int nb, int gram, double *__restrict__ part, int pstride) {
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n, nbt = nb + (gram ? 1 : 0);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int g = lane >> 4, i = lane & 15, l = i & 7;
+  const int bw = blockIdx.y * 8 + 2 * wid;  // the wave's first block
+  if (bw >= nbt) return;                     // uniform over the wave; no barriers in this kernel
+  const double *V = which ? S.R : S.W;
+  const int b = bw + (i >> 3);
+  const double *Ab = b < nb ? S.Q + (size_t)b * n * NB : (b == nb && gram ? V : nullptr);
+  d4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+  for (int r = tm.row0; r < tm.row0 + BT; r += 16) {
+    double a[4], bv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int row = r + 4 * t + g;
+      const bool in = row < n;
+      a[t] = (in && Ab) ? Ab[(size_t)row * NB + l] : 0.0;
+      bv[t] = in ? V[(size_t)row * NB + l] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bv[t], acc, 0, 0, 0);
+  }
+  if (i < 8) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int mrow = g + 4 * v, blk = bw + (mrow >> 3);
+      if (blk < nbt) part[((size_t)blockIdx.x * pstride + blk) * 64 + (mrow & 7) * 8 + i] = acc[v];
+    }
+  }
+}
+
+// block per (system, basis block b < nbt): C (dst 0: C1, 1: C2) block b = sum of the system's tile
+// partials (the four waves take every fourth tile). Systems outside the map have no tiles.
+__global__ __launch_bounds__(256) void kb_bdots_final(const GSys *__restrict__ sys, const double *__restrict__ part,
+                                                      int pstride, int dst) {
+  __shared__ double red[4][64];
+  const GSys &S = sys[blockIdx.x];
+  if (S.nt4 == 0) return;  // uniform over the block
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, b = blockIdx.y;
+  double c = 0.0;
+  for (int tl = wv; tl < S.nt4; tl += 4) c += part[((size_t)(S.t40 + tl) * pstride + b) * 64 + lane];
+  red[wv][lane] = c;
+  __syncthreads();
+  if (wv == 0) (dst ? S.C2 : S.C1)[(size_t)b * 64 + lane] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+// thread per row, all NB columns (16-byte loads; the coefficients are uniform over the block):
+//   mode 0: W -= sum_(b < nb) Q_b C1_b
+//   mode 1: Q_nb = (W - sum_(b < nb) Q_b C2_b) Ri          (the next basis block)
+//   mode 2: X += sum_(b < kk) Q_b Y_b, Y in C1, kk the rounds the system used (at most nb)
+//   mode 3: Q_0 = R Ri                                      (start of a cycle)
+__global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int nb,
+                                                  int mode) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const TileMap tm = tmap[blockIdx.x];
+  const GSys &S = sys[tm.sys];
+  const int n = S.n;
+  const int row = tm.row0 + threadIdx.x;
+  if (row >= n) return;  // no barriers in this kernel
+  int nbb = mode == 3 ? 0 : nb;
+  if (mode == 2) {  // the rounds used: the live columns freeze together (none live: no update)
+    int kk = -1;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const double *sm = S.small + 8 * k;
+      if (sm[4] > 0.0) kk = max(kk, sm[5] > 0.0 ? min((int)sm[5], nb) : nb);
+    }
+    nbb = max(kk, 0);
+  }
+  const double *Cf = mode == 1 ? S.C2 : S.C1;
+  const double *src = mode == 3 ? S.R : (mode == 2 ? S.X : S.W);
+  d2 x[4], s[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    x[p] = ((const d2 *)(src + (size_t)row * NB))[p];
+    s[p] = d2{0.0, 0.0};
+  }
+#pragma unroll 2
+  for (int b = 0; b < nbb; ++b) {
+    const d2 *q = (const d2 *)(S.Q + ((size_t)b * n + row) * NB);
+    d2 qv[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) qv[p] = q[p];
+    const double *C = Cf + (size_t)b * 64;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const double ql = (l & 1) ? qv[l >> 1].y : qv[l >> 1].x;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) s[p] += ql * d2{C[l * 8 + 2 * p], C[l * 8 + 2 * p + 1]};
+    }
+  }
+  if (mode == 0 || mode == 2) {
+    d2 *dst = (d2 *)((mode == 0 ? S.W : S.X) + (size_t)row * NB);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) dst[p] = mode == 0 ? x[p] - s[p] : x[p] + s[p];
+    return;
+  }
+  double w[8];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const d2 v = x[p] - s[p];
+    w[2 * p] = v.x;
+    w[2 * p + 1] = v.y;
+  }
+  const size_t blk = mode == 1 ? (size_t)nb : 0;  // mode 3 writes Q_0
+  d2 *dst = (d2 *)(S.Q + (blk * n + row) * NB);
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    d2 o = {0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o += w[c] * d2{S.Ri[c * 8 + 2 * p], S.Ri[c * 8 + 2 * p + 1]};
+    dst[p] = o;
+  }
+}
+
+// Cholesky factor of an L x L Gram matrix G (row-major, stride 8) with dropped columns: column c is
+// dropped (zero row, zero diagonal) when it is dead or its pivot is at most drop * nrm2[c]; rows of
+// R are written to R (stride 8), the inverse over the kept columns to Ri (full 8 x 8, zero
+// elsewhere). Returns 1 when a live column was dropped. One thread.
+__device__ inline int chol_drop(const double *G, const double *nrm2, const bool *live, int L, double drop, double *R,
+                                double *Ri) {
+  int brk = 0;
+  for (int a = 0; a < 64; ++a) {
+    R[a] = 0.0;
+    Ri[a] = 0.0;
+  }
+  for (int c = 0; c < L; ++c) {
+    double d = G[c * 8 + c];
+    for (int t = 0; t < c; ++t) d -= R[t * 8 + c] * R[t * 8 + c];
+    if (!live[c] || !(d > drop * nrm2[c])) {
+      if (live[c]) brk = 1;
+      continue;  // row c stays zero
+    }
+    const double r = sqrt(d);
+    R[c * 8 + c] = r;
+    for (int c2 = c + 1; c2 < L; ++c2) {
+      double v = G[c * 8 + c2];
+      for (int t = 0; t < c; ++t) v -= R[t * 8 + c] * R[t * 8 + c2];
+      R[c * 8 + c2] = v / r;
+    }
+  }
+  // Ri = R^-1 over the kept columns (upper triangular), column by column
+  for (int k = 0; k < L; ++k) {
+    if (R[k * 8 + k] == 0.0) continue;
+    Ri[k * 8 + k] = 1.0 / R[k * 8 + k];
+    for (int a = k - 1; a >= 0; --a) {
+      if (R[a * 8 + a] == 0.0) continue;
+      double v = 0.0;
+      for (int t = a + 1; t <= k; ++t) v += R[a * 8 + t] * Ri[t * 8 + k];
+      Ri[a * 8 + k] = -v / R[a * 8 + a];
+    }
+  }
+  return brk;
+}
+
+// wave per system of the cycle: Q_0 = R R0^-1 from the Gram matrix of the residual block (C2 block 0),
+// g = (R0; 0). Dead columns (converged or beyond nrhs) take no part.
+__global__ __launch_bounds__(64) void kb_binit(const GSys *__restrict__ sys, int m, int L) {
+  __shared__ double sR[64], sRi[64];
+  const GSys &S = sys[blockIdx.x];
+  if (!S.incyc) return;  // uniform over the block
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    bool live[NB];
+    double nrm2[NB];
+    for (int c = 0; c < NB; ++c) {
+      live[c] = c < L && S.small[8 * c + 4] > 0.0;
+      nrm2[c] = S.C2[c * 9];
+    }
+    (void)chol_drop(S.C2, nrm2, live, L, 1e-24, sR, sRi);
+  }
+  __syncthreads();
+  S.Ri[lane] = sRi[lane];
+  // rows 0 .. L - 1 of g hold R0 (column k of the live k), the rest zero
+  for (int idx = lane; idx < L * (m + 1) * NB; idx += 64) {
+    const int r = idx / NB, k = idx % NB;
+    S.gB[idx] = (r < L && k < L) ? sR[r * 8 + k] : 0.0;
+  }
+}
+
+// wave per system of the current map: the small dense part of round j. Lane (c, i) = 8 c + i:
+// group c handles new Hessenberg column L j + c (and later right-hand side c), sub-lane i the row
+// q + i of the reflector window at step q.
+__global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, int j, int L, double tol) {
+  __shared__ double sG[64], sR[64], sRi[64], sN[NB];
+  __shared__ double sv[NB][NB];  // reflectors of the round's new columns: tau, v_1 .. v_L
+  __shared__ int sbrk, sok[NB];
+  const GSys &S = sys[blockIdx.x];
+  if (S.nt4 == 0) return;  // uniform over the block
+  const int lane = threadIdx.x, c = lane >> 3, i = lane & 7;
+  const double *C1 = S.C1, *C2 = S.C2;
+  // G2 = G1 - sum_b C2_b^T C2_b; squared norms of the product's columns before the sweeps
+  {
+    double gs = C2[(size_t)(j + 1) * 64 + c * 8 + i];
+    for (int b = 0; b <= j; ++b) {
+      const double *cb = C2 + (size_t)b * 64;
+      for (int l = 0; l < L; ++l) gs -= cb[l * 8 + c] * cb[l * 8 + i];
+    }
+    sG[lane] = gs;
+    if (c == i) {
+      double w = C2[(size_t)(j + 1) * 64 + c * 9];
+      for (int b = 0; b <= j; ++b)
+        for (int l = 0; l < L; ++l) {
+          const double v = C1[(size_t)b * 64 + l * 8 + c];
+          w += v * v;
+        }
+      sN[c] = w;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    bool live[NB];
+    for (int k = 0; k < NB; ++k) live[k] = k < L && S.small[8 * k + 4] > 0.0;
+    sbrk = chol_drop(sG, sN, live, L, 1e-20, sR, sRi);
+  }
+  __syncthreads();
+  S.Ri[lane] = sRi[lane];
+  const int LJ = L * j, rows = L * (j + 2);
+  const bool mine = c < L;  // a new column of this round
+  // H[r][LJ + c] as built: C1 + C2 rows of the blocks b <= j, then the subdiagonal block R_(j+1)
+  auto hval = [&](int r) -> double {
+    if (!mine || r >= rows) return 0.0;
+    if (r < L * (j + 1)) {
+      const int b = r / L, l = r - b * L;
+      return C1[(size_t)b * 64 + l * 8 + c] + C2[(size_t)b * 64 + l * 8 + c];
+    }
+    const int cc = r - L * (j + 1);
+    return cc <= c ? sR[cc * 8 + c] : 0.0;
+  };
+  auto gsum = [&](double x) {  // over the 8 sub-lanes of a group
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 4, 64);
+    return x;
+  };
+  double w = i <= L ? hval(i) : 0.0;  // window: sub-lane i holds row q + i
+  const int qc = LJ + c;
+  double *Rcol = S.Rp + (size_t)qc * (qc + 1) / 2;
+  for (int q = 0; q < LJ + L; ++q) {
+    double tau, vi;
+    if (q < LJ) {  // reflector of an earlier round
+      const double *h = S.hv + (size_t)q * 8;
+      tau = h[0];
+      vi = i == 0 ? 1.0 : (i <= L ? h[i] : 0.0);
+    } else {
+      const int cq = q - LJ;  // the new column whose reflector this step forms
+      const double x0 = __shfl(w, 8 * c, 64);
+      const double s2 = gsum(i >= 1 && i <= L ? w * w : 0.0);
+      if (c == cq) {
+        double alpha = x0, t = 0.0, sc = 0.0;
+        if (s2 > 0.0) {
+          const double nx = sqrt(x0 * x0 + s2);
+          alpha = x0 > 0.0 ? -nx : nx;
+          t = (alpha - x0) / alpha;
+          sc = 1.0 / (x0 - alpha);
+        }
+        if (i == 0) {
+          sv[cq][0] = t;
+          Rcol[q] = alpha;
+        } else if (i <= L) {
+          sv[cq][i] = w * sc;
+        }
+        if (i <= L) S.hv[(size_t)q * 8 + i] = i == 0 ? t : w * sc;
+      }
+      __syncthreads();
+      tau = sv[cq][0];
+      vi = i == 0 ? 1.0 : (i <= L ? sv[cq][i] : 0.0);
+    }
+    const bool act = mine && q < qc;  // group c applies the reflectors of the columns before its own
+    const double s = gsum(act ? vi * w : 0.0);
+    if (act) {
+      w -= tau * s * vi;
+      if (i == 0) Rcol[q] = w;  // row q of column qc is final
+    }
+    const double up = __shfl_down(w, 1, 8);
+    const double enter = (act && i == L) ? hval(q + 1 + L) : 0.0;
+    if (act) w = i < L ? up : enter;
+  }
+  __syncthreads();
+  // the round's reflectors on the right-hand sides: group k, window rows LJ + t + i
+  const int k = c;
+  const bool klive = k < L && S.small[8 * k + 4] > 0.0;
+  double gw = (klive && i <= L) ? S.gB[(size_t)(LJ + i) * NB + k] : 0.0;
+  for (int t = 0; t < L; ++t) {
+    const double tau = sv[t][0], vi = i == 0 ? 1.0 : (i <= L ? sv[t][i] : 0.0);
+    const double s = gsum(vi * gw);
+    gw -= tau * s * vi;
+    if (klive && i == 0) S.gB[(size_t)(LJ + t) * NB + k] = gw;
+    const double up = __shfl_down(gw, 1, 8);
+    const int r = LJ + t + 1 + L;
+    const double enter = (klive && i == L && r < rows) ? S.gB[(size_t)r * NB + k] : 0.0;
+    gw = i < L ? up : (i == L ? enter : 0.0);
+  }
+  // rows L (j + 1) .. L (j + 2) - 1: the residual of right-hand side k
+  if (klive && i < L) S.gB[(size_t)(L * (j + 1) + i) * NB + k] = gw;
+  const double est = sqrt(gsum(i < L ? gw * gw : 0.0));
+  double *sm = S.small + 8 * k;
+  if (i == 0) {
+    sok[k] = 1;
+    if (klive) {
+      sm[2] = est;
+      sok[k] = est <= 0.1 * tol * sm[0] ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (i == 0 && klive && sm[5] < 0.0) {
+    bool all = sbrk != 0;
+    if (!all) {
+      all = true;
+      for (int kk = 0; kk < NB; ++kk) all = all && sok[kk];
+    }
+    if (all) sm[5] = (double)(j + 1);  // rounds to use; the product skips the system from now on
+  }
+}
+
+// wave per system of the cycle: Y = R^-1 g over the rounds used (in place in g), then the solution
+// blocks Y_b[l][k] into C1. Lane 8 p + k owns the entries (r, k) with r = p (mod 8) of g, so that
+// each element of g is only ever read back by the lane that wrote it.
+__global__ __launch_bounds__(64) void kb_bsolve(const GSys *__restrict__ sys, int steps, int L) {
+  __shared__ double sy[NB];
+  const GSys &S = sys[blockIdx.x];
+  if (!S.incyc) return;  // uniform over the block
+  const int lane = threadIdx.x, p = lane >> 3, k = lane & 7;
+  int kk = -1;
+  for (int c = 0; c < L; ++c) {
+    const double *sm = S.small + 8 * c;
+    if (sm[4] > 0.0) kk = max(kk, sm[5] > 0.0 ? min((int)sm[5], steps) : steps);
+  }
+  kk = max(kk, 0);
+  const int N = L * kk;
+  for (int q = N - 1; q >= 0; --q) {
+    const double *Rc = S.Rp + (size_t)q * (q + 1) / 2;
+    if (p == (q & 7)) {
+      double *gq = S.gB + (size_t)q * NB + k;
+      const double d = Rc[q];
+      const double y = d != 0.0 ? *gq / d : 0.0;
+      *gq = y;
+      sy[k] = y;
+    }
+    __syncthreads();
+    const double yk = sy[k];
+    for (int r = p; r < q; r += 8) S.gB[(size_t)r * NB + k] -= Rc[r] * yk;
+    __syncthreads();
+  }
+  for (int r = p; r < N; r += 8) {
+    const int b = r / L, l = r - b * L;
+    S.C1[(size_t)b * 64 + l * 8 + k] = S.gB[(size_t)r * NB + k];
+  }
+  for (int idx = lane; idx < kk * 64; idx += 64)
+    if (((idx >> 3) & 7) >= L) S.C1[idx] = 0.0;
+}
+
 }  // namespace
 
 int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int nrhs, int restart, int maxit,
@@ -863,7 +1246,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // the structured Stokes operator of the fast path (FP32 symmetric blocks) is the only operator
   if (!fp32) return andr_fail(ctx, ANDR_ERR_ARG, "GMRES takes the structured Stokes operator only");
   size_t vec_elems = 0, basis_elems = 0, zt_elems = 0;
-  int nw_full = 0, nt_full = 0, net_full = 0;
+  int nw_full = 0, nt_full = 0, net_full = 0, nt4_full = 0;
   for (int s = 0; s < nsys; ++s) {
     const int n = in[s].n, ne = in[s].op.ne;
     if (!in[s].op.S || !in[s].op.minv || 3 * ne != n)
@@ -872,24 +1255,35 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     basis_elems += (size_t)(m + 1) * n * NB;
     nw_full += 4 * ((ne + 63) / 64);
     nt_full += (n + TILE - 1) / TILE;
+    nt4_full += (n + BT - 1) / BT;
     net_full += (ne + 31) / 32;
     zt_elems += (size_t)3 * NB * ((ne + 31) / 32 * 32);
   }
   // per system: Hessenberg, rotations, g, coefficients; the per-rhs state (8 doubles per rhs) of all
   // systems sits in one separate array so that the host reads it with one copy
-  auto small_of = [&](int mc) {  // H, Hu, cs, sn, g, cw per rhs
+  // Block GMRES (one Krylov space per system for its nrhs columns; the reflector window of its
+  // least-squares update holds nrhs + 1 rows in 8 lanes, so nrhs <= 7) unless
+  // ANDR_GMRES_INDEPENDENT=1, which runs one Krylov space per column.
+  const bool blk = getenv("ANDR_GMRES_INDEPENDENT") == nullptr && nrhs <= NB - 1;
+  const int L = nrhs;
+  auto small_of = [&](int mc) {
+    if (blk)  // Rp, hv, gB, C1, C2, Ri
+      return (size_t)L * mc * ((size_t)L * mc + 1) / 2 + (size_t)8 * L * mc + (size_t)L * (mc + 1) * NB +
+             (size_t)(mc + 1) * 64 + (size_t)(mc + 2) * 64 + 64;
+    // H, Hu, cs, sn, g, cw per rhs
     return (size_t)NB * (2 * (size_t)(mc + 1) * mc + 2 * mc + (mc + 1) + cw_stride(mc));
   };
   const size_t small_per = small_of(m);
   const size_t state_per = (size_t)NB * 8;
-  DevBuf<double> R, W, Q, SM, ST8, partW, partT, SDEF, partS;
+  DevBuf<double> R, W, Q, SM, ST8, partW, partT, SDEF, partS, partB;
   DevBuf<double> ZT;
   DevBuf<float> ZT32;
   DevBuf<GSys> dsys;
   DevBuf<int> ddone;
   if (R.alloc(vec_elems, ctx) || W.alloc(vec_elems, ctx) || Q.alloc(basis_elems, ctx) ||
       SM.alloc(small_per * nsys, ctx) || ST8.alloc(state_per * nsys, ctx) || partW.alloc((size_t)nw_full * NB, ctx) ||
-      partT.alloc((size_t)nt_full * part_stride(m) * NB, ctx) ||
+      (!blk && partT.alloc((size_t)nt_full * part_stride(m) * NB, ctx)) ||
+      (blk && partB.alloc((size_t)nt4_full * (m + 2) * 64, ctx)) ||
       dsys.alloc(nsys, ctx) || ddone.alloc(1, ctx))
     return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES workspace (%zu basis doubles)", basis_elems);
   // Krylov products on the FP32 matrix cores at relative residuals of 1e-6 and above (their ~1e-7
@@ -908,28 +1302,30 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   // matrix-core waves of 32 element rows, et = element tiles of 32 (z = M^-1 v and wall sums)
   struct MapSet {
     DevBuf<WaveMap> w;
-    DevBuf<TileMap> t, et;
+    DevBuf<TileMap> t, et, t4;
     DevBuf<MWave> mw;
     DevBuf<int> w0, nw;
-    int n_w = 0, n_t = 0, n_et = 0, n_mw = 0, max_seg = 1;
+    int n_w = 0, n_t = 0, n_et = 0, n_mw = 0, n_t4 = 0, max_seg = 1;
     size_t mw_cap = 0;
-    std::vector<int> tile0, ntiles, et0, net, nseg;
+    std::vector<int> tile0, ntiles, et0, net, nseg, t40, nt4;
     std::vector<size_t> poff;
     size_t p_elems = 0;
   };
   MapSet cyc, live;
   for (MapSet *ms : {&cyc, &live})
     if (ms->w.alloc(nw_full, ctx) || ms->t.alloc(nt_full, ctx) || ms->w0.alloc(nsys, ctx) || ms->nw.alloc(nsys, ctx) ||
-        ms->et.alloc(net_full, ctx))
+        ms->et.alloc(net_full, ctx) || (blk && ms->t4.alloc(nt4_full, ctx)))
       return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES work maps");
   std::vector<GSys> hs(nsys);
   DevBuf<double> Pseg;
   size_t pseg_cap = 0;
   auto build = [&](const std::vector<int> &list, MapSet &ms, int target_waves = MF_TARGET_WAVES) -> int {
     std::vector<WaveMap> wm;
-    std::vector<TileMap> tm, em;
+    std::vector<TileMap> tm, em, t4;
     std::vector<MWave> mw;
     std::vector<int> w0(nsys, 0), nw(nsys, 0);
+    ms.t40.assign(nsys, 0);
+    ms.nt4.assign(nsys, 0);
     ms.tile0.assign(nsys, 0);
     ms.ntiles.assign(nsys, 0);
     ms.et0.assign(nsys, 0);
@@ -949,6 +1345,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       ms.tile0[s] = (int)tm.size();
       for (int r = 0; r < n; r += TILE) tm.push_back({s, r});
       ms.ntiles[s] = (int)tm.size() - ms.tile0[s];
+      if (blk) {
+        ms.t40[s] = (int)t4.size();
+        for (int r = 0; r < n; r += BT) t4.push_back({s, r});
+        ms.nt4[s] = (int)t4.size() - ms.t40[s];
+      }
       ms.et0[s] = (int)em.size();
       for (int e = 0; e < ne; e += 32) em.push_back({s, e});
       ms.net[s] = (int)em.size() - ms.et0[s];
@@ -967,6 +1368,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     ms.n_t = (int)tm.size();
     ms.n_et = (int)em.size();
     ms.n_mw = (int)mw.size();
+    ms.n_t4 = (int)t4.size();
+    if (ms.n_t4) ANDR_HIP(ctx, hipMemcpyAsync(ms.t4.p, t4.data(), sizeof(TileMap) * t4.size(), hipMemcpyHostToDevice, st));
     if (ms.n_et) ANDR_HIP(ctx, hipMemcpyAsync(ms.et.p, em.data(), sizeof(TileMap) * em.size(), hipMemcpyHostToDevice, st));
     if (mw.size() > ms.mw_cap) {
       if (ms.mw.alloc(mw.size(), ctx)) return andr_fail(ctx, ANDR_ERR_ALLOC, "GMRES matrix-core map");
@@ -989,6 +1392,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     for (int s = 0; s < nsys; ++s) {
       hs[s].tile0 = ms.tile0[s];
       hs[s].ntiles = ms.ntiles[s];
+      hs[s].t40 = blk ? ms.t40[s] : 0;
+      hs[s].nt4 = blk ? ms.nt4[s] : 0;
       hs[s].et0 = ms.et0[s];
       hs[s].net = ms.net[s];
       hs[s].nseg = ms.nseg[s];
@@ -1032,6 +1437,20 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       bo += (size_t)(mc + 1) * in[s].n * NB;
       double *sp = SM.p + so;
       so += small_of(mc);
+      if (blk) {
+        g.Rp = sp;
+        sp += (size_t)L * mc * ((size_t)L * mc + 1) / 2;
+        g.hv = sp;
+        sp += (size_t)8 * L * mc;
+        g.gB = sp;
+        sp += (size_t)L * (mc + 1) * NB;
+        g.C1 = sp;
+        sp += (size_t)(mc + 1) * 64;
+        g.C2 = sp;
+        sp += (size_t)(mc + 2) * 64;
+        g.Ri = sp;
+        continue;
+      }
       g.H = sp;
       sp += (size_t)NB * (mc + 1) * mc;
       g.Hu = sp;
@@ -1051,9 +1470,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       nsum += in[s].n;
       ntl += (in[s].n + TILE - 1) / TILE;
     }
+    size_t nt4l = 0;
+    for (int s : list) nt4l += (in[s].n + BT - 1) / BT;
     for (int mc = MMAX; mc > m; mc -= 8)
       if ((size_t)(mc + 1) * nsum * NB <= basis_elems && list.size() * small_of(mc) <= small_per * nsys &&
-          ntl * part_stride(mc) <= (size_t)nt_full * part_stride(m))
+          (blk ? nt4l * (mc + 2) <= (size_t)nt4_full * (m + 2) : ntl * part_stride(mc) <= (size_t)nt_full * part_stride(m)))
         return mc;
     return m;
   };
@@ -1224,8 +1645,15 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     first_cycle = false;
     if ((rc = use_tiles(live))) return rc;
     std::fill(skip_counted.begin(), skip_counted.end(), 0);
-    hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
-    hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
+    if (blk) {  // Q_0 R0 = R (Cholesky QR of the residual block), g = (R0; 0)
+      hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, 1), dim3(256), 0, st, dsys.p, live.t4.p, 1, 0, 1, partB.p, mc + 2);
+      hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1);
+      hipLaunchKernelGGL(kb_binit, dim3(nsys), dim3(64), 0, st, dsys.p, mc, L);
+      hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, 0, 3);
+    } else {
+      hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
+      hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
+    }
     {  // a cycle that starts with few rows (a small batch) takes the tuned column split at once
       const int target = krylov_target(cyc_list, 0);
       if (target < 0) return target;  // a failed timing run
@@ -1243,15 +1671,32 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       for (int s = 0; s < nsys; ++s)
         if (live.ntiles[s]) nlive += in[s].n;
       const double vbytes = nlive * NB * sizeof(double);
-      const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (2.0 * j + 6.0) * vbytes,
-                                        2.0 * 4.0 * j * nlive * NB);
-      hipLaunchKernelGGL(kb_adots, dim3(live.n_t, std::max(1, (j + ICH - 1) / ICH)), dim3(256), 0, st, dsys.p,
-                         live.t.p, j, mc, partT.p, 0);
-      hipLaunchKernelGGL(kb_adots_final, dim3(nsys, (2 * j + 3 + 7) / 8), dim3(256), 0, st, dsys.p, j, mc, partT.p);
-      hipLaunchKernelGGL(kb_arnoldi, dim3(nsys * NB), dim3(64), sizeof(double) * (5 * mc + 2), st, dsys.p, j, mc, tol,
-                         0);
-      hipLaunchKernelGGL(kb_aupdate, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, mc);
-      andr_prof_end(ctx, tok_o);
+      if (blk) {
+        // two dot sweeps and two update sweeps over the j + 1 basis blocks (classical Gram-Schmidt
+        // twice); the second update also normalises the new block
+        const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (4.0 * j + 10.0) * vbytes,
+                                          2.0 * 64.0 * (4.0 * j + 5.0) * nlive);
+        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 8) / 8), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1, 0,
+                           partB.p, mc + 2);
+        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 0);
+        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 0);
+        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 9) / 8), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1, 1,
+                           partB.p, mc + 2);
+        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 2), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1);
+        hipLaunchKernelGGL(kb_bsmall, dim3(nsys), dim3(64), 0, st, dsys.p, j, L, tol);
+        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 1);
+        andr_prof_end(ctx, tok_o);
+      } else {
+        const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (2.0 * j + 6.0) * vbytes,
+                                          2.0 * 4.0 * j * nlive * NB);
+        hipLaunchKernelGGL(kb_adots, dim3(live.n_t, std::max(1, (j + ICH - 1) / ICH)), dim3(256), 0, st, dsys.p,
+                           live.t.p, j, mc, partT.p, 0);
+        hipLaunchKernelGGL(kb_adots_final, dim3(nsys, (2 * j + 3 + 7) / 8), dim3(256), 0, st, dsys.p, j, mc, partT.p);
+        hipLaunchKernelGGL(kb_arnoldi, dim3(nsys * NB), dim3(64), sizeof(double) * (5 * mc + 2), st, dsys.p, j, mc,
+                           tol, 0);
+        hipLaunchKernelGGL(kb_aupdate, dim3(live.n_t), dim3(256), 0, st, dsys.p, live.t.p, j, mc);
+        andr_prof_end(ctx, tok_o);
+      }
       steps = j + 1;
       if (steps % 8 == 0 && steps < steps_max) {  // drop systems whose columns are all resolved
         ANDR_HIP(ctx, hipGetLastError());
@@ -1276,7 +1721,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         }
       }
     }
-    if (steps == steps_max && live.n_t > 0) {
+    if (!blk && steps == steps_max && live.n_t > 0) {
       // close the cycle: the last column is final once its candidate is reorthogonalised
       hipLaunchKernelGGL(kb_adots, dim3(live.n_t, std::max(1, (steps + ICH - 1) / ICH)), dim3(256), 0, st, dsys.p,
                          live.t.p, steps, mc, partT.p, 1);
@@ -1286,8 +1731,13 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
                          tol, 1);
     }
     total += steps;
-    hipLaunchKernelGGL(kb_solve_y, dim3(nsys * NB), dim3(64), sizeof(double) * (mc + 1), st, dsys.p, steps, mc);
-    hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, mc);
+    if (blk) {
+      hipLaunchKernelGGL(kb_bsolve, dim3(nsys), dim3(64), 0, st, dsys.p, steps, L);
+      hipLaunchKernelGGL(kb_bupdate, dim3(cyc.n_t4), dim3(256), 0, st, dsys.p, cyc.t4.p, steps, 2);
+    } else {
+      hipLaunchKernelGGL(kb_solve_y, dim3(nsys * NB), dim3(64), sizeof(double) * (mc + 1), st, dsys.p, steps, mc);
+      hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, mc);
+    }
     ANDR_HIP(ctx, hipGetLastError());
     if ((rc = read_small())) return rc;
     for (int s : cyc_list)

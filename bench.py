@@ -311,7 +311,8 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] fast form ({prec_name}): {value:.2f} tensors/s, {ms_per_step:.0f} ms/step")
     # ablation on the same batch, one step each after a warm call: the fixed-order FP32 rule instead
-    # of the learned rule order (mixed form), and the learned form with FP64 Krylov products
+    # of the learned rule order (mixed form), the learned form with FP64 Krylov products, and with
+    # one Krylov space per right-hand side instead of block GMRES
     ablation = {"learned_step_ms": round(ms_per_step, 1)}
     if prec == api.PREC_LEARNED and not args.no_ablation:
         ctx.resistance_batch_resident(resident, precision=api.PREC_MIXED)
@@ -327,6 +328,14 @@ def main():
             ablation[f"{prec_name}_fp64_krylov_products_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
         finally:
             os.environ.pop("ANDR_GMRES_FP64_PRODUCTS", None)
+        os.environ["ANDR_GMRES_INDEPENDENT"] = "1"
+        try:
+            ctx.resistance_batch_resident(resident, precision=prec)
+            ta = time.perf_counter()
+            ctx.resistance_batch_resident(resident, precision=prec)
+            ablation[f"{prec_name}_independent_gmres_step_ms"] = round(1e3 * (time.perf_counter() - ta), 1)
+        finally:
+            os.environ.pop("ANDR_GMRES_INDEPENDENT", None)
     log(f"[bench] ablation: {ablation}")
     # the same step from host meshes (upload over PCIe inside the call), reported beside value
     th = time.perf_counter()
@@ -414,8 +423,8 @@ def main():
         "ablation": dict(ablation, plain_fp64_lsqr_tensors_per_s_on_sample=round(plain_rate, 3),
                          fast_tensors_per_s_on_sample=round(len(sample) / t_fast, 3),
                          note="steps of the whole batch: learned rule order (value), fixed-order FP32 rule "
-                              "(mixed), FP64 instead of FP32 Krylov products; the plain FP64 LSQR path and the "
-                              "fast form on the stratified sample"),
+                              "(mixed), FP64 instead of FP32 Krylov products, one Krylov space per column instead "
+                              "of block GMRES; the plain FP64 LSQR path and the fast form on the stratified sample"),
         "host_meshes": {"tensors_per_s": round(host_rate, 3),
                         "note": "one step from host meshes, mesh upload over PCIe inside the call"},
         "accuracy": {"max_tensor_err_vs_fp64": float(f"{max(errs):.3e}"),
