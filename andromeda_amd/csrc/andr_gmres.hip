@@ -890,19 +890,25 @@ __global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, co
   const double *V = which ? S.R : S.W;
   const int b = bw + (i >> 3);
   const double *Ab = b < nb ? S.Q + (size_t)b * n * NB : (b == nb && gram ? V : nullptr);
+  // rows past the end (last tile of a system) and lanes without a block load a valid row and feed
+  // zeros to the A operand, so that the 32 loads of an iteration go out unconditionally
+  const double *Ap = Ab ? Ab : V;
+  const bool aon = Ab != nullptr;
+  const int rlast = n - 1;
   d4v acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-  for (int r = tm.row0; r < tm.row0 + BT; r += 16) {
-    double a[4], bv[4];
+  for (int r = tm.row0; r < tm.row0 + BT; r += 64) {
+    double a[16], bv[16];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int row = r + 4 * t + g;
-      const bool in = row < n;
-      a[t] = (in && Ab) ? Ab[(size_t)row * NB + l] : 0.0;
-      bv[t] = in ? V[(size_t)row * NB + l] : 0.0;
+    for (int t = 0; t < 16; ++t) {
+      const int rr = min(r + 4 * t + g, rlast);
+      a[t] = Ap[(size_t)rr * NB + l];
+      bv[t] = V[(size_t)rr * NB + l];
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bv[t], acc, 0, 0, 0);
+    for (int t = 0; t < 16; ++t) {
+      const bool in = aon && r + 4 * t + g <= rlast;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? a[t] : 0.0, bv[t], acc, 0, 0, 0);
+    }
   }
   if (i < 8) {
 #pragma unroll
@@ -951,7 +957,11 @@ __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, 
     }
     nbb = max(kk, 0);
   }
-  const double *Cf = mode == 1 ? S.C2 : S.C1;
+  // the coefficients are uniform over the block and read-only here: loads through the constant
+  // address space go to the scalar cache and feed the FMAs as scalar operands
+  typedef const __attribute__((address_space(4))) double cdouble;
+  cdouble *Cf = (cdouble *)(uintptr_t)(mode == 1 ? S.C2 : S.C1);
+  cdouble *Ric = (cdouble *)(uintptr_t)S.Ri;
   const double *src = mode == 3 ? S.R : (mode == 2 ? S.X : S.W);
   d2 x[4], s[4];
 #pragma unroll
@@ -965,7 +975,7 @@ __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, 
     d2 qv[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) qv[p] = q[p];
-    const double *C = Cf + (size_t)b * 64;
+    cdouble *C = Cf + (size_t)b * 64;
 #pragma unroll
     for (int l = 0; l < 8; ++l) {
       const double ql = (l & 1) ? qv[l >> 1].y : qv[l >> 1].x;
@@ -992,7 +1002,7 @@ __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, 
   for (int p = 0; p < 4; ++p) {
     d2 o = {0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < 8; ++c) o += w[c] * d2{S.Ri[c * 8 + 2 * p], S.Ri[c * 8 + 2 * p + 1]};
+    for (int c = 0; c < 8; ++c) o += w[c] * d2{Ric[c * 8 + 2 * p], Ric[c * 8 + 2 * p + 1]};
     dst[p] = o;
   }
 }
