@@ -874,48 +874,64 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 // direction left (pivot below 1e-20 of its squared norm) ends the cycle: the restart test then takes
 // the true residual. Dead columns (converged, or beyond nrhs) stay exact zeros throughout.
 //
-// Dot sweep: per (tile of BT rows, 8 blocks) partials of C_b = Q_b^T V, b < nb, and with gram V^T V
-// as block nb. Wave w takes blocks 8 y + 2 w and 8 y + 2 w + 1 as the two 8-row halves of the FP64
-// matrix-core A operand (lane (g, i): A[i][g] = Q_(b(i))[row + g][i & 7], B[g][i] = V[row + g][i & 7]);
-// register v of lane (g, i < 8) ends as C_b[l][i] with b, l from row g + 4 v of the product.
+// Dot sweep: per (tile of BT rows, 16 blocks) partials of C_b = Q_b^T V, b < nb, and with gram V^T V
+// as block nb. The tile's rows of V are staged in LDS (zero past the end of the system). Wave w
+// takes blocks 16 y + 4 w .. + 3 as two FP64 matrix-core A operands of two blocks each (lane (g, i):
+// A[i][g] = Q_(b(i))[row + g][i & 7], B[g][i] = V[row + g][i & 7]); register v of lane (g, i < 8)
+// ends as C_b[l][i] with b, l from row g + 4 v of the product. Rows past the end meet zeros in V,
+// so the A operands load a valid row unconditionally.
 __global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap,
                                                 int which, int nb, int gram, double *__restrict__ part, int pstride) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  __shared__ double vs[BT * NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
   const int n = S.n, nbt = nb + (gram ? 1 : 0);
+  const double *V = which ? S.R : S.W;
+  {
+    const int row = tm.row0 + threadIdx.x;
+    d2 *dst = (d2 *)(vs + threadIdx.x * NB);
+    const d2 *src = (const d2 *)(V + (size_t)min(row, n - 1) * NB);
+#pragma unroll
+    for (int p = 0; p < NB / 2; ++p) dst[p] = row < n ? src[p] : d2{0.0, 0.0};
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g = lane >> 4, i = lane & 15, l = i & 7;
-  const int bw = blockIdx.y * 8 + 2 * wid;  // the wave's first block
-  if (bw >= nbt) return;                     // uniform over the wave; no barriers in this kernel
-  const double *V = which ? S.R : S.W;
-  const int b = bw + (i >> 3);
-  const double *Ab = b < nb ? S.Q + (size_t)b * n * NB : (b == nb && gram ? V : nullptr);
-  // rows past the end (last tile of a system) and lanes without a block load a valid row and feed
-  // zeros to the A operand, so that the 32 loads of an iteration go out unconditionally
-  const double *Ap = Ab ? Ab : V;
-  const bool aon = Ab != nullptr;
+  const int bw = blockIdx.y * 16 + 4 * wid;  // the wave's first block
+  if (bw >= nbt) return;                      // after the only barrier
+  const double *Ap[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = bw + 2 * h + (i >> 3);
+    const double *Ab = b < nb ? S.Q + (size_t)b * n * NB : (b == nb && gram ? V : nullptr);
+    Ap[h] = Ab ? Ab : V;  // lanes without a block compute partials that are not written
+  }
   const int rlast = n - 1;
-  d4v acc = {0.0, 0.0, 0.0, 0.0};
-  for (int r = tm.row0; r < tm.row0 + BT; r += 64) {
-    double a[16], bv[16];
+  d4v acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  for (int r = 0; r < BT; r += 64) {
+    double a[2][16], bv[16];
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-      const int rr = min(r + 4 * t + g, rlast);
-      a[t] = Ap[(size_t)rr * NB + l];
-      bv[t] = V[(size_t)rr * NB + l];
+      const size_t rr = (size_t)min(tm.row0 + r + 4 * t + g, rlast) * NB + l;
+      a[0][t] = Ap[0][rr];
+      a[1][t] = Ap[1][rr];
+      bv[t] = vs[(r + 4 * t + g) * NB + l];
     }
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-      const bool in = aon && r + 4 * t + g <= rlast;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? a[t] : 0.0, bv[t], acc, 0, 0, 0);
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][t], bv[t], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][t], bv[t], acc[1], 0, 0, 0);
     }
   }
   if (i < 8) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int mrow = g + 4 * v, blk = bw + (mrow >> 3);
-      if (blk < nbt) part[((size_t)blockIdx.x * pstride + blk) * 64 + (mrow & 7) * 8 + i] = acc[v];
-    }
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int mrow = g + 4 * v, blk = bw + 2 * h + (mrow >> 3);
+        if (blk < nbt) part[((size_t)blockIdx.x * pstride + blk) * 64 + (mrow & 7) * 8 + i] = acc[h][v];
+      }
   }
 }
 
@@ -1686,12 +1702,12 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         // twice); the second update also normalises the new block
         const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (4.0 * j + 10.0) * vbytes,
                                           2.0 * 64.0 * (4.0 * j + 5.0) * nlive);
-        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 8) / 8), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1, 0,
-                           partB.p, mc + 2);
+        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 16) / 16), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1,
+                           0, partB.p, mc + 2);
         hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 0);
         hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 0);
-        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 9) / 8), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1, 1,
-                           partB.p, mc + 2);
+        hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 17) / 16), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1,
+                           1, partB.p, mc + 2);
         hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 2), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1);
         hipLaunchKernelGGL(kb_bsmall, dim3(nsys), dim3(64), 0, st, dsys.p, j, L, tol);
         hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 1);

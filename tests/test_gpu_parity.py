@@ -206,9 +206,9 @@ def test_block_gmres_vs_independent(ctx, monkeypatch):
     allb = [m for m, _ in M.batch_bodies(512, 64)]
     bodies = [allb[i] for i in (29, 204, 523, 534)]  # superellipsoids of 500 and 2420, fibres of 496 and 2000
     plain = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
-    blk = ctx.resistance_batch(bodies, precision=api.PREC_LEARNED)
+    blk = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
     monkeypatch.setenv("ANDR_GMRES_INDEPENDENT", "1")
-    ind = ctx.resistance_batch(bodies, precision=api.PREC_LEARNED)
+    ind = ctx.resistance_batch(bodies, precision=api.PREC_MIXED)
     monkeypatch.delenv("ANDR_GMRES_INDEPENDENT")
     for p, a, b in zip(plain, blk, ind):
         print(f"rounds {a.nit.max()} vs steps {b.nit.tolist()}; err block {tensor_error(a.FT, p.FT):.1e} "
