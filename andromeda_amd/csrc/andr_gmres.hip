@@ -918,11 +918,15 @@ __global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, co
       a[1][t] = Ap[1][rr];
       bv[t] = vs[(r + 4 * t + g) * NB + l];
     }
+    // all 48 loads of the iteration go out before its products (left to itself the scheduler
+    // moves each load next to its use and a wave keeps few loads in flight)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][t], bv[t], acc[0], 0, 0, 0);
       acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][t], bv[t], acc[1], 0, 0, 0);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (i < 8) {
 #pragma unroll
@@ -985,19 +989,30 @@ __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, 
     x[p] = ((const d2 *)(src + (size_t)row * NB))[p];
     s[p] = d2{0.0, 0.0};
   }
-#pragma unroll 2
-  for (int b = 0; b < nbb; ++b) {
-    const d2 *q = (const d2 *)(S.Q + ((size_t)b * n + row) * NB);
-    d2 qv[4];
+  // blocks in groups of four: the group's 16 row loads go out before its products (past the last
+  // block the loads repeat it and are not used)
+  const size_t vstride = (size_t)n * NB;
+  for (int b0 = 0; b0 < nbb; b0 += 4) {
+    d2 qv[4][4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) qv[p] = q[p];
-    cdouble *C = Cf + (size_t)b * 64;
+    for (int u = 0; u < 4; ++u) {
+      const d2 *q = (const d2 *)(S.Q + (size_t)min(b0 + u, nbb - 1) * vstride + (size_t)row * NB);
 #pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const double ql = (l & 1) ? qv[l >> 1].y : qv[l >> 1].x;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) s[p] += ql * d2{C[l * 8 + 2 * p], C[l * 8 + 2 * p + 1]};
+      for (int p = 0; p < 4; ++p) qv[u][p] = q[p];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (b0 + u >= nbb) break;  // uniform
+      cdouble *C = Cf + (size_t)(b0 + u) * 64;
+#pragma unroll
+      for (int l = 0; l < 8; ++l) {
+        const double ql = (l & 1) ? qv[u][l >> 1].y : qv[u][l >> 1].x;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) s[p] += ql * d2{C[l * 8 + 2 * p], C[l * 8 + 2 * p + 1]};
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (mode == 0 || mode == 2) {
     d2 *dst = (d2 *)((mode == 0 ? S.W : S.X) + (size_t)row * NB);
