@@ -30,6 +30,10 @@ static_assert(NB == 8, "the matrix-core product packs two 8-column halves into t
 constexpr int TILE = 64;   // rows per vector tile (256 threads: 32 rows x 8 right-hand sides per pass)
 constexpr int ICH = 8;   // basis vectors per block of the projection kernel
 constexpr int BT = 256;  // rows per tile of the block Gram-Schmidt sweeps
+// A cycle stops when the least-squares estimate of every column's residual is below EST_MARGIN tol |b|:
+// the other half of the tolerance covers the difference between the estimate and the true FP64
+// residual that the FP32 Krylov products leave (the restart test checks the true residual).
+constexpr double EST_MARGIN = 0.5;
 
 struct GSys {  // device view of one system
   const void *A;
@@ -654,7 +658,7 @@ __global__ __launch_bounds__(64) void kb_arnoldi(const GSys *__restrict__ sys, i
       g[j - 1] = c * g[j - 1];
       const double est = fabs(g[j]);
       sm[2] = est;
-      if (live && (est <= 0.1 * tol * sm[0] || fabs(b) <= 1e-14 * rr)) sm[5] = (double)j;  // steps to use
+      if (live && (est <= EST_MARGIN * tol * sm[0] || fabs(b) <= 1e-14 * rr)) sm[5] = (double)j;  // steps to use
     }
     __syncthreads();
     double *hr = Hr + (size_t)(j - 1) * (m + 1);
@@ -1224,7 +1228,7 @@ __global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, in
     sok[k] = 1;
     if (klive) {
       sm[2] = est;
-      sok[k] = est <= 0.1 * tol * sm[0] ? 1 : 0;
+      sok[k] = est <= EST_MARGIN * tol * sm[0] ? 1 : 0;
     }
   }
   __syncthreads();
