@@ -249,7 +249,8 @@ typedef struct {
 #define ANDR_PROBLEMS_STRAIN 1
 
 typedef struct {
-  int32_t nit[6];       /* iterations per rigid-body problem */
+  int32_t nit[6];       /* iterations per rigid-body problem (LSQR iterations; GMRES: block rounds,
+                           each reading the body's matrix once for all six problems) */
   int32_t status;       /* max solver status over the six problems: 0 converged; LSQR (plain path):
                            its istop when > 3 (4 ill-conditioned, 5 iteration limit); GMRES (mixed
                            and learned): 4 stagnated at a relative residual <= 1e-6 above eps (the
