@@ -865,6 +865,15 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 }
 
 // ---- Block GMRES ------------------------------------------------------------------------------
+// a column of the system still iterates in the current cycle (the sweeps skip a system whose columns
+// have all converged: its later rounds are never used)
+__device__ inline bool sys_iterating(const GSys &S) {
+  bool need = false;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) need |= S.small[8 * k + 4] > 0.0 && S.small[8 * k + 5] < 0.0;
+  return need;
+}
+
 // One Krylov space for the L right-hand sides of a system: round j multiplies the basis block Q_j
 // (n x L, orthonormal columns; a column that takes no part is zero) and orthogonalises W = A M^-1 Q_j
 // against Q_0 .. Q_j by block classical Gram-Schmidt applied twice:
@@ -885,11 +894,13 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 // ends as C_b[l][i] with b, l from row g + 4 v of the product. Rows past the end meet zeros in V,
 // so the A operands load a valid row unconditionally.
 __global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap,
-                                                int which, int nb, int gram, double *__restrict__ part, int pstride) {
+                                                int which, int nb, int gram, double *__restrict__ part, int pstride,
+                                                int all) {
   typedef double d2 __attribute__((ext_vector_type(2)));
   __shared__ double vs[BT * NB];
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
+  if (!all && !sys_iterating(S)) return;  // uniform over the block
   const int n = S.n, nbt = nb + (gram ? 1 : 0);
   const double *V = which ? S.R : S.W;
   {
@@ -946,10 +957,10 @@ __global__ __launch_bounds__(256) void kb_bdots(const GSys *__restrict__ sys, co
 // block per (system, basis block b < nbt): C (dst 0: C1, 1: C2) block b = sum of the system's tile
 // partials (the four waves take every fourth tile). Systems outside the map have no tiles.
 __global__ __launch_bounds__(256) void kb_bdots_final(const GSys *__restrict__ sys, const double *__restrict__ part,
-                                                      int pstride, int dst) {
+                                                      int pstride, int dst, int all) {
   __shared__ double red[4][64];
   const GSys &S = sys[blockIdx.x];
-  if (S.nt4 == 0) return;  // uniform over the block
+  if (S.nt4 == 0 || (!all && !sys_iterating(S))) return;  // uniform over the block
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, b = blockIdx.y;
   double c = 0.0;
   for (int tl = wv; tl < S.nt4; tl += 4) c += part[((size_t)(S.t40 + tl) * pstride + b) * 64 + lane];
@@ -964,10 +975,11 @@ __global__ __launch_bounds__(256) void kb_bdots_final(const GSys *__restrict__ s
 //   mode 2: X += sum_(b < kk) Q_b Y_b, Y in C1, kk the rounds the system used (at most nb)
 //   mode 3: Q_0 = R Ri                                      (start of a cycle)
 __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, const TileMap *__restrict__ tmap, int nb,
-                                                  int mode) {
+                                                  int mode, int all) {
   typedef double d2 __attribute__((ext_vector_type(2)));
   const TileMap tm = tmap[blockIdx.x];
   const GSys &S = sys[tm.sys];
+  if (mode < 2 && !all && !sys_iterating(S)) return;  // the sweeps of a round; no barriers here
   const int n = S.n;
   const int row = tm.row0 + threadIdx.x;
   if (row >= n) return;  // no barriers in this kernel
@@ -1110,12 +1122,13 @@ __global__ __launch_bounds__(64) void kb_binit(const GSys *__restrict__ sys, int
 // wave per system of the current map: the small dense part of round j. Lane (c, i) = 8 c + i:
 // group c handles new Hessenberg column L j + c (and later right-hand side c), sub-lane i the row
 // q + i of the reflector window at step q.
-__global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, int j, int L, double tol) {
+__global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, int j, int L, double tol, int all) {
   __shared__ double sG[64], sR[64], sRi[64], sN[NB];
   __shared__ double sv[NB][NB];  // reflectors of the round's new columns: tau, v_1 .. v_L
+  __shared__ double shv[NB][NB], sen[NB][NB];  // a chunk of 8 earlier reflectors, entering rows
   __shared__ int sbrk, sok[NB];
   const GSys &S = sys[blockIdx.x];
-  if (S.nt4 == 0) return;  // uniform over the block
+  if (S.nt4 == 0 || (!all && !sys_iterating(S))) return;  // uniform over the block
   const int lane = threadIdx.x, c = lane >> 3, i = lane & 7;
   const double *C1 = S.C1, *C2 = S.C2;
   // G2 = G1 - sum_b C2_b^T C2_b; squared norms of the product's columns before the sweeps
@@ -1165,40 +1178,55 @@ __global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, in
   double w = i <= L ? hval(i) : 0.0;  // window: sub-lane i holds row q + i
   const int qc = LJ + c;
   double *Rcol = S.Rp + (size_t)qc * (qc + 1) / 2;
-  for (int q = 0; q < LJ + L; ++q) {
-    double tau, vi;
-    if (q < LJ) {  // reflector of an earlier round
-      const double *h = S.hv + (size_t)q * 8;
-      tau = h[0];
-      vi = i == 0 ? 1.0 : (i <= L ? h[i] : 0.0);
-    } else {
-      const int cq = q - LJ;  // the new column whose reflector this step forms
-      const double x0 = __shfl(w, 8 * c, 64);
-      const double s2 = gsum(i >= 1 && i <= L ? w * w : 0.0);
-      if (c == cq) {
-        double alpha = x0, t = 0.0, sc = 0.0;
-        if (s2 > 0.0) {
-          const double nx = sqrt(x0 * x0 + s2);
-          alpha = x0 > 0.0 ? -nx : nx;
-          t = (alpha - x0) / alpha;
-          sc = 1.0 / (x0 - alpha);
-        }
-        if (i == 0) {
-          sv[cq][0] = t;
-          Rcol[q] = alpha;
-        } else if (i <= L) {
-          sv[cq][i] = w * sc;
-        }
-        if (i <= L) S.hv[(size_t)q * 8 + i] = i == 0 ? t : w * sc;
+  // the reflectors of earlier rounds, 8 steps at a time: lane (c, i) fetches entry i of reflector
+  // q0 + c and the row that enters group c's window at step q0 + i, so that the chain of steps
+  // reads LDS only
+  for (int q0 = 0; q0 < LJ; q0 += 8) {
+    const int qn = min(8, LJ - q0);
+    const double hvv = c < qn ? S.hv[(size_t)(q0 + c) * 8 + i] : 0.0;
+    const double env = i < qn ? hval(q0 + i + 1 + L) : 0.0;
+    __syncthreads();
+    shv[c][i] = hvv;
+    sen[c][i] = env;
+    __syncthreads();
+    for (int t = 0; t < qn; ++t) {
+      const double tau = shv[t][0], vi = i == 0 ? 1.0 : (i <= L ? shv[t][i] : 0.0);
+      const double sd = gsum(mine ? vi * w : 0.0);
+      if (mine) {
+        w -= tau * sd * vi;
+        if (i == 0) Rcol[q0 + t] = w;  // row q0 + t of column qc is final
       }
-      __syncthreads();
-      tau = sv[cq][0];
-      vi = i == 0 ? 1.0 : (i <= L ? sv[cq][i] : 0.0);
+      const double up = __shfl_down(w, 1, 8);
+      if (mine) w = i < L ? up : (i == L ? sen[c][t] : 0.0);
     }
+  }
+  // the round's own columns: column LJ + cq forms its reflector at step LJ + cq, the later ones apply it
+  for (int q = LJ; q < LJ + L; ++q) {
+    const int cq = q - LJ;  // the new column whose reflector this step forms
+    const double x0 = __shfl(w, 8 * c, 64);
+    const double s2 = gsum(i >= 1 && i <= L ? w * w : 0.0);
+    if (c == cq) {
+      double alpha = x0, t = 0.0, sc = 0.0;
+      if (s2 > 0.0) {
+        const double nx = sqrt(x0 * x0 + s2);
+        alpha = x0 > 0.0 ? -nx : nx;
+        t = (alpha - x0) / alpha;
+        sc = 1.0 / (x0 - alpha);
+      }
+      if (i == 0) {
+        sv[cq][0] = t;
+        Rcol[q] = alpha;
+      } else if (i <= L) {
+        sv[cq][i] = w * sc;
+      }
+      if (i <= L) S.hv[(size_t)q * 8 + i] = i == 0 ? t : w * sc;
+    }
+    __syncthreads();
+    const double tau = sv[cq][0], vi = i == 0 ? 1.0 : (i <= L ? sv[cq][i] : 0.0);
     const bool act = mine && q < qc;  // group c applies the reflectors of the columns before its own
-    const double s = gsum(act ? vi * w : 0.0);
+    const double sd = gsum(act ? vi * w : 0.0);
     if (act) {
-      w -= tau * s * vi;
+      w -= tau * sd * vi;
       if (i == 0) Rcol[q] = w;  // row q of column qc is final
     }
     const double up = __shfl_down(w, 1, 8);
@@ -1544,8 +1572,8 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
   ANDR_HIP(ctx, hipMemcpyAsync(d_arn.p, h_arn.data(), sizeof(double) * h_arn.size(), hipMemcpyHostToDevice, st));
 
   const int gs = (nsys * NB + 63) / 64;
-  // ANDR_GMRES_NO_SKIP=1: converged systems keep their products until the next map rebuild (the
-  // steps after convergence are never used, so the results are the same bit for bit)
+  // ANDR_GMRES_NO_SKIP=1: converged systems keep their products (and block sweeps) until the next map
+  // rebuild (the rounds after convergence are never used, so the results are the same bit for bit)
   const int noskip = getenv("ANDR_GMRES_NO_SKIP") ? 1 : 0;
   // which_x 0: residual R = B - A M^-1 X (R holds B); 1: Krylov product W = A M^-1 Q_j
   auto gemv = [&](const MapSet &ms, int which_x, int jq, const double *scales, bool prof = true) {
@@ -1691,10 +1719,10 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     if ((rc = use_tiles(live))) return rc;
     std::fill(skip_counted.begin(), skip_counted.end(), 0);
     if (blk) {  // Q_0 R0 = R (Cholesky QR of the residual block), g = (R0; 0)
-      hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, 1), dim3(256), 0, st, dsys.p, live.t4.p, 1, 0, 1, partB.p, mc + 2);
-      hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1);
+      hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, 1), dim3(256), 0, st, dsys.p, live.t4.p, 1, 0, 1, partB.p, mc + 2, 1);
+      hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1, 1);
       hipLaunchKernelGGL(kb_binit, dim3(nsys), dim3(64), 0, st, dsys.p, mc, L);
-      hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, 0, 3);
+      hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, 0, 3, 1);
     } else {
       hipLaunchKernelGGL(kb_ginit, dim3(gs), dim3(64), 0, st, dsys.p, nsys, mc);
       hipLaunchKernelGGL(kb_q0, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p);
@@ -1722,14 +1750,14 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
         const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (4.0 * j + 10.0) * vbytes,
                                           2.0 * 64.0 * (4.0 * j + 5.0) * nlive);
         hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 16) / 16), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1,
-                           0, partB.p, mc + 2);
-        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 0);
-        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 0);
+                           0, partB.p, mc + 2, noskip);
+        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 1), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 0, noskip);
+        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 0, noskip);
         hipLaunchKernelGGL(kb_bdots, dim3(live.n_t4, (j + 17) / 16), dim3(256), 0, st, dsys.p, live.t4.p, 0, j + 1,
-                           1, partB.p, mc + 2);
-        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 2), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1);
-        hipLaunchKernelGGL(kb_bsmall, dim3(nsys), dim3(64), 0, st, dsys.p, j, L, tol);
-        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 1);
+                           1, partB.p, mc + 2, noskip);
+        hipLaunchKernelGGL(kb_bdots_final, dim3(nsys, j + 2), dim3(256), 0, st, dsys.p, partB.p, mc + 2, 1, noskip);
+        hipLaunchKernelGGL(kb_bsmall, dim3(nsys), dim3(64), 0, st, dsys.p, j, L, tol, noskip);
+        hipLaunchKernelGGL(kb_bupdate, dim3(live.n_t4), dim3(256), 0, st, dsys.p, live.t4.p, j + 1, 1, noskip);
         andr_prof_end(ctx, tok_o);
       } else {
         const int tok_o = andr_prof_begin(ctx, "gmres_orthogonalise", (2.0 * j + 6.0) * vbytes,
@@ -1778,7 +1806,7 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     total += steps;
     if (blk) {
       hipLaunchKernelGGL(kb_bsolve, dim3(nsys), dim3(64), 0, st, dsys.p, steps, L);
-      hipLaunchKernelGGL(kb_bupdate, dim3(cyc.n_t4), dim3(256), 0, st, dsys.p, cyc.t4.p, steps, 2);
+      hipLaunchKernelGGL(kb_bupdate, dim3(cyc.n_t4), dim3(256), 0, st, dsys.p, cyc.t4.p, steps, 2, 1);
     } else {
       hipLaunchKernelGGL(kb_solve_y, dim3(nsys * NB), dim3(64), sizeof(double) * (mc + 1), st, dsys.p, steps, mc);
       hipLaunchKernelGGL(kb_xupdate, dim3(cyc.n_t), dim3(256), 0, st, dsys.p, cyc.t.p, steps, mc);
