@@ -1057,7 +1057,7 @@ __global__ __launch_bounds__(256) void kb_bupdate(const GSys *__restrict__ sys, 
 // Cholesky factor of an L x L Gram matrix G (row-major, stride 8) with dropped columns: column c is
 // dropped (zero row, zero diagonal) when it is dead or its pivot is at most drop * nrm2[c]; rows of
 // R are written to R (stride 8), the inverse over the kept columns to Ri (full 8 x 8, zero
-// elsewhere). Returns 1 when a live column was dropped. One thread.
+// elsewhere). Returns 1 when a live column with nrm2 > 0 was dropped. One thread.
 __device__ inline int chol_drop(const double *G, const double *nrm2, const bool *live, int L, double drop, double *R,
                                 double *Ri) {
   int brk = 0;
@@ -1069,7 +1069,9 @@ __device__ inline int chol_drop(const double *G, const double *nrm2, const bool 
     double d = G[c * 8 + c];
     for (int t = 0; t < c; ++t) d -= R[t * 8 + c] * R[t * 8 + c];
     if (!live[c] || !(d > drop * nrm2[c])) {
-      if (live[c]) brk = 1;
+      // a live column whose product still had a norm lost its new direction: breakdown; a zero
+      // product (the column has no basis vector, e.g. a right-hand side that repeats another) is not
+      if (live[c] && nrm2[c] > 0.0) brk = 1;
       continue;  // row c stays zero
     }
     const double r = sqrt(d);
