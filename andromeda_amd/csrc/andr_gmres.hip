@@ -1668,10 +1668,21 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
       fz = std::max(fz, (int)sm[5]);
     }
     skip_counted[s] = 1;
-    if (fz < 0 || jlast <= fz) return;
-    const double ne = in[s].op.ne, nskip = jlast - fz;
+    // fz: steps used (independent: set at step fz, whose product ran) or rounds used (block: set in
+    // round fz - 1); the launches after it up to jlast skipped the system
+    const int first_skipped = blk ? fz : fz + 1;
+    if (fz < 0 || jlast < first_skipped) return;
+    const double ne = in[s].op.ne, nskip = jlast - first_skipped + 1;
     andr_prof_adjust(ctx, "kb_gemv_sym", -nskip * (6.0 * ne * ne * 4.0 + in[s].n * NB * 8.0),
                      -nskip * 2.0 * 1.5 * 6.0 * ne * ne * nrhs);
+    if (blk) {  // the block sweeps skip the same rounds, and the normalising update of round fz - 1
+      double units = (fz - 1) + 3.0, fl = 0.0;
+      for (int jj = fz; jj <= jlast; ++jj) {
+        units += 4.0 * jj + 10.0;
+        fl += 4.0 * jj + 5.0;
+      }
+      andr_prof_adjust(ctx, "gmres_orthogonalise", -units * in[s].n * NB * 8.0, -2.0 * 64.0 * fl * in[s].n);
+    }
   };
   std::vector<int> cyc_list(nsys);
   for (int s = 0; s < nsys; ++s) cyc_list[s] = s;
