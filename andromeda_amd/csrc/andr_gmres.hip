@@ -1128,28 +1128,40 @@ __global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, in
   __shared__ double sG[64], sR[64], sRi[64], sN[NB];
   __shared__ double sv[NB][NB];  // reflectors of the round's new columns: tau, v_1 .. v_L
   __shared__ double shv[NB][NB], sen[NB][NB];  // a chunk of 8 earlier reflectors, entering rows
+  __shared__ double sC1[8][64], sC2[8][64];       // a chunk of 8 coefficient blocks
   __shared__ int sbrk, sok[NB];
   const GSys &S = sys[blockIdx.x];
   if (S.nt4 == 0 || (!all && !sys_iterating(S))) return;  // uniform over the block
   const int lane = threadIdx.x, c = lane >> 3, i = lane & 7;
   const double *C1 = S.C1, *C2 = S.C2;
-  // G2 = G1 - sum_b C2_b^T C2_b; squared norms of the product's columns before the sweeps
+  // G2 = G1 - sum_b C2_b^T C2_b; squared norms of the product's columns before the sweeps. The
+  // coefficient blocks pass through LDS 8 at a time (coalesced loads, all in flight together).
   {
     double gs = C2[(size_t)(j + 1) * 64 + c * 8 + i];
-    for (int b = 0; b <= j; ++b) {
-      const double *cb = C2 + (size_t)b * 64;
-      for (int l = 0; l < L; ++l) gs -= cb[l * 8 + c] * cb[l * 8 + i];
+    double w = C2[(size_t)(j + 1) * 64 + c * 9];
+    for (int b0 = 0; b0 <= j; b0 += 8) {
+      const int nb8 = min(8, j + 1 - b0);
+      double v1[8], v2[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        v1[u] = u < nb8 ? C1[(size_t)(b0 + u) * 64 + lane] : 0.0;
+        v2[u] = u < nb8 ? C2[(size_t)(b0 + u) * 64 + lane] : 0.0;
+      }
+      __syncthreads();  // the previous chunk's reads are done
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sC1[u][lane] = v1[u];
+        sC2[u][lane] = v2[u];
+      }
+      __syncthreads();
+      for (int u = 0; u < nb8; ++u)
+        for (int l = 0; l < L; ++l) {
+          gs -= sC2[u][l * 8 + c] * sC2[u][l * 8 + i];
+          w += sC1[u][l * 8 + c] * sC1[u][l * 8 + c];
+        }
     }
     sG[lane] = gs;
-    if (c == i) {
-      double w = C2[(size_t)(j + 1) * 64 + c * 9];
-      for (int b = 0; b <= j; ++b)
-        for (int l = 0; l < L; ++l) {
-          const double v = C1[(size_t)b * 64 + l * 8 + c];
-          w += v * v;
-        }
-      sN[c] = w;
-    }
+    if (c == i) sN[c] = w;
   }
   __syncthreads();
   if (lane == 0) {
