@@ -200,9 +200,10 @@ def test_converged_system_skip_is_exact(ctx, monkeypatch):
 def test_block_gmres_vs_independent(ctx, monkeypatch):
     """Block GMRES (one Krylov space for the six rigid-body problems of a body, the default) and
     one Krylov space per problem (ANDR_GMRES_INDEPENDENT=1) solve the same systems: on superellipsoids
-    and fibres of the batched workload both stay within 1e-6 of the plain FP64 path and within 1e-7
-    of each other, and the block form needs fewer rounds than the slowest column of the other needs
-    steps (a round and a step each read the body's matrix once)."""
+    and fibres of the batched workload both stay within 1e-6 of the plain FP64 path and within 2e-7
+    of each other (each stops at half the tolerance on its own least-squares estimate, ~1e-8 to 7e-8
+    from the plain path), and the block form needs fewer rounds than the slowest column of the other
+    needs steps (a round and a step each read the body's matrix once)."""
     allb = [m for m, _ in M.batch_bodies(512, 64)]
     bodies = [allb[i] for i in (29, 204, 523, 534)]  # superellipsoids of 500 and 2420, fibres of 496 and 2000
     plain = ctx.resistance_batch(bodies, precision=api.PREC_FP64, eps=1e-12)
@@ -215,7 +216,7 @@ def test_block_gmres_vs_independent(ctx, monkeypatch):
               f"independent {tensor_error(b.FT, p.FT):.1e}")
         assert a.status == 0 and b.status == 0
         assert tensor_error(a.FT, p.FT) < 1e-6 and tensor_error(b.FT, p.FT) < 1e-6
-        assert tensor_error(a.FT, b.FT) < 1e-7
+        assert tensor_error(a.FT, b.FT) < 2e-7
         assert a.nit.max() < b.nit.max()
 
 
