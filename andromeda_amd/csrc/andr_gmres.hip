@@ -865,14 +865,10 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 }
 
 // ---- Block GMRES ------------------------------------------------------------------------------
-// a column of the system still iterates in the current cycle (the sweeps skip a system whose columns
-// have all converged: its later rounds are never used)
-__device__ inline bool sys_iterating(const GSys &S) {
-  bool need = false;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) need |= S.small[8 * k + 4] > 0.0 && S.small[8 * k + 5] < 0.0;
-  return need;
-}
+// a column of the system still iterates in the current block cycle (the sweeps skip a system whose
+// columns have all converged: its later rounds are never used). Slot 6 of column 0's state holds it:
+// set by kb_binit, cleared by kb_bsmall in the round that freezes every live column at once.
+__device__ inline bool sys_iterating(const GSys &S) { return S.small[6] > 0.0; }
 
 // One Krylov space for the L right-hand sides of a system: round j multiplies the basis block Q_j
 // (n x L, orthonormal columns; a column that takes no part is zero) and orthogonalises W = A M^-1 Q_j
@@ -1104,13 +1100,15 @@ __global__ __launch_bounds__(64) void kb_binit(const GSys *__restrict__ sys, int
   if (!S.incyc) return;  // uniform over the block
   const int lane = threadIdx.x;
   if (lane == 0) {
-    bool live[NB];
+    bool live[NB], any = false;
     double nrm2[NB];
     for (int c = 0; c < NB; ++c) {
       live[c] = c < L && S.small[8 * c + 4] > 0.0;
+      any = any || live[c];
       nrm2[c] = S.C2[c * 9];
     }
     (void)chol_drop(S.C2, nrm2, live, L, 1e-24, sR, sRi);
+    S.small[6] = any ? 1.0 : 0.0;  // iterating in this cycle
   }
   __syncthreads();
   S.Ri[lane] = sRi[lane];
@@ -1280,7 +1278,10 @@ __global__ __launch_bounds__(64) void kb_bsmall(const GSys *__restrict__ sys, in
       all = true;
       for (int kk = 0; kk < NB; ++kk) all = all && sok[kk];
     }
-    if (all) sm[5] = (double)(j + 1);  // rounds to use; the product skips the system from now on
+    if (all) {
+      sm[5] = (double)(j + 1);  // rounds to use; the product skips the system from now on
+      S.small[6] = 0.0;         // and the sweeps (every live column writes the same value)
+    }
   }
 }
 
