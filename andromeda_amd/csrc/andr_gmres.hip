@@ -86,15 +86,22 @@ struct TileMap {
 typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int MAXW = ANDR_OP_MAXW;
 
+// a column of the system still iterates in the current block cycle (the sweeps skip a system whose
+// columns have all converged: its later rounds are never used). Slot 6 of column 0's state holds it:
+// set by kb_binit, cleared by kb_bsmall in the round that freezes every live column at once.
+__device__ inline bool sys_iterating(const GSys &S) { return S.small[6] > 0.0; }
+
 // ---- structured Stokes operator --------------------------------------------------------------
 // per tile of 32 elements (256 threads = 32 elements x 8 right-hand sides): z_e = M^-1_e v_e for the
 // three components of element e, v = Q_j (mode 0, z to ZT or ZT32) or v = X (mode 1, z to ZT in
 // FP64); partial sums of n_e . z_e per wall (1..MAXW) and rhs for the null-space term.
 __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, const TileMap *__restrict__ emap, int mode,
-                                               int jq, double *__restrict__ part_s) {
+                                               int jq, double *__restrict__ part_s, int all) {
   __shared__ double red[4][MAXW][NB];
   const TileMap tm = emap[blockIdx.x];
   const GSys &S = sys[tm.sys];
+  // block GMRES: a converged system's later products are never used (the product skips its matrix)
+  if (!all && !sys_iterating(S)) return;  // uniform over the block
   const StokesOp &O = S.op;
   const int ne = O.ne, n = S.n;
   const int k = threadIdx.x & 7, el = threadIdx.x >> 3;
@@ -145,10 +152,12 @@ __global__ __launch_bounds__(256) void kb_prep(const GSys *__restrict__ sys, con
 
 // block per system, thread per (wall, rhs) and eighth of the element tiles: sdef = sum of the
 // system's tile partials
-__global__ __launch_bounds__(256) void kb_sred(const GSys *__restrict__ sys, const double *__restrict__ part_s) {
+__global__ __launch_bounds__(256) void kb_sred(const GSys *__restrict__ sys, const double *__restrict__ part_s,
+                                               int all) {
   static_assert(MAXW * NB == 32, "eight partial sums per (wall, rhs)");
   __shared__ double red[8][32];
   const GSys &S = sys[blockIdx.x];
+  if (!all && !sys_iterating(S)) return;  // as kb_prep (uniform over the block)
   const int r = threadIdx.x & 31, p = threadIdx.x >> 5;
   double a = 0.0;
   for (int i = p; i < S.net; i += 8) a += part_s[(size_t)(S.et0 + i) * MAXW * NB + r];
@@ -865,10 +874,6 @@ __global__ __launch_bounds__(256) void kb_q0(const GSys *__restrict__ sys, const
 }
 
 // ---- Block GMRES ------------------------------------------------------------------------------
-// a column of the system still iterates in the current block cycle (the sweeps skip a system whose
-// columns have all converged: its later rounds are never used). Slot 6 of column 0's state holds it:
-// set by kb_binit, cleared by kb_bsmall in the round that freezes every live column at once.
-__device__ inline bool sys_iterating(const GSys &S) { return S.small[6] > 0.0; }
 
 // One Krylov space for the L right-hand sides of a system: round j multiplies the basis block Q_j
 // (n x L, orthonormal columns; a column that takes no part is zero) and orthogonalises W = A M^-1 Q_j
@@ -1605,8 +1610,11 @@ int andr_gmres_batch(andr_ctx *ctx, const GmresSys *in, int nsys, bool fp32, int
     const int tok = prof ? andr_prof_begin(ctx, krylov ? "kb_gemv_sym" : "gmres_residual", n2 * 4.0 + 2.0 * n1 * NB * 8.0,
                                            2.0 * 1.5 * n2 * nrhs)
                          : -1;
-    hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p);
-    hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p);
+    // the block solver's Krylov products skip converged systems' operands too (the flag read by
+    // sys_iterating is kept by the block kernels only)
+    const int all = (krylov && blk) ? noskip : 1;
+    hipLaunchKernelGGL(kb_prep, dim3(ms.n_et), dim3(256), 0, st, dsys.p, ms.et.p, krylov ? 0 : 1, jq, partS.p, all);
+    hipLaunchKernelGGL(kb_sred, dim3(nsys), dim3(256), 0, st, dsys.p, partS.p, all);
     if (krylov && f32prod)
       hipLaunchKernelGGL(kb_gemv_sym<true>, dim3((ms.n_mw + 3) / 4), dim3(256), 0, st, dsys.p, ms.mw.p, ms.n_mw,
                          noskip);
